@@ -1,0 +1,823 @@
+// Host side of the MI355X IVF-flat library: the index object behind the C-ABI of
+// include/harmony_ivf.h. It mirrors harmony::Index (src/Index.h:13-149): train (k-means),
+// add (assignment + list order), search (coarse -> warm-up bound -> pruned list scan ->
+// top-k), and keeps every list on the GPU in the list-major, dimension-major layout the
+// kernels read. All device work is issued on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/harmony_ivf.h"
+#include "hv_device.h"
+#include "hv_kernels.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) cap = b;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct HvError {
+    int code;
+    std::string msg;
+};
+
+#define HV_HIP(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            throw HvError{HV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)};              \
+    } while (0)
+
+#define HV_CHECK_LAUNCH(what)                                                                             \
+    do {                                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                                \
+        if (e_ != hipSuccess) throw HvError{HV_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+void require(bool ok, int code, const std::string& msg) {
+    if (!ok) throw HvError{code, msg};
+}
+
+// SplitMix64 + Fisher-Yates: the shuffle used for the k-means subsample and init
+// (replacing std::shuffle(default_random_engine(seed)) of Clustering.cpp:58,81 with a
+// portable stream).
+struct SplitMix64 {
+    uint64_t s;
+    explicit SplitMix64(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+};
+
+void shuffle_indices(std::vector<int64_t>& v, uint64_t seed) {
+    SplitMix64 rng(seed);
+    for (int64_t i = (int64_t)v.size() - 1; i > 0; --i) {
+        int64_t j = (int64_t)(rng.next() % (uint64_t)(i + 1));
+        std::swap(v[i], v[j]);
+    }
+}
+
+// Subsample (Clustering.cpp:48-68) and init (72-89): returns the sample row indices and the
+// sample rows that seed the centroids.
+void kmeans_pick(int64_t n, int64_t nlist, int64_t max_pts, uint64_t seed, std::vector<int64_t>& sample_rows,
+                 std::vector<int64_t>& init_rows) {
+    sample_rows.resize(n);
+    std::iota(sample_rows.begin(), sample_rows.end(), 0);
+    if (max_pts > 0 && n > nlist * max_pts) {
+        shuffle_indices(sample_rows, seed);
+        sample_rows.resize(nlist * max_pts);
+    }
+    const int64_t ns = (int64_t)sample_rows.size();
+    init_rows.resize(ns);
+    std::iota(init_rows.begin(), init_rows.end(), 0);
+    shuffle_indices(init_rows, seed);
+    init_rows.resize(nlist);
+}
+
+// Dimension blocks of node.cpp:639-646 (B blocks of floor(d/B) dims, the last takes the
+// rest), each cut into staging chunks of at most kChunkDims dims.
+hv::ChunkPlan make_plan(int d, int B) {
+    hv::ChunkPlan p;
+    std::memset(&p, 0, sizeof(p));
+    const int w = d / B;
+    for (int b = 0; b < B; ++b) {
+        const int lo = b * w;
+        const int hi = (b == B - 1) ? d : lo + w;
+        for (int c = lo; c < hi; c += hv::kChunkDims) {
+            require(p.n < hv::kMaxChunks, HV_ERR_CAPACITY, "too many dimension chunks (d too large for B)");
+            p.lo[p.n] = c;
+            p.len[p.n] = std::min(hv::kChunkDims, hi - c);
+            p.ends_block[p.n] = (c + hv::kChunkDims >= hi) ? 1 : 0;
+            p.prune_after[p.n] = (p.ends_block[p.n] && b != B - 1) ? 1 : 0;
+            p.n++;
+        }
+    }
+    return p;
+}
+
+}  // namespace
+
+struct hv_index {
+    int64_t d = 0, nlist = 0;
+    int metric = HV_METRIC_L2;
+    int opt_level = HV_OPT_NONE, added_opt_level = HV_OPT_NONE;
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    int scan_grid = 0;
+
+    std::vector<int> perm;  // empty = identity
+    DevBuf d_perm;
+
+    bool trained = false;
+    std::vector<float> centroids;  // index dim order
+    DevBuf d_centroids;
+
+    bool added = false;
+    int64_t ntotal = 0;
+    std::vector<int64_t> list_off, code_off, ids;  // ids in list order
+    std::vector<int> list_stride, rank_order;
+    DevBuf d_codes, d_list_off, d_code_off, d_list_stride, d_ids, d_rank_order;
+
+    hv_search_params sp{};
+    std::vector<float> ext_bounds;
+
+    // search scratch (grow-only)
+    DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
+        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats;
+    int64_t last_n = 0, last_nprobe = 0;
+};
+
+namespace {
+
+void set_err(hv_index_t* h, const HvError& e) {
+    if (h) h->err = e.msg;
+}
+
+template <typename F>
+int guarded(hv_index_t* h, F&& f) {
+    try {
+        f();
+        if (h) h->err.clear();
+        return HV_OK;
+    } catch (const HvError& e) {
+        set_err(h, e);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (h) h->err = "host allocation failed";
+        return HV_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        if (h) h->err = e.what();
+        return HV_ERR_INVALID_ARG;
+    }
+}
+
+void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+    HV_HIP(b.ensure(bytes));
+    if (bytes) HV_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+// Nearest centroid of every row of xp (index dim order, on the device): labels + distances.
+void assign_rows(hv_index_t* h, const float* xp, int64_t n, int* labels, float* best) {
+    const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (h->nlist * 4)));
+    HV_HIP(h->s_dist.ensure(sizeof(float) * per * h->nlist));
+    for (int64_t r0 = 0; r0 < n; r0 += per) {
+        const int64_t rows = std::min(per, n - r0);
+        hv::launch_pair_l2_full(xp + r0 * h->d, rows, h->d_centroids.as<float>(), h->nlist, (int)h->d,
+                                h->s_dist.as<float>(), h->nlist, h->stream);
+        hv::launch_argmin_rows(h->s_dist.as<float>(), rows, (int)h->nlist, labels + r0, best ? best + r0 : nullptr,
+                               h->stream);
+        HV_CHECK_LAUNCH("assignment");
+    }
+}
+
+// Upload rows in caller dim order and return a device pointer to them in index dim order.
+const float* upload_rows(hv_index_t* h, const float* x, int64_t n, DevBuf& raw, DevBuf& permuted) {
+    upload(raw, x, sizeof(float) * n * h->d, h->stream);
+    if (h->perm.empty()) return raw.as<float>();
+    HV_HIP(permuted.ensure(sizeof(float) * n * h->d));
+    hv::launch_permute_rows(raw.as<float>(), n, (int)h->d, h->d_perm.as<int>(), permuted.as<float>(), h->stream);
+    HV_CHECK_LAUNCH("permute");
+    return permuted.as<float>();
+}
+
+void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D, int64_t* I, hv_stats* st) {
+    require(h->trained && h->added, HV_ERR_STATE, "search before train/add");
+    require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
+    require(k >= 1 && k <= hv::kMaxK, HV_ERR_CAPACITY, "k must be in [1, 1024]");
+    require((h->sp.opt_level & h->added_opt_level) == h->sp.opt_level, HV_ERR_INVALID_ARG,
+            "opt_level is not subset of added_opt_level");  // Index.cpp:1642-1645
+    if (n == 0) return;
+    require(q_dev && D && I, HV_ERR_INVALID_ARG, "null buffer");
+    const int64_t d = h->d, nlist = h->nlist;
+    int64_t nprobe = std::min<int64_t>(h->sp.nprobe, nlist);  // Index.cpp:1646-1648
+    require(nprobe >= 1, HV_ERR_INVALID_ARG, "nprobe must be >= 1");
+    const int B = h->sp.pruning ? h->sp.dim_blocks : 1;
+    require(h->sp.dim_blocks >= 1 && h->sp.dim_blocks <= d, HV_ERR_INVALID_ARG, "dim_blocks must be in [1, d]");
+    require(nlist <= hv::kMaxSelectRow, HV_ERR_CAPACITY, "nlist > 8192");
+    const hv::ChunkPlan plan = make_plan((int)d, B);
+    int64_t wl = std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
+    int64_t wls = h->sp.warmup_list_size == 0 ? k : h->sp.warmup_list_size;  // query.cpp:328-333
+    if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
+        require(wl * wls <= hv::kMaxWarmup, HV_ERR_CAPACITY, "warmup_list * warmup_list_size > 8192");
+    if (h->sp.bound_mode == HV_BOUND_EXTERNAL)
+        require((int64_t)h->ext_bounds.size() == n, HV_ERR_STATE, "external bounds not set for this batch");
+    int64_t cap = h->sp.survivor_cap;
+    if (cap == 0) cap = std::min<int64_t>(8192, std::max<int64_t>(4096, hv_next_pow2((int)(4 * k))));
+    require(cap >= k && cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap must be in [k, 8192]");
+    require((int64_t)n * nprobe < (1ll << 31), HV_ERR_CAPACITY, "n * nprobe >= 2^31");
+
+    hipStream_t s = h->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    HV_HIP(hipEventRecord(h->ev[0], s));
+
+    // 1. queries in index dim order
+    const float* Qp = q_dev;
+    if (!h->perm.empty()) {
+        HV_HIP(h->s_qp.ensure(sizeof(float) * n * d));
+        hv::launch_permute_rows(q_dev, n, (int)d, h->d_perm.as<int>(), h->s_qp.as<float>(), s);
+        Qp = h->s_qp.as<float>();
+    }
+    // 2. coarse: full distance rows + nprobe select (Index.cpp:967-997)
+    HV_HIP(h->s_probes.ensure(sizeof(int) * n * nprobe));
+    int* probes = h->s_probes.as<int>();
+    {
+        const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (nlist * 4)));
+        HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
+        for (int64_t r0 = 0; r0 < n; r0 += per) {
+            const int64_t rows = std::min(per, n - r0);
+            hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d,
+                                    h->s_dist.as<float>(), nlist, s);
+            hv::launch_select_rows(h->s_dist.as<float>(), rows, (int)nlist, (int)nprobe, probes + r0 * nprobe,
+                                   nullptr, s);
+        }
+        HV_CHECK_LAUNCH("coarse");
+    }
+    HV_HIP(hipEventRecord(h->ev[1], s));
+    // 3. bound
+    HV_HIP(h->s_bound.ensure(sizeof(float) * n));
+    float* bound = h->s_bound.as<float>();
+    if (h->sp.bound_mode == HV_BOUND_EXTERNAL) {
+        HV_HIP(hipMemcpyAsync(bound, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+    } else if (wl > 0) {
+        hv::launch_warmup_bound(Qp, n, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                                h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), probes, (int)nprobe,
+                                (int)wl, (int)wls, (int)k, plan, bound, s);
+    } else {
+        int fm;
+        const float fmax = FLT_MAX;
+        std::memcpy(&fm, &fmax, 4);
+        hv::launch_fill_i32(reinterpret_cast<int*>(bound), n, fm, s);
+    }
+    HV_CHECK_LAUNCH("bound");
+    HV_HIP(hipEventRecord(h->ev[2], s));
+    // 4. grouping by (list, rank) + work plan
+    const int64_t nb = nlist * nprobe;
+    HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
+    HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
+    HV_HIP(h->s_bucket_off.ensure(sizeof(int) * (nb + 1)));
+    HV_HIP(h->s_scan_tmp.ensure(sizeof(int) * hv::scan_tmp_ints(nb)));
+    HV_HIP(h->s_pairs.ensure(sizeof(int) * n * nprobe));
+    const int64_t qcap64 = n * nprobe / hv::kQueryTile + nlist + 1;
+    require(qcap64 < (1ll << 28), HV_ERR_CAPACITY, "too many work items");
+    const int qcap = (int)qcap64;
+    HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * hv::kNumXcd * qcap));
+    HV_HIP(h->s_queue.ensure(sizeof(int) * 2 * hv::kNumXcd));
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 4));
+    unsigned long long* stats = h->s_stats.as<unsigned long long>();
+    hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
+    hv::launch_fill_i32(h->s_fill.as<int>(), nb, 0, s);
+    hv::launch_fill_u64(stats, 4, 0ull, s);
+    hv::launch_count_pairs(probes, n, (int)nprobe, h->s_bucket.as<int>(), s);
+    hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
+    hv::launch_scatter_pairs(probes, n, (int)nprobe, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
+                             h->s_pairs.as<int>(), s);
+    int* queue_len = h->s_queue.as<int>();
+    int* queue_head = queue_len + hv::kNumXcd;
+    hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, h->d_rank_order.as<int>(),
+                          h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue_len, queue_head,
+                          stats, s);
+    HV_CHECK_LAUNCH("grouping");
+    HV_HIP(hipEventRecord(h->ev[3], s));
+    // 5. list scan
+    HV_HIP(h->s_surv.ensure(sizeof(unsigned long long) * n * cap));
+    HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
+    hv::launch_fill_i32(h->s_surv_cnt.as<int>(), n, 0, s);
+    hv::launch_list_scan(Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                         h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), plan,
+                         B > 1 ? 1 : 0, bound, h->s_pairs.as<int>(), h->s_items.as<hv::ScanItem>(), queue_len,
+                         queue_head, qcap, h->s_surv.as<unsigned long long>(), h->s_surv_cnt.as<unsigned int>(),
+                         (int)cap, stats, h->scan_grid, s);
+    HV_CHECK_LAUNCH("list scan");
+    HV_HIP(hipEventRecord(h->ev[4], s));
+    // 6. top-k selection (+ exact path for overflowed buffers)
+    HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 1)));
+    int* ovf_cnt = h->s_overflow.as<int>();
+    int* ovf_list = ovf_cnt + 1;
+    hv::launch_fill_i32(ovf_cnt, 1, 0, s);
+    hv::launch_select_survivors(h->s_surv.as<unsigned long long>(), h->s_surv_cnt.as<unsigned int>(), n, (int)cap,
+                                (int)k, D, I, ovf_list, ovf_cnt, stats, s);
+    HV_CHECK_LAUNCH("select");
+    int novf = 0;
+    HV_HIP(hipMemcpyAsync(&novf, ovf_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+    HV_HIP(hipStreamSynchronize(s));
+    if (novf > 0) {
+        hv::launch_query_exact(ovf_list, novf, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                               h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(),
+                               probes, (int)nprobe, plan, B > 1 ? 1 : 0, bound, (int)k, D, I, s);
+        HV_CHECK_LAUNCH("exact fallback");
+    }
+    HV_HIP(hipEventRecord(h->ev[5], s));
+    unsigned long long hs[4] = {0, 0, 0, 0};
+    HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HV_HIP(hipStreamSynchronize(s));
+    h->last_n = n;
+    h->last_nprobe = nprobe;
+    if (st) {
+        float ms[5];
+        for (int i = 0; i < 5; ++i) HV_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+        st->ms_coarse += ms[0];
+        st->ms_warmup += ms[1];
+        st->ms_group += ms[2];
+        st->ms_scan += ms[3];
+        st->ms_select += ms[4];
+        st->candidates += (int64_t)hs[0];
+        st->block_evals += (int64_t)hs[1];
+        st->block_evals_max += (int64_t)hs[0] * B;
+        st->survivors += (int64_t)hs[2];
+        st->overflow_queries += novf;
+        st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_level) {
+    g_create_error.clear();
+    if (d <= 0 || nlist <= 0) {
+        g_create_error = "d and nlist must be positive";
+        return nullptr;
+    }
+    if (metric != HV_METRIC_L2) {
+        g_create_error = "only METRIC_L2 is implemented";
+        return nullptr;
+    }
+    if (opt_level < 0 || opt_level > HV_OPT_ALL) {
+        g_create_error = "invalid opt_level";
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_error = "no GPU device visible";
+        return nullptr;
+    }
+    hv_index_t* h = new hv_index_t();
+    h->d = d;
+    h->nlist = nlist;
+    h->metric = metric;
+    h->opt_level = opt_level;
+    hv_default_search_params(&h->sp);
+    if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        delete h;
+        return nullptr;
+    }
+    for (auto& e : h->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            g_create_error = "hipEventCreate failed";
+            delete h;
+            return nullptr;
+        }
+    }
+    h->scan_grid = hv::scan_grid_size(h->device);
+    return h;
+}
+
+void hv_destroy(hv_index_t* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
+                      &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
+                      &h->s_bound, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
+                      &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats};
+    for (DevBuf* b : bufs) b->release();
+    for (auto& e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char* hv_last_error(const hv_index_t* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int hv_set_dim_order(hv_index_t* h, const int32_t* perm) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(!h->trained, HV_ERR_STATE, "dim order must be set before train");
+        if (!perm) {
+            h->perm.clear();
+            return;
+        }
+        std::vector<int> p(perm, perm + h->d), seen(h->d, 0);
+        for (int v : p) {
+            require(v >= 0 && v < h->d && !seen[v], HV_ERR_INVALID_ARG, "not a permutation of 0..d-1");
+            seen[v] = 1;
+        }
+        h->perm = p;
+        upload(h->d_perm, p.data(), sizeof(int) * h->d, h->stream);
+        HV_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+void hv_default_kmeans_params(hv_kmeans_params* p) {
+    if (!p) return;
+    p->niter = 10;
+    p->seed = 0;
+    p->max_points_per_centroid = 256;
+}
+
+int hv_set_centroids(hv_index_t* h, const float* c) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(c != nullptr, HV_ERR_INVALID_ARG, "null centroids");
+        h->centroids.assign(h->nlist * h->d, 0.f);
+        for (int64_t l = 0; l < h->nlist; ++l)
+            for (int64_t t = 0; t < h->d; ++t)
+                h->centroids[l * h->d + t] = c[l * h->d + (h->perm.empty() ? t : h->perm[t])];
+        upload(h->d_centroids, h->centroids.data(), sizeof(float) * h->nlist * h->d, h->stream);
+        HV_HIP(hipStreamSynchronize(h->stream));
+        h->trained = true;
+        h->added = false;
+    });
+}
+
+int hv_train(hv_index_t* h, int64_t n, const float* x, const hv_kmeans_params* kp) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(x != nullptr, HV_ERR_INVALID_ARG, "null x");
+        require(n >= h->nlist, HV_ERR_INVALID_ARG, "need at least nlist training vectors");
+        hv_kmeans_params p;
+        hv_default_kmeans_params(&p);
+        if (kp) p = *kp;
+        require(p.niter >= 0 && p.max_points_per_centroid >= 0, HV_ERR_INVALID_ARG, "bad k-means params");
+        const int64_t d = h->d, nlist = h->nlist;
+        std::vector<int64_t> rows, init;
+        kmeans_pick(n, nlist, p.max_points_per_centroid, p.seed, rows, init);
+        const int64_t ns = (int64_t)rows.size();
+        require(ns < (1ll << 31), HV_ERR_CAPACITY, "training sample too large");
+        std::vector<float> sample(ns * d);
+        for (int64_t i = 0; i < ns; ++i) std::memcpy(&sample[i * d], x + rows[i] * d, sizeof(float) * d);
+        DevBuf raw, perm_buf, labels, off_buf, order_buf;
+        const float* xp = upload_rows(h, sample.data(), ns, raw, perm_buf);
+        h->centroids.assign(nlist * d, 0.f);
+        for (int64_t c = 0; c < nlist; ++c)
+            for (int64_t t = 0; t < d; ++t)
+                h->centroids[c * d + t] = sample[init[c] * d + (h->perm.empty() ? t : h->perm[t])];
+        upload(h->d_centroids, h->centroids.data(), sizeof(float) * nlist * d, h->stream);
+        HV_HIP(labels.ensure(sizeof(int) * ns));
+        HV_HIP(off_buf.ensure(sizeof(int64_t) * (nlist + 1)));
+        HV_HIP(order_buf.ensure(sizeof(int) * ns));
+        std::vector<int> lab(ns), order(ns);
+        std::vector<int64_t> off(nlist + 1);
+        for (int64_t it = 0; it < p.niter; ++it) {
+            assign_rows(h, xp, ns, labels.as<int>(), nullptr);
+            HV_HIP(hipMemcpyAsync(lab.data(), labels.p, sizeof(int) * ns, hipMemcpyDeviceToHost, h->stream));
+            HV_HIP(hipStreamSynchronize(h->stream));
+            // stable counting sort of the sample by cluster (ascending point order inside)
+            std::fill(off.begin(), off.end(), 0);
+            for (int64_t i = 0; i < ns; ++i) off[lab[i] + 1]++;
+            for (int64_t c = 0; c < nlist; ++c) off[c + 1] += off[c];
+            std::vector<int64_t> cur(off.begin(), off.end() - 1);
+            for (int64_t i = 0; i < ns; ++i) order[cur[lab[i]]++] = (int)i;
+            HV_HIP(hipMemcpyAsync(off_buf.p, off.data(), sizeof(int64_t) * (nlist + 1), hipMemcpyHostToDevice,
+                                  h->stream));
+            HV_HIP(hipMemcpyAsync(order_buf.p, order.data(), sizeof(int) * ns, hipMemcpyHostToDevice, h->stream));
+            hv::launch_centroid_update_f64(xp, (int)d, order_buf.as<int>(), off_buf.as<int64_t>(), (int)nlist,
+                                           h->d_centroids.as<float>(), h->stream);
+            HV_CHECK_LAUNCH("centroid update");
+        }
+        HV_HIP(hipMemcpyAsync(h->centroids.data(), h->d_centroids.p, sizeof(float) * nlist * d,
+                              hipMemcpyDeviceToHost, h->stream));
+        HV_HIP(hipStreamSynchronize(h->stream));
+        raw.release();
+        perm_buf.release();
+        labels.release();
+        off_buf.release();
+        order_buf.release();
+        h->trained = true;
+        h->added = false;
+    });
+}
+
+// Index::add (Index.cpp:581-696): every vector goes to its nearest centroid (ties to the
+// lower id); inside a list, vectors are ordered by ascending distance to the centroid, ties
+// by id. Like the reference (IVF::reset, Index.cpp:636-639), add rebuilds the lists from the
+// given vectors, whose ids are their positions 0..n-1.
+int hv_add(hv_index_t* h, int64_t n, const float* x) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(h->trained, HV_ERR_STATE, "add before train");
+        require(x != nullptr && n > 0, HV_ERR_INVALID_ARG, "empty add");
+        require(n < 0xffffffffll, HV_ERR_CAPACITY, "ntotal must be < 2^32 - 1");
+        const int64_t d = h->d, nlist = h->nlist;
+        DevBuf raw, perm_buf, labels, best, order_buf;
+        const float* xp = upload_rows(h, x, n, raw, perm_buf);
+        HV_HIP(labels.ensure(sizeof(int) * n));
+        HV_HIP(best.ensure(sizeof(float) * n));
+        assign_rows(h, xp, n, labels.as<int>(), best.as<float>());
+        std::vector<int> lab(n);
+        std::vector<float> dist(n);
+        HV_HIP(hipMemcpyAsync(lab.data(), labels.p, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        HV_HIP(hipMemcpyAsync(dist.data(), best.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+        HV_HIP(hipStreamSynchronize(h->stream));
+        labels.release();
+        best.release();
+        std::vector<int64_t> off(nlist + 1, 0);
+        for (int64_t i = 0; i < n; ++i) off[lab[i] + 1]++;
+        for (int64_t l = 0; l < nlist; ++l) off[l + 1] += off[l];
+        std::vector<int64_t> cur(off.begin(), off.end() - 1);
+        std::vector<int> order(n);
+        for (int64_t i = 0; i < n; ++i) order[cur[lab[i]]++] = (int)i;
+        for (int64_t l = 0; l < nlist; ++l)
+            std::sort(order.begin() + off[l], order.begin() + off[l + 1], [&](int a, int b) {
+                return dist[a] != dist[b] ? dist[a] < dist[b] : a < b;
+            });
+        h->list_off = off;
+        h->list_stride.assign(nlist, 0);
+        h->code_off.assign(nlist + 1, 0);
+        for (int64_t l = 0; l < nlist; ++l) {
+            const int64_t sz = off[l + 1] - off[l];
+            h->list_stride[l] = (int)((sz + 15) / 16 * 16);
+            h->code_off[l + 1] = h->code_off[l] + d * h->list_stride[l];
+        }
+        h->ids.assign(order.begin(), order.end());
+        h->rank_order.resize(nlist);
+        std::iota(h->rank_order.begin(), h->rank_order.end(), 0);
+        std::stable_sort(h->rank_order.begin(), h->rank_order.end(), [&](int a, int b) {
+            return off[a + 1] - off[a] > off[b + 1] - off[b];
+        });
+        std::vector<uint32_t> ids32(order.begin(), order.end());
+        upload(order_buf, order.data(), sizeof(int) * n, h->stream);
+        upload(h->d_list_off, off.data(), sizeof(int64_t) * (nlist + 1), h->stream);
+        upload(h->d_code_off, h->code_off.data(), sizeof(int64_t) * (nlist + 1), h->stream);
+        upload(h->d_list_stride, h->list_stride.data(), sizeof(int) * nlist, h->stream);
+        upload(h->d_ids, ids32.data(), sizeof(uint32_t) * n, h->stream);
+        upload(h->d_rank_order, h->rank_order.data(), sizeof(int) * nlist, h->stream);
+        HV_HIP(h->d_codes.ensure(sizeof(float) * std::max<int64_t>(1, h->code_off[nlist])));
+        hv::launch_build_layout(xp, (int)d, order_buf.as<int>(), h->d_list_off.as<int64_t>(),
+                                h->d_code_off.as<int64_t>(), h->d_list_stride.as<int>(), (int)nlist,
+                                h->d_codes.as<float>(), h->stream);
+        HV_CHECK_LAUNCH("layout");
+        HV_HIP(hipStreamSynchronize(h->stream));
+        raw.release();
+        perm_buf.release();
+        order_buf.release();
+        h->ntotal = n;
+        h->added = true;
+        h->added_opt_level = h->opt_level;
+    });
+}
+
+void hv_default_search_params(hv_search_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nprobe = 1;
+    p->opt_level = HV_OPT_NONE;
+    p->dim_blocks = 4;
+    p->pruning = 1;
+    p->bound_mode = HV_BOUND_WARMUP;
+    p->warmup_list = 10;
+    p->warmup_list_size = 0;
+    p->precision = HV_PRECISION_FP32;
+    p->survivor_cap = 0;
+}
+
+int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(p != nullptr, HV_ERR_INVALID_ARG, "null params");
+        require(p->nprobe >= 1, HV_ERR_INVALID_ARG, "nprobe must be >= 1");
+        require(p->dim_blocks >= 1 && p->dim_blocks <= h->d, HV_ERR_INVALID_ARG, "dim_blocks must be in [1, d]");
+        require(p->opt_level >= 0 && p->opt_level <= HV_OPT_ALL, HV_ERR_INVALID_ARG, "invalid opt_level");
+        require(p->bound_mode == HV_BOUND_WARMUP || p->bound_mode == HV_BOUND_EXTERNAL, HV_ERR_INVALID_ARG,
+                "invalid bound_mode");
+        require(p->warmup_list >= 0 && p->warmup_list_size >= 0, HV_ERR_INVALID_ARG, "negative warm-up size");
+        require(p->precision == HV_PRECISION_FP32, HV_ERR_INVALID_ARG, "unsupported precision");
+        require(p->survivor_cap >= 0 && p->survivor_cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap > 8192");
+        h->sp = *p;
+    });
+}
+
+int hv_get_search_params(const hv_index_t* h, hv_search_params* p) {
+    if (!h || !p) return HV_ERR_INVALID_ARG;
+    *p = h->sp;
+    return HV_OK;
+}
+
+int hv_set_external_bounds(hv_index_t* h, int64_t n, const float* bounds) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(n >= 0 && (bounds || n == 0), HV_ERR_INVALID_ARG, "bad bounds");
+        h->ext_bounds.assign(bounds, bounds + n);
+    });
+}
+
+int hv_search_device(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D_dev, int64_t* I_dev,
+                     hv_stats* st) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        if (st) std::memset(st, 0, sizeof(*st));
+        do_search(h, n, q_dev, k, D_dev, I_dev, st);
+    });
+}
+
+int hv_search(hv_index_t* h, int64_t n, const float* q, int64_t k, float* D, int64_t* I, hv_stats* st) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        if (st) std::memset(st, 0, sizeof(*st));
+        require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
+        require(k >= 1 && k <= hv::kMaxK, HV_ERR_CAPACITY, "k must be in [1, 1024]");
+        if (n == 0) return;
+        require(q && D && I, HV_ERR_INVALID_ARG, "null buffer");
+        auto t0 = std::chrono::steady_clock::now();
+        upload(h->s_q, q, sizeof(float) * n * h->d, h->stream);
+        HV_HIP(h->s_D.ensure(sizeof(float) * n * k));
+        HV_HIP(h->s_I.ensure(sizeof(int64_t) * n * k));
+        HV_HIP(hipStreamSynchronize(h->stream));
+        auto t1 = std::chrono::steady_clock::now();
+        do_search(h, n, h->s_q.as<float>(), k, h->s_D.as<float>(), h->s_I.as<int64_t>(), st);
+        auto t2 = std::chrono::steady_clock::now();
+        HV_HIP(hipMemcpyAsync(D, h->s_D.p, sizeof(float) * n * k, hipMemcpyDeviceToHost, h->stream));
+        HV_HIP(hipMemcpyAsync(I, h->s_I.p, sizeof(int64_t) * n * k, hipMemcpyDeviceToHost, h->stream));
+        HV_HIP(hipStreamSynchronize(h->stream));
+        auto t3 = std::chrono::steady_clock::now();
+        if (st) {
+            st->ms_upload = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            st->ms_download = std::chrono::duration<double, std::milli>(t3 - t2).count();
+            st->ms_total = std::chrono::duration<double, std::milli>(t3 - t0).count();
+        }
+    });
+}
+
+int hv_ground_truth_f64(hv_index_t* h, int64_t n, const float* q, int64_t k, double* D, int64_t* I) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
+        if (n == 0) return;
+        require(q && D && I, HV_ERR_INVALID_ARG, "null buffer");
+        require(k >= 1 && k <= hv::kMaxK, HV_ERR_CAPACITY, "k must be in [1, 1024]");
+        const int64_t d = h->d;
+        const int64_t cap = 4096;
+        // 1. the current search gives k real candidates per query; their FP32 distances
+        //    bound the k-th exact distance. The FP32 chain of d+2 rounded non-negative terms
+        //    is within a relative (d+2)*2^-24 of the exact value, so 4x that margin is safe.
+        upload(h->s_q, q, sizeof(float) * n * d, h->stream);
+        HV_HIP(h->s_D.ensure(sizeof(float) * n * k));
+        HV_HIP(h->s_I.ensure(sizeof(int64_t) * n * k));
+        do_search(h, n, h->s_q.as<float>(), k, h->s_D.as<float>(), h->s_I.as<int64_t>(), nullptr);
+        std::vector<float> d32(n * k);
+        std::vector<int64_t> i32(n * k);
+        HV_HIP(hipMemcpy(d32.data(), h->s_D.p, sizeof(float) * n * k, hipMemcpyDeviceToHost));
+        HV_HIP(hipMemcpy(i32.data(), h->s_I.p, sizeof(int64_t) * n * k, hipMemcpyDeviceToHost));
+        const double margin = 1.0 + 4.0 * (double)(d + 2) * std::ldexp(1.0, -24);
+        std::vector<double> tau(n);
+        for (int64_t i = 0; i < n; ++i)
+            tau[i] = i32[i * k + k - 1] >= 0 ? (double)d32[i * k + k - 1] * margin : INFINITY;
+        // 2. FP64 scan of every list, keeping distances <= tau
+        const float* Qp = h->s_q.as<float>();
+        if (!h->perm.empty()) Qp = h->s_qp.as<float>();  // permuted by the search above
+        DevBuf b_tau, b_sd, b_sid, b_cnt;
+        upload(b_tau, tau.data(), sizeof(double) * n, h->stream);
+        HV_HIP(b_sd.ensure(sizeof(double) * n * cap));
+        HV_HIP(b_sid.ensure(sizeof(uint32_t) * n * cap));
+        HV_HIP(b_cnt.ensure(sizeof(unsigned int) * n));
+        hv::launch_fill_i32(b_cnt.as<int>(), n, 0, h->stream);
+        hv::launch_gt_f64(Qp, n, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                          h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(),
+                          (int)h->nlist, b_tau.as<double>(), (int)cap, b_sd.as<double>(), b_sid.as<uint32_t>(),
+                          b_cnt.as<unsigned int>(), h->stream);
+        HV_CHECK_LAUNCH("fp64 ground truth");
+        std::vector<unsigned int> cnt(n);
+        HV_HIP(hipMemcpyAsync(cnt.data(), b_cnt.p, sizeof(unsigned int) * n, hipMemcpyDeviceToHost, h->stream));
+        HV_HIP(hipStreamSynchronize(h->stream));
+        for (int64_t i = 0; i < n; ++i)
+            require(cnt[i] <= cap, HV_ERR_CAPACITY, "ground truth: more than 4096 candidates under the bound");
+        std::vector<double> sd(n * cap);
+        std::vector<uint32_t> sid(n * cap);
+        HV_HIP(hipMemcpy(sd.data(), b_sd.p, sizeof(double) * n * cap, hipMemcpyDeviceToHost));
+        HV_HIP(hipMemcpy(sid.data(), b_sid.p, sizeof(uint32_t) * n * cap, hipMemcpyDeviceToHost));
+        // 3. sort survivors by (distance, id)
+        std::vector<std::pair<double, int64_t>> row;
+        for (int64_t i = 0; i < n; ++i) {
+            row.clear();
+            for (unsigned int j = 0; j < cnt[i]; ++j) row.emplace_back(sd[i * cap + j], (int64_t)sid[i * cap + j]);
+            const int64_t m = std::min<int64_t>(k, (int64_t)row.size());
+            std::partial_sort(row.begin(), row.begin() + m, row.end());
+            for (int64_t j = 0; j < k; ++j) {
+                D[i * k + j] = j < m ? row[j].first : INFINITY;
+                I[i * k + j] = j < m ? row[j].second : -1;
+            }
+        }
+    });
+}
+
+int64_t hv_ntotal(const hv_index_t* h) { return h ? h->ntotal : -1; }
+int64_t hv_dim(const hv_index_t* h) { return h ? h->d : -1; }
+int64_t hv_nlist(const hv_index_t* h) { return h ? h->nlist : -1; }
+
+int hv_get_centroids(const hv_index_t* h, float* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (!h->trained) return HV_ERR_STATE;
+    for (int64_t l = 0; l < h->nlist; ++l)
+        for (int64_t t = 0; t < h->d; ++t)
+            out[l * h->d + (h->perm.empty() ? t : h->perm[t])] = h->centroids[l * h->d + t];
+    return HV_OK;
+}
+
+int hv_get_list_offsets(const hv_index_t* h, int64_t* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (!h->added) return HV_ERR_STATE;
+    std::memcpy(out, h->list_off.data(), sizeof(int64_t) * (h->nlist + 1));
+    return HV_OK;
+}
+
+int hv_get_list_ids(const hv_index_t* h, int64_t* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (!h->added) return HV_ERR_STATE;
+    std::memcpy(out, h->ids.data(), sizeof(int64_t) * h->ntotal);
+    return HV_OK;
+}
+
+int hv_get_list_codes(const hv_index_t* h, float* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (!h->added) return HV_ERR_STATE;
+    hv_index_t* hm = const_cast<hv_index_t*>(h);
+    return guarded(hm, [&] {
+        std::vector<float> slab(h->code_off[h->nlist]);
+        HV_HIP(hipMemcpy(slab.data(), h->d_codes.p, sizeof(float) * slab.size(), hipMemcpyDeviceToHost));
+        for (int64_t l = 0; l < h->nlist; ++l) {
+            const int64_t n = h->list_off[l + 1] - h->list_off[l];
+            const int64_t ps = h->list_stride[l];
+            for (int64_t t = 0; t < n; ++t)
+                for (int64_t i = 0; i < h->d; ++i)
+                    out[(h->list_off[l] + t) * h->d + (h->perm.empty() ? i : h->perm[i])] =
+                        slab[h->code_off[l] + i * ps + t];
+        }
+    });
+}
+
+int hv_get_last_probes(const hv_index_t* h, int64_t* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (h->last_n == 0) return HV_ERR_STATE;
+    std::vector<int> p(h->last_n * h->last_nprobe);
+    if (hipMemcpy(p.data(), h->s_probes.p, sizeof(int) * p.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return HV_ERR_DEVICE;
+    for (size_t i = 0; i < p.size(); ++i) out[i] = p[i];
+    return HV_OK;
+}
+
+int hv_get_last_bounds(const hv_index_t* h, float* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (h->last_n == 0) return HV_ERR_STATE;
+    if (hipMemcpy(out, h->s_bound.p, sizeof(float) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
+        return HV_ERR_DEVICE;
+    return HV_OK;
+}
+
+int hv_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+int hv_device_malloc(void** ptr, int64_t bytes) {
+    if (!ptr || bytes < 0) return HV_ERR_INVALID_ARG;
+    return hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1)) == hipSuccess ? HV_OK : HV_ERR_DEVICE;
+}
+int hv_device_free(void* ptr) { return hipFree(ptr) == hipSuccess ? HV_OK : HV_ERR_DEVICE; }
+int hv_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+    return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice) == hipSuccess ? HV_OK : HV_ERR_DEVICE;
+}
+int hv_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+    return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess ? HV_OK : HV_ERR_DEVICE;
+}
+
+}  // extern "C"

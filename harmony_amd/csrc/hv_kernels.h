@@ -1,0 +1,84 @@
+// Host-side launch wrappers for the HIP kernels (implemented in hv_*.hip). All launches go on
+// the given stream; none of them synchronise.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hv {
+
+constexpr int kMaxChunks = 96;   // staging chunks per vector (dimension blocks split by kChunkDims)
+constexpr int kChunkDims = 32;   // dims staged in LDS per step
+constexpr int kQueryTile = 64;   // queries per scan work item
+constexpr int kVecTile = 128;    // vectors per scan tile
+constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+constexpr int kMaxK = 1024;      // largest k of the GPU top-k
+constexpr int kMaxSelectRow = 8192;   // largest row (nlist) of the coarse select
+constexpr int kMaxWarmup = 8192;      // largest warm-up candidate set per query
+
+// Staging plan of the dimension loop: the B pruning blocks, each cut into chunks of at most
+// kChunkDims dims. ends_block[c] = 1 when chunk c closes a block; prune_after[c] = 1 when the
+// bound is checked after it (a block that is not the last).
+struct ChunkPlan {
+    int n;
+    int lo[kMaxChunks];
+    int len[kMaxChunks];
+    int ends_block[kMaxChunks];
+    int prune_after[kMaxChunks];
+};
+
+struct ScanItem {
+    int list;
+    int pair_begin;
+    int pair_end;
+    int pad;
+};
+
+// --- distances / selects
+void launch_pair_l2_full(const float* Q, int64_t nq, const float* C, int64_t nc, int d, float* out, int64_t ldo,
+                         hipStream_t s);
+void launch_select_rows(const float* dist, int64_t n, int m, int p, int* out_ids, float* out_d, hipStream_t s);
+void launch_argmin_rows(const float* dist, int64_t n, int m, int* labels, float* best, hipStream_t s);
+
+// --- search path
+void launch_permute_rows(const float* src, int64_t n, int d, const int* perm, float* dst, hipStream_t s);
+void launch_warmup_bound(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,
+                         const int64_t* list_off, const int* list_stride, const int* probes, int nprobe, int wl,
+                         int wls, int k, const ChunkPlan& plan, float* bound, hipStream_t s);
+void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int* bucket_cnt, hipStream_t s);
+// exclusive scan of n ints into out (n+1 entries, out[n] = total); tmp >= scan_tmp_ints(n)
+int64_t scan_tmp_ints(int64_t n);
+void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s);
+void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* bucket_off, int* fill,
+                          int* pairs, hipStream_t s);
+void launch_plan_items(const int* bucket_off, int nlist, int nprobe, const int* list_rank_order,
+                       const int64_t* list_off, int qcap, ScanItem* items, int* queue_len, int* queue_head,
+                       unsigned long long* stats, hipStream_t s);
+int scan_grid_size(int device);
+void launch_list_scan(const float* Qp, int d, const float* codes, const int64_t* code_off,
+                      const int64_t* list_off, const int* list_stride, const uint32_t* ids,
+                      const ChunkPlan& plan, int prune, const float* bound, const int* pairs,
+                      const ScanItem* items, const int* queue_len, int* queue_head, int qcap,
+                      unsigned long long* surv, unsigned int* surv_cnt, int cap, unsigned long long* stats,
+                      int grid, hipStream_t s);
+void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap,
+                             int k, float* D, int64_t* I, int* overflow_list, int* overflow_cnt,
+                             unsigned long long* stats, hipStream_t s);
+void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes,
+                        const int64_t* code_off, const int64_t* list_off, const int* list_stride,
+                        const uint32_t* ids, const int* probes, int nprobe, const ChunkPlan& plan, int prune,
+                        const float* bound, int k, float* D, int64_t* I, hipStream_t s);
+
+// --- FP64 ground truth
+void launch_gt_f64(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,
+                   const int64_t* list_off, const int* list_stride, const uint32_t* ids, int nlist,
+                   const double* tau, int cap, double* sd, uint32_t* sid, unsigned int* cnt, hipStream_t s);
+
+// --- index build
+void launch_centroid_update_f64(const float* x, int d, const int* perm, const int64_t* cl_off, int nlist,
+                                float* centroids, hipStream_t s);
+void launch_build_layout(const float* x, int d, const int* order, const int64_t* list_off,
+                         const int64_t* code_off, const int* list_stride, int nlist, float* codes, hipStream_t s);
+
+}  // namespace hv

@@ -1,0 +1,207 @@
+"""Python mirror of harmony::Index (src/Index.h:13-149) over the C-ABI.
+
+Same names and argument meaning as the reference class where they exist:
+  Index(d, nlist, nprobe, metric, opt_level)      Index.h:15-25
+  train(x, niter, seed, max_points_per_centroid)  Index.h:69 / Clustering.h:13-18
+  add(x)                                          Index.h:80 (ids are positions 0..n-1)
+  preSearch(warmup_list, warmup_list_size, ...)   Index.h:90 (the warm-up / pruning knobs)
+  search(queries, k) -> (D, I)                    Index.h:82
+  nprobe, opt_level                               mutable public members (query.cpp:683,686)
+Errors raise HvError (the reference throws std::runtime_error / exits; Index.cpp:1642-1645).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (HV_BOUND_EXTERNAL, HV_BOUND_WARMUP, HV_METRIC_L2, HV_OPT_NONE, HvError, KmeansParams,
+                   SearchParams, Stats, check, lib)
+
+
+def _f32(a, shape_last=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape_last is not None and (a.ndim != 2 or a.shape[1] != shape_last):
+        raise ValueError("expected an [n][%d] float32 array, got %r" % (shape_last, a.shape))
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceArray:
+    """A raw device allocation owned by the library's allocator (no GPU framework needed)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(None, lib().hv_device_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        check(None, lib().hv_memcpy_htod(d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        check(None, lib().hv_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().hv_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Index:
+    def __init__(self, d, nlist, nprobe=1, metric=HV_METRIC_L2, opt_level=HV_OPT_NONE, dim_order=None):
+        self._h = None
+        L = lib()
+        h = L.hv_create(int(d), int(nlist), int(metric), int(opt_level))
+        if not h:
+            raise HvError(_lib.HV_ERR_NO_DEVICE, (L.hv_last_error(None) or b"").decode())
+        self._h = ctypes.c_void_p(h)
+        self.d = int(d)
+        self.nlist = int(nlist)
+        self.metric = metric
+        self._params = SearchParams()
+        L.hv_default_search_params(ctypes.byref(self._params))
+        self._params.nprobe = int(nprobe)
+        self._params.opt_level = int(opt_level)
+        if dim_order is not None:
+            perm = np.ascontiguousarray(dim_order, dtype=np.int32)
+            check(self._h, L.hv_set_dim_order(self._h, _ptr(perm)))
+        self._push()
+
+    # -- lifetime
+    def close(self):
+        if self._h:
+            lib().hv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- knobs
+    def _push(self):
+        check(self._h, lib().hv_set_search_params(self._h, ctypes.byref(self._params)))
+
+    @property
+    def nprobe(self):
+        return self._params.nprobe
+
+    @nprobe.setter
+    def nprobe(self, v):
+        self._params.nprobe = int(v)
+        self._push()
+
+    @property
+    def opt_level(self):
+        return self._params.opt_level
+
+    @opt_level.setter
+    def opt_level(self, v):
+        self._params.opt_level = int(v)
+        self._push()
+
+    def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
+                  survivor_cap=0):
+        """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59)."""
+        p = self._params
+        p.warmup_list = int(warmup_list)
+        p.warmup_list_size = int(warmup_list_size)
+        p.pruning = 1 if pruning else 0
+        p.dim_blocks = int(dim_blocks)
+        p.bound_mode = int(bound_mode)
+        p.survivor_cap = int(survivor_cap)
+        self._push()
+
+    def set_external_bounds(self, bounds):
+        b = np.ascontiguousarray(bounds, dtype=np.float32)
+        check(self._h, lib().hv_set_external_bounds(self._h, b.size, _ptr(b)))
+
+    # -- build
+    def train(self, x, niter=10, seed=0, max_points_per_centroid=256):
+        x = _f32(x, self.d)
+        kp = KmeansParams(int(niter), int(seed), int(max_points_per_centroid))
+        check(self._h, lib().hv_train(self._h, x.shape[0], _ptr(x), ctypes.byref(kp)))
+
+    def set_centroids(self, c):
+        c = _f32(c, self.d)
+        check(self._h, lib().hv_set_centroids(self._h, _ptr(c)))
+
+    def add(self, x):
+        x = _f32(x, self.d)
+        check(self._h, lib().hv_add(self._h, x.shape[0], _ptr(x)))
+
+    # -- search
+    def search(self, queries, k, return_stats=False):
+        q = _f32(queries, self.d)
+        n = q.shape[0]
+        D = np.empty((n, k), dtype=np.float32)
+        I = np.empty((n, k), dtype=np.int64)
+        st = Stats()
+        check(self._h, lib().hv_search(self._h, n, _ptr(q), int(k), _ptr(D), _ptr(I), ctypes.byref(st)))
+        return (D, I, st.as_dict()) if return_stats else (D, I)
+
+    def search_device(self, q_dev, n, k, D_dev, I_dev):
+        """Queries and results already in HBM (DeviceArray). Returns the stats dict."""
+        st = Stats()
+        check(self._h, lib().hv_search_device(self._h, int(n), q_dev.ptr, int(k), D_dev.ptr, I_dev.ptr,
+                                              ctypes.byref(st)))
+        return st.as_dict()
+
+    def ground_truth_f64(self, queries, k):
+        q = _f32(queries, self.d)
+        n = q.shape[0]
+        D = np.empty((n, k), dtype=np.float64)
+        I = np.empty((n, k), dtype=np.int64)
+        check(self._h, lib().hv_ground_truth_f64(self._h, n, _ptr(q), int(k), _ptr(D), _ptr(I)))
+        return D, I
+
+    # -- introspection
+    @property
+    def ntotal(self):
+        return lib().hv_ntotal(self._h)
+
+    def centroids(self):
+        out = np.empty((self.nlist, self.d), dtype=np.float32)
+        check(self._h, lib().hv_get_centroids(self._h, _ptr(out)))
+        return out
+
+    def list_offsets(self):
+        out = np.empty(self.nlist + 1, dtype=np.int64)
+        check(self._h, lib().hv_get_list_offsets(self._h, _ptr(out)))
+        return out
+
+    def list_ids(self):
+        out = np.empty(self.ntotal, dtype=np.int64)
+        check(self._h, lib().hv_get_list_ids(self._h, _ptr(out)))
+        return out
+
+    def list_codes(self):
+        out = np.empty((self.ntotal, self.d), dtype=np.float32)
+        check(self._h, lib().hv_get_list_codes(self._h, _ptr(out)))
+        return out
+
+    def last_probes(self, n):
+        out = np.empty((n, min(self.nprobe, self.nlist)), dtype=np.int64)
+        check(self._h, lib().hv_get_last_probes(self._h, _ptr(out)))
+        return out
+
+    def last_bounds(self, n):
+        out = np.empty(n, dtype=np.float32)
+        check(self._h, lib().hv_get_last_bounds(self._h, _ptr(out)))
+        return out

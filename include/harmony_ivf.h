@@ -1,0 +1,151 @@
+/*
+ * harmony_ivf.h — C-ABI of the MI355X IVF-flat search library (libharmony_amd.so).
+ *
+ * This is the drop-in boundary for Harmony's per-node IVF-flat search path. The reference
+ * exposes it as the C++ class harmony::Index (src/Index.h:13-149) plus the scanner operator
+ * interface IVFScanBase (src/IVFScan.hpp:15-65); each entry point below names the member it
+ * replaces. Conventions:
+ *   - no exceptions cross the ABI; every call returns an int status (HV_OK == 0) and the
+ *     message of the last failure is available from hv_last_error();
+ *   - host pointers are caller-owned, read/written synchronously inside the call;
+ *   - device memory is owned by the handle; one HIP stream per handle; one search at a time
+ *     per handle (the reference's Index::search is not re-entrant either, Index.h:142-147);
+ *   - results are ordered by (distance, id), padded with (FLT_MAX, -1) (heap.hpp:13-25).
+ *   - ids are the insertion positions 0..ntotal-1 (Index.cpp:663); ntotal < 2^32 - 1.
+ */
+#ifndef HARMONY_IVF_H
+#define HARMONY_IVF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HV_OK 0
+#define HV_ERR_INVALID_ARG (-1) /* bad size / pointer / parameter */
+#define HV_ERR_STATE (-2)       /* call out of order (search before add, ...) */
+#define HV_ERR_DEVICE (-3)      /* HIP runtime failure */
+#define HV_ERR_IO (-4)          /* file open / read / format */
+#define HV_ERR_CAPACITY (-5)    /* size beyond a documented limit */
+#define HV_ERR_NO_DEVICE (-6)   /* no GPU visible */
+
+/* MetricType (src/common.h:20-21). Only L2 is implemented on the GPU path. */
+#define HV_METRIC_IP 0
+#define HV_METRIC_L2 1
+
+/* OptLevel bits (src/common.h:23-32). The L2 search ignores them, as the reference does
+ * (its TRIANGLE / SUBNN code in IVFScan.hpp:223-428 is commented out); they are kept so the
+ * opt-level subset check of Index::search (Index.cpp:1642-1645) behaves the same. */
+#define HV_OPT_NONE 0
+#define HV_OPT_TRIANGLE 1
+#define HV_OPT_SUBNN_L2 2
+#define HV_OPT_SUBNN_IP 4
+#define HV_OPT_ALL 7
+
+/* bound modes for the pruning bound */
+#define HV_BOUND_WARMUP 0   /* Index::warmUpSearch (Index.cpp:999-1051) */
+#define HV_BOUND_EXTERNAL 1 /* caller-provided per-query bound (--fullWarmUp, query.cpp:716-725) */
+
+/* precision of the candidate distances */
+#define HV_PRECISION_FP32 0
+
+typedef struct hv_index hv_index_t;
+
+/* k-means parameters (ClusteringParameters, src/Clustering.h:13-18). */
+typedef struct {
+    int64_t niter;                   /* Lloyd iterations (reference Index::train uses 20) */
+    uint64_t seed;                   /* shuffle seed for subsample + init */
+    int64_t max_points_per_centroid; /* subsample cap = nlist * this (0 = no cap) */
+} hv_kmeans_params;
+
+/* Search parameters: the mutable knobs of harmony::Index (nprobe, opt_level) and of
+ * Index::Param (pruning, fullWarmUp, heapTops; src/Index.h:48-68), plus the preSearch
+ * warm-up sizes (src/Index.h:90) and the per-node dimension-block count (node.cpp:639). */
+typedef struct {
+    int64_t nprobe;           /* clamped to nlist (Index.cpp:1646-1648) */
+    int32_t opt_level;        /* must be a subset of the added opt level */
+    int32_t dim_blocks;       /* B >= 1; blocks of floor(d/B) dims, last takes the rest */
+    int32_t pruning;          /* 1: drop a candidate once a partial sum exceeds its bound */
+    int32_t bound_mode;       /* HV_BOUND_* */
+    int64_t warmup_list;      /* lists used for the warm-up bound (default 10) */
+    int64_t warmup_list_size; /* vectors per warm-up list (0 -> k, query.cpp:328-333) */
+    int32_t precision;        /* HV_PRECISION_* */
+    int32_t reserved0;
+    int64_t survivor_cap;     /* per-query candidate buffer (0 = automatic) */
+} hv_search_params;
+
+/* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */
+typedef struct {
+    double ms_total;         /* host wall time of the call */
+    double ms_upload;        /* host->device queries (hv_search only) */
+    double ms_coarse;        /* coarse distances + nprobe select */
+    double ms_warmup;        /* warm-up bound */
+    double ms_group;         /* (query, list) grouping + work plan */
+    double ms_scan;          /* list scan */
+    double ms_select;        /* per-query top-k selection */
+    double ms_download;      /* device->host results (hv_search only) */
+    int64_t candidates;      /* sum over queries of the probed list sizes (C_q) */
+    int64_t block_evals;     /* dimension blocks evaluated (executed work, wave granularity) */
+    int64_t block_evals_max; /* candidates * B (unpruned work) */
+    int64_t survivors;       /* candidates that reached the top-k selection */
+    int64_t overflow_queries;/* queries whose survivors exceeded the buffer (exact fallback) */
+    int64_t work_items;      /* scan work items (list x query tile) */
+} hv_stats;
+
+/* ---- lifetime (Index::Index, Index.h:15-25) ---- */
+hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_level);
+void hv_destroy(hv_index_t* h);
+const char* hv_last_error(const hv_index_t* h); /* h may be NULL: last hv_create failure */
+
+/* Dimension order applied to every vector on entry (blocks then follow this order). perm is
+ * a permutation of 0..d-1; NULL restores the identity. Must precede hv_train. */
+int hv_set_dim_order(hv_index_t* h, const int32_t* perm);
+
+/* ---- build (Index::train / Index::add, Index.h:69,80) ---- */
+void hv_default_kmeans_params(hv_kmeans_params* p);
+int hv_train(hv_index_t* h, int64_t n, const float* x, const hv_kmeans_params* p);
+int hv_set_centroids(hv_index_t* h, const float* centroids); /* nlist*d, in caller dim order */
+int hv_add(hv_index_t* h, int64_t n, const float* x);
+
+/* ---- search (Index::search, Index.h:82; preSearch knobs Index.h:90) ---- */
+void hv_default_search_params(hv_search_params* p);
+int hv_set_search_params(hv_index_t* h, const hv_search_params* p);
+int hv_get_search_params(const hv_index_t* h, hv_search_params* p);
+/* per-query external bound for HV_BOUND_EXTERNAL (heapTops, Index.h:60) */
+int hv_set_external_bounds(hv_index_t* h, int64_t n, const float* bounds);
+/* host buffers: q [n][d] f32 row-major, D [n][k] f32, I [n][k] int64 */
+int hv_search(hv_index_t* h, int64_t n, const float* q, int64_t k, float* D, int64_t* I, hv_stats* st);
+/* device buffers (already in HBM): same layouts */
+int hv_search_device(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D_dev, int64_t* I_dev,
+                     hv_stats* st);
+
+/* Exact kNN in FP64 over every vector of the index, ordered by (distance, id): the ground
+ * truth recall is measured against (the reference computes it with faiss IndexIVFFlat at
+ * nprobe = nlist, query.cpp:481-514). Runs the current search first to bound each query's
+ * k-th distance, then a full FP64 scan keeps the candidates under the bound. Host buffers. */
+int hv_ground_truth_f64(hv_index_t* h, int64_t n, const float* q, int64_t k, double* D, int64_t* I);
+
+/* ---- introspection (what the reference keeps public in Index / IVF, Index.h:117-119) ---- */
+int64_t hv_ntotal(const hv_index_t* h);
+int64_t hv_dim(const hv_index_t* h);
+int64_t hv_nlist(const hv_index_t* h);
+int hv_get_centroids(const hv_index_t* h, float* out);       /* nlist*d, caller dim order */
+int hv_get_list_offsets(const hv_index_t* h, int64_t* out);  /* nlist+1 */
+int hv_get_list_ids(const hv_index_t* h, int64_t* out);      /* ntotal, list order */
+int hv_get_list_codes(const hv_index_t* h, float* out);      /* ntotal*d, list order, caller dims */
+int hv_get_last_probes(const hv_index_t* h, int64_t* out);   /* n*nprobe of the last search */
+int hv_get_last_bounds(const hv_index_t* h, float* out);     /* n bounds of the last search */
+
+/* ---- device memory plumbing for callers without a GPU framework ---- */
+int hv_device_count(void);
+int hv_device_malloc(void** ptr, int64_t bytes);
+int hv_device_free(void* ptr);
+int hv_memcpy_htod(void* dst, const void* src, int64_t bytes);
+int hv_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HARMONY_IVF_H */

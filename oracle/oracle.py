@@ -1,0 +1,211 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle.so) and of the reference's own heap
+compiled from source (oracle/_ref/libref_heap.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TIE_CANONICAL = 0
+TIE_VERBATIM = 1
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int64
+_C = ctypes.c_int
+
+_SIG = {
+    "or_heap_init": (None, [_I, _P, _P]),
+    "or_heap_push": (None, [_C, _I, _P, _P, ctypes.c_float, _I]),
+    "or_heap_replace_top": (None, [_C, _I, _P, _P, ctypes.c_float, _I]),
+    "or_heap_sort": (None, [_C, _I, _P, _P]),
+    "or_l2": (ctypes.c_float, [_P, _P, _I]),
+    "or_l2_blocked": (ctypes.c_float, [_P, _P, _I, _I]),
+    "or_coarse": (None, [_I, _P, _I, _I, _P, _I, _C, _C, _P, _P]),
+    "or_warmup_bound": (None, [_I, _P, _I, _P, _P, _I, _I, _P, _I, _I, _I, _C, _C, _P]),
+    "or_search": (_C, [_I, _P, _I, _I, _P, _P, _P, _P, _I, _I, _I, _C, _I, _I, _C, _C, _C, _P, _P, _P, _P]),
+    "or_bruteforce_f64": (None, [_I, _P, _I, _P, _I, _I, _C, _P, _P]),
+    "or_assign": (None, [_I, _P, _I, _I, _P, _C, _P, _P]),
+    "or_build_lists": (None, [_I, _P, _P, _I, _P, _P]),
+    "or_kmeans_init": (None, [_I, _P, _I, _I, _I, ctypes.c_uint64, _P, _P, _P]),
+    "or_kmeans_step": (None, [_I, _P, _I, _I, _P, _C]),
+    "or_kmeans": (None, [_I, _P, _I, _I, _I, _I, ctypes.c_uint64, _C, _P]),
+    "or_recall": (ctypes.c_double, [_I, _I, _P, _P, _I]),
+    "or_r2": (ctypes.c_double, [_I, _I, _P, _P, _P, _I]),
+}
+
+_cache = {}
+
+
+def _load(name):
+    if name not in _cache:
+        path = os.path.join(HERE, "build", name)
+        if not os.path.isfile(path):
+            raise RuntimeError("%s not built: run `python oracle/build.py`" % path)
+        L = ctypes.CDLL(path)
+        for fn, (res, args) in _SIG.items():
+            f = getattr(L, fn)
+            f.restype = res
+            f.argtypes = args
+        _cache[name] = L
+    return _cache[name]
+
+
+def lib(fast=False):
+    """The exact oracle, or with fast=True the build with the reference's Release flags (the
+    timed CPU baseline), AVX-512 flavour when the host has avx512f."""
+    if not fast:
+        return _load("liboracle.so")
+    flags = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = line
+                    break
+    except OSError:
+        pass
+    return _load("liboracle_fast_v4.so" if " avx512f " in flags + " " else "liboracle_fast_v3.so")
+
+
+def ref_heap():
+    """The reference's heap.hpp compiled from its own source, or None when not built."""
+    path = os.path.join(HERE, "_ref", "libref_heap.so")
+    if not os.path.isfile(path):
+        return None
+    if "ref" not in _cache:
+        L = ctypes.CDLL(path)
+        L.ref_init_result.argtypes = [_I, _P, _P]
+        L.ref_heap_replace_top.argtypes = [_I, _P, _P, ctypes.c_float, _I]
+        L.ref_sort_result.argtypes = [_I, _P, _P]
+        for f in (L.ref_init_result, L.ref_heap_replace_top, L.ref_sort_result):
+            f.restype = None
+        _cache["ref"] = L
+    return _cache["ref"]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class IndexData:
+    """An IVF index as flat arrays: centroids [nlist][d], list_off [nlist+1], ids [nb] and codes
+    [nb][d] in list order (each list in its within-list order)."""
+
+    def __init__(self, centroids, list_off, ids, codes):
+        self.centroids = _f32(centroids)
+        self.list_off = _i64(list_off)
+        self.ids = _i64(ids)
+        self.codes = _f32(codes)
+        self.nlist, self.d = self.centroids.shape
+
+    @classmethod
+    def from_gpu_index(cls, index):
+        return cls(index.centroids(), index.list_offsets(), index.list_ids(), index.list_codes())
+
+
+def coarse(q, centroids, nprobe, tie=TIE_CANONICAL, nthreads=0, fast=False):
+    q, c = _f32(q), _f32(centroids)
+    nq, d = q.shape
+    probes = np.empty((nq, nprobe), np.int64)
+    pd = np.empty((nq, nprobe), np.float32)
+    lib(fast).or_coarse(nq, _p(q), d, c.shape[0], _p(c), nprobe, tie, nthreads, _p(probes), _p(pd))
+    return probes, pd
+
+
+def search(idx, q, k, nprobe, dim_blocks=4, pruning=True, warmup_list=10, warmup_list_size=0,
+           tie=TIE_CANONICAL, bound_blocked=True, nthreads=0, fast=False):
+    """(D, I, probes, bound) of the per-node search path (see or_search)."""
+    q = _f32(q)
+    nq, d = q.shape
+    npr = min(nprobe, idx.nlist)
+    D = np.empty((nq, k), np.float32)
+    I = np.empty((nq, k), np.int64)
+    probes = np.empty((nq, npr), np.int64)
+    bound = np.empty(nq, np.float32)
+    rc = lib(fast).or_search(nq, _p(q), d, idx.nlist, _p(idx.centroids), _p(idx.list_off), _p(idx.ids),
+                             _p(idx.codes), k, nprobe, dim_blocks, 1 if pruning else 0, warmup_list,
+                             warmup_list_size, tie, 1 if bound_blocked else 0, nthreads, _p(D), _p(I), _p(probes),
+                             _p(bound))
+    if rc != 0:
+        raise ValueError("or_search rejected its arguments")
+    return D, I, probes, bound
+
+
+def bruteforce_f64(xb, q, k, nthreads=0):
+    xb, q = _f32(xb), _f32(q)
+    nq, d = q.shape
+    D = np.empty((nq, k), np.float64)
+    I = np.empty((nq, k), np.int64)
+    lib().or_bruteforce_f64(xb.shape[0], _p(xb), nq, _p(q), d, k, nthreads, _p(D), _p(I))
+    return D, I
+
+
+def assign(x, centroids, nthreads=0):
+    x, c = _f32(x), _f32(centroids)
+    n, d = x.shape
+    lab = np.empty(n, np.int64)
+    dist = np.empty(n, np.float32)
+    lib().or_assign(n, _p(x), d, c.shape[0], _p(c), nthreads, _p(lab), _p(dist))
+    return lab, dist
+
+
+def build_lists(labels, dist, nlist):
+    labels, dist = _i64(labels), _f32(dist)
+    n = labels.size
+    off = np.empty(nlist + 1, np.int64)
+    order = np.empty(n, np.int64)
+    lib().or_build_lists(n, _p(labels), _p(dist), nlist, _p(off), _p(order))
+    return off, order
+
+
+def kmeans(x, nlist, niter=10, max_pts=256, seed=0, nthreads=0):
+    x = _f32(x)
+    n, d = x.shape
+    c = np.empty((nlist, d), np.float32)
+    lib().or_kmeans(n, _p(x), d, nlist, niter, max_pts, seed, nthreads, _p(c))
+    return c
+
+
+def kmeans_init(x, nlist, max_pts=256, seed=0):
+    x = _f32(x)
+    n, d = x.shape
+    cap = nlist * max_pts if (max_pts > 0 and n > nlist * max_pts) else n
+    sample = np.empty((cap, d), np.float32)
+    ns = ctypes.c_int64(0)
+    c = np.empty((nlist, d), np.float32)
+    lib().or_kmeans_init(n, _p(x), d, nlist, max_pts, seed, _p(sample), ctypes.byref(ns), _p(c))
+    return sample[: ns.value], c
+
+
+def kmeans_step(x, centroids, nthreads=0):
+    x = _f32(x)
+    c = np.array(centroids, dtype=np.float32, copy=True)
+    lib().or_kmeans_step(x.shape[0], _p(x), x.shape[1], c.shape[0], _p(c), nthreads)
+    return c
+
+
+def build_index(xb, nlist, niter=10, max_pts=256, seed=0, nthreads=0):
+    """The oracle's own train + add (k-means, assignment, list order)."""
+    c = kmeans(xb, nlist, niter, max_pts, seed, nthreads)
+    lab, dist = assign(xb, c, nthreads)
+    off, order = build_lists(lab, dist, nlist)
+    return IndexData(c, off, order, _f32(xb)[order])
+
+
+def recall(I, GT, k=None):
+    I, GT = _i64(I), _i64(GT)
+    nq, kk = I.shape
+    k = kk if k is None else k
+    return lib().or_recall(nq, k, _p(I), _p(GT), GT.shape[1])
