@@ -79,6 +79,8 @@ SIGNATURES = {
     "hv_search": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_search_device": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_ground_truth_f64": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P]),
+    "hv_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "hv_load": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "hv_ntotal": (_I64, [_P]),
     "hv_dim": (_I64, [_P]),
     "hv_nlist": (_I64, [_P]),

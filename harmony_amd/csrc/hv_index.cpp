@@ -219,6 +219,46 @@ const float* upload_rows(hv_index_t* h, const float* x, int64_t n, DevBuf& raw, 
     return permuted.as<float>();
 }
 
+// Installs the inverted lists on the device: rows of xp (index dim order, on the device) are
+// gathered through order[] into the list-major, dimension-major slabs; off[nlist+1] delimits
+// the lists and ids[] (list order) are the vector ids.
+void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order, const std::vector<int64_t>& off,
+                   const std::vector<int64_t>& ids) {
+    const int64_t d = h->d, nlist = h->nlist, n = (int64_t)order.size();
+    h->list_off = off;
+    h->list_stride.assign(nlist, 0);
+    h->code_off.assign(nlist + 1, 0);
+    for (int64_t l = 0; l < nlist; ++l) {
+        const int64_t sz = off[l + 1] - off[l];
+        require(sz < (1ll << 30), HV_ERR_CAPACITY, "list longer than 2^30");
+        h->list_stride[l] = (int)((sz + 15) / 16 * 16);
+        h->code_off[l + 1] = h->code_off[l] + d * h->list_stride[l];
+    }
+    h->ids = ids;
+    h->rank_order.resize(nlist);
+    std::iota(h->rank_order.begin(), h->rank_order.end(), 0);
+    std::stable_sort(h->rank_order.begin(), h->rank_order.end(),
+                     [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+    std::vector<uint32_t> ids32(ids.begin(), ids.end());
+    DevBuf order_buf;
+    upload(order_buf, order.data(), sizeof(int) * n, h->stream);
+    upload(h->d_list_off, off.data(), sizeof(int64_t) * (nlist + 1), h->stream);
+    upload(h->d_code_off, h->code_off.data(), sizeof(int64_t) * (nlist + 1), h->stream);
+    upload(h->d_list_stride, h->list_stride.data(), sizeof(int) * nlist, h->stream);
+    upload(h->d_ids, ids32.data(), sizeof(uint32_t) * n, h->stream);
+    upload(h->d_rank_order, h->rank_order.data(), sizeof(int) * nlist, h->stream);
+    HV_HIP(h->d_codes.ensure(sizeof(float) * std::max<int64_t>(1, h->code_off[nlist])));
+    if (n)
+        hv::launch_build_layout(xp, (int)d, order_buf.as<int>(), h->d_list_off.as<int64_t>(),
+                                h->d_code_off.as<int64_t>(), h->d_list_stride.as<int>(), (int)nlist,
+                                h->d_codes.as<float>(), h->stream);
+    HV_CHECK_LAUNCH("layout");
+    HV_HIP(hipStreamSynchronize(h->stream));
+    order_buf.release();
+    h->ntotal = n;
+    h->added = true;
+}
+
 void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D, int64_t* I, hv_stats* st) {
     require(h->trained && h->added, HV_ERR_STATE, "search before train/add");
     require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
@@ -561,39 +601,163 @@ int hv_add(hv_index_t* h, int64_t n, const float* x) {
             std::sort(order.begin() + off[l], order.begin() + off[l + 1], [&](int a, int b) {
                 return dist[a] != dist[b] ? dist[a] < dist[b] : a < b;
             });
-        h->list_off = off;
-        h->list_stride.assign(nlist, 0);
-        h->code_off.assign(nlist + 1, 0);
-        for (int64_t l = 0; l < nlist; ++l) {
-            const int64_t sz = off[l + 1] - off[l];
-            h->list_stride[l] = (int)((sz + 15) / 16 * 16);
-            h->code_off[l + 1] = h->code_off[l] + d * h->list_stride[l];
-        }
-        h->ids.assign(order.begin(), order.end());
-        h->rank_order.resize(nlist);
-        std::iota(h->rank_order.begin(), h->rank_order.end(), 0);
-        std::stable_sort(h->rank_order.begin(), h->rank_order.end(), [&](int a, int b) {
-            return off[a + 1] - off[a] > off[b + 1] - off[b];
-        });
-        std::vector<uint32_t> ids32(order.begin(), order.end());
-        upload(order_buf, order.data(), sizeof(int) * n, h->stream);
-        upload(h->d_list_off, off.data(), sizeof(int64_t) * (nlist + 1), h->stream);
-        upload(h->d_code_off, h->code_off.data(), sizeof(int64_t) * (nlist + 1), h->stream);
-        upload(h->d_list_stride, h->list_stride.data(), sizeof(int) * nlist, h->stream);
-        upload(h->d_ids, ids32.data(), sizeof(uint32_t) * n, h->stream);
-        upload(h->d_rank_order, h->rank_order.data(), sizeof(int) * nlist, h->stream);
-        HV_HIP(h->d_codes.ensure(sizeof(float) * std::max<int64_t>(1, h->code_off[nlist])));
-        hv::launch_build_layout(xp, (int)d, order_buf.as<int>(), h->d_list_off.as<int64_t>(),
-                                h->d_code_off.as<int64_t>(), h->d_list_stride.as<int>(), (int)nlist,
-                                h->d_codes.as<float>(), h->stream);
-        HV_CHECK_LAUNCH("layout");
-        HV_HIP(hipStreamSynchronize(h->stream));
+        install_lists(h, xp, order, off, std::vector<int64_t>(order.begin(), order.end()));
         raw.release();
         perm_buf.release();
-        order_buf.release();
-        h->ntotal = n;
-        h->added = true;
         h->added_opt_level = h->opt_level;
+    });
+}
+
+extern "C++" {
+namespace {
+
+template <typename T>
+void read_exact(FILE* f, T* dst, size_t count, const char* what) {
+    if (count && std::fread(dst, sizeof(T), count, f) != count)
+        throw HvError{HV_ERR_IO, std::string("truncated index file while reading ") + what};
+}
+
+template <typename T>
+void write_exact(FILE* f, const T* src, size_t count) {
+    if (count && std::fwrite(src, sizeof(T), count, f) != count) throw HvError{HV_ERR_IO, "short write"};
+}
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+}  // extern "C++"
+
+// Index::save_index (Index.cpp:1708-1732) + IVF::save_IVF (IVF.cpp:76-99).
+int hv_save(const hv_index_t* hc, const char* path) {
+    if (!hc) return HV_ERR_INVALID_ARG;
+    hv_index_t* h = const_cast<hv_index_t*>(hc);
+    return guarded(h, [&] {
+        require(path != nullptr, HV_ERR_INVALID_ARG, "null path");
+        require(h->trained && h->added, HV_ERR_STATE, "save before train/add");
+        const int64_t d = h->d, nlist = h->nlist;
+        std::vector<float> codes(h->ntotal * d), cent(nlist * d);
+        if (h->ntotal) {
+            int rc = hv_get_list_codes(h, codes.data());
+            if (rc != HV_OK) throw HvError{rc, h->err};
+        }
+        hv_get_centroids(h, cent.data());
+        FILE* f = std::fopen(path, "wb");
+        require(f != nullptr, HV_ERR_IO, std::string("Cannot open file ") + path);
+        FileCloser fc{f};
+        const uint64_t hd[2] = {(uint64_t)d, (uint64_t)nlist};
+        const int32_t en[2] = {h->metric, h->added_opt_level};
+        const uint64_t sub[3] = {0, 1, 1};
+        write_exact(f, hd, 2);
+        write_exact(f, en, 2);
+        write_exact(f, sub, 3);
+        write_exact(f, cent.data(), cent.size());
+        std::vector<uint64_t> ids;
+        std::vector<float> norms;
+        for (int64_t l = 0; l < nlist; ++l) {
+            const int64_t b = h->list_off[l], n = h->list_off[l + 1] - b;
+            if (n == 0) continue;
+            const uint64_t rec[4] = {(uint64_t)l, (uint64_t)n, (uint64_t)d, 0};
+            const int32_t opt = 0;
+            write_exact(f, rec, 4);
+            write_exact(f, &opt, 1);
+            ids.assign(h->ids.begin() + b, h->ids.begin() + b + n);
+            write_exact(f, ids.data(), ids.size());
+            write_exact(f, codes.data() + b * d, (size_t)(n * d));
+            norms.resize(n);
+            for (int64_t t = 0; t < n; ++t) {  // candidate_norms = <x, x> (Index.cpp:689-696)
+                float s = 0.f;
+                for (int64_t i = 0; i < d; ++i) s = std::fmaf(codes[(b + t) * d + i], codes[(b + t) * d + i], s);
+                norms[t] = s;
+            }
+            write_exact(f, norms.data(), norms.size());
+        }
+    });
+}
+
+// Index::load_index (Index.cpp:1735-1770) + IVF::load_IVF (IVF.cpp:101-134).
+int hv_load(hv_index_t* h, const char* path) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(path != nullptr, HV_ERR_INVALID_ARG, "null path");
+        FILE* f = std::fopen(path, "rb");
+        require(f != nullptr, HV_ERR_IO, std::string("Cannot open file ") + path);
+        FileCloser fc{f};
+        uint64_t hd[2];
+        int32_t en[2];
+        uint64_t sub[3];
+        read_exact(f, hd, 2, "header");
+        read_exact(f, en, 2, "header");
+        read_exact(f, sub, 3, "header");
+        const int64_t d = (int64_t)hd[0], nlist = (int64_t)hd[1];
+        require(d > 0 && d <= 65536 && nlist > 0 && nlist <= (1 << 24), HV_ERR_IO, "implausible d / nlist in header");
+        require(en[0] == HV_METRIC_L2, HV_ERR_INVALID_ARG, "only METRIC_L2 indexes are supported");
+        std::vector<float> cent(nlist * d);
+        read_exact(f, cent.data(), cent.size(), "centroids");
+        std::vector<std::vector<uint64_t>> lids(nlist);
+        std::vector<std::vector<float>> lcodes(nlist);
+        int64_t total = 0;
+        for (;;) {
+            uint64_t lid;
+            if (std::fread(&lid, sizeof(lid), 1, f) != 1) {
+                require(std::feof(f), HV_ERR_IO, "read error");
+                break;
+            }
+            uint64_t rec[3];
+            int32_t opt;
+            read_exact(f, rec, 3, "list header");
+            read_exact(f, &opt, 1, "list header");
+            const int64_t n = (int64_t)rec[0], ld = (int64_t)rec[1], sk = (int64_t)rec[2];
+            require((int64_t)lid < nlist && ld == d && n >= 0 && n < (1ll << 32), HV_ERR_IO, "bad list record");
+            require(lids[lid].empty(), HV_ERR_IO, "list stored twice");
+            lids[lid].resize(n);
+            lcodes[lid].resize(n * d);
+            read_exact(f, lids[lid].data(), n, "ids");
+            read_exact(f, lcodes[lid].data(), n * d, "codes");
+            std::vector<float> skip(n);
+            read_exact(f, skip.data(), n, "norms");
+            // optional arrays (IVF.cpp:117-134): skipped, the L2 search does not use them
+            int64_t extra = 0;
+            if ((opt & HV_OPT_TRIANGLE) || (opt & HV_OPT_SUBNN_IP)) extra += 2 * 4 * n;
+            if (opt & HV_OPT_SUBNN_L2) extra += n * sk * (8 + 4);
+            if (opt & HV_OPT_SUBNN_IP) extra += 2 * n * sk * (8 + 4);
+            require(std::fseek(f, (long)extra, SEEK_CUR) == 0, HV_ERR_IO, "truncated optional arrays");
+            total += n;
+        }
+        require(total < 0xffffffffll, HV_ERR_CAPACITY, "ntotal must be < 2^32 - 1");
+        // re-initialise the handle to the file's shape
+        if (d != h->d) {
+            h->perm.clear();
+            h->d_perm.release();
+        }
+        h->d = d;
+        h->nlist = nlist;
+        h->trained = h->added = false;
+        h->added_opt_level = en[1];
+        h->opt_level = en[1];
+        h->last_n = 0;
+        int rc = hv_set_centroids(h, cent.data());
+        if (rc != HV_OK) throw HvError{rc, h->err};
+        std::vector<float> rows(total * d);
+        std::vector<int64_t> off(nlist + 1, 0), ids(total);
+        for (int64_t l = 0; l < nlist; ++l) {
+            off[l + 1] = off[l] + (int64_t)lids[l].size();
+            std::copy(lcodes[l].begin(), lcodes[l].end(), rows.begin() + off[l] * d);
+            for (size_t t = 0; t < lids[l].size(); ++t) {
+                require(lids[l][t] < 0xffffffffull, HV_ERR_CAPACITY, "id >= 2^32 - 1");
+                ids[off[l] + t] = (int64_t)lids[l][t];
+            }
+        }
+        DevBuf raw, perm_buf;
+        const float* xp = total ? upload_rows(h, rows.data(), total, raw, perm_buf) : nullptr;
+        std::vector<int> order(total);
+        std::iota(order.begin(), order.end(), 0);
+        install_lists(h, xp, order, off, ids);
+        h->added_opt_level = en[1];
     });
 }
 

@@ -171,6 +171,16 @@ class Index:
         check(self._h, lib().hv_ground_truth_f64(self._h, n, _ptr(q), int(k), _ptr(D), _ptr(I)))
         return D, I
 
+    # -- persistence (Index::save_index / load_index, Index.h:83-85)
+    def save_index(self, path):
+        check(self._h, lib().hv_save(self._h, str(path).encode()))
+
+    def load_index(self, path):
+        check(self._h, lib().hv_load(self._h, str(path).encode()))
+        L = lib()
+        self.d = int(L.hv_dim(self._h))
+        self.nlist = int(L.hv_nlist(self._h))
+
     # -- introspection
     @property
     def ntotal(self):

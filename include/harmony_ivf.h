@@ -127,6 +127,18 @@ int hv_search_device(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, fl
  * k-th distance, then a full FP64 scan keeps the candidates under the bound. Host buffers. */
 int hv_ground_truth_f64(hv_index_t* h, int64_t n, const float* q, int64_t k, double* D, int64_t* I);
 
+/* ---- persistence: the reference's index file (Index::save_index / load_index,
+ * Index.cpp:1708-1770; per-list IVF record, IVF.cpp:76-134). Layout, little-endian:
+ *   size_t d, size_t nlist, int32 metric, int32 added_opt_level, size_t sub_k, sub_nlist,
+ *   sub_nprobe, float centroids[nlist][d], then per non-empty list: size_t list_id, size_t
+ *   list_size, size_t d, size_t sub_k, int32 opt_level, size_t ids[list_size],
+ *   float codes[list_size][d], float norms[list_size] (+ optional arrays per opt_level bits),
+ *   read until end of file. hv_save writes per-list opt_level 0 (no optional arrays);
+ *   hv_load skips optional arrays, replaces the handle's d / nlist with the file's and keeps
+ *   each list's vector order. Codes are stored in caller dimension order. */
+int hv_save(const hv_index_t* h, const char* path);
+int hv_load(hv_index_t* h, const char* path);
+
 /* ---- introspection (what the reference keeps public in Index / IVF, Index.h:117-119) ---- */
 int64_t hv_ntotal(const hv_index_t* h);
 int64_t hv_dim(const hv_index_t* h);

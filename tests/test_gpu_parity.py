@@ -205,3 +205,16 @@ def test_w2_ground_truth_and_recall(w2):
     rec = O.recall(I, GI)
     say("w2 recall@100 at nprobe 64 on 40 queries: %.4f" % rec)
     assert rec > 0.9
+
+
+def test_w1_matches_golden_fixture(w1):
+    import os
+    xb, xq, g, o = w1
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "w1_golden.npz"),
+                   allow_pickle=False)
+    g.nprobe = 10
+    g.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4)
+    D, I = g.search(xq, 10)
+    np.testing.assert_array_equal(I, gold["I"])
+    np.testing.assert_array_equal(D, gold["D"])
+    np.testing.assert_array_equal(g.list_offsets(), gold["list_off"])
