@@ -45,7 +45,8 @@ class SearchParams(ctypes.Structure):
     _fields_ = [("nprobe", ctypes.c_int64), ("opt_level", ctypes.c_int32), ("dim_blocks", ctypes.c_int32),
                 ("pruning", ctypes.c_int32), ("bound_mode", ctypes.c_int32), ("warmup_list", ctypes.c_int64),
                 ("warmup_list_size", ctypes.c_int64), ("precision", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("survivor_cap", ctypes.c_int64)]
+                ("refine_bound", ctypes.c_int32), ("survivor_cap", ctypes.c_int64), ("tile_shape", ctypes.c_int32),
+                ("tile_shape_first", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -1,7 +1,12 @@
-"""Build libharmony_amd.so (host C++ + HIP kernels for gfx950) in-tree with hipcc."""
+"""Build libharmony_amd.so (host C++ + HIP kernels for gfx950) in-tree with hipcc.
+
+python -m harmony_amd.build [--incremental]
+  --incremental  recompile only sources newer than their object (headers count for all)
+"""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -9,7 +14,8 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libharmony_amd.so")
 ARCH = os.environ.get("HV_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hv_build.hip", "hv_scan.hip", "hv_gt.hip", "hv_index.cpp"]
+SOURCES = ["hv_build.hip", "hv_scan.hip", "hv_select.hip", "hv_gt.hip", "hv_index.cpp"]
+HEADERS = ["hv_device.h", "hv_kernels.h", os.path.join("..", "..", "include", "harmony_ivf.h")]
 
 # -packed-fp32-ops off: hipcc -O3 would otherwise pair FP32 ops into v_pk_* forms.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
@@ -22,14 +28,28 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-def build(verbose=True):
+def _stale(src, obj):
+    if not os.path.isfile(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, h) for h in HEADERS]
+    return any(os.path.getmtime(p) > t for p in deps if os.path.isfile(p))
+
+
+def build(incremental=False):
     os.makedirs(OBJ, exist_ok=True)
+    jobs = []
     objs = []
     for src in SOURCES:
+        path = os.path.join(CSRC, src)
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"]
-        _run(["hipcc"] + lang + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+        if incremental and not _stale(path, obj):
+            continue
+        jobs.append(["hipcc", "-x", "hip"] + HIPFLAGS + ["-c", path, "-o", obj])
+    with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), 4))) as ex:
+        for f in [ex.submit(_run, j) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-o", tmp])
     os.replace(tmp, LIB)
@@ -37,5 +57,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv[1:])
     sys.exit(0)

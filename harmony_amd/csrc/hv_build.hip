@@ -79,32 +79,6 @@ void launch_pair_l2_full(const float* Q, int64_t nq, const float* C, int64_t nc,
 }
 
 // ---------------------------------------------------------------------------------------
-// Per row of dist[n][m]: the p smallest (distance, column) pairs in ascending order. This is
-// lite_scan_codes into a p-heap + sort_result (IVFScan.hpp:108-138, heap.hpp:84-90) with the
-// boundary tie going to the lower column, as the reference's strict '<' in column order does.
-// One workgroup per row, bitonic sort of the whole row in LDS.
-__global__ __launch_bounds__(256) void k_select_rows(const float* __restrict__ dist, int m, int p,
-                                                     int* __restrict__ out_ids, float* __restrict__ out_d) {
-    extern __shared__ unsigned long long s_keys[];
-    const int64_t row = blockIdx.x;
-    const int P = hv_next_pow2(max(m, 2));
-    for (int c = threadIdx.x; c < P; c += blockDim.x)
-        s_keys[c] = c < m ? hv_key(dist[row * m + c], (uint32_t)c) : HV_KEY_PAD;
-    block_bitonic_sort(s_keys, P);
-    for (int i = threadIdx.x; i < p; i += blockDim.x) {
-        const unsigned long long k = s_keys[i];
-        out_ids[row * p + i] = (int)(k & 0xffffffffu);
-        if (out_d) out_d[row * p + i] = hv_key_dist(k);
-    }
-}
-
-void launch_select_rows(const float* dist, int64_t n, int m, int p, int* out_ids, float* out_d, hipStream_t s) {
-    if (n <= 0) return;
-    const size_t lds = sizeof(unsigned long long) * hv_next_pow2(m < 2 ? 2 : m);
-    hipLaunchKernelGGL(k_select_rows, dim3((unsigned)n), dim3(256), lds, s, dist, m, p, out_ids, out_d);
-}
-
-// ---------------------------------------------------------------------------------------
 // Per row: argmin over (distance, column) — nearest centroid with ties to the lower id.
 // One wavefront per row, 64-bit key min-reduction across the wavefront.
 __global__ __launch_bounds__(256) void k_argmin_rows(const float* __restrict__ dist, int64_t n, int m,

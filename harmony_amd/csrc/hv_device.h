@@ -48,3 +48,70 @@ __host__ __device__ inline int hv_next_pow2(int v) {
     while (p < v) p <<= 1;
     return p;
 }
+
+// Wavefront-wide ascending bitonic sort of 64*E keys held in registers: element i lives in
+// lane i / E, slot i % E. Strides below E exchange within a lane; larger strides exchange with
+// lane ^ (stride / E) through cross-lane shuffles. No LDS, no barriers.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
+    const int lane = threadIdx.x & 63;
+    constexpr int N = 64 * E;
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= E) {
+                const int lx = stride / E;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = lane * E + e;
+                    const unsigned long long o = __shfl_xor(v[e], lx, 64);
+                    const bool up = (i & size) == 0;
+                    const bool lower = (i & stride) == 0;
+                    const unsigned long long mn = o < v[e] ? o : v[e];
+                    const unsigned long long mx = o < v[e] ? v[e] : o;
+                    v[e] = (lower == up) ? mn : mx;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if (e & stride) continue;
+                    const int e2 = e | stride;
+                    const bool up = ((lane * E + e) & size) == 0;
+                    const unsigned long long a = v[e], b = v[e2];
+                    const bool sw = (a > b) == up;
+                    v[e] = sw ? b : a;
+                    v[e2] = sw ? a : b;
+                }
+            }
+        }
+    }
+}
+
+// Sorts the n (<= 2048) keys produced by key_of(j), j in [0, n), with one wavefront and hands
+// every element of the sorted order to emit(rank, key); ranks >= n hold HV_KEY_PAD. All 64 lanes
+// of the wavefront must call it with the same n.
+template <int E, typename KeyOf, typename Emit>
+__device__ __forceinline__ void wave_sort_emit_e(int n, KeyOf&& key_of, Emit&& emit) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = e * 64 + lane;  // any placement works: the sort orders them
+        v[e] = j < n ? key_of(j) : HV_KEY_PAD;
+    }
+    wave_bitonic_sort<E>(v);
+#pragma unroll
+    for (int e = 0; e < E; ++e) emit(lane * E + e, v[e]);
+}
+
+template <typename KeyOf, typename Emit>
+__device__ __forceinline__ void wave_sort_emit(int n, KeyOf&& key_of, Emit&& emit) {
+    if (n <= 64) wave_sort_emit_e<1>(n, key_of, emit);
+    else if (n <= 128) wave_sort_emit_e<2>(n, key_of, emit);
+    else if (n <= 256) wave_sort_emit_e<4>(n, key_of, emit);
+    else if (n <= 512) wave_sort_emit_e<8>(n, key_of, emit);
+    else if (n <= 1024) wave_sort_emit_e<16>(n, key_of, emit);
+    else wave_sort_emit_e<32>(n, key_of, emit);
+}
+constexpr int kWaveSortMax = 2048;

@@ -111,6 +111,10 @@ void kmeans_pick(int64_t n, int64_t nlist, int64_t max_pts, uint64_t seed, std::
     init_rows.resize(nlist);
 }
 
+// Longest vector range of one scan work item: long lists are split so that no workgroup holds
+// a whole large list while the others run dry.
+constexpr int kVecSplit = 1024;
+
 // Dimension blocks of node.cpp:639-646 (B blocks of floor(d/B) dims, the last takes the
 // rest), each cut into staging chunks of at most kChunkDims dims.
 hv::ChunkPlan make_plan(int d, int B) {
@@ -142,7 +146,7 @@ struct hv_index {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
-    int scan_grid = 0;
+    int scan_grid[hv::kNumShapes] = {};
 
     std::vector<int> perm;  // empty = identity
     DevBuf d_perm;
@@ -152,7 +156,7 @@ struct hv_index {
     DevBuf d_centroids;
 
     bool added = false;
-    int64_t ntotal = 0;
+    int64_t ntotal = 0, max_list = 0;
     std::vector<int64_t> list_off, code_off, ids;  // ids in list order
     std::vector<int> list_stride, rank_order;
     DevBuf d_codes, d_list_off, d_code_off, d_list_stride, d_ids, d_rank_order;
@@ -161,7 +165,7 @@ struct hv_index {
     std::vector<float> ext_bounds;
 
     // search scratch (grow-only)
-    DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
+    DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats;
     int64_t last_n = 0, last_nprobe = 0;
 };
@@ -256,6 +260,8 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
     HV_HIP(hipStreamSynchronize(h->stream));
     order_buf.release();
     h->ntotal = n;
+    h->max_list = 0;
+    for (int64_t l = 0; l < nlist; ++l) h->max_list = std::max<int64_t>(h->max_list, off[l + 1] - off[l]);
     h->added = true;
 }
 
@@ -312,81 +318,110 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("coarse");
     }
     HV_HIP(hipEventRecord(h->ev[1], s));
-    // 3. bound
-    HV_HIP(h->s_bound.ensure(sizeof(float) * n));
-    float* bound = h->s_bound.as<float>();
-    if (h->sp.bound_mode == HV_BOUND_EXTERNAL) {
-        HV_HIP(hipMemcpyAsync(bound, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-    } else if (wl > 0) {
-        hv::launch_warmup_bound(Qp, n, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
-                                h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), probes, (int)nprobe,
-                                (int)wl, (int)wls, (int)k, plan, bound, s);
-    } else {
-        int fm;
-        const float fmax = FLT_MAX;
-        std::memcpy(&fm, &fmax, 4);
-        hv::launch_fill_i32(reinterpret_cast<int*>(bound), n, fm, s);
-    }
-    HV_CHECK_LAUNCH("bound");
-    HV_HIP(hipEventRecord(h->ev[2], s));
-    // 4. grouping by (list, rank) + work plan
+    // 3. grouping by (list, rank)
     const int64_t nb = nlist * nprobe;
     HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
     HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
     HV_HIP(h->s_bucket_off.ensure(sizeof(int) * (nb + 1)));
     HV_HIP(h->s_scan_tmp.ensure(sizeof(int) * hv::scan_tmp_ints(nb)));
     HV_HIP(h->s_pairs.ensure(sizeof(int) * n * nprobe));
-    const int64_t qcap64 = n * nprobe / hv::kQueryTile + nlist + 1;
-    require(qcap64 < (1ll << 28), HV_ERR_CAPACITY, "too many work items");
-    const int qcap = (int)qcap64;
-    HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * hv::kNumXcd * qcap));
-    HV_HIP(h->s_queue.ensure(sizeof(int) * 2 * hv::kNumXcd));
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 4));
+    // work items of one phase: at most (query tiles) x (vector ranges); query tiles hold >= 16
+    // queries, vector ranges >= kVecSplit vectors
+    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / kVecSplit + 1);
+    require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
+    HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
+    HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 8));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
     hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
     hv::launch_fill_i32(h->s_fill.as<int>(), nb, 0, s);
-    hv::launch_fill_u64(stats, 4, 0ull, s);
+    hv::launch_fill_u64(stats, 8, 0ull, s);
     hv::launch_count_pairs(probes, n, (int)nprobe, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
     hv::launch_scatter_pairs(probes, n, (int)nprobe, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
                              h->s_pairs.as<int>(), s);
-    int* queue_len = h->s_queue.as<int>();
-    int* queue_head = queue_len + hv::kNumXcd;
-    hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, h->d_rank_order.as<int>(),
-                          h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue_len, queue_head,
-                          stats, s);
     HV_CHECK_LAUNCH("grouping");
-    HV_HIP(hipEventRecord(h->ev[3], s));
-    // 5. list scan
+    HV_HIP(hipEventRecord(h->ev[2], s));
+
+    int* queue = h->s_queue.as<int>();
+    // survivor buffer: room for the warm-up set too
+    if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0) cap = std::max<int64_t>(cap, hv_next_pow2((int)(wl * wls)));
     HV_HIP(h->s_surv.ensure(sizeof(unsigned long long) * n * cap));
     HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
-    hv::launch_fill_i32(h->s_surv_cnt.as<int>(), n, 0, s);
-    hv::launch_list_scan(Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
-                         h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), plan,
-                         B > 1 ? 1 : 0, bound, h->s_pairs.as<int>(), h->s_items.as<hv::ScanItem>(), queue_len,
-                         queue_head, qcap, h->s_surv.as<unsigned long long>(), h->s_surv_cnt.as<unsigned int>(),
-                         (int)cap, stats, h->scan_grid, s);
-    HV_CHECK_LAUNCH("list scan");
+    unsigned long long* surv = h->s_surv.as<unsigned long long>();
+    unsigned int* surv_cnt = h->s_surv_cnt.as<unsigned int>();
+    hv::launch_fill_i32(reinterpret_cast<int*>(surv_cnt), n, 0, s);
+    // one scan phase over probe ranks [r0, r1), at most vmax vectors per list
+    // Tile shape: the caller's, or by the expected number of queries per list in the phase
+    // (few queries per list -> narrow query tiles, long vector tiles).
+    auto pick_shape = [&](int r0, int r1) {
+        if (h->sp.tile_shape > 0 && r1 == (int)nprobe && r1 - r0 > 1) return h->sp.tile_shape - 1;
+        if (h->sp.tile_shape_first > 0 && r0 == 0 && r1 == 1) return h->sp.tile_shape_first - 1;
+        const double g = (double)n * (r1 - r0) / (double)nlist;
+        return g <= 16 ? 5 : (g <= 40 ? 2 : 0);
+    };
+    auto scan_phase = [&](int r0, int r1, int vmax, int prune, int accept_all, unsigned long long* st_slot) {
+        const int shape = pick_shape(r0, r1);
+        if (h->scan_grid[shape] == 0) h->scan_grid[shape] = hv::scan_grid_size(h->device, shape, (int)d);
+        const int vt = hv::scan_vector_tile(shape);
+        const int vsplit = (kVecSplit + vt - 1) / vt * vt;
+        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmax,
+                              hv::scan_query_tile(shape), vsplit, h->d_rank_order.as<int>(),
+                              h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
+        hv::launch_list_scan(shape, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                             h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), plan,
+                             prune, h->s_bound.as<float>(), h->s_pairs.as<int>(), h->s_items.as<hv::ScanItem>(),
+                             queue, vmax, vsplit, accept_all, surv, surv_cnt, (int)cap, st_slot, h->scan_grid[shape],
+                             s);
+        HV_CHECK_LAUNCH("list scan");
+    };
+    // 4. bound: the reference's warm-up bound (heapTops), or the caller's
+    HV_HIP(h->s_bound.ensure(sizeof(float) * n));
+    HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
+    float* bound = h->s_bound.as<float>();
+    float* bound_ref = h->s_bound_ref.as<float>();
+    if (h->sp.bound_mode == HV_BOUND_EXTERNAL) {
+        HV_HIP(hipMemcpyAsync(bound_ref, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+        HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    } else if (wl > 0) {
+        scan_phase(0, (int)wl, (int)wls, 0, 1, stats + 4);
+        hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, s);
+    } else {
+        int fm;
+        const float fmax = FLT_MAX;
+        std::memcpy(&fm, &fmax, 4);
+        hv::launch_fill_i32(reinterpret_cast<int*>(bound), n, fm, s);
+        hv::launch_fill_i32(reinterpret_cast<int*>(bound_ref), n, fm, s);
+    }
+    HV_CHECK_LAUNCH("bound");
+    HV_HIP(hipEventRecord(h->ev[3], s));
+    // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
+    //    the remaining probed lists
+    const int prune = B > 1 ? 1 : 0;
+    int r_next = 0;
+    if (h->sp.refine_bound && nprobe > 1) {
+        scan_phase(0, 1, INT32_MAX, prune, 0, stats);
+        hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
+        HV_CHECK_LAUNCH("compact");
+        r_next = 1;
+    }
+    scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, stats);
     HV_HIP(hipEventRecord(h->ev[4], s));
     // 6. top-k selection (+ exact path for overflowed buffers)
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 1)));
     int* ovf_cnt = h->s_overflow.as<int>();
     int* ovf_list = ovf_cnt + 1;
     hv::launch_fill_i32(ovf_cnt, 1, 0, s);
-    hv::launch_select_survivors(h->s_surv.as<unsigned long long>(), h->s_surv_cnt.as<unsigned int>(), n, (int)cap,
-                                (int)k, D, I, ovf_list, ovf_cnt, stats, s);
+    hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, ovf_list, ovf_cnt, stats, s);
     HV_CHECK_LAUNCH("select");
-    int novf = 0;
-    HV_HIP(hipMemcpyAsync(&novf, ovf_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-    HV_HIP(hipStreamSynchronize(s));
-    if (novf > 0) {
-        hv::launch_query_exact(ovf_list, novf, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
-                               h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(),
-                               probes, (int)nprobe, plan, B > 1 ? 1 : 0, bound, (int)k, D, I, s);
-        HV_CHECK_LAUNCH("exact fallback");
-    }
+    hv::launch_query_exact(ovf_list, ovf_cnt, (int)std::min<int64_t>(n, 256), Qp, (int)d, h->d_codes.as<float>(),
+                           h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(),
+                           h->d_ids.as<uint32_t>(), probes, (int)nprobe, plan, prune, bound, (int)k, D, I, s);
+    HV_CHECK_LAUNCH("exact fallback");
     HV_HIP(hipEventRecord(h->ev[5], s));
-    unsigned long long hs[4] = {0, 0, 0, 0};
+    int novf = 0;
+    unsigned long long hs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HV_HIP(hipMemcpyAsync(&novf, ovf_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
     HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
     h->last_n = n;
@@ -395,8 +430,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         float ms[5];
         for (int i = 0; i < 5; ++i) HV_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
         st->ms_coarse += ms[0];
-        st->ms_warmup += ms[1];
-        st->ms_group += ms[2];
+        st->ms_group += ms[1];
+        st->ms_warmup += ms[2];
         st->ms_scan += ms[3];
         st->ms_select += ms[4];
         st->candidates += (int64_t)hs[0];
@@ -449,7 +484,6 @@ hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_leve
             return nullptr;
         }
     }
-    h->scan_grid = hv::scan_grid_size(h->device);
     return h;
 }
 
@@ -458,7 +492,7 @@ void hv_destroy(hv_index_t* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
-                      &h->s_bound, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
+                      &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
@@ -772,6 +806,7 @@ void hv_default_search_params(hv_search_params* p) {
     p->warmup_list = 10;
     p->warmup_list_size = 0;
     p->precision = HV_PRECISION_FP32;
+    p->refine_bound = 1;
     p->survivor_cap = 0;
 }
 
@@ -787,6 +822,9 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
         require(p->warmup_list >= 0 && p->warmup_list_size >= 0, HV_ERR_INVALID_ARG, "negative warm-up size");
         require(p->precision == HV_PRECISION_FP32, HV_ERR_INVALID_ARG, "unsupported precision");
         require(p->survivor_cap >= 0 && p->survivor_cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap > 8192");
+        require(p->tile_shape >= 0 && p->tile_shape <= hv::kNumShapes && p->tile_shape_first >= 0 &&
+                    p->tile_shape_first <= hv::kNumShapes,
+                HV_ERR_INVALID_ARG, "invalid tile_shape");
         h->sp = *p;
     });
 }
@@ -962,7 +1000,7 @@ int hv_get_last_probes(const hv_index_t* h, int64_t* out) {
 int hv_get_last_bounds(const hv_index_t* h, float* out) {
     if (!h || !out) return HV_ERR_INVALID_ARG;
     if (h->last_n == 0) return HV_ERR_STATE;
-    if (hipMemcpy(out, h->s_bound.p, sizeof(float) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(out, h->s_bound_ref.p, sizeof(float) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
         return HV_ERR_DEVICE;
     return HV_OK;
 }

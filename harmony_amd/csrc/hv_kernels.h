@@ -8,9 +8,8 @@ namespace hv {
 
 constexpr int kMaxChunks = 96;   // staging chunks per vector (dimension blocks split by kChunkDims)
 constexpr int kChunkDims = 32;   // dims staged in LDS per step
-constexpr int kQueryTile = 64;   // queries per scan work item
-constexpr int kVecTile = 128;    // vectors per scan tile
-constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+constexpr int kNumShapes = 7;    // compiled list-scan tile shapes (see hv_scan.hip)
+constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 constexpr int kMaxK = 1024;      // largest k of the GPU top-k
 constexpr int kMaxSelectRow = 8192;   // largest row (nlist) of the coarse select
 constexpr int kMaxWarmup = 8192;      // largest warm-up candidate set per query
@@ -28,9 +27,9 @@ struct ChunkPlan {
 
 struct ScanItem {
     int list;
-    int pair_begin;
+    int pair_begin;  // range of the grouped (query) pairs
     int pair_end;
-    int pad;
+    int vec_begin;   // first vector of the item's vector range
 };
 
 // --- distances / selects
@@ -41,9 +40,10 @@ void launch_argmin_rows(const float* dist, int64_t n, int m, int* labels, float*
 
 // --- search path
 void launch_permute_rows(const float* src, int64_t n, int d, const int* perm, float* dst, hipStream_t s);
-void launch_warmup_bound(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,
-                         const int64_t* list_off, const int* list_stride, const int* probes, int nprobe, int wl,
-                         int wls, int k, const ChunkPlan& plan, float* bound, hipStream_t s);
+void launch_kth_survivors(const unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k,
+                          float* bound_ref, float* bound, hipStream_t s);
+void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k, float* bound,
+                         hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int* bucket_cnt, hipStream_t s);
@@ -52,20 +52,23 @@ int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s);
 void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* bucket_off, int* fill,
                           int* pairs, hipStream_t s);
-void launch_plan_items(const int* bucket_off, int nlist, int nprobe, const int* list_rank_order,
-                       const int64_t* list_off, int qcap, ScanItem* items, int* queue_len, int* queue_head,
-                       unsigned long long* stats, hipStream_t s);
-int scan_grid_size(int device);
-void launch_list_scan(const float* Qp, int d, const float* codes, const int64_t* code_off,
+// queue: 3 x 8 ints (per-XCD length, head, base) + nlist ints (first item of each list rank)
+void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmax, int qtile, int vsplit,
+                       const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
+                       int* queue, unsigned long long* stats, hipStream_t s);
+int scan_query_tile(int shape);   // queries per work item of a tile shape
+int scan_vector_tile(int shape);  // vectors per tile of a tile shape
+int scan_grid_size(int device, int shape, int d);
+void launch_list_scan(int shape, const float* Qp, int d, const float* codes, const int64_t* code_off,
                       const int64_t* list_off, const int* list_stride, const uint32_t* ids,
                       const ChunkPlan& plan, int prune, const float* bound, const int* pairs,
-                      const ScanItem* items, const int* queue_len, int* queue_head, int qcap,
-                      unsigned long long* surv, unsigned int* surv_cnt, int cap, unsigned long long* stats,
-                      int grid, hipStream_t s);
+                      const ScanItem* items, int* queue, int vmax, int vsplit, int accept_all,
+                      unsigned long long* surv, unsigned int* surv_cnt, int cap, unsigned long long* stats, int grid,
+                      hipStream_t s);
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap,
                              int k, float* D, int64_t* I, int* overflow_list, int* overflow_cnt,
                              unsigned long long* stats, hipStream_t s);
-void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes,
+void launch_query_exact(const int* qlist, const int* qcount, int grid, const float* Qp, int d, const float* codes,
                         const int64_t* code_off, const int64_t* list_off, const int* list_stride,
                         const uint32_t* ids, const int* probes, int nprobe, const ChunkPlan& plan, int prune,
                         const float* bound, int k, float* D, int64_t* I, hipStream_t s);

@@ -36,56 +36,6 @@ __device__ __forceinline__ float blocked_l2_col(const float* qs, const float* __
 }
 
 // ---------------------------------------------------------------------------------------
-// Warm-up bound (Index::warmUpSearch, Index.cpp:999-1051; heapTops = heap top, 1303-1305):
-// the k-th smallest distance over the first min(|L|, wls) vectors of each of the first wl
-// probed lists, FLT_MAX when fewer than k vectors were seen. Distances use the same blocked
-// sum as the scan, so every warm-up vector passes its own query's bound. One workgroup per
-// query; the distances are sorted in LDS.
-__global__ __launch_bounds__(256) void k_warmup_bound(const float* __restrict__ Qp, int d,
-                                                      const float* __restrict__ codes,
-                                                      const int64_t* __restrict__ code_off,
-                                                      const int64_t* __restrict__ list_off,
-                                                      const int* __restrict__ list_stride,
-                                                      const int* __restrict__ probes, int nprobe, int wl, int wls,
-                                                      int k, ChunkPlan plan, float* __restrict__ bound) {
-    extern __shared__ __align__(16) unsigned char s_raw[];
-    float* qs = reinterpret_cast<float*>(s_raw);
-    unsigned int* dbuf = reinterpret_cast<unsigned int*>(s_raw + sizeof(float) * ((d + 3) & ~3));
-    const int64_t q = blockIdx.x;
-    for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Qp[q * d + i];
-    __syncthreads();
-    int pos = 0;
-    for (int j = 0; j < wl; ++j) {
-        const int l = probes[q * nprobe + j];
-        const int n = (int)(list_off[l + 1] - list_off[l]);
-        const int m = min(n, wls);
-        const float* X = codes + code_off[l];
-        const int ps = list_stride[l];
-        for (int t = threadIdx.x; t < m; t += blockDim.x)
-            dbuf[pos + t] = __float_as_uint(blocked_l2_col(qs, X, ps, t, plan, false, 0.f));
-        pos += m;
-    }
-    if (pos < k) {
-        if (threadIdx.x == 0) bound[q] = 3.402823466e+38f;
-        return;
-    }
-    const int P = hv_next_pow2(max(pos, 2));
-    for (int i = pos + threadIdx.x; i < P; i += blockDim.x) dbuf[i] = 0xffffffffu;
-    block_bitonic_sort(dbuf, P);
-    if (threadIdx.x == 0) bound[q] = __uint_as_float(dbuf[k - 1]);
-}
-
-void launch_warmup_bound(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,
-                         const int64_t* list_off, const int* list_stride, const int* probes, int nprobe, int wl,
-                         int wls, int k, const ChunkPlan& plan, float* bound, hipStream_t s) {
-    if (nq <= 0) return;
-    const int pmax = hv_next_pow2(max(wl * wls, 2));
-    const size_t lds = sizeof(float) * ((d + 3) & ~3) + sizeof(unsigned int) * pmax;
-    hipLaunchKernelGGL(k_warmup_bound, dim3((unsigned)nq), dim3(256), lds, s, Qp, d, codes, code_off, list_off,
-                       list_stride, probes, nprobe, wl, wls, k, plan, bound);
-}
-
-// ---------------------------------------------------------------------------------------
 // Grouping: (query, probe rank) pairs bucketed by (list, rank). Within a list, queries that
 // rank the list first come first, so a query tile holds queries at similar distance from the
 // list and prunes (or survives) together.
@@ -194,87 +144,127 @@ void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStre
 }
 
 // ---------------------------------------------------------------------------------------
-// Work plan: a list with g probing queries becomes ceil(g / 64) items (list, query tile).
-// Lists, taken in decreasing size order, are dealt round-robin to 8 per-XCD queues, so the
-// tiles of one list run on one XCD and share its L2, while every XCD gets a similar mix of
-// list sizes. One workgroup of 1024 threads; thread t owns ranks 8t..8t+7 (one per queue).
-__global__ __launch_bounds__(1024) void k_plan_items(const int* __restrict__ bucket_off, int nlist, int nprobe,
-                                                     const int* __restrict__ rank_order,
-                                                     const int64_t* __restrict__ list_off, int qcap,
-                                                     ScanItem* __restrict__ items, int* __restrict__ queue_len,
-                                                     int* __restrict__ queue_head,
+// Work plan: a list with g probing queries (in probe ranks [r0, r1)) and n vectors (at most
+// vmax) becomes ceil(g / qtile) x ceil(n / vsplit) items (list, query tile, vector range), so
+// no item is longer than vsplit vectors. Lists, taken in decreasing size order, are dealt
+// round-robin to 8 per-XCD queues, so the tiles of one list run on one XCD and share its L2,
+// while every XCD gets a similar mix of list sizes. The queues lie back to back in `items`.
+// queue[0..8) = lengths, [8..16) = heads, [16..24) = bases, [24 + r] = first item of the list
+// of rank r. k_plan_count (one workgroup of 1024 threads, thread t owns ranks 8t..8t+7, one
+// per queue) sizes the queues; k_plan_write (one workgroup per list) fills them.
+struct PlanArgs {
+    const int* bucket_off;
+    const int* rank_order;
+    const int64_t* list_off;
+    int nlist, nprobe, r0, r1, vmax, qtile, vsplit;
+};
+
+__device__ __forceinline__ void plan_list(const PlanArgs& a, int r, int& l, int& gb, int& ge, int& nlv, int& nqt,
+                                          int& nvr) {
+    l = a.rank_order[r];
+    gb = a.bucket_off[(int64_t)l * a.nprobe + a.r0];
+    ge = a.bucket_off[(int64_t)l * a.nprobe + a.r1];
+    nlv = (int)min(a.list_off[l + 1] - a.list_off[l], (long long)a.vmax);
+    nqt = (ge - gb + a.qtile - 1) / a.qtile;
+    nvr = (nlv + a.vsplit - 1) / a.vsplit;
+}
+
+__global__ __launch_bounds__(1024) void k_plan_count(PlanArgs a, int* __restrict__ queue,
                                                      unsigned long long* __restrict__ stats) {
     __shared__ int s_scan[kNumXcd][1024];
+    __shared__ int s_base[kNumXcd];
     const int t = threadIdx.x;
-    int cnt[kNumXcd], lst[kNumXcd], gb[kNumXcd], ge[kNumXcd];
+    int cnt[kNumXcd];
     unsigned long long cand = 0;
 #pragma unroll
     for (int x = 0; x < kNumXcd; ++x) {
         const int r = t * kNumXcd + x;
         cnt[x] = 0;
-        lst[x] = -1;
-        gb[x] = ge[x] = 0;
-        if (r < nlist) {
-            const int l = rank_order[r];
-            lst[x] = l;
-            gb[x] = bucket_off[(int64_t)l * nprobe];
-            ge[x] = bucket_off[(int64_t)(l + 1) * nprobe];
-            cnt[x] = (ge[x] - gb[x] + kQueryTile - 1) / kQueryTile;
-            cand += (unsigned long long)(ge[x] - gb[x]) * (unsigned long long)(list_off[l + 1] - list_off[l]);
+        if (r < a.nlist) {
+            int l, gb, ge, nlv, nqt, nvr;
+            plan_list(a, r, l, gb, ge, nlv, nqt, nvr);
+            cnt[x] = nqt * nvr;
+            cand += (unsigned long long)(ge - gb) * (unsigned long long)nlv;
         }
         s_scan[x][t] = cnt[x];
     }
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int a[kNumXcd];
+        int v[kNumXcd];
 #pragma unroll
-        for (int x = 0; x < kNumXcd; ++x) a[x] = t >= off ? s_scan[x][t - off] : 0;
+        for (int x = 0; x < kNumXcd; ++x) v[x] = t >= off ? s_scan[x][t - off] : 0;
         __syncthreads();
 #pragma unroll
-        for (int x = 0; x < kNumXcd; ++x) s_scan[x][t] += a[x];
+        for (int x = 0; x < kNumXcd; ++x) s_scan[x][t] += v[x];
         __syncthreads();
     }
+    if (t == 0) {
+        int b = 0;
+        for (int x = 0; x < kNumXcd; ++x) {
+            s_base[x] = b;
+            queue[x] = s_scan[x][1023];
+            queue[kNumXcd + x] = 0;
+            queue[2 * kNumXcd + x] = b;
+            b += s_scan[x][1023];
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int x = 0; x < kNumXcd; ++x) {
-        const int start = s_scan[x][t] - cnt[x];
-        for (int c = 0; c < cnt[x]; ++c) {
-            ScanItem it;
-            it.list = lst[x];
-            it.pair_begin = gb[x] + c * kQueryTile;
-            it.pair_end = min(ge[x], it.pair_begin + kQueryTile);
-            it.pad = 0;
-            items[(int64_t)x * qcap + start + c] = it;
-        }
+        const int r = t * kNumXcd + x;
+        if (r < a.nlist) queue[3 * kNumXcd + r] = s_base[x] + s_scan[x][t] - cnt[x];
     }
-    if (t == 1023) {
-#pragma unroll
-        for (int x = 0; x < kNumXcd; ++x) {
-            queue_len[x] = s_scan[x][1023];
-            queue_head[x] = 0;
-        }
-    }
-    // candidates = sum over pairs of the probed list's size
+    // candidates = sum over pairs of the probed list's (capped) size
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_xor(cand, off, 64);
     if ((t & 63) == 0 && cand) atomicAdd(&stats[0], cand);
 }
 
-void launch_plan_items(const int* bucket_off, int nlist, int nprobe, const int* list_rank_order,
-                       const int64_t* list_off, int qcap, ScanItem* items, int* queue_len, int* queue_head,
-                       unsigned long long* stats, hipStream_t s) {
-    hipLaunchKernelGGL(k_plan_items, dim3(1), dim3(1024), 0, s, bucket_off, nlist, nprobe, list_rank_order,
-                       list_off, qcap, items, queue_len, queue_head, stats);
+__global__ __launch_bounds__(64) void k_plan_write(PlanArgs a, const int* __restrict__ queue, int64_t qcap,
+                                                   ScanItem* __restrict__ items) {
+    const int r = blockIdx.x;
+    int l, gb, ge, nlv, nqt, nvr;
+    plan_list(a, r, l, gb, ge, nlv, nqt, nvr);
+    const int64_t pos0 = queue[3 * kNumXcd + r];
+    const int total = nqt * nvr;
+    for (int j = threadIdx.x; j < total; j += blockDim.x) {
+        const int c = j / nvr, v = j % nvr;
+        ScanItem it;
+        it.list = l;
+        it.pair_begin = gb + c * a.qtile;
+        it.pair_end = min(ge, it.pair_begin + a.qtile);
+        it.vec_begin = v * a.vsplit;
+        if (pos0 + j < qcap) items[pos0 + j] = it;
+    }
+}
+
+void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmax, int qtile, int vsplit,
+                       const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
+                       int* queue, unsigned long long* stats, hipStream_t s) {
+    PlanArgs a{bucket_off, list_rank_order, list_off, nlist, nprobe, r0, r1, vmax, qtile, vsplit};
+    hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1024), 0, s, a, queue, stats);
+    hipLaunchKernelGGL(k_plan_write, dim3((unsigned)nlist), dim3(64), 0, s, a, queue, qcap, items);
 }
 
 // ---------------------------------------------------------------------------------------
-// The list scan. A persistent grid; each workgroup pulls (list, query tile) items from its
-// XCD's queue (then from the others'). Per item, the list is swept in tiles of 128 vectors;
-// per tile, the 64 queries x 128 vectors distances are accumulated over the staging chunks of
-// the dimension blocks. Each thread owns a 4 query x 8 vector register tile, so one 16-byte
-// LDS read of queries and two of vectors feed 32 subtract + 32 FMA. After each block but the
-// last, pairs whose partial sum exceeds the query's bound are dropped; a wavefront whose 16
-// queries x 128 vectors are all dropped stops computing, and the workgroup leaves the tile as
-// soon as all four wavefronts have. Surviving pairs with distance <= bound are appended to
-// their query's candidate buffer as (distance, id) keys.
+// The list scan. A persistent grid; each workgroup pulls (list, query tile, vector range)
+// items from its XCD's queue (then from the others'). Per item, the range is swept in tiles of
+// VT vectors; per tile, the QT queries x VT vectors distances are accumulated over the staging
+// chunks of the dimension blocks. Each thread owns a QPT query x VPT vector register tile, so
+// one or two 16-byte LDS reads of queries and vectors feed QPT*VPT subtract + FMA pairs.
+//
+// The query tile is staged once per item (all dimensions, when it fits in LDS: QRES) or per
+// chunk. Work is a stream of (tile, chunk) jobs; the global loads of the predicted next job are
+// issued before the current one is computed, so their latency hides behind the FMAs. At a
+// pruning point the prediction is the next tile's first chunk when the previous tile died at
+// that point (far lists: nearly every tile), else the current tile's next chunk.
+//
+// After each block but the last, pairs whose partial sum exceeds the query's bound are
+// dropped; inside a block a wavefront re-checks fl(dis + partial) every 8 dimensions (partial
+// sums only grow, so such a pair ends above its bound). A wavefront whose pairs are all dropped
+// stops computing; the workgroup leaves the tile once every wavefront has. Surviving pairs with
+// distance <= bound (or every pair, with accept_all) are appended to their query's candidate
+// buffer as (distance, id) keys.
+template <int QPT, int VPT, int TQ, bool QRES>
 __global__ __launch_bounds__(256) void k_list_scan(const float* __restrict__ Qp, int d,
                                                    const float* __restrict__ codes,
                                                    const int64_t* __restrict__ code_off,
@@ -283,20 +273,30 @@ __global__ __launch_bounds__(256) void k_list_scan(const float* __restrict__ Qp,
                                                    const uint32_t* __restrict__ ids, ChunkPlan plan, int prune,
                                                    const float* __restrict__ bound, const int* __restrict__ pairs,
                                                    const ScanItem* __restrict__ items,
-                                                   const int* __restrict__ queue_len, int* __restrict__ queue_head,
-                                                   int qcap, unsigned long long* __restrict__ surv,
+                                                   int* __restrict__ queue, int vmax, int vsplit, int accept_all,
+                                                   unsigned long long* __restrict__ surv,
                                                    unsigned int* __restrict__ surv_cnt, int cap,
                                                    unsigned long long* __restrict__ stats) {
-    constexpr int QT = kQueryTile, VT = kVecTile, KC = kChunkDims, QS = QT + 4;
-    __shared__ __align__(16) float Qs[KC][QS];
+    constexpr int TV = 256 / TQ, QT = TQ * QPT, VT = TV * VPT, KC = kChunkDims, QS = QT + 4;
+    constexpr int QLD = QRES ? 1 : (QT * KC + 255) / 256;  // query values staged per thread per chunk
+    constexpr int XLD = (KC * VT / 4 + 255) / 256;         // float4 vector loads per thread per chunk
+    static_assert(QPT == 2 || QPT == 4 || QPT == 8, "QPT");
+    static_assert(VPT == 4 || VPT == 8, "VPT");
+    static_assert(QPT * VPT <= 32, "pair mask is 32 bits");
+    extern __shared__ __align__(16) float s_q[];  // [d][QS] (QRES) or [KC][QS]
     __shared__ __align__(16) float Xs[KC][VT];
     __shared__ float s_bound[QT];
     __shared__ int s_qid[QT];
     __shared__ int s_item;
+    float(*Qs)[QS] = reinterpret_cast<float(*)[QS]>(s_q);
     const int tid = threadIdx.x;
-    const int tq = tid >> 4, tv = tid & 15;
+    const int tq = tid / TV, tv = tid % TV;
     const int xcd = blockIdx.x % kNumXcd;
     unsigned long long evals = 0;
+    const int* __restrict__ queue_len = queue;
+    int* __restrict__ queue_head = queue + kNumXcd;
+    const int* __restrict__ queue_base = queue + 2 * kNumXcd;
+    bool predict_die = true;
 
     for (;;) {
         if (tid == 0) {
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_list_scan(const float* __restrict__ Qp,
             for (int o = 0; o < kNumXcd && it < 0; ++o) {
                 const int x = (xcd + o) % kNumXcd;
                 const int slot = atomicAdd(&queue_head[x], 1);
-                if (slot < queue_len[x]) it = x * qcap + slot;
+                if (slot < queue_len[x]) it = queue_base[x] + slot;
             }
             s_item = it;
         }
@@ -325,94 +325,210 @@ __global__ __launch_bounds__(256) void k_list_scan(const float* __restrict__ Qp,
         }
         const int l = item.list;
         const int64_t lbeg = list_off[l];
-        const int nl = (int)(list_off[l + 1] - lbeg);
+        const int nl = (int)min(list_off[l + 1] - lbeg, (long long)vmax);
         const int ps = list_stride[l];
         const float* __restrict__ X = codes + code_off[l];
         __syncthreads();
-        float myb[4];
+        if constexpr (QRES) {
+            // the whole query tile, row-coalesced reads, transposed into [dim][query]
+            for (int e = tid; e < QT * d; e += 256) {
+                const int j = e / d, i = e - j * d;
+                Qs[i][j] = j < nQ ? Qp[(int64_t)s_qid[j] * d + i] : 0.f;
+            }
+        }
+        float myb[QPT];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) myb[i] = s_bound[tq * 4 + i];
+        for (int i = 0; i < QPT; ++i) myb[i] = s_bound[tq * QPT + i];
 
-        for (int vt = 0; vt < nl; vt += VT) {
-            uint32_t valid = 0;
+        // ---- per-tile state
+        int vt = item.vec_begin;
+        const int vend = min(nl, item.vec_begin + vsplit);
+        uint32_t valid = 0, alive = 0;
+        int nvalid = 0;
+        bool wave_alive = false;
+        float dis[QPT][VPT], p[QPT][VPT];
+        auto begin_tile = [&]() {
+            valid = 0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < QPT; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (tq * 4 + i < nQ && vt + tv * 8 + j < nl) valid |= 1u << (i * 8 + j);
-            uint32_t alive = valid;
-            const int nvalid = __builtin_popcount(valid);
-            bool wave_alive = __any(alive != 0);
-            float dis[4][8], p[4][8];
+                for (int j = 0; j < VPT; ++j)
+                    if (tq * QPT + i < nQ && vt + tv * VPT + j < nl) valid |= 1u << (i * VPT + j);
+            alive = valid;
+            nvalid = __builtin_popcount(valid);
+            wave_alive = __any(alive != 0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < QPT; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) dis[i][j] = p[i][j] = 0.f;
-
-            for (int c = 0; c < plan.n; ++c) {
-                const int lo = plan.lo[c], len = plan.len[c];
-                for (int e = tid; e < QT * KC; e += 256) {
-                    const int j = e & (QT - 1), kk = e / QT;
-                    Qs[kk][j] = kk < len ? Qp[(int64_t)s_qid[j] * d + lo + kk] : 0.f;
-                }
-                for (int e = tid; e < KC * (VT / 4); e += 256) {
-                    const int kk = e / (VT / 4), v4 = e % (VT / 4);
-                    const int vv = vt + v4 * 4;
-                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (kk < len && vv < ps) v = *reinterpret_cast<const float4*>(X + (int64_t)(lo + kk) * ps + vv);
-                    *reinterpret_cast<float4*>(&Xs[kk][v4 * 4]) = v;
-                }
-                __syncthreads();
-                if (wave_alive) {
-                    for (int kk = 0; kk < len; ++kk) {
-                        const float4 qa = *reinterpret_cast<const float4*>(&Qs[kk][tq * 4]);
-                        const float4 x0 = *reinterpret_cast<const float4*>(&Xs[kk][tv * 8]);
-                        const float4 x1 = *reinterpret_cast<const float4*>(&Xs[kk][tv * 8 + 4]);
-                        const float qv[4] = {qa.x, qa.y, qa.z, qa.w};
-                        const float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                for (int j = 0; j < VPT; ++j) dis[i][j] = p[i][j] = 0.f;
+        };
+        // ---- prefetch registers and the job they hold
+        float qv[QLD];
+        float4 xv[XLD];
+        int reg_vt = -1, reg_c = -1;
+        auto load_job = [&](int jvt, int jc) {
+            const int lo = plan.lo[jc], len = plan.len[jc];
+            if constexpr (!QRES) {
 #pragma unroll
-                        for (int i = 0; i < 4; ++i)
-#pragma unroll
-                            for (int j = 0; j < 8; ++j) {
-                                const float t = qv[i] - xv[j];
-                                p[i][j] = __builtin_fmaf(t, t, p[i][j]);
-                            }
-                    }
-                }
-                __syncthreads();
-                if (plan.ends_block[c]) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            dis[i][j] += p[i][j];
-                            p[i][j] = 0.f;
-                        }
-                    if (wave_alive) evals += (unsigned long long)nvalid;
-                    if (prune && plan.prune_after[c]) {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-#pragma unroll
-                            for (int j = 0; j < 8; ++j)
-                                if (dis[i][j] > myb[i]) alive &= ~(1u << (i * 8 + j));
-                        wave_alive = __any(alive != 0);
-                        if (!__syncthreads_or(alive != 0)) break;
-                    }
+                for (int u = 0; u < QLD; ++u) {
+                    const int e = tid + 256 * u;
+                    const int kk = e / QT;
+                    qv[u] = (e < QT * KC && kk < len) ? Qp[(int64_t)s_qid[e % QT] * d + lo + kk] : 0.f;
                 }
             }
-            if (alive) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+            for (int u = 0; u < XLD; ++u) {
+                const int e = tid + 256 * u;
+                const int kk = e / (VT / 4), v4 = e % (VT / 4);
+                const int vv = jvt + v4 * 4;
+                xv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (e < KC * VT / 4 && kk < len && vv < ps)
+                    xv[u] = *reinterpret_cast<const float4*>(X + (int64_t)(lo + kk) * ps + vv);
+            }
+            reg_vt = jvt;
+            reg_c = jc;
+        };
+
+        int c = 0;
+        begin_tile();
+        load_job(vt, 0);
+        while (true) {
+            // the registers hold job (vt, c): move it to LDS
+            if constexpr (!QRES) {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if (((alive >> (i * 8 + j)) & 1u) && dis[i][j] <= myb[i]) {
-                            const int q = s_qid[tq * 4 + i];
-                            const unsigned int slot = atomicAdd(&surv_cnt[q], 1u);
+                for (int u = 0; u < QLD; ++u) {
+                    const int e = tid + 256 * u;
+                    if (e < QT * KC) Qs[e / QT][e % QT] = qv[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < XLD; ++u) {
+                const int e = tid + 256 * u;
+                if (e < KC * VT / 4) *reinterpret_cast<float4*>(&Xs[e / (VT / 4)][(e % (VT / 4)) * 4]) = xv[u];
+            }
+            __syncthreads();
+            const bool prune_point = prune && plan.prune_after[c];
+            if (prune_point && predict_die) {
+                if (vt + VT < vend) load_job(vt + VT, 0);
+            } else if (c + 1 < plan.n) {
+                load_job(vt, c + 1);
+            } else if (vt + VT < vend) {
+                load_job(vt + VT, 0);
+            }
+            const int len = plan.len[c];
+            const int qrow0 = QRES ? plan.lo[c] : 0;
+            if (wave_alive) {
+                auto step = [&](int kk) {
+                    float qf[QPT], xf[VPT];
+                    if constexpr (QPT == 2) {
+                        const float2 a = *reinterpret_cast<const float2*>(&Qs[qrow0 + kk][tq * QPT]);
+                        qf[0] = a.x;
+                        qf[1] = a.y;
+                    } else {
+#pragma unroll
+                        for (int h = 0; h < QPT / 4; ++h) {
+                            const float4 a = *reinterpret_cast<const float4*>(&Qs[qrow0 + kk][tq * QPT + 4 * h]);
+                            qf[4 * h] = a.x;
+                            qf[4 * h + 1] = a.y;
+                            qf[4 * h + 2] = a.z;
+                            qf[4 * h + 3] = a.w;
+                        }
+                    }
+#pragma unroll
+                    for (int h = 0; h < VPT / 4; ++h) {
+                        const float4 b = *reinterpret_cast<const float4*>(&Xs[kk][tv * VPT + 4 * h]);
+                        xf[4 * h] = b.x;
+                        xf[4 * h + 1] = b.y;
+                        xf[4 * h + 2] = b.z;
+                        xf[4 * h + 3] = b.w;
+                    }
+#pragma unroll
+                    for (int i = 0; i < QPT; ++i)
+#pragma unroll
+                        for (int j = 0; j < VPT; ++j) {
+                            const float t = qf[i] - xf[j];
+                            p[i][j] = __builtin_fmaf(t, t, p[i][j]);
+                        }
+                };
+                for (int s0 = 0; s0 < len; s0 += 8) {
+                    const int s1 = min(s0 + 8, len);
+#pragma nounroll
+                    for (int kk = s0; kk < s1; ++kk) step(kk);
+                    if (prune && s1 < len) {
+#pragma unroll
+                        for (int i = 0; i < QPT; ++i)
+#pragma unroll
+                            for (int j = 0; j < VPT; ++j)
+                                if (dis[i][j] + p[i][j] > myb[i]) alive &= ~(1u << (i * VPT + j));
+                        if (!__any(alive != 0)) break;
+                    }
+                }
+                wave_alive = __any(alive != 0);
+            }
+            __syncthreads();
+            bool died = false;
+            if (plan.ends_block[c]) {
+#pragma unroll
+                for (int i = 0; i < QPT; ++i)
+#pragma unroll
+                    for (int j = 0; j < VPT; ++j) {
+                        dis[i][j] += p[i][j];
+                        p[i][j] = 0.f;
+                    }
+                if (wave_alive) evals += (unsigned long long)nvalid;
+                if (prune_point) {
+#pragma unroll
+                    for (int i = 0; i < QPT; ++i)
+#pragma unroll
+                        for (int j = 0; j < VPT; ++j)
+                            if (dis[i][j] > myb[i]) alive &= ~(1u << (i * VPT + j));
+                    wave_alive = __any(alive != 0);
+                    died = !__syncthreads_or(alive != 0);
+                    predict_die = died;
+                }
+            }
+            const bool tile_done = died || c + 1 == plan.n;
+            if (tile_done && !died && __any(alive != 0)) {
+                // Append the survivors. The TV consecutive lanes sharing a query row reserve
+                // their slots with one atomic per row: a prefix sum of the per-lane counts over
+                // the group, the last lane adds the group total to the query's counter.
+#pragma unroll
+                for (int i = 0; i < QPT; ++i) {
+                    uint32_t m = 0;
+#pragma unroll
+                    for (int j = 0; j < VPT; ++j)
+                        if (((alive >> (i * VPT + j)) & 1u) && (accept_all || dis[i][j] <= myb[i])) m |= 1u << j;
+                    const int cnt = __builtin_popcount(m);
+                    int incl = cnt;
+#pragma unroll
+                    for (int off = 1; off < TV; off <<= 1) {
+                        const int y = __shfl_up(incl, off, TV);
+                        if (tv >= off) incl += y;
+                    }
+                    const int total = __shfl(incl, TV - 1, TV);
+                    const int q = s_qid[tq * QPT + i];
+                    unsigned int base = 0;
+                    if (tv == TV - 1 && total > 0) base = atomicAdd(&surv_cnt[q], (unsigned int)total);
+                    base = (unsigned int)__shfl((int)base, TV - 1, TV);
+                    unsigned int slot = base + (unsigned int)(incl - cnt);
+#pragma unroll
+                    for (int j = 0; j < VPT; ++j)
+                        if ((m >> j) & 1u) {
                             if (slot < (unsigned int)cap)
-                                surv[(int64_t)q * cap + slot] = hv_key(dis[i][j], ids[lbeg + vt + tv * 8 + j]);
+                                surv[(int64_t)q * cap + slot] = hv_key(dis[i][j], ids[lbeg + vt + tv * VPT + j]);
+                            ++slot;
                         }
-                    }
+                }
             }
+            const int nvt = tile_done ? vt + VT : vt;
+            const int nc = tile_done ? 0 : c + 1;
+            if (nvt >= vend) break;
+            if (reg_vt != nvt || reg_c != nc) load_job(nvt, nc);
+            if (tile_done) {
+                vt = nvt;
+                begin_tile();
+            }
+            c = nc;
         }
         __syncthreads();
     }
@@ -420,71 +536,95 @@ __global__ __launch_bounds__(256) void k_list_scan(const float* __restrict__ Qp,
     if ((tid & 63) == 0 && evals) atomicAdd(&stats[1], evals);
 }
 
-int scan_grid_size(int device) {
+// Compiled tile shapes: {queries per thread, vectors per thread, threads along queries}.
+struct ShapeInfo {
+    int qpt, vpt, tq;
+};
+static const ShapeInfo kShapes[kNumShapes] = {
+    {4, 8, 16},  // 0: 64 queries x 128 vectors
+    {4, 4, 16},  // 1: 64 x 64
+    {2, 8, 16},  // 2: 32 x 128
+    {2, 4, 16},  // 3: 32 x 64
+    {4, 4, 4},   // 4: 16 x 256
+    {2, 4, 8},   // 5: 16 x 128
+    {8, 4, 8},   // 6: 64 x 128 (8 x 4 per thread)
+};
+
+int scan_query_tile(int shape) { return kShapes[shape].qpt * kShapes[shape].tq; }
+int scan_vector_tile(int shape) { return kShapes[shape].vpt * (256 / kShapes[shape].tq); }
+
+// The query tile stays in LDS for the whole item when it takes at most this many bytes.
+constexpr int kQresBytes = 40 * 1024;
+static bool use_qres(int shape, int d) { return (size_t)d * (scan_query_tile(shape) + 4) * 4 <= kQresBytes; }
+static size_t qlds_bytes(int shape, int d) {
+    return (size_t)(use_qres(shape, d) ? d : kChunkDims) * (scan_query_tile(shape) + 4) * sizeof(float);
+}
+
+template <int QPT, int VPT, int TQ, bool QRES>
+static int grid_for(int device, size_t lds) {
     int ncu = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
         ncu = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_list_scan, 256, 0) != hipSuccess || per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_list_scan<QPT, VPT, TQ, QRES>, 256, lds) !=
+            hipSuccess ||
+        per_cu <= 0)
         per_cu = 2;
     return ncu * per_cu;
 }
 
-void launch_list_scan(const float* Qp, int d, const float* codes, const int64_t* code_off, const int64_t* list_off,
-                      const int* list_stride, const uint32_t* ids, const ChunkPlan& plan, int prune,
-                      const float* bound, const int* pairs, const ScanItem* items, const int* queue_len,
-                      int* queue_head, int qcap, unsigned long long* surv, unsigned int* surv_cnt, int cap,
+#define HV_SHAPES(X)      \
+    X(0, 4, 8, 16)        \
+    X(1, 4, 4, 16)        \
+    X(2, 2, 8, 16)        \
+    X(3, 2, 4, 16)        \
+    X(4, 4, 4, 4)         \
+    X(5, 2, 4, 8)         \
+    X(6, 8, 4, 8)
+
+int scan_grid_size(int device, int shape, int d) {
+    const size_t lds = qlds_bytes(shape, d);
+    const bool qres = use_qres(shape, d);
+#define HV_GRID_CASE(ID, A, B, C) \
+    case ID: return qres ? grid_for<A, B, C, true>(device, lds) : grid_for<A, B, C, false>(device, lds);
+    switch (shape) {
+        HV_SHAPES(HV_GRID_CASE)
+        default: return 512;
+    }
+#undef HV_GRID_CASE
+}
+
+void launch_list_scan(int shape, const float* Qp, int d, const float* codes, const int64_t* code_off,
+                      const int64_t* list_off, const int* list_stride, const uint32_t* ids, const ChunkPlan& plan,
+                      int prune, const float* bound, const int* pairs, const ScanItem* items, int* queue, int vmax,
+                      int vsplit, int accept_all, unsigned long long* surv, unsigned int* surv_cnt, int cap,
                       unsigned long long* stats, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_list_scan, dim3((unsigned)grid), dim3(256), 0, s, Qp, d, codes, code_off, list_off,
-                       list_stride, ids, plan, prune, bound, pairs, items, queue_len, queue_head, qcap, surv,
-                       surv_cnt, cap, stats);
+    const size_t lds = qlds_bytes(shape, d);
+    const bool qres = use_qres(shape, d);
+#define HV_SCAN_LAUNCH(A, B, C, R)                                                                             \
+    hipLaunchKernelGGL((k_list_scan<A, B, C, R>), dim3((unsigned)grid), dim3(256), lds, s, Qp, d, codes, code_off, \
+                       list_off, list_stride, ids, plan, prune, bound, pairs, items, queue, vmax, vsplit,           \
+                       accept_all, surv, surv_cnt, cap, stats)
+#define HV_SCAN_CASE(ID, A, B, C)                \
+    case ID:                                     \
+        if (qres) HV_SCAN_LAUNCH(A, B, C, true); \
+        else HV_SCAN_LAUNCH(A, B, C, false);     \
+        break;
+    switch (shape) {
+        HV_SHAPES(HV_SCAN_CASE)
+        default: break;
+    }
+#undef HV_SCAN_CASE
+#undef HV_SCAN_LAUNCH
 }
 
 // ---------------------------------------------------------------------------------------
-// Per-query top-k of the surviving candidates: sort the (distance, id) keys in LDS, write
-// the first k, pad with (FLT_MAX, -1) (sort_result, heap.hpp:84-90, on (distance, id)).
-// Queries whose buffer overflowed are listed for the exact per-query kernel instead.
-__global__ __launch_bounds__(256) void k_select_survivors(const unsigned long long* __restrict__ surv,
-                                                          const unsigned int* __restrict__ surv_cnt, int cap, int k,
-                                                          float* __restrict__ D, int64_t* __restrict__ I,
-                                                          int* __restrict__ overflow_list,
-                                                          int* __restrict__ overflow_cnt,
-                                                          unsigned long long* __restrict__ stats) {
-    extern __shared__ unsigned long long s_keys[];
-    const int64_t q = blockIdx.x;
-    const unsigned int n = surv_cnt[q];
-    if (n > (unsigned int)cap) {
-        if (threadIdx.x == 0) {
-            const int slot = atomicAdd(overflow_cnt, 1);
-            overflow_list[slot] = (int)q;
-        }
-        return;
-    }
-    const int P = hv_next_pow2(max(max((int)n, k), 2));
-    for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < (int)n ? surv[q * cap + i] : HV_KEY_PAD;
-    block_bitonic_sort(s_keys, P);
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
-        const unsigned long long key = s_keys[i];
-        D[q * k + i] = hv_key_dist(key);
-        I[q * k + i] = hv_key_id(key);
-    }
-    if (threadIdx.x == 0 && n) atomicAdd(&stats[2], (unsigned long long)n);
-}
-
-void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
-                             float* D, int64_t* I, int* overflow_list, int* overflow_cnt, unsigned long long* stats,
-                             hipStream_t s) {
-    if (nq <= 0) return;
-    const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
-    hipLaunchKernelGGL(k_select_survivors, dim3((unsigned)nq), dim3(256), lds, s, surv, surv_cnt, cap, k, D, I,
-                       overflow_list, overflow_cnt, stats);
-}
-
-// ---------------------------------------------------------------------------------------
-// Exact per-query path for queries whose candidate buffer overflowed: one workgroup walks the
-// query's probed lists, one candidate per thread, keeping a sorted top-k in LDS and merging a
+// Exact per-query path for queries whose candidate buffer overflowed (the list and its length
+// are on the device; workgroups stride over it, so the host never waits for the count): one
+// workgroup walks the query's probed lists, one candidate per thread, keeping a sorted top-k in LDS and merging a
 // 1024-entry buffer of candidates that beat the current k-th into it by bitonic sort. Same
 // arithmetic, pruning and (distance <= bound) filter as the scan, so the result is the same.
-__global__ __launch_bounds__(256) void k_query_exact(const int* __restrict__ qlist, const float* __restrict__ Qp,
+__global__ __launch_bounds__(256) void k_query_exact(const int* __restrict__ qlist, const int* __restrict__ qcount,
+                                                     const float* __restrict__ Qp,
                                                      int d, const float* __restrict__ codes,
                                                      const int64_t* __restrict__ code_off,
                                                      const int64_t* __restrict__ list_off,
@@ -499,8 +639,11 @@ __global__ __launch_bounds__(256) void k_query_exact(const int* __restrict__ qli
     float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * (TOP + BUF));
     __shared__ int s_cnt;
     __shared__ unsigned long long s_kth;
-    const int64_t q = qlist[blockIdx.x];
+    const int nql = *qcount;
+  for (int w = blockIdx.x; w < nql; w += gridDim.x) {
+    const int64_t q = qlist[w];
     const float b = bound[q];
+    __syncthreads();
     for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Qp[q * d + i];
     for (int i = threadIdx.x; i < TOP + BUF; i += blockDim.x) s[i] = HV_KEY_PAD;
     if (threadIdx.x == 0) {
@@ -544,15 +687,16 @@ __global__ __launch_bounds__(256) void k_query_exact(const int* __restrict__ qli
         D[q * k + i] = hv_key_dist(s[i]);
         I[q * k + i] = hv_key_id(s[i]);
     }
+  }
 }
 
-void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes,
+void launch_query_exact(const int* qlist, const int* qcount, int grid, const float* Qp, int d, const float* codes,
                         const int64_t* code_off, const int64_t* list_off, const int* list_stride,
                         const uint32_t* ids, const int* probes, int nprobe, const ChunkPlan& plan, int prune,
                         const float* bound, int k, float* D, int64_t* I, hipStream_t s) {
-    if (nql <= 0) return;
+    if (grid <= 0) return;
     const size_t lds = sizeof(unsigned long long) * 2048 + sizeof(float) * ((d + 3) & ~3);
-    hipLaunchKernelGGL(k_query_exact, dim3((unsigned)nql), dim3(256), lds, s, qlist, Qp, d, codes, code_off,
+    hipLaunchKernelGGL(k_query_exact, dim3((unsigned)grid), dim3(256), lds, s, qlist, qcount, Qp, d, codes, code_off,
                        list_off, list_stride, ids, probes, nprobe, plan, prune, bound, k, D, I);
 }
 

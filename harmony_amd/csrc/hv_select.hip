@@ -1,0 +1,226 @@
+// Per-row / per-query selections of the IVF-flat path: the coarse nprobe select, the warm-up
+// bound (k-th distance), the top-k compaction that tightens the bound, and the final top-k.
+// Every one sorts 64-bit (distance, id) keys — distance bits high, id low, so the integer
+// order is the (distance, id) order — and keeps a prefix. Rows of up to 2048 keys are sorted
+// by one wavefront in registers (wave_bitonic_sort); larger rows by a workgroup in LDS.
+#include "hv_device.h"
+#include "hv_kernels.h"
+
+namespace hv {
+
+constexpr float kFltMax = 3.402823466e+38f;
+
+// ---------------------------------------------------------------------------------------
+// Coarse select (Index.cpp:967-997 via lite_scan_codes into an nprobe-heap + sort_result,
+// IVFScan.hpp:108-138, heap.hpp:84-90): the p smallest (distance, column) of each row, in
+// ascending order; the boundary tie goes to the lower column, as the reference's strict '<' in
+// column order does.
+__global__ __launch_bounds__(256) void k_select_rows_wave(const float* __restrict__ dist, int64_t n, int m, int p,
+                                                          int* __restrict__ out_ids, float* __restrict__ out_d) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const float* __restrict__ r = dist + row * m;
+    wave_sort_emit(
+        m, [&](int j) { return hv_key(r[j], (uint32_t)j); },
+        [&](int i, unsigned long long key) {
+            if (i < p) {
+                out_ids[row * p + i] = (int)(key & 0xffffffffu);
+                if (out_d) out_d[row * p + i] = hv_key_dist(key);
+            }
+        });
+}
+
+__global__ __launch_bounds__(256) void k_select_rows_block(const float* __restrict__ dist, int m, int p,
+                                                           int* __restrict__ out_ids, float* __restrict__ out_d) {
+    extern __shared__ unsigned long long s_keys[];
+    const int64_t row = blockIdx.x;
+    const int P = hv_next_pow2(max(m, 2));
+    for (int c = threadIdx.x; c < P; c += blockDim.x)
+        s_keys[c] = c < m ? hv_key(dist[row * m + c], (uint32_t)c) : HV_KEY_PAD;
+    block_bitonic_sort(s_keys, P);
+    for (int i = threadIdx.x; i < p; i += blockDim.x) {
+        const unsigned long long k = s_keys[i];
+        out_ids[row * p + i] = (int)(k & 0xffffffffu);
+        if (out_d) out_d[row * p + i] = hv_key_dist(k);
+    }
+}
+
+void launch_select_rows(const float* dist, int64_t n, int m, int p, int* out_ids, float* out_d, hipStream_t s) {
+    if (n <= 0) return;
+    if (m <= kWaveSortMax) {
+        hipLaunchKernelGGL(k_select_rows_wave, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, dist, n, m, p, out_ids,
+                           out_d);
+    } else {
+        const size_t lds = sizeof(unsigned long long) * hv_next_pow2(m);
+        hipLaunchKernelGGL(k_select_rows_block, dim3((unsigned)n), dim3(256), lds, s, dist, m, p, out_ids, out_d);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Warm-up bound (Index::warmUpSearch, Index.cpp:999-1051; heapTops = heap top, 1303-1305):
+// the k-th smallest of the query's warm-up distances, FLT_MAX when fewer than k were seen.
+// Resets the query's candidate count for the scan that follows.
+__global__ __launch_bounds__(256) void k_kth_wave(const unsigned long long* __restrict__ surv,
+                                                  unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
+                                                  float* __restrict__ bound_ref, float* __restrict__ bound) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const int n = (int)min(cnt[q], (unsigned int)cap);
+    if (n > kWaveSortMax) return;  // k_kth_block
+    const unsigned long long* __restrict__ src = surv + q * cap;
+    if (n >= k) {
+        wave_sort_emit(
+            n, [&](int j) { return src[j]; },
+            [&](int i, unsigned long long key) {
+                if (i == k - 1) bound_ref[q] = bound[q] = hv_key_dist(key);
+            });
+    } else if ((threadIdx.x & 63) == 0) {
+        bound_ref[q] = bound[q] = kFltMax;
+    }
+    if ((threadIdx.x & 63) == 0) cnt[q] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_kth_block(const unsigned long long* __restrict__ surv,
+                                                   unsigned int* __restrict__ cnt, int cap, int k,
+                                                   float* __restrict__ bound_ref, float* __restrict__ bound) {
+    extern __shared__ unsigned long long s_keys[];
+    const int64_t q = blockIdx.x;
+    const int n = (int)min(cnt[q], (unsigned int)cap);
+    if (n <= kWaveSortMax) return;  // k_kth_wave
+    const int P = hv_next_pow2(n);
+    for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < n ? surv[q * cap + i] : HV_KEY_PAD;
+    block_bitonic_sort(s_keys, P);
+    if (threadIdx.x == 0) {
+        bound_ref[q] = bound[q] = hv_key_dist(s_keys[k - 1]);
+        cnt[q] = 0;
+    }
+}
+
+void launch_kth_survivors(const unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k,
+                          float* bound_ref, float* bound, hipStream_t s) {
+    if (nq <= 0) return;
+    if (cap > kWaveSortMax) {  // before the wave kernel resets the counts
+        const size_t lds = sizeof(unsigned long long) * hv_next_pow2(cap);
+        hipLaunchKernelGGL(k_kth_block, dim3((unsigned)nq), dim3(256), lds, s, surv, cnt, cap, k, bound_ref, bound);
+    }
+    hipLaunchKernelGGL(k_kth_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, cnt, nq, cap, k, bound_ref,
+                       bound);
+}
+
+// ---------------------------------------------------------------------------------------
+// After the scan of each query's nearest list: keep only its k best keys and lower the bound to
+// their k-th distance when that is tighter. The k-th of real candidates bounds the final k-th
+// distance, so a tighter bound prunes the remaining lists harder without changing the result,
+// and any key outside these k has k better keys and cannot be a result. A query whose buffer
+// overflowed keeps its keys (it goes to the exact path) but still gets the tighter bound of
+// the keys that were stored.
+__global__ __launch_bounds__(256) void k_compact_wave(unsigned long long* __restrict__ surv,
+                                                      unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
+                                                      float* __restrict__ bound) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const unsigned int c = cnt[q];
+    const int n = (int)min(c, (unsigned int)cap);
+    if (n == 0 || n > kWaveSortMax) return;
+    unsigned long long* __restrict__ buf = surv + q * cap;
+    const bool keep = c <= (unsigned int)cap;
+    const int m = min(n, k);
+    wave_sort_emit(
+        n, [&](int j) { return buf[j]; },
+        [&](int i, unsigned long long key) {
+            if (keep && i < m) buf[i] = key;
+            if (i == k - 1 && n >= k) bound[q] = fminf(bound[q], hv_key_dist(key));
+        });
+    if (keep && (threadIdx.x & 63) == 0) cnt[q] = (unsigned int)m;
+}
+
+__global__ __launch_bounds__(256) void k_compact_block(unsigned long long* __restrict__ surv,
+                                                       unsigned int* __restrict__ cnt, int cap, int k,
+                                                       float* __restrict__ bound) {
+    extern __shared__ unsigned long long s_keys[];
+    const int64_t q = blockIdx.x;
+    const unsigned int c = cnt[q];
+    const int n = (int)min(c, (unsigned int)cap);
+    if (n <= kWaveSortMax) return;  // k_compact_wave
+    const int P = hv_next_pow2(n);
+    for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < n ? surv[q * cap + i] : HV_KEY_PAD;
+    block_bitonic_sort(s_keys, P);
+    if (c <= (unsigned int)cap) {
+        const int m = min(n, k);
+        for (int i = threadIdx.x; i < m; i += blockDim.x) surv[q * cap + i] = s_keys[i];
+        __syncthreads();
+        if (threadIdx.x == 0) cnt[q] = (unsigned int)m;
+    }
+    if (threadIdx.x == 0) bound[q] = fminf(bound[q], hv_key_dist(s_keys[k - 1]));
+}
+
+void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k, float* bound,
+                         hipStream_t s) {
+    if (nq <= 0) return;
+    if (cap > kWaveSortMax) {  // before the wave kernel shrinks the counts
+        const size_t lds = sizeof(unsigned long long) * hv_next_pow2(cap);
+        hipLaunchKernelGGL(k_compact_block, dim3((unsigned)nq), dim3(256), lds, s, surv, cnt, cap, k, bound);
+    }
+    hipLaunchKernelGGL(k_compact_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, cnt, nq, cap, k, bound);
+}
+
+// ---------------------------------------------------------------------------------------
+// Final top-k of the surviving candidates: the first k keys in (distance, id) order, padded
+// with (FLT_MAX, -1) (heap_init + sort_result, heap.hpp:13-90, merge of Index.cpp:1327-1343).
+// Queries whose buffer overflowed are listed for the exact per-query kernel instead.
+__global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* __restrict__ surv,
+                                                     const unsigned int* __restrict__ surv_cnt, int64_t nq, int cap,
+                                                     int k, float* __restrict__ D, int64_t* __restrict__ I,
+                                                     int* __restrict__ overflow_list, int* __restrict__ overflow_cnt,
+                                                     unsigned long long* __restrict__ stats) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const unsigned int n = surv_cnt[q];
+    const bool lane0 = (threadIdx.x & 63) == 0;
+    if (n > (unsigned int)cap) {
+        if (lane0) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
+        return;
+    }
+    if (lane0 && n) atomicAdd(&stats[2], (unsigned long long)n);
+    const int m = max((int)n, k);
+    if (m > kWaveSortMax) return;  // k_select_block
+    const unsigned long long* __restrict__ src = surv + q * cap;
+    wave_sort_emit(
+        m, [&](int j) { return j < (int)n ? src[j] : HV_KEY_PAD; },
+        [&](int i, unsigned long long key) {
+            if (i < k) {
+                D[q * k + i] = hv_key_dist(key);
+                I[q * k + i] = hv_key_id(key);
+            }
+        });
+}
+
+__global__ __launch_bounds__(256) void k_select_block(const unsigned long long* __restrict__ surv,
+                                                      const unsigned int* __restrict__ surv_cnt, int cap, int k,
+                                                      float* __restrict__ D, int64_t* __restrict__ I) {
+    extern __shared__ unsigned long long s_keys[];
+    const int64_t q = blockIdx.x;
+    const unsigned int n = surv_cnt[q];
+    if (n > (unsigned int)cap || max((int)n, k) <= kWaveSortMax) return;  // overflow list / k_select_wave
+    const int P = hv_next_pow2(max((int)n, k));
+    for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < (int)n ? surv[q * cap + i] : HV_KEY_PAD;
+    block_bitonic_sort(s_keys, P);
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        D[q * k + i] = hv_key_dist(s_keys[i]);
+        I[q * k + i] = hv_key_id(s_keys[i]);
+    }
+}
+
+void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
+                             float* D, int64_t* I, int* overflow_list, int* overflow_cnt, unsigned long long* stats,
+                             hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_select_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, surv_cnt, nq, cap, k, D,
+                       I, overflow_list, overflow_cnt, stats);
+    if (cap > kWaveSortMax || k > kWaveSortMax) {
+        const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
+        hipLaunchKernelGGL(k_select_block, dim3((unsigned)nq), dim3(256), lds, s, surv, surv_cnt, cap, k, D, I);
+    }
+}
+
+}  // namespace hv

@@ -117,9 +117,12 @@ class Index:
         self._push()
 
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
-                  survivor_cap=0):
+                  survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0):
         """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59)."""
         p = self._params
+        p.refine_bound = 1 if refine_bound else 0
+        p.tile_shape = int(tile_shape)
+        p.tile_shape_first = int(tile_shape_first)
         p.warmup_list = int(warmup_list)
         p.warmup_list_size = int(warmup_list_size)
         p.pruning = 1 if pruning else 0

@@ -72,8 +72,12 @@ typedef struct {
     int64_t warmup_list;      /* lists used for the warm-up bound (default 10) */
     int64_t warmup_list_size; /* vectors per warm-up list (0 -> k, query.cpp:328-333) */
     int32_t precision;        /* HV_PRECISION_* */
-    int32_t reserved0;
+    int32_t refine_bound;     /* 1: after each query's nearest list, lower its bound to the running
+                                 k-th distance (min with the warm-up bound; results unchanged) */
     int64_t survivor_cap;     /* per-query candidate buffer (0 = automatic) */
+    int32_t tile_shape;       /* list-scan tile shape of the remaining-lists phase: 0 = automatic,
+                                 s > 0 = compiled shape s-1 */
+    int32_t tile_shape_first; /* same, for the nearest-list phase */
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */

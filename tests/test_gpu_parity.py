@@ -46,12 +46,13 @@ def test_w1_build_matches_oracle(w1):
     np.testing.assert_array_equal(g.list_codes(), o.codes)
 
 
-@pytest.mark.parametrize("B,pruning", [(4, True), (4, False), (1, True), (3, True), (8, True)])
-def test_w1_search_matches_oracle(w1, B, pruning):
+@pytest.mark.parametrize("B,pruning,refine", [(4, True, True), (4, True, False), (4, False, True), (1, True, True),
+                                              (3, True, True), (8, True, False)])
+def test_w1_search_matches_oracle(w1, B, pruning, refine):
     xb, xq, g, o = w1
-    say("w1 search B=%d pruning=%s" % (B, pruning))
+    say("w1 search B=%d pruning=%s refine=%s" % (B, pruning, refine))
     g.nprobe = 10
-    g.preSearch(warmup_list=10, warmup_list_size=0, pruning=pruning, dim_blocks=B)
+    g.preSearch(warmup_list=10, warmup_list_size=0, pruning=pruning, dim_blocks=B, refine_bound=refine)
     D, I, st = g.search(xq, 10, return_stats=True)
     oD, oI, oprobes, obound = O.search(o, xq, 10, nprobe=10, dim_blocks=B, pruning=pruning)
     np.testing.assert_array_equal(g.last_probes(xq.shape[0]), oprobes)
