@@ -24,6 +24,7 @@ HV_OPT_ALL = 7
 HV_BOUND_WARMUP = 0
 HV_BOUND_EXTERNAL = 1
 HV_PRECISION_FP32 = 0
+HV_PRECISION_BF16 = 1
 
 
 class HipExtensionMissing(RuntimeError):

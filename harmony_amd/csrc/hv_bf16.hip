@@ -1,0 +1,566 @@
+// BF16 precision form of the list scan: candidate distances on the matrix cores, exact FP32
+// re-rank of the survivors.
+//
+// For a query q probing list l (centroid c), and a vector x of the list, with residuals
+// rq = q - c and rx = x - c:  |q - x|^2 = |rq|^2 + |rx|^2 - 2 rq.rx.  The residual codes rx are
+// stored in BF16 (d zero-padded to dpad = 32 * NKS) with |rx|^2 in FP32; |rq|^2 is the coarse
+// distance of (q, l). rq.rx is accumulated in FP32 by v_mfma_f32_16x16x32_bf16 (16 queries x
+// 16 vectors x 32 dimensions per instruction).
+//
+// Code layout (per list, MFMA B-operand order): vector block vb (16 vectors), K-step s (32 dims),
+// lane t = 16 g + c holds dims s*32 + g*8 .. +8 of vector vb*16 + c, as 8 BF16 (16 bytes):
+//   element (v, k) -> (((v / 16) * NKS + k / 32) * 64 + ((k % 32) / 8) * 16 + v % 16) * 8 + k % 8
+// so one K-step of one block is one contiguous 1 KB load of the wavefront.
+//
+// Rounding makes the estimate d~ inexact; the kernel bounds the error of every pair by
+//   err = kappa1 * |rq| * |rx| + kappa2 * (|rq|^2 + |rx|^2)
+// (BF16 rounding of both residuals, FP32 accumulation, and the gap between the real distance
+// and the FP32 chain the exact path computes; the host derives kappa1/kappa2 from d). So
+// lb = d~ - err <= exact <= d~ + err = ub for every pair, and a pair can be dropped when
+// lb > bound without changing the result. Survivors are re-ranked with the exact FP32 distance
+// of the plain path, so the BF16 form returns the same ids and distances as the plain path.
+#include <algorithm>
+
+#include "hv_device.h"
+#include "hv_kernels.h"
+
+namespace hv {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+// ---------------------------------------------------------------------------------------
+// Index build: BF16 residual codes and FP32 residual norms of every list (one workgroup per
+// list), and the FP32 rows in list order for the re-rank.
+__global__ __launch_bounds__(256) void k_build_resid(const float* __restrict__ x, int d, int dpad,
+                                                     const int* __restrict__ order,
+                                                     const int64_t* __restrict__ list_off,
+                                                     const float* __restrict__ centroids,
+                                                     const int64_t* __restrict__ roff, __bf16* __restrict__ rcodes,
+                                                     float* __restrict__ rnorm) {
+    const int l = blockIdx.x;
+    const int64_t b = list_off[l];
+    const int n = (int)(list_off[l + 1] - b);
+    const int n16 = (n + 15) / 16 * 16;
+    const float* __restrict__ c = centroids + (int64_t)l * d;
+    __bf16* __restrict__ out = rcodes + roff[l];
+    const int nks = dpad / 32;
+    for (int64_t e = threadIdx.x; e < (int64_t)n16 * dpad; e += blockDim.x) {
+        const int v = (int)(e / dpad), k = (int)(e - (int64_t)v * dpad);
+        float r = 0.f;
+        if (v < n && k < d) r = x[(int64_t)order[b + v] * d + k] - c[k];
+        const int64_t o = ((((int64_t)(v >> 4) * nks + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + (v & 15)) << 3) + (k & 7);
+        out[o] = (__bf16)r;
+    }
+    for (int v = threadIdx.x; v < n; v += blockDim.x) {
+        const float* __restrict__ xv = x + (int64_t)order[b + v] * d;
+        float acc = 0.f;
+        for (int k = 0; k < d; ++k) {
+            const float t = xv[k] - c[k];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+        rnorm[b + v] = acc;
+    }
+}
+
+void launch_build_resid(const float* x, int d, int dpad, const int* order, const int64_t* list_off,
+                        const float* centroids, const int64_t* roff, int nlist, __bf16* rcodes, float* rnorm,
+                        hipStream_t s) {
+    if (nlist <= 0) return;
+    hipLaunchKernelGGL(k_build_resid, dim3((unsigned)nlist), dim3(256), 0, s, x, d, dpad, order, list_off, centroids,
+                       roff, rcodes, rnorm);
+}
+
+__global__ void k_gather_rows(const float* __restrict__ rows, int64_t n, int d, const int* __restrict__ order,
+                              float* __restrict__ by_pos) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * d) return;
+    const int64_t p = e / d;
+    const int k = (int)(e - p * d);
+    by_pos[e] = rows[(int64_t)order[p] * d + k];
+}
+
+void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, float* by_pos, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((n * d + 255) / 256)), dim3(256), 0, s, rows, n, d, order,
+                       by_pos);
+}
+
+// ---------------------------------------------------------------------------------------
+// Query residuals of every (query, probe rank) slot: BF16 (q - c) rows of dpad dims (zero past
+// d), one thread per 8 dims.
+__global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp, const float* __restrict__ centroids,
+                                                    const int* __restrict__ probes, int64_t nslots, int nprobe, int d,
+                                                    int dpad, bf16x8* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g8 = dpad / 8;
+    if (t >= nslots * g8) return;
+    const int64_t slot = t / g8;
+    const int k0 = (int)(t - slot * g8) * 8;
+    const float* __restrict__ qv = Qp + (slot / nprobe) * d;
+    const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = k0 + e;
+        r[e] = (__bf16)(k < d ? qv[k] - cv[k] : 0.f);
+    }
+    out[t] = r;
+}
+
+void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, int64_t nq, int nprobe, int d,
+                       int dpad, __bf16* out, hipStream_t s) {
+    const int64_t total = nq * nprobe * (dpad / 8);
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_pair_resid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Qp, centroids, probes,
+                       nq * nprobe, nprobe, d, dpad, reinterpret_cast<bf16x8*>(out));
+}
+
+// ---------------------------------------------------------------------------------------
+// The BF16 filter scan. Same work items as the FP32 scan (list, <= 64 queries, vector range),
+// assigned statically: workgroup b takes the items j, j + G/8, j + 2G/8, ... of the queue of its
+// XCD (b % 8), j = b / 8, G = grid size (the queues deal lists round-robin by size, so each
+// stride of items is similar work). The item's queries form up to 4 blocks of 16. A wavefront
+// holds the BF16 residuals of up to QBM query blocks in registers (NKS K-steps of 8 values per
+// lane and block) and streams 16-vector blocks, each loaded once and multiplied into every
+// block it holds; the wavefronts of the workgroup split the vector blocks (and the query blocks
+// when there are more than QBM). After the MFMAs each lane tests the 4 (query, vector) pairs of
+// each block it holds. Keys carry the vector's position in the list-ordered arrays, not its id.
+//   mode 0: append the pairs with lb <= bound to the query's buffer, key = lb.
+//   mode 1: the query's nearest list (probe rank 0 only): every pair of vector v is written to
+//           slot v (no atomics; the caller limits the vectors to v < cap and sets the counts with
+//           k_first_counts), key = lb, aux = ub.
+template <int NKS>
+__global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qres,
+                                                   const float* __restrict__ pdist, int nprobe,
+                                                   const __bf16* __restrict__ rcodes,
+                                                   const int64_t* __restrict__ roff,
+                                                   const float* __restrict__ rnorm,
+                                                   const int64_t* __restrict__ list_off,
+                                                   const float* __restrict__ bound,
+                                                   const int* __restrict__ pair_slot,
+                                                   const ScanItem* __restrict__ items, const int* __restrict__ queue,
+                                                   int vmax, int vsplit, int mode, float kappa1, float kappa2,
+                                                   unsigned long long* __restrict__ surv, float* __restrict__ aux,
+                                                   unsigned int* __restrict__ surv_cnt, int cap,
+                                                   unsigned long long* __restrict__ stats) {
+    constexpr int QBM = NKS <= 4 ? 4 : (NKS <= 8 ? 2 : 1);  // query blocks per wavefront
+    constexpr bool kPrefetch = NKS <= 8;                     // larger K keeps occupancy instead
+    __shared__ int s_q[64], s_slot[64];
+    __shared__ __align__(16) float s_nrq[64];
+    __shared__ __align__(16) float s_srq[64];
+    __shared__ __align__(16) float s_bound[64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, col16 = lane & 15;
+    const int xcd = blockIdx.x % kNumXcd;
+    const int per_xcd = max(1, (int)gridDim.x / kNumXcd);
+    const int qlen = queue[xcd], qbase = queue[2 * kNumXcd + xcd];
+    unsigned long long pairs_done = 0;
+
+    for (int slot_i = (int)blockIdx.x / kNumXcd; slot_i < qlen; slot_i += per_xcd) {
+        const ScanItem item = items[qbase + slot_i];
+        const int nQ = item.pair_end - item.pair_begin;
+        if (tid < 64) {
+            int q = 0, slot = 0;
+            float nr = 0.f, b = -1.f;  // rows past nQ: bound -1 keeps nothing
+            if (tid < nQ) {
+                slot = pair_slot[item.pair_begin + tid];
+                q = slot / nprobe;
+                nr = pdist[slot];
+                b = mode == 1 ? 0.f : bound[q];
+            }
+            s_q[tid] = q;
+            s_slot[tid] = slot;
+            s_nrq[tid] = nr;
+            s_srq[tid] = __builtin_amdgcn_sqrtf(nr);
+            s_bound[tid] = b;
+        }
+        const int l = item.list;
+        const int64_t lbeg = list_off[l];
+        const int nl = (int)min(list_off[l + 1] - lbeg, (long long)vmax);
+        const int vend = min(nl, item.vec_begin + vsplit);
+        __syncthreads();
+
+        const int QB = (nQ + 15) / 16;
+        const int ngrp = (QB + QBM - 1) / QBM;  // query-block groups: 1, 2, 3 or 4
+        const int ngp = ngrp == 1 ? 1 : (ngrp == 2 ? 2 : 4);
+        const int wpg = 4 / ngp;  // wavefronts per group, splitting its vector blocks
+        const int grp = w / wpg, vsub = w % wpg;
+        if (grp < ngrp && nl > 0) {
+            const int qb0 = grp * QBM;
+            const int nqb = min(QBM, QB - qb0);
+            // A operands: query row (qb0 + j)*16 + col16, dims s*32 + g*8 .. +8. Rows past nQ read
+            // slot s_slot[0]; their bound is -1.
+            bf16x8 a[QBM][NKS];
+#pragma unroll
+            for (int j = 0; j < QBM; ++j) {
+                const int qrow = (qb0 + j) * 16 + col16;
+                const bf16x8* __restrict__ qa = reinterpret_cast<const bf16x8*>(qres) +
+                                                (int64_t)s_slot[qrow < nQ ? qrow : 0] * (NKS * 4) + g;
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) a[j][s] = qa[s * 4];
+            }
+            const bf16x8* __restrict__ R = reinterpret_cast<const bf16x8*>(rcodes + roff[l]) + lane;
+            const int vstep = 16 * wpg;
+            int v0 = item.vec_begin + vsub * 16;
+            // software pipeline: the codes and norm of the next block are in flight while the
+            // current block is computed
+            bf16x8 bn[NKS];
+            float nxn = 0.f;
+            if (kPrefetch && v0 < vend) {
+                const int64_t blk = (int64_t)(v0 >> 4) * NKS * 64;
+#pragma unroll
+                for (int s = 0; s < NKS; ++s) bn[s] = R[blk + s * 64];
+                nxn = rnorm[lbeg + min(v0 + col16, nl - 1)];
+            }
+            for (; v0 < vend; v0 += vstep) {
+                bf16x8 bc[NKS];
+                float nx;
+                const int v1 = kPrefetch ? v0 + vstep : v0;
+                if (kPrefetch) {
+#pragma unroll
+                    for (int s = 0; s < NKS; ++s) bc[s] = bn[s];
+                    nx = nxn;
+                }
+                if (v1 < vend) {
+                    const int64_t blk = (int64_t)(v1 >> 4) * NKS * 64;
+#pragma unroll
+                    for (int s = 0; s < NKS; ++s) bn[s] = R[blk + s * 64];
+                    nxn = rnorm[lbeg + min(v1 + col16, nl - 1)];
+                }
+                if (!kPrefetch) {
+#pragma unroll
+                    for (int s = 0; s < NKS; ++s) bc[s] = bn[s];
+                    nx = nxn;
+                }
+                floatx4 acc[QBM];
+#pragma unroll
+                for (int j = 0; j < QBM; ++j) {
+                    acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+                    if (j < nqb) {
+#pragma unroll
+                        for (int s = 0; s < NKS; ++s)
+                            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j][s], bc[s], acc[j], 0, 0, 0);
+                    }
+                }
+                const int v = v0 + col16;
+                const bool cvalid = v < vend;
+                const float sx = __builtin_amdgcn_sqrtf(nx);
+                const uint32_t pos = (uint32_t)(lbeg + v);
+                pairs_done += 256 * nqb;
+#pragma unroll
+                for (int j = 0; j < QBM; ++j) {
+                    if (j >= nqb) break;
+                    int r0 = (qb0 + j) * 16 + g * 4;
+                    // re-read the rows' state from LDS each block rather than holding all of it in
+                    // registers across the loop (occupancy)
+                    asm volatile("" : "+v"(r0));
+                    const float4 nr4 = *reinterpret_cast<const float4*>(&s_nrq[r0]);
+                    const float4 sr4 = *reinterpret_cast<const float4*>(&s_srq[r0]);
+                    const float4 b4 = *reinterpret_cast<const float4*>(&s_bound[r0]);
+                    const float nrq[4] = {nr4.x, nr4.y, nr4.z, nr4.w};
+                    const float srq[4] = {sr4.x, sr4.y, sr4.z, sr4.w};
+                    const float bq[4] = {b4.x, b4.y, b4.z, b4.w};
+                    float lbv[4], ubv[4];
+                    bool kp[4];
+                    bool anyk = false;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float dt = nrq[i] + nx - 2.f * acc[j][i];
+                        const float err = kappa1 * srq[i] * sx + kappa2 * (nrq[i] + nx);
+                        lbv[i] = fmaxf(dt - err, 0.f);
+                        ubv[i] = fmaxf(dt + err, 0.f);
+                        kp[i] = cvalid && bq[i] >= 0.f && (mode == 1 || lbv[i] <= bq[i]);
+                        anyk |= kp[i];
+                    }
+                    if (mode == 1) {
+                        // direct slots: each query row has one list here
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            if (kp[i] && v < cap) {
+                                const int64_t o = (int64_t)s_q[r0 + i] * cap + v;
+                                surv[o] = hv_key(lbv[i], pos);
+                                aux[o] = ubv[i];
+                            }
+                        }
+                        continue;
+                    }
+                    if (__ballot(anyk) == 0ull) continue;
+                    // appends: one atomic per (query row, 16-lane group), all four issued before use
+                    unsigned int gm[4], base[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        gm[i] = (unsigned int)((__ballot(kp[i]) >> (g * 16)) & 0xffffu);
+                        base[i] = 0u;
+                    }
+                    if (col16 == 0) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (gm[i])
+                                base[i] = atomicAdd(&surv_cnt[s_q[r0 + i]], (unsigned int)__builtin_popcount(gm[i]));
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const unsigned int bs = (unsigned int)__shfl((int)base[i], g * 16, 64);
+                        if (kp[i]) {
+                            const unsigned int slot = bs + (unsigned int)__builtin_popcount(gm[i] & ((1u << col16) - 1u));
+                            if (slot < (unsigned int)cap) {
+                                const int64_t o = (int64_t)s_q[r0 + i] * cap + slot;
+                                surv[o] = hv_key(lbv[i], pos);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if ((tid & 63) == 0 && pairs_done) atomicAdd(&stats[3], pairs_done);
+}
+
+// Candidate counts after the direct-slot scan of the nearest lists (vectors v < cap).
+__global__ void k_first_counts(const int* __restrict__ probes, int64_t nq, int nprobe,
+                               const int64_t* __restrict__ list_off, int cap, unsigned int* __restrict__ cnt) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int l = probes[q * nprobe];
+    cnt[q] = (unsigned int)min(list_off[l + 1] - list_off[l], (long long)cap);
+}
+
+void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
+                         unsigned int* cnt, hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_first_counts, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, probes, nq, nprobe,
+                       list_off, cap, cnt);
+}
+
+int bf16_ksteps(int d) { return (d + 31) / 32; }
+
+void launch_bf16_scan(int grid, const __bf16* qres, int d, const float* pdist, int nprobe,
+                      const __bf16* rcodes, const int64_t* roff, const float* rnorm, const int64_t* list_off,
+                      const float* bound, const int* pair_slot, const ScanItem* items, const int* queue, int vmax,
+                      int vsplit, int mode, float kappa1, float kappa2, unsigned long long* surv, float* aux,
+                      unsigned int* surv_cnt, int cap, unsigned long long* stats, hipStream_t s) {
+#define HV_BF16_CASE(K)                                                                                         \
+    case K:                                                                                                     \
+        hipLaunchKernelGGL(k_bf16_scan<K>, dim3((unsigned)grid), dim3(256), 0, s, qres, pdist, nprobe, rcodes, roff, \
+                           rnorm, list_off, bound, pair_slot, items, queue, vmax, vsplit, mode, kappa1, kappa2, surv,  \
+                           aux, surv_cnt, cap, stats);                                                               \
+        break;
+    switch (bf16_ksteps(d)) {
+        HV_BF16_CASE(1)
+        HV_BF16_CASE(2)
+        HV_BF16_CASE(3)
+        HV_BF16_CASE(4)
+        HV_BF16_CASE(5)
+        HV_BF16_CASE(6)
+        HV_BF16_CASE(7)
+        HV_BF16_CASE(8)
+        HV_BF16_CASE(9)
+        HV_BF16_CASE(10)
+        HV_BF16_CASE(11)
+        HV_BF16_CASE(12)
+        HV_BF16_CASE(13)
+        HV_BF16_CASE(14)
+        HV_BF16_CASE(15)
+        HV_BF16_CASE(16)
+        default: break;
+    }
+#undef HV_BF16_CASE
+}
+
+int bf16_grid_size(int device) {
+    int ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bf16_scan<4>, 256, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    // the static item assignment deals the items of each XCD queue over grid / 8 workgroups
+    return std::max(kNumXcd, ncu * per_cu / kNumXcd * kNumXcd);
+}
+
+// ---------------------------------------------------------------------------------------
+// After the BF16 scan of each query's nearest list: the k-th smallest upper bound bounds the
+// k-th exact distance, so the query's bound drops to it when tighter; the stored pairs whose
+// lower bound exceeds the new bound cannot be results and are dropped (stable in-place
+// compaction). One wavefront per query; the k-th upper bound by a 4-pass radix select (8-bit
+// digits of the FP32 bits, which order non-negative floats) over a 256-bin LDS histogram.
+// An overflowed query (count > cap) keeps its count: its bound still tightens (the k-th over the
+// stored subset is >= the k-th over all), and the exact path redoes it.
+__global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
+                                                      const float* __restrict__ aux,
+                                                      unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
+                                                      float* __restrict__ bound) {
+    __shared__ unsigned int s_hist[4][256];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= nq) return;  // whole wavefront: no block-level barrier below
+    unsigned int* __restrict__ hist = s_hist[w];
+    const unsigned int c = cnt[q];
+    const int n = (int)min(c, (unsigned int)cap);
+    float b = bound[q];
+    unsigned long long* __restrict__ buf = surv + q * cap;
+    const unsigned int* __restrict__ ub = reinterpret_cast<const unsigned int*>(aux + q * cap);
+    if (n >= k) {
+        unsigned int prefix = 0u, mask = 0u;
+        int kk = k;  // rank (1-based) still to find inside the current prefix
+        for (int shift = 24; shift >= 0; shift -= 8) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) hist[lane * 4 + t] = 0u;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int j = lane; j < n; j += 64) {
+                unsigned int u = ub[j];
+                u = (int)u < 0 ? 0u : u;  // -0.0 orders as 0
+                if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            unsigned int h[4], sum = 0u;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                h[t] = hist[lane * 4 + t];
+                sum += h[t];
+            }
+            unsigned int incl = sum;  // inclusive prefix over lanes
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const unsigned int excl = incl - sum;
+            // the lane whose bins hold rank kk
+            const bool mine = excl < (unsigned int)kk && (unsigned int)kk <= incl;
+            int digit = 0;
+            unsigned int below = excl;
+            if (mine) {
+                unsigned int run = excl;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (run + h[t] >= (unsigned int)kk) {
+                        digit = lane * 4 + t;
+                        below = run;
+                        break;
+                    }
+                    run += h[t];
+                }
+            }
+            const unsigned long long who = __ballot(mine);
+            const int src = __builtin_ctzll(who);
+            digit = __shfl(digit, src, 64);
+            below = (unsigned int)__shfl((int)below, src, 64);
+            kk -= (int)below;
+            prefix |= (unsigned int)digit << shift;
+            mask |= 255u << shift;
+        }
+        b = fminf(b, __uint_as_float(prefix));
+    }
+    if (c > (unsigned int)cap) {
+        if (lane == 0) bound[q] = b;
+        return;
+    }
+    // stable in-place compaction of the pairs with lb <= b: each round reads its 64 keys before
+    // writing, and writes land at or before the round's first position
+    int out = 0;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        const unsigned long long key = j < n ? buf[j] : HV_KEY_PAD;
+        const bool keep = j < n && hv_key_dist(key) <= b;
+        const unsigned long long m = __ballot(keep);
+        const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (keep) buf[out + before] = key;
+        out += __builtin_popcountll(m);
+    }
+    if (lane == 0) {
+        cnt[q] = (unsigned int)out;
+        bound[q] = b;
+    }
+}
+
+void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
+                         float* bound, hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, aux, cnt, nq, cap, k,
+                       bound);
+}
+
+// ---------------------------------------------------------------------------------------
+// Exact re-rank: every stored pair's key (bound, position) becomes (exact FP32 distance, id), the
+// distance being the plain path's arithmetic (blocked FMA chains in dimension order) on the FP32
+// rows kept in list order. One wavefront per query, one pair per lane; each staging chunk
+// (<= 32 dims) of the row is loaded in one batch, then chained. V4: d and every chunk start are
+// multiples of 4.
+template <bool V4>
+__global__ __launch_bounds__(256) void k_rerank(const float* __restrict__ Qp, int d,
+                                                const float* __restrict__ rows_by_pos,
+                                                const uint32_t* __restrict__ ids, ChunkPlan plan,
+                                                unsigned long long* __restrict__ surv,
+                                                const unsigned int* __restrict__ cnt, int64_t nq, int cap) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const int lane = threadIdx.x & 63;
+    const unsigned int c = cnt[q];
+    if (c > (unsigned int)cap) return;  // overflowed: the exact path handles the query
+    const float* __restrict__ qv = Qp + q * d;
+    unsigned long long* __restrict__ buf = surv + q * cap;
+    for (int j = lane; j < (int)c; j += 64) {
+        const uint32_t pos = (uint32_t)(buf[j] & 0xffffffffu);
+        const uint32_t id = ids[pos];
+        const float* __restrict__ xv = rows_by_pos + (int64_t)pos * d;
+        float dis = 0.f, p = 0.f;
+        for (int ch = 0; ch < plan.n; ++ch) {
+            const int lo = plan.lo[ch], len = plan.len[ch];
+            float xr[kChunkDims], qr[kChunkDims];
+            if (V4) {
+                // a float4 starting at or past d only covers dims past the chunk (d % 4 == 0)
+#pragma unroll
+                for (int t = 0; t < kChunkDims / 4; ++t) {
+                    const int i = min(lo + 4 * t, d - 4);
+                    const float4 xx = *reinterpret_cast<const float4*>(xv + i);
+                    const float4 qq = *reinterpret_cast<const float4*>(qv + i);
+                    xr[4 * t] = xx.x;
+                    xr[4 * t + 1] = xx.y;
+                    xr[4 * t + 2] = xx.z;
+                    xr[4 * t + 3] = xx.w;
+                    qr[4 * t] = qq.x;
+                    qr[4 * t + 1] = qq.y;
+                    qr[4 * t + 2] = qq.z;
+                    qr[4 * t + 3] = qq.w;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < kChunkDims; ++t) {
+                    const int i = min(lo + t, d - 1);
+                    xr[t] = xv[i];
+                    qr[t] = qv[i];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kChunkDims; ++t) {
+                if (t < len) {
+                    const float u = qr[t] - xr[t];
+                    p = __builtin_fmaf(u, u, p);
+                }
+            }
+            if (plan.ends_block[ch]) {
+                dis += p;
+                p = 0.f;
+            }
+        }
+        buf[j] = hv_key(dis, id);
+    }
+}
+
+void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
+                   unsigned long long* surv, const unsigned int* cnt, int64_t nq, int cap, hipStream_t s) {
+    if (nq <= 0) return;
+    bool v4 = d >= 4 && (d & 3) == 0;
+    for (int ch = 0; ch < plan.n; ++ch) v4 = v4 && (plan.lo[ch] & 3) == 0;
+    const dim3 grid((unsigned)((nq + 3) / 4));
+    if (v4)
+        hipLaunchKernelGGL(k_rerank<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, nq, cap);
+    else
+        hipLaunchKernelGGL(k_rerank<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, nq, cap);
+}
+
+}  // namespace hv

@@ -160,13 +160,16 @@ struct hv_index {
     std::vector<int64_t> list_off, code_off, ids;  // ids in list order
     std::vector<int> list_stride, rank_order;
     DevBuf d_codes, d_list_off, d_code_off, d_list_stride, d_ids, d_rank_order;
+    // BF16 residual codes per list (MFMA operand order), their FP32 norms, and the FP32 rows in list order
+    DevBuf d_rcodes, d_roff, d_rnorm, d_rows;
+    int dpad = 0, bf16_grid = 0;
 
     hv_search_params sp{};
     std::vector<float> ext_bounds;
 
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
-        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats;
+        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats, s_pdist, s_pair_slot, s_aux, s_qres, s_runs;
     int64_t last_n = 0, last_nprobe = 0;
 };
 
@@ -257,6 +260,21 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
                                 h->d_code_off.as<int64_t>(), h->d_list_stride.as<int>(), (int)nlist,
                                 h->d_codes.as<float>(), h->stream);
     HV_CHECK_LAUNCH("layout");
+    // BF16 precision form: residual codes and norms, FP32 rows by id for the exact re-rank
+    h->dpad = hv::bf16_ksteps((int)d) * 32;
+    std::vector<int64_t> roff(nlist + 1, 0);
+    for (int64_t l = 0; l < nlist; ++l) roff[l + 1] = roff[l] + (int64_t)h->list_stride[l] * h->dpad;
+    upload(h->d_roff, roff.data(), sizeof(int64_t) * (nlist + 1), h->stream);
+    HV_HIP(h->d_rcodes.ensure(sizeof(__bf16) * std::max<int64_t>(1, roff[nlist])));
+    HV_HIP(h->d_rnorm.ensure(sizeof(float) * std::max<int64_t>(1, n)));
+    HV_HIP(h->d_rows.ensure(sizeof(float) * std::max<int64_t>(1, n * d)));
+    if (n) {
+        hv::launch_build_resid(xp, (int)d, h->dpad, order_buf.as<int>(), h->d_list_off.as<int64_t>(),
+                               h->d_centroids.as<float>(), h->d_roff.as<int64_t>(), (int)nlist,
+                               h->d_rcodes.as<__bf16>(), h->d_rnorm.as<float>(), h->stream);
+        hv::launch_gather_rows(xp, n, (int)d, order_buf.as<int>(), h->d_rows.as<float>(), h->stream);
+    }
+    HV_CHECK_LAUNCH("bf16 layout");
     HV_HIP(hipStreamSynchronize(h->stream));
     order_buf.release();
     h->ntotal = n;
@@ -304,7 +322,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     // 2. coarse: full distance rows + nprobe select (Index.cpp:967-997)
     HV_HIP(h->s_probes.ensure(sizeof(int) * n * nprobe));
+    HV_HIP(h->s_pdist.ensure(sizeof(float) * n * nprobe));
     int* probes = h->s_probes.as<int>();
+    float* pdist = h->s_pdist.as<float>();
     {
         const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (nlist * 4)));
         HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
@@ -313,7 +333,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d,
                                     h->s_dist.as<float>(), nlist, s);
             hv::launch_select_rows(h->s_dist.as<float>(), rows, (int)nlist, (int)nprobe, probes + r0 * nprobe,
-                                   nullptr, s);
+                                   pdist + r0 * nprobe, s);
         }
         HV_CHECK_LAUNCH("coarse");
     }
@@ -325,12 +345,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bucket_off.ensure(sizeof(int) * (nb + 1)));
     HV_HIP(h->s_scan_tmp.ensure(sizeof(int) * hv::scan_tmp_ints(nb)));
     HV_HIP(h->s_pairs.ensure(sizeof(int) * n * nprobe));
+    HV_HIP(h->s_pair_slot.ensure(sizeof(int) * n * nprobe));
     // work items of one phase: at most (query tiles) x (vector ranges); query tiles hold >= 16
     // queries, vector ranges >= kVecSplit vectors
     const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / kVecSplit + 1);
     require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
+    // stats: [0..3] counters of the scan phases, [4..7] of the warm-up
     HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 8));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
     hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
@@ -339,7 +361,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     hv::launch_count_pairs(probes, n, (int)nprobe, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
     hv::launch_scatter_pairs(probes, n, (int)nprobe, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
-                             h->s_pairs.as<int>(), s);
+                             h->s_pairs.as<int>(), h->s_pair_slot.as<int>(), s);
     HV_CHECK_LAUNCH("grouping");
     HV_HIP(hipEventRecord(h->ev[2], s));
 
@@ -365,7 +387,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (h->scan_grid[shape] == 0) h->scan_grid[shape] = hv::scan_grid_size(h->device, shape, (int)d);
         const int vt = hv::scan_vector_tile(shape);
         const int vsplit = (kVecSplit + vt - 1) / vt * vt;
-        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmax,
+        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, 0, vmax,
                               hv::scan_query_tile(shape), vsplit, h->d_rank_order.as<int>(),
                               h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         hv::launch_list_scan(shape, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
@@ -375,7 +397,38 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              s);
         HV_CHECK_LAUNCH("list scan");
     };
-    // 4. bound: the reference's warm-up bound (heapTops), or the caller's
+    // BF16 precision form: one phase of the MFMA filter scan over probe ranks [r0, r1)
+    const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
+    float* aux = nullptr;
+    float kappa1 = 0.f, kappa2 = 0.f;
+    if (bf16) {
+        HV_HIP(h->s_aux.ensure(sizeof(float) * n * cap));
+        aux = h->s_aux.as<float>();
+        // error bound of the BF16 estimate (see hv_bf16.hip): BF16 unit roundoff 2^-8 on both
+        // residuals, FP32 accumulation over d terms, FP32 norms, and the FP32 chain of the exact path
+        const double ubf = std::ldexp(1.0, -8), u = std::ldexp(1.0, -24);
+        const double gd = d * u / (1.0 - d * u);
+        kappa1 = (float)(1.01 * 2.0 * (2.0 * (ubf + u) + (ubf + u) * (ubf + u) + gd * (1.0 + ubf) * (1.0 + ubf)));
+        kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
+        if (h->bf16_grid == 0) h->bf16_grid = hv::bf16_grid_size(h->device);
+        HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
+        hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, n, (int)nprobe, (int)d, h->dpad,
+                              h->s_qres.as<__bf16>(), s);
+        HV_CHECK_LAUNCH("pair residuals");
+    }
+    auto bf16_phase = [&](int r0, int r1, int vmin, int vmax, int mode, unsigned long long* st_slot) {
+        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, kVecSplit,
+                              h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
+                              h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
+        hv::launch_bf16_scan(h->bf16_grid, h->s_qres.as<__bf16>(), (int)d, pdist, (int)nprobe,
+                             h->d_rcodes.as<__bf16>(), h->d_roff.as<int64_t>(), h->d_rnorm.as<float>(),
+                             h->d_list_off.as<int64_t>(), h->s_bound.as<float>(), h->s_pair_slot.as<int>(),
+                             h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplit, mode, kappa1, kappa2, surv, aux,
+                             surv_cnt, (int)cap, st_slot, s);
+        HV_CHECK_LAUNCH("bf16 scan");
+    };
+    // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
+    //    warm-up scan: it scans each query's nearest list unpruned and takes the bound from there.
     HV_HIP(h->s_bound.ensure(sizeof(float) * n));
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     float* bound = h->s_bound.as<float>();
@@ -383,7 +436,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     if (h->sp.bound_mode == HV_BOUND_EXTERNAL) {
         HV_HIP(hipMemcpyAsync(bound_ref, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    } else if (wl > 0) {
+    } else if (wl > 0 && !bf16) {
         scan_phase(0, (int)wl, (int)wls, 0, 1, stats + 4);
         hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, s);
     } else {
@@ -398,14 +451,33 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
     //    the remaining probed lists
     const int prune = B > 1 ? 1 : 0;
-    int r_next = 0;
-    if (h->sp.refine_bound && nprobe > 1) {
-        scan_phase(0, 1, INT32_MAX, prune, 0, stats);
-        hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
+    if (bf16) {
+        // nearest list: vectors below cap go to their own slot unfiltered; the k-th smallest upper
+        // bound then lowers the bound and the pairs whose lower bound exceeds it are dropped
+        // (vector ranges start on 16-vector blocks)
+        const int vdirect = (int)(cap / 16 * 16);
+        if (vdirect > 0) bf16_phase(0, 1, 0, vdirect, 1, stats);
+        hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
+        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
         HV_CHECK_LAUNCH("compact");
-        r_next = 1;
+        HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        // the rest of the nearest lists (longer than the direct part), then the other probed lists
+        if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
+        if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats);
+        // exact FP32 distances of the survivors (the plain path's arithmetic)
+        hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
+                          (int)cap, s);
+        HV_CHECK_LAUNCH("rerank");
+    } else {
+        int r_next = 0;
+        if (h->sp.refine_bound && nprobe > 1) {
+            scan_phase(0, 1, INT32_MAX, prune, 0, stats);
+            hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
+            HV_CHECK_LAUNCH("compact");
+            r_next = 1;
+        }
+        scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, stats);
     }
-    scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, stats);
     HV_HIP(hipEventRecord(h->ev[4], s));
     // 6. top-k selection (+ exact path for overflowed buffers)
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 1)));
@@ -414,14 +486,21 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     hv::launch_fill_i32(ovf_cnt, 1, 0, s);
     hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, ovf_list, ovf_cnt, stats, s);
     HV_CHECK_LAUNCH("select");
-    hv::launch_query_exact(ovf_list, ovf_cnt, (int)std::min<int64_t>(n, 256), Qp, (int)d, h->d_codes.as<float>(),
-                           h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(),
-                           h->d_ids.as<uint32_t>(), probes, (int)nprobe, plan, prune, bound, (int)k, D, I, s);
-    HV_CHECK_LAUNCH("exact fallback");
-    HV_HIP(hipEventRecord(h->ev[5], s));
+    // the overflowed queries (rare) take the exact path, spread over (query, probed list) pairs
     int novf = 0;
-    unsigned long long hs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HV_HIP(hipMemcpyAsync(&novf, ovf_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+    HV_HIP(hipStreamSynchronize(s));
+    if (novf > 0) {
+        const int nseg = hv::exact_segments(h->max_list);
+        HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * novf * nprobe * nseg * k));
+        hv::launch_query_exact(ovf_list, novf, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                               h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), probes,
+                               (int)nprobe, nseg, plan, prune, bound, (int)k, h->s_runs.as<unsigned long long>(), D, I,
+                               s);
+        HV_CHECK_LAUNCH("exact fallback");
+    }
+    HV_HIP(hipEventRecord(h->ev[5], s));
+    unsigned long long hs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
     h->last_n = n;
@@ -493,7 +572,8 @@ void hv_destroy(hv_index_t* h) {
     DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
-                      &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats};
+                      &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats,
+                      &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres, &h->s_runs, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -820,7 +900,8 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
         require(p->bound_mode == HV_BOUND_WARMUP || p->bound_mode == HV_BOUND_EXTERNAL, HV_ERR_INVALID_ARG,
                 "invalid bound_mode");
         require(p->warmup_list >= 0 && p->warmup_list_size >= 0, HV_ERR_INVALID_ARG, "negative warm-up size");
-        require(p->precision == HV_PRECISION_FP32, HV_ERR_INVALID_ARG, "unsupported precision");
+        require(p->precision == HV_PRECISION_FP32 || p->precision == HV_PRECISION_BF16, HV_ERR_INVALID_ARG,
+                "unsupported precision");
         require(p->survivor_cap >= 0 && p->survivor_cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap > 8192");
         require(p->tile_shape >= 0 && p->tile_shape <= hv::kNumShapes && p->tile_shape_first >= 0 &&
                     p->tile_shape_first <= hv::kNumShapes,

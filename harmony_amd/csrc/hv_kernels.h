@@ -51,10 +51,11 @@ void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int* bucket_c
 int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s);
 void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* bucket_off, int* fill,
-                          int* pairs, hipStream_t s);
+                          int* pairs, int* pair_slot, hipStream_t s);
 // queue: 3 x 8 ints (per-XCD length, head, base) + nlist ints (first item of each list rank)
-void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmax, int qtile, int vsplit,
-                       const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
+// vector range [vmin, vmax) of each list (vmin a multiple of the scan's vector tile)
+void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
+                       int vsplit, const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
                        int* queue, unsigned long long* stats, hipStream_t s);
 int scan_query_tile(int shape);   // queries per work item of a tile shape
 int scan_vector_tile(int shape);  // vectors per tile of a tile shape
@@ -68,10 +69,37 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap,
                              int k, float* D, int64_t* I, int* overflow_list, int* overflow_cnt,
                              unsigned long long* stats, hipStream_t s);
-void launch_query_exact(const int* qlist, const int* qcount, int grid, const float* Qp, int d, const float* codes,
-                        const int64_t* code_off, const int64_t* list_off, const int* list_stride,
-                        const uint32_t* ids, const int* probes, int nprobe, const ChunkPlan& plan, int prune,
-                        const float* bound, int k, float* D, int64_t* I, hipStream_t s);
+// exact path for the nql overflowed queries of qlist, lists cut into exact_segments(max list
+// length) segments; runs >= nql * nprobe * nseg * k keys of scratch
+int exact_segments(int64_t max_list);
+void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
+                        const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
+                        int nprobe, int nseg, const ChunkPlan& plan, int prune, const float* bound, int k,
+                        unsigned long long* runs, float* D, int64_t* I, hipStream_t s);
+
+// --- BF16 precision form (hv_bf16.hip)
+int bf16_ksteps(int d);  // 32-dim MFMA steps per vector (dpad = 32 * ksteps)
+int bf16_grid_size(int device);
+void launch_build_resid(const float* x, int d, int dpad, const int* order, const int64_t* list_off,
+                        const float* centroids, const int64_t* roff, int nlist, __bf16* rcodes, float* rnorm,
+                        hipStream_t s);
+// rows in list order: by_pos[p] = rows[order[p]]
+void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, float* by_pos, hipStream_t s);
+// BF16 (q - c) rows of dpad dims for every (query, probe rank) slot
+void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, int64_t nq, int nprobe, int d,
+                       int dpad, __bf16* out, hipStream_t s);
+void launch_bf16_scan(int grid, const __bf16* qres, int d, const float* pdist, int nprobe,
+                      const __bf16* rcodes, const int64_t* roff, const float* rnorm, const int64_t* list_off,
+                      const float* bound, const int* pair_slot, const ScanItem* items, const int* queue, int vmax,
+                      int vsplit, int mode, float kappa1, float kappa2, unsigned long long* surv, float* aux,
+                      unsigned int* surv_cnt, int cap, unsigned long long* stats, hipStream_t s);
+// counts after the direct-slot scan of the nearest lists (vectors v < cap)
+void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
+                         unsigned int* cnt, hipStream_t s);
+void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
+                         float* bound, hipStream_t s);
+void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
+                   unsigned long long* surv, const unsigned int* cnt, int64_t nq, int cap, hipStream_t s);
 
 // --- FP64 ground truth
 void launch_gt_f64(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,

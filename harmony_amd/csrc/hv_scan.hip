@@ -10,6 +10,8 @@
 // distance <= bound, so the scan keeps exactly the non-pruned candidates with distance
 // <= bound and the selection sorts them by (distance, id). The result equals the k smallest
 // (distance, id) over all candidates, with pruning on or off.
+#include <algorithm>
+
 #include "hv_device.h"
 #include "hv_kernels.h"
 
@@ -21,10 +23,17 @@ __device__ __forceinline__ float blocked_l2_col(const float* qs, const float* __
                                                 const ChunkPlan& plan, bool prune, float bound) {
     float dis = 0.f, p = 0.f;
     for (int c = 0; c < plan.n; ++c) {
-        const int lo = plan.lo[c], hi = lo + plan.len[c];
-        for (int i = lo; i < hi; ++i) {
-            const float t2 = qs[i] - X[(int64_t)i * stride + t];
-            p = __builtin_fmaf(t2, t2, p);
+        const int lo = plan.lo[c], len = plan.len[c];
+        // the chunk's loads first (one memory latency per chunk), then its FMA chain in order
+        float xv[kChunkDims];
+#pragma unroll
+        for (int u = 0; u < kChunkDims; ++u) xv[u] = X[(int64_t)(lo + min(u, len - 1)) * stride + t];
+#pragma unroll
+        for (int u = 0; u < kChunkDims; ++u) {
+            if (u < len) {
+                const float t2 = qs[lo + u] - xv[u];
+                p = __builtin_fmaf(t2, t2, p);
+            }
         }
         if (plan.ends_block[c]) {
             dis += p;
@@ -53,22 +62,26 @@ void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int* bucket_c
                        bucket_cnt);
 }
 
+// pairs[pos] = query, pair_slot[pos] = query * nprobe + rank (the probe slot, for the coarse
+// distance of the pair)
 __global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, int nprobe,
-                                const int* __restrict__ off, int* __restrict__ fill, int* __restrict__ pairs) {
+                                const int* __restrict__ off, int* __restrict__ fill, int* __restrict__ pairs,
+                                int* __restrict__ pair_slot) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
     const int j = (int)(e % nprobe);
     const int64_t b = (int64_t)probes[e] * nprobe + j;
     const int pos = off[b] + atomicAdd(&fill[b], 1);
     pairs[pos] = (int)(e / nprobe);
+    pair_slot[pos] = (int)e;
 }
 
 void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* bucket_off, int* fill, int* pairs,
-                          hipStream_t s) {
+                          int* pair_slot, hipStream_t s) {
     const int64_t tot = nq * nprobe;
     if (tot <= 0) return;
     hipLaunchKernelGGL(k_scatter_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe,
-                       bucket_off, fill, pairs);
+                       bucket_off, fill, pairs, pair_slot);
 }
 
 // Exclusive scan of n ints (three passes: tile scan, scan of tile sums, add).
@@ -144,8 +157,8 @@ void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStre
 }
 
 // ---------------------------------------------------------------------------------------
-// Work plan: a list with g probing queries (in probe ranks [r0, r1)) and n vectors (at most
-// vmax) becomes ceil(g / qtile) x ceil(n / vsplit) items (list, query tile, vector range), so
+// Work plan: a list with g probing queries (in probe ranks [r0, r1)) and n vectors (those of
+// [vmin, vmax)) becomes ceil(g / qtile) x ceil(n / vsplit) items (list, query tile, vector range), so
 // no item is longer than vsplit vectors. Lists, taken in decreasing size order, are dealt
 // round-robin to 8 per-XCD queues, so the tiles of one list run on one XCD and share its L2,
 // while every XCD gets a similar mix of list sizes. The queues lie back to back in `items`.
@@ -156,7 +169,7 @@ struct PlanArgs {
     const int* bucket_off;
     const int* rank_order;
     const int64_t* list_off;
-    int nlist, nprobe, r0, r1, vmax, qtile, vsplit;
+    int nlist, nprobe, r0, r1, vmin, vmax, qtile, vsplit;
 };
 
 __device__ __forceinline__ void plan_list(const PlanArgs& a, int r, int& l, int& gb, int& ge, int& nlv, int& nqt,
@@ -164,7 +177,7 @@ __device__ __forceinline__ void plan_list(const PlanArgs& a, int r, int& l, int&
     l = a.rank_order[r];
     gb = a.bucket_off[(int64_t)l * a.nprobe + a.r0];
     ge = a.bucket_off[(int64_t)l * a.nprobe + a.r1];
-    nlv = (int)min(a.list_off[l + 1] - a.list_off[l], (long long)a.vmax);
+    nlv = max(0, (int)min(a.list_off[l + 1] - a.list_off[l], (long long)a.vmax) - a.vmin);
     nqt = (ge - gb + a.qtile - 1) / a.qtile;
     nvr = (nlv + a.vsplit - 1) / a.vsplit;
 }
@@ -232,15 +245,15 @@ __global__ __launch_bounds__(64) void k_plan_write(PlanArgs a, const int* __rest
         it.list = l;
         it.pair_begin = gb + c * a.qtile;
         it.pair_end = min(ge, it.pair_begin + a.qtile);
-        it.vec_begin = v * a.vsplit;
+        it.vec_begin = a.vmin + v * a.vsplit;
         if (pos0 + j < qcap) items[pos0 + j] = it;
     }
 }
 
-void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmax, int qtile, int vsplit,
-                       const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
+void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
+                       int vsplit, const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
                        int* queue, unsigned long long* stats, hipStream_t s) {
-    PlanArgs a{bucket_off, list_rank_order, list_off, nlist, nprobe, r0, r1, vmax, qtile, vsplit};
+    PlanArgs a{bucket_off, list_rank_order, list_off, nlist, nprobe, r0, r1, vmin, vmax, qtile, vsplit};
     hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1024), 0, s, a, queue, stats);
     hipLaunchKernelGGL(k_plan_write, dim3((unsigned)nlist), dim3(64), 0, s, a, queue, qcap, items);
 }
@@ -618,86 +631,229 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 }
 
 // ---------------------------------------------------------------------------------------
-// Exact per-query path for queries whose candidate buffer overflowed (the list and its length
-// are on the device; workgroups stride over it, so the host never waits for the count): one
-// workgroup walks the query's probed lists, one candidate per thread, keeping a sorted top-k in LDS and merging a
-// 1024-entry buffer of candidates that beat the current k-th into it by bitonic sort. Same
-// arithmetic, pruning and (distance <= bound) filter as the scan, so the result is the same.
-__global__ __launch_bounds__(256) void k_query_exact(const int* __restrict__ qlist, const int* __restrict__ qcount,
-                                                     const float* __restrict__ Qp,
-                                                     int d, const float* __restrict__ codes,
+// Exact path for queries whose candidate buffer overflowed. k_exact_lists: one workgroup per
+// (overflowed query, probed list, segment of kExactSpan vectors), one candidate per thread;
+// candidates that pass the bound and
+// beat the current k-th key are collected in LDS (up to kExactSeg per segment of the list), the
+// k-th smallest key of the collected set is found by a block-wide radix select (8 passes of 8
+// bits over the 64-bit keys, which are unique), and only the keys up to it are kept. It writes
+// its k best keys (sorted, padded). k_exact_merge: one workgroup per query reduces the
+// nprobe * nseg runs to the final top-k the same way. Same arithmetic, pruning and
+// (distance <= bound) filter as the scan, so the result is the same.
+constexpr int kExactTop = 1024, kExactSeg = 4096, kExactSpan = 1024;
+
+// k-th smallest (1-based) of the na keys of a and the nb keys of b (block-wide; every thread
+// gets it)
+__device__ unsigned long long block_radix_kth(const unsigned long long* a, int na, const unsigned long long* b, int nb,
+                                              int k, unsigned int* hist, unsigned long long* shared_out) {
+    unsigned long long prefix = 0ull, mask = 0ull;
+    int kk = k;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0u;
+        __syncthreads();
+        for (int i = threadIdx.x; i < na + nb; i += blockDim.x) {
+            const unsigned long long key = i < na ? a[i] : b[i - na];
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // wavefront 0: 4 bins per lane, inclusive scan, find the bin holding rank kk
+            const int lane = threadIdx.x;
+            unsigned int h[4], sum = 0u;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                h[t] = hist[lane * 4 + t];
+                sum += h[t];
+            }
+            unsigned int incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const unsigned int excl = incl - sum;
+            if (excl < (unsigned int)kk && (unsigned int)kk <= incl) {
+                unsigned int run = excl;
+                int digit = lane * 4 + 3;
+                unsigned int below = run + h[0] + h[1] + h[2];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (run + h[t] >= (unsigned int)kk) {
+                        digit = lane * 4 + t;
+                        below = run;
+                        break;
+                    }
+                    run += h[t];
+                }
+                shared_out[0] = ((unsigned long long)digit << 32) | below;
+            }
+        }
+        __syncthreads();
+        const unsigned long long r = shared_out[0];
+        kk -= (int)(r & 0xffffffffull);
+        prefix |= (r >> 32) << shift;
+        mask |= 255ull << shift;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// Block-wide reduction of the kept keys s[0, s_top) and the collected keys s_col[0, s_cnt) to the
+// k smallest (all of them when there are at most k): s_top, s_kth (the k-th key once there are
+// k) and s_cnt = 0 on return.
+__device__ void topk_reduce(unsigned long long* s, const unsigned long long* s_col, unsigned long long* s_out, int k,
+                            int& s_top, int& s_cnt, unsigned long long& s_kth, unsigned int* s_hist,
+                            unsigned long long* s_sel) {
+    const int top = s_top, m = s_cnt;
+    if (top + m > k) {
+        // the k-th smallest of the kept and the collected keys; keep the k keys <= it
+        const unsigned long long kv = block_radix_kth(s, top, s_col, m, k, s_hist, s_sel);
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < top + m; i += blockDim.x) {
+            const unsigned long long key = i < top ? s[i] : s_col[i - top];
+            if (key <= kv) s_out[atomicAdd(&s_cnt, 1)] = key;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += blockDim.x) s[i] = s_out[i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_top = k;
+            s_kth = kv;
+            s_cnt = 0;
+        }
+    } else {
+        for (int i = threadIdx.x; i < m; i += blockDim.x) s[top + i] = s_col[i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_top = top + m;
+            s_cnt = 0;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qlist, int nql,
+                                                     const float* __restrict__ Qp, int d,
+                                                     const float* __restrict__ codes,
                                                      const int64_t* __restrict__ code_off,
                                                      const int64_t* __restrict__ list_off,
                                                      const int* __restrict__ list_stride,
                                                      const uint32_t* __restrict__ ids, const int* __restrict__ probes,
-                                                     int nprobe, ChunkPlan plan, int prune,
-                                                     const float* __restrict__ bound, int k, float* __restrict__ D,
-                                                     int64_t* __restrict__ I) {
-    constexpr int TOP = 1024, BUF = 1024;
+                                                     int nprobe, int nseg, ChunkPlan plan, int prune,
+                                                     const float* __restrict__ bound, int k,
+                                                     unsigned long long* __restrict__ runs) {
     extern __shared__ __align__(16) unsigned char s_raw[];
-    unsigned long long* s = reinterpret_cast<unsigned long long*>(s_raw);  // [TOP + BUF]
-    float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * (TOP + BUF));
-    __shared__ int s_cnt;
+    // kept keys [kExactTop] | keys collected this segment [kExactSeg] | reduction output [kExactTop]
+    unsigned long long* s = reinterpret_cast<unsigned long long*>(s_raw);
+    unsigned long long* s_col = s + kExactTop;
+    unsigned long long* s_out = s_col + kExactSeg;
+    float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * (2 * kExactTop + kExactSeg));
+    __shared__ unsigned int s_hist[256];
+    __shared__ unsigned long long s_sel;
+    __shared__ int s_cnt, s_top;
     __shared__ unsigned long long s_kth;
-    const int nql = *qcount;
-  for (int w = blockIdx.x; w < nql; w += gridDim.x) {
-    const int64_t q = qlist[w];
-    const float b = bound[q];
-    __syncthreads();
-    for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Qp[q * d + i];
-    for (int i = threadIdx.x; i < TOP + BUF; i += blockDim.x) s[i] = HV_KEY_PAD;
-    if (threadIdx.x == 0) {
-        s_cnt = 0;
-        s_kth = HV_KEY_PAD;
-    }
-    __syncthreads();
-    for (int j = 0; j < nprobe; ++j) {
+    const int64_t per_q = (int64_t)nprobe * nseg;
+    for (int64_t w = blockIdx.x; w < (int64_t)nql * per_q; w += gridDim.x) {
+        const int64_t q = qlist[w / per_q];
+        const int j = (int)(w % per_q) / nseg, seg = (int)(w % nseg);
+        const float b = bound[q];
+        __syncthreads();
+        for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Qp[q * d + i];
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_top = 0;
+            s_kth = HV_KEY_PAD;
+        }
+        __syncthreads();
         const int l = probes[q * nprobe + j];
         const int64_t lb = list_off[l];
-        const int n = (int)(list_off[l + 1] - lb);
+        const int n = (int)min(list_off[l + 1] - lb, (long long)(seg + 1) * kExactSpan);
         const float* X = codes + code_off[l];
         const int ps = list_stride[l];
-        for (int t0 = 0; t0 < n; t0 += blockDim.x) {
+        for (int t0 = seg * kExactSpan; t0 < n; t0 += blockDim.x) {
             const int t = t0 + threadIdx.x;
+            const unsigned long long kth = s_kth;
             if (t < n) {
                 const float dis = blocked_l2_col(qs, X, ps, t, plan, prune != 0, b);
                 if (dis <= b) {
                     const unsigned long long key = hv_key(dis, ids[lb + t]);
-                    if (key < s_kth) {
-                        const int pos = atomicAdd(&s_cnt, 1);
-                        s[TOP + pos] = key;
-                    }
+                    if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
                 }
             }
             __syncthreads();
-            if (s_cnt > BUF - (int)blockDim.x) {
-                block_bitonic_sort(s, TOP + BUF);
-                for (int i = k + threadIdx.x; i < TOP + BUF; i += blockDim.x) s[i] = HV_KEY_PAD;
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    s_cnt = 0;
-                    s_kth = s[k - 1];
-                }
-                __syncthreads();
-            }
+            // reduce when the next round might not fit, and at the end
+            if (s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= n)
+                topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
         }
+        // sort the kept keys (padded to a power of two) and write the run
+        const int top = s_top;
+        const int p2 = hv_next_pow2(max(k, 2));
+        for (int i = top + threadIdx.x; i < p2; i += blockDim.x) s[i] = HV_KEY_PAD;
+        __syncthreads();
+        block_bitonic_sort(s, p2);
+        for (int i = threadIdx.x; i < k; i += blockDim.x) runs[w * k + i] = s[i];
     }
-    block_bitonic_sort(s, TOP + BUF);
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
-        D[q * k + i] = hv_key_dist(s[i]);
-        I[q * k + i] = hv_key_id(s[i]);
-    }
-  }
 }
 
-void launch_query_exact(const int* qlist, const int* qcount, int grid, const float* Qp, int d, const float* codes,
-                        const int64_t* code_off, const int64_t* list_off, const int* list_stride,
-                        const uint32_t* ids, const int* probes, int nprobe, const ChunkPlan& plan, int prune,
-                        const float* bound, int k, float* D, int64_t* I, hipStream_t s) {
-    if (grid <= 0) return;
-    const size_t lds = sizeof(unsigned long long) * 2048 + sizeof(float) * ((d + 3) & ~3);
-    hipLaunchKernelGGL(k_query_exact, dim3((unsigned)grid), dim3(256), lds, s, qlist, qcount, Qp, d, codes, code_off,
-                       list_off, list_stride, ids, probes, nprobe, plan, prune, bound, k, D, I);
+__global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qlist, int nql, int nruns, int k,
+                                                     const unsigned long long* __restrict__ runs,
+                                                     float* __restrict__ D, int64_t* __restrict__ I) {
+    __shared__ unsigned long long s[2 * kExactTop + kExactSeg];
+    unsigned long long* s_col = s + kExactTop;
+    unsigned long long* s_out = s_col + kExactSeg;
+    __shared__ unsigned int s_hist[256];
+    __shared__ unsigned long long s_sel;
+    __shared__ int s_cnt, s_top;
+    __shared__ unsigned long long s_kth;
+    for (int w = blockIdx.x; w < nql; w += gridDim.x) {
+        const int64_t q = qlist[w];
+        const unsigned long long* __restrict__ src = runs + (int64_t)w * nruns * k;
+        const int total = nruns * k;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_top = 0;
+            s_kth = HV_KEY_PAD;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < total; t0 += blockDim.x) {
+            const int t = t0 + threadIdx.x;
+            const unsigned long long kth = s_kth;
+            if (t < total) {
+                const unsigned long long key = src[t];
+                if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
+            }
+            __syncthreads();
+            if (s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= total)
+                topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
+        }
+        const int top = s_top;
+        const int p2 = hv_next_pow2(max(k, 2));
+        for (int i = top + threadIdx.x; i < p2; i += blockDim.x) s[i] = HV_KEY_PAD;
+        __syncthreads();
+        block_bitonic_sort(s, p2);
+        for (int i = threadIdx.x; i < k; i += blockDim.x) {
+            D[q * k + i] = hv_key_dist(s[i]);
+            I[q * k + i] = hv_key_id(s[i]);
+        }
+    }
+}
+
+int exact_segments(int64_t max_list) { return (int)std::max<int64_t>(1, (max_list + kExactSpan - 1) / kExactSpan); }
+
+void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
+                        const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
+                        int nprobe, int nseg, const ChunkPlan& plan, int prune, const float* bound, int k,
+                        unsigned long long* runs, float* D, int64_t* I, hipStream_t s) {
+    if (nql <= 0) return;
+    const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
+    const int64_t pairs = (int64_t)nql * nprobe * nseg;
+    hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, 65536)), dim3(256), lds, s, qlist, nql,
+                       Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nseg, plan, prune, bound, k,
+                       runs);
+    hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql, 65536)), dim3(256), 0, s, qlist, nql, nprobe * nseg,
+                       k, runs, D, I);
 }
 
 }  // namespace hv

@@ -117,9 +117,14 @@ class Index:
         self._push()
 
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
-                  survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0):
-        """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59)."""
+                  survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None):
+        """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59).
+
+        precision: HV_PRECISION_FP32 (default) or HV_PRECISION_BF16 (MFMA filter + FP32 distances
+        of the survivors; same results). None keeps the current setting."""
         p = self._params
+        if precision is not None:
+            p.precision = int(precision)
         p.refine_bound = 1 if refine_bound else 0
         p.tile_shape = int(tile_shape)
         p.tile_shape_first = int(tile_shape_first)

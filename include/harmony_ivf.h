@@ -49,7 +49,9 @@ extern "C" {
 #define HV_BOUND_EXTERNAL 1 /* caller-provided per-query bound (--fullWarmUp, query.cpp:716-725) */
 
 /* precision of the candidate distances */
-#define HV_PRECISION_FP32 0
+#define HV_PRECISION_FP32 0 /* FP32 blocked partial sums on every candidate (reference arithmetic) */
+#define HV_PRECISION_BF16 1 /* BF16 MFMA filter with a certified error bound, then the FP32 distance
+                               of each survivor: same results as HV_PRECISION_FP32 */
 
 typedef struct hv_index hv_index_t;
 
