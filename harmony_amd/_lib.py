@@ -54,7 +54,8 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("ms_total", "ms_upload", "ms_coarse", "ms_warmup", "ms_group",
                                                "ms_scan", "ms_select", "ms_download")] + \
                [(n, ctypes.c_int64) for n in ("candidates", "block_evals", "block_evals_max", "survivors",
-                                              "overflow_queries", "work_items")]
+                                              "overflow_queries", "work_items")] + \
+               [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -352,12 +352,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
-    // stats: [0..3] counters of the scan phases, [4..7] of the warm-up
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 8));
+    // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
+    // (the remaining probed lists)
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 12));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
     hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
     hv::launch_fill_i32(h->s_fill.as<int>(), nb, 0, s);
-    hv::launch_fill_u64(stats, 8, 0ull, s);
+    hv::launch_fill_u64(stats, 12, 0ull, s);
     hv::launch_count_pairs(probes, n, (int)nprobe, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
     hv::launch_scatter_pairs(probes, n, (int)nprobe, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
@@ -382,6 +383,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         const double g = (double)n * (r1 - r0) / (double)nlist;
         return g <= 16 ? 5 : (g <= 40 ? 2 : 0);
     };
+    bool main_recorded = false;  // events 6-7 bracket the main phase's scan kernel
     auto scan_phase = [&](int r0, int r1, int vmax, int prune, int accept_all, unsigned long long* st_slot) {
         const int shape = pick_shape(r0, r1);
         if (h->scan_grid[shape] == 0) h->scan_grid[shape] = hv::scan_grid_size(h->device, shape, (int)d);
@@ -390,12 +392,16 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, 0, vmax,
                               hv::scan_query_tile(shape), vsplit, h->d_rank_order.as<int>(),
                               h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
+        const bool main_phase = st_slot == stats + 8;
+        main_recorded |= main_phase;
+        if (main_phase) HV_HIP(hipEventRecord(h->ev[6], s));
         hv::launch_list_scan(shape, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
                              h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), plan,
                              prune, h->s_bound.as<float>(), h->s_pairs.as<int>(), h->s_items.as<hv::ScanItem>(),
                              queue, vmax, vsplit, accept_all, surv, surv_cnt, (int)cap, st_slot, h->scan_grid[shape],
                              s);
         HV_CHECK_LAUNCH("list scan");
+        if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
     };
     // BF16 precision form: one phase of the MFMA filter scan over probe ranks [r0, r1)
     const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
@@ -420,12 +426,16 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, kVecSplit,
                               h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
                               h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
+        const bool main_phase = st_slot == stats + 8;
+        main_recorded |= main_phase;
+        if (main_phase) HV_HIP(hipEventRecord(h->ev[6], s));
         hv::launch_bf16_scan(h->bf16_grid, h->s_qres.as<__bf16>(), (int)d, pdist, (int)nprobe,
                              h->d_rcodes.as<__bf16>(), h->d_roff.as<int64_t>(), h->d_rnorm.as<float>(),
                              h->d_list_off.as<int64_t>(), h->s_bound.as<float>(), h->s_pair_slot.as<int>(),
                              h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplit, mode, kappa1, kappa2, surv, aux,
                              surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("bf16 scan");
+        if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
     };
     // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
     //    warm-up scan: it scans each query's nearest list unpruned and takes the bound from there.
@@ -463,7 +473,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
-        if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats);
+        if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
         // exact FP32 distances of the survivors (the plain path's arithmetic)
         hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
                           (int)cap, s);
@@ -476,7 +486,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             HV_CHECK_LAUNCH("compact");
             r_next = 1;
         }
-        scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, stats);
+        scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
     }
     HV_HIP(hipEventRecord(h->ev[4], s));
     // 6. top-k selection (+ exact path for overflowed buffers)
@@ -500,7 +510,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("exact fallback");
     }
     HV_HIP(hipEventRecord(h->ev[5], s));
-    unsigned long long hs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long hs[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
     h->last_n = n;
@@ -513,11 +523,18 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         st->ms_warmup += ms[2];
         st->ms_scan += ms[3];
         st->ms_select += ms[4];
-        st->candidates += (int64_t)hs[0];
-        st->block_evals += (int64_t)hs[1];
-        st->block_evals_max += (int64_t)hs[0] * B;
+        const int64_t cand = (int64_t)(hs[0] + hs[8]);
+        st->candidates += cand;
+        st->block_evals += (int64_t)(hs[1] + hs[9]);
+        st->block_evals_max += cand * B;
         st->survivors += (int64_t)hs[2];
         st->overflow_queries += novf;
+        if (main_recorded) {
+            float mm = 0.f;
+            HV_HIP(hipEventElapsedTime(&mm, h->ev[6], h->ev[7]));
+            st->ms_main += mm;
+            st->main_pairs += (int64_t)hs[8];
+        }
         st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 }

@@ -98,6 +98,8 @@ typedef struct {
     int64_t survivors;       /* candidates that reached the top-k selection */
     int64_t overflow_queries;/* queries whose survivors exceeded the buffer (exact fallback) */
     int64_t work_items;      /* scan work items (list x query tile) */
+    double ms_main;          /* the dominant kernel: the scan of the remaining probed lists */
+    int64_t main_pairs;      /* (query, vector) pairs of that kernel (unpruned) */
 } hv_stats;
 
 /* ---- lifetime (Index::Index, Index.h:15-25) ---- */
