@@ -6,4 +6,4 @@ reference's harmony::Index class over the C-ABI.
 """
 from ._lib import HV_PRECISION_BF16, HV_PRECISION_FP32, HipExtensionMissing, HvError, lib  # noqa: F401
 from .index import DeviceArray, Index  # noqa: F401
-from . import datagen, metrics  # noqa: F401
+from . import datagen, learned, metrics  # noqa: F401

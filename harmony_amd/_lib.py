@@ -25,6 +25,10 @@ HV_BOUND_WARMUP = 0
 HV_BOUND_EXTERNAL = 1
 HV_PRECISION_FP32 = 0
 HV_PRECISION_BF16 = 1
+HV_MLP_IN = 65
+HV_MLP_HIDDEN = 128
+HV_MLP_OUT = 2
+HV_MLP_PARAMS = HV_MLP_IN * HV_MLP_HIDDEN + HV_MLP_HIDDEN + HV_MLP_HIDDEN * HV_MLP_OUT + HV_MLP_OUT
 
 
 class HipExtensionMissing(RuntimeError):
@@ -47,7 +51,9 @@ class SearchParams(ctypes.Structure):
                 ("pruning", ctypes.c_int32), ("bound_mode", ctypes.c_int32), ("warmup_list", ctypes.c_int64),
                 ("warmup_list_size", ctypes.c_int64), ("precision", ctypes.c_int32),
                 ("refine_bound", ctypes.c_int32), ("survivor_cap", ctypes.c_int64), ("tile_shape", ctypes.c_int32),
-                ("tile_shape_first", ctypes.c_int32)]
+                ("tile_shape_first", ctypes.c_int32), ("learned", ctypes.c_int32),
+                ("learned_nprobe_scale", ctypes.c_float), ("learned_bound_scale", ctypes.c_float),
+                ("learned_min_nprobe", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -55,7 +61,7 @@ class Stats(ctypes.Structure):
                                                "ms_scan", "ms_select", "ms_download")] + \
                [(n, ctypes.c_int64) for n in ("candidates", "block_evals", "block_evals_max", "survivors",
                                               "overflow_queries", "work_items")] + \
-               [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64)]
+               [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64), ("fallback_queries", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -93,6 +99,17 @@ SIGNATURES = {
     "hv_get_list_codes": (ctypes.c_int, [_P, _P]),
     "hv_get_last_probes": (ctypes.c_int, [_P, _P]),
     "hv_get_last_bounds": (ctypes.c_int, [_P, _P]),
+    "hv_get_last_nprobe": (ctypes.c_int, [_P, _P]),
+    "hv_last_probe_stride": (_I64, [_P]),
+    "hv_query_features": (ctypes.c_int, [_P, _I64, _P, _P]),
+    "hv_mlp_set_norm": (ctypes.c_int, [_P, _P]),
+    "hv_mlp_get_norm": (ctypes.c_int, [_P, _P]),
+    "hv_mlp_init": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "hv_mlp_set_params": (ctypes.c_int, [_P, _P]),
+    "hv_mlp_get_params": (ctypes.c_int, [_P, _P]),
+    "hv_mlp_grad": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
+    "hv_mlp_train": (ctypes.c_int, [_P, _I64, _P, _P, _I32, _I32, ctypes.c_float, ctypes.c_uint64, _P]),
+    "hv_mlp_predict": (ctypes.c_int, [_P, _I64, _P, _P]),
     "hv_device_count": (ctypes.c_int, []),
     "hv_device_malloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), _I64]),
     "hv_device_free": (ctypes.c_int, [_P]),

@@ -90,12 +90,14 @@ void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, f
 // Query residuals of every (query, probe rank) slot: BF16 (q - c) rows of dpad dims (zero past
 // d), one thread per 8 dims.
 __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp, const float* __restrict__ centroids,
-                                                    const int* __restrict__ probes, int64_t nslots, int nprobe, int d,
-                                                    int dpad, bf16x8* __restrict__ out) {
+                                                    const int* __restrict__ probes, const int* __restrict__ qn,
+                                                    int64_t nslots, int nprobe, int d, int dpad,
+                                                    bf16x8* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int g8 = dpad / 8;
     if (t >= nslots * g8) return;
     const int64_t slot = t / g8;
+    if (qn && (int)(slot % nprobe) >= qn[slot / nprobe]) return;  // rank not searched
     const int k0 = (int)(t - slot * g8) * 8;
     const float* __restrict__ qv = Qp + (slot / nprobe) * d;
     const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
@@ -108,11 +110,11 @@ __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp
     out[t] = r;
 }
 
-void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, int64_t nq, int nprobe, int d,
-                       int dpad, __bf16* out, hipStream_t s) {
+void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
+                       int nprobe, int d, int dpad, __bf16* out, hipStream_t s) {
     const int64_t total = nq * nprobe * (dpad / 8);
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_pair_resid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Qp, centroids, probes,
+    hipLaunchKernelGGL(k_pair_resid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Qp, centroids, probes, qn,
                        nq * nprobe, nprobe, d, dpad, reinterpret_cast<bf16x8*>(out));
 }
 

@@ -167,10 +167,17 @@ struct hv_index {
     hv_search_params sp{};
     std::vector<float> ext_bounds;
 
+    // learned probe/bound predictor: FP32 parameters and Adam moments, BF16 serving weights,
+    // feature/target standardization
+    bool mlp_params = false, mlp_norm = false;
+    DevBuf m_params, m_grad, m_adam_m, m_adam_v, m_w1b, m_w2b, m_norm;
+
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
-        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats, s_pdist, s_pair_slot, s_aux, s_qres, s_runs;
-    int64_t last_n = 0, last_nprobe = 0;
+        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats, s_pdist, s_pair_slot, s_aux, s_qres, s_runs,
+        s_qn, s_feat, s_pred, s_seed, s_fb, s_fbq, s_fbD, s_fbI, s_save;
+    int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
+    bool last_learned = false;
 };
 
 namespace {
@@ -283,6 +290,22 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
     h->added = true;
 }
 
+// Coarse quantizer: full query x centroid distance rows (in slices of <= 256 MB) and the P
+// nearest lists of each query, ascending (Index.cpp:967-997).
+void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist) {
+    const int64_t d = h->d, nlist = h->nlist;
+    hipStream_t s = h->stream;
+    const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (nlist * 4)));
+    HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
+    for (int64_t r0 = 0; r0 < n; r0 += per) {
+        const int64_t rows = std::min(per, n - r0);
+        hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d, h->s_dist.as<float>(),
+                                nlist, s);
+        hv::launch_select_rows(h->s_dist.as<float>(), rows, (int)nlist, (int)P, probes + r0 * P, pdist + r0 * P, s);
+    }
+    HV_CHECK_LAUNCH("coarse");
+}
+
 void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D, int64_t* I, hv_stats* st) {
     require(h->trained && h->added, HV_ERR_STATE, "search before train/add");
     require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
@@ -298,11 +321,20 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     require(h->sp.dim_blocks >= 1 && h->sp.dim_blocks <= d, HV_ERR_INVALID_ARG, "dim_blocks must be in [1, d]");
     require(nlist <= hv::kMaxSelectRow, HV_ERR_CAPACITY, "nlist > 8192");
     const hv::ChunkPlan plan = make_plan((int)d, B);
-    int64_t wl = std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
+    // learned predictor: per-query probe counts (at most nprobe) and seeded bounds; the probe
+    // lists are then selected at least 64 deep (its features are the 64 nearest distances)
+    const bool learned = h->sp.learned != 0;
+    if (learned)
+        require(h->mlp_params && h->mlp_norm, HV_ERR_STATE,
+                "learned search needs a predictor (hv_mlp_init/train or set_params, and hv_mlp_set_norm)");
+    const int64_t nprobe_cap = nprobe;
+    if (learned) nprobe = std::min<int64_t>(nlist, std::max<int64_t>(nprobe, hv::kNearest));
+    int64_t wl = learned ? 0 : std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
     int64_t wls = h->sp.warmup_list_size == 0 ? k : h->sp.warmup_list_size;  // query.cpp:328-333
     if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
         require(wl * wls <= hv::kMaxWarmup, HV_ERR_CAPACITY, "warmup_list * warmup_list_size > 8192");
-    if (h->sp.bound_mode == HV_BOUND_EXTERNAL)
+    const bool external = !learned && h->sp.bound_mode == HV_BOUND_EXTERNAL;
+    if (external)
         require((int64_t)h->ext_bounds.size() == n, HV_ERR_STATE, "external bounds not set for this batch");
     int64_t cap = h->sp.survivor_cap;
     if (cap == 0) cap = std::min<int64_t>(8192, std::max<int64_t>(4096, hv_next_pow2((int)(4 * k))));
@@ -325,17 +357,22 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_pdist.ensure(sizeof(float) * n * nprobe));
     int* probes = h->s_probes.as<int>();
     float* pdist = h->s_pdist.as<float>();
-    {
-        const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (nlist * 4)));
-        HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
-        for (int64_t r0 = 0; r0 < n; r0 += per) {
-            const int64_t rows = std::min(per, n - r0);
-            hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d,
-                                    h->s_dist.as<float>(), nlist, s);
-            hv::launch_select_rows(h->s_dist.as<float>(), rows, (int)nlist, (int)nprobe, probes + r0 * nprobe,
-                                   pdist + r0 * nprobe, s);
-        }
-        HV_CHECK_LAUNCH("coarse");
+    coarse_select(h, n, Qp, nprobe, probes, pdist);
+    const int* qn = nullptr;
+    float* seed = nullptr;
+    if (learned) {
+        HV_HIP(h->s_feat.ensure(sizeof(float) * n * hv::kMlpIn));
+        HV_HIP(h->s_qn.ensure(sizeof(int) * n));
+        HV_HIP(h->s_seed.ensure(sizeof(float) * n));
+        HV_HIP(h->s_pred.ensure(sizeof(float) * n * 2));
+        hv::launch_features(pdist, (int)nprobe, Qp, (int)d, n, h->s_feat.as<float>(), s);
+        hv::launch_mlp_serve(h->s_feat.as<float>(), n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
+                             h->m_params.as<float>(), h->m_norm.as<float>(), h->sp.learned_nprobe_scale,
+                             h->sp.learned_bound_scale, std::max(1, h->sp.learned_min_nprobe), (int)nprobe_cap,
+                             h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(), s);
+        HV_CHECK_LAUNCH("predictor");
+        qn = h->s_qn.as<int>();
+        seed = h->s_seed.as<float>();
     }
     HV_HIP(hipEventRecord(h->ev[1], s));
     // 3. grouping by (list, rank)
@@ -359,16 +396,17 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
     hv::launch_fill_i32(h->s_fill.as<int>(), nb, 0, s);
     hv::launch_fill_u64(stats, 12, 0ull, s);
-    hv::launch_count_pairs(probes, n, (int)nprobe, h->s_bucket.as<int>(), s);
+    hv::launch_count_pairs(probes, n, (int)nprobe, qn, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
-    hv::launch_scatter_pairs(probes, n, (int)nprobe, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
+    hv::launch_scatter_pairs(probes, n, (int)nprobe, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
                              h->s_pairs.as<int>(), h->s_pair_slot.as<int>(), s);
     HV_CHECK_LAUNCH("grouping");
     HV_HIP(hipEventRecord(h->ev[2], s));
 
     int* queue = h->s_queue.as<int>();
     // survivor buffer: room for the warm-up set too
-    if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0) cap = std::max<int64_t>(cap, hv_next_pow2((int)(wl * wls)));
+    if (!learned && h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
+        cap = std::max<int64_t>(cap, hv_next_pow2((int)(wl * wls)));
     HV_HIP(h->s_surv.ensure(sizeof(unsigned long long) * n * cap));
     HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
     unsigned long long* surv = h->s_surv.as<unsigned long long>();
@@ -418,7 +456,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
         if (h->bf16_grid == 0) h->bf16_grid = hv::bf16_grid_size(h->device);
         HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
-        hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, n, (int)nprobe, (int)d, h->dpad,
+        hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)d, h->dpad,
                               h->s_qres.as<__bf16>(), s);
         HV_CHECK_LAUNCH("pair residuals");
     }
@@ -443,7 +481,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     float* bound = h->s_bound.as<float>();
     float* bound_ref = h->s_bound_ref.as<float>();
-    if (h->sp.bound_mode == HV_BOUND_EXTERNAL) {
+    if (learned) {
+        HV_HIP(hipMemcpyAsync(bound_ref, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        HV_HIP(hipMemcpyAsync(bound, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    } else if (external) {
         HV_HIP(hipMemcpyAsync(bound_ref, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     } else if (wl > 0 && !bf16) {
@@ -505,8 +546,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * novf * nprobe * nseg * k));
         hv::launch_query_exact(ovf_list, novf, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
                                h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), probes,
-                               (int)nprobe, nseg, plan, prune, bound, (int)k, h->s_runs.as<unsigned long long>(), D, I,
-                               s);
+                               (int)nprobe, qn, nseg, plan, prune, bound, (int)k, h->s_runs.as<unsigned long long>(), D,
+                               I, s);
         HV_CHECK_LAUNCH("exact fallback");
     }
     HV_HIP(hipEventRecord(h->ev[5], s));
@@ -514,7 +555,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
     h->last_n = n;
-    h->last_nprobe = nprobe;
+    h->last_nprobe = nprobe_cap;
+    h->last_stride = nprobe;
+    h->last_learned = learned;
     if (st) {
         float ms[5];
         for (int i = 0; i < 5; ++i) HV_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
@@ -535,8 +578,58 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             st->ms_main += mm;
             st->main_pairs += (int64_t)hs[8];
         }
-        st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    if (learned) {
+        // queries whose k-th result is missing or above the seeded bound (a neighbour may have
+        // been pruned) are redone on the plain path: all nprobe lists, warm-up bound
+        HV_HIP(h->s_fb.ensure(sizeof(int) * (n + 1)));
+        int* fb_cnt = h->s_fb.as<int>();
+        int* fb_list = fb_cnt + 1;
+        hv::launch_fill_i32(fb_cnt, 1, 0, s);
+        hv::launch_flag_fallback(D, I, (int)k, seed, n, fb_list, fb_cnt, s);
+        int nfb = 0;
+        HV_HIP(hipMemcpyAsync(&nfb, fb_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+        HV_HIP(hipStreamSynchronize(s));
+        if (st) st->fallback_queries += nfb;
+        if (nfb > 0) {
+            HV_HIP(h->s_fbq.ensure(sizeof(float) * nfb * d));
+            HV_HIP(h->s_fbD.ensure(sizeof(float) * nfb * k));
+            HV_HIP(h->s_fbI.ensure(sizeof(int64_t) * nfb * k));
+            hv::launch_gather_rows(q_dev, nfb, (int)d, fb_list, h->s_fbq.as<float>(), s);
+            // keep this search's probe lists and bounds for introspection
+            const size_t pbytes = sizeof(int) * n * nprobe, bbytes = sizeof(float) * n;
+            HV_HIP(h->s_save.ensure(pbytes + bbytes));
+            HV_HIP(hipMemcpyAsync(h->s_save.p, probes, pbytes, hipMemcpyDeviceToDevice, s));
+            HV_HIP(hipMemcpyAsync(static_cast<char*>(h->s_save.p) + pbytes, bound_ref, bbytes,
+                                  hipMemcpyDeviceToDevice, s));
+            const hv_search_params keep = h->sp;
+            const double total_before = st ? st->ms_total : 0.0;  // this call's wall time covers the redo
+            h->sp.learned = 0;
+            h->sp.bound_mode = HV_BOUND_WARMUP;
+            try {
+                do_search(h, nfb, h->s_fbq.as<float>(), k, h->s_fbD.as<float>(), h->s_fbI.as<int64_t>(), st);
+            } catch (...) {
+                h->sp = keep;
+                throw;
+            }
+            h->sp = keep;
+            if (st) st->ms_total = total_before;
+            hv::launch_scatter_results(h->s_fbD.as<float>(), h->s_fbI.as<int64_t>(), fb_list, nfb, (int)k, D, I, s);
+            hv::launch_scatter_const(h->s_qn.as<int>(), fb_list, nfb, (int)nprobe_cap, s);  // they probed nprobe
+            HV_HIP(h->s_probes.ensure(pbytes));
+            HV_HIP(h->s_bound_ref.ensure(bbytes));
+            HV_HIP(hipMemcpyAsync(h->s_probes.p, h->s_save.p, pbytes, hipMemcpyDeviceToDevice, s));
+            HV_HIP(hipMemcpyAsync(h->s_bound_ref.p, static_cast<char*>(h->s_save.p) + pbytes, bbytes,
+                                  hipMemcpyDeviceToDevice, s));
+            HV_CHECK_LAUNCH("fallback");
+            HV_HIP(hipStreamSynchronize(s));
+            h->last_n = n;
+            h->last_nprobe = nprobe_cap;
+            h->last_stride = nprobe;
+            h->last_learned = true;
+        }
+    }
+    if (st) st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // namespace
@@ -590,7 +683,8 @@ void hv_destroy(hv_index_t* h) {
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats,
-                      &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres, &h->s_runs, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
+                      &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres, &h->s_runs, &h->s_qn,
+                      &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_fbq, &h->s_fbD, &h->s_fbI, &h->s_save, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -905,6 +999,10 @@ void hv_default_search_params(hv_search_params* p) {
     p->precision = HV_PRECISION_FP32;
     p->refine_bound = 1;
     p->survivor_cap = 0;
+    p->learned = 0;
+    p->learned_nprobe_scale = 1.0f;
+    p->learned_bound_scale = 1.0f;
+    p->learned_min_nprobe = 1;
 }
 
 int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
@@ -923,6 +1021,9 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
         require(p->tile_shape >= 0 && p->tile_shape <= hv::kNumShapes && p->tile_shape_first >= 0 &&
                     p->tile_shape_first <= hv::kNumShapes,
                 HV_ERR_INVALID_ARG, "invalid tile_shape");
+        require(p->learned == 0 || p->learned == 1, HV_ERR_INVALID_ARG, "learned must be 0 or 1");
+        require(p->learned_nprobe_scale > 0.f && p->learned_min_nprobe >= 1, HV_ERR_INVALID_ARG,
+                "learned_nprobe_scale must be > 0 and learned_min_nprobe >= 1");
         h->sp = *p;
     });
 }
@@ -1088,7 +1189,7 @@ int hv_get_list_codes(const hv_index_t* h, float* out) {
 int hv_get_last_probes(const hv_index_t* h, int64_t* out) {
     if (!h || !out) return HV_ERR_INVALID_ARG;
     if (h->last_n == 0) return HV_ERR_STATE;
-    std::vector<int> p(h->last_n * h->last_nprobe);
+    std::vector<int> p(h->last_n * h->last_stride);
     if (hipMemcpy(p.data(), h->s_probes.p, sizeof(int) * p.size(), hipMemcpyDeviceToHost) != hipSuccess)
         return HV_ERR_DEVICE;
     for (size_t i = 0; i < p.size(); ++i) out[i] = p[i];
@@ -1101,6 +1202,240 @@ int hv_get_last_bounds(const hv_index_t* h, float* out) {
     if (hipMemcpy(out, h->s_bound_ref.p, sizeof(float) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
         return HV_ERR_DEVICE;
     return HV_OK;
+}
+
+int hv_get_last_nprobe(const hv_index_t* h, int32_t* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (h->last_n == 0) return HV_ERR_STATE;
+    if (h->last_learned) {
+        if (hipMemcpy(out, h->s_qn.p, sizeof(int32_t) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
+            return HV_ERR_DEVICE;
+    } else {
+        for (int64_t i = 0; i < h->last_n; ++i) out[i] = (int32_t)h->last_nprobe;
+    }
+    return HV_OK;
+}
+
+int64_t hv_last_probe_stride(const hv_index_t* h) { return h ? h->last_stride : 0; }
+
+// ---- learned probe/bound predictor
+int hv_query_features(hv_index_t* h, int64_t n, const float* q, float* feat) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(h->trained, HV_ERR_STATE, "features need the centroids (train first)");
+        require(n >= 0 && (n == 0 || (q && feat)), HV_ERR_INVALID_ARG, "bad buffers");
+        if (n == 0) return;
+        const int64_t d = h->d;
+        const int64_t P = std::min<int64_t>(h->nlist, hv::kNearest);
+        hipStream_t s = h->stream;
+        upload(h->s_q, q, sizeof(float) * n * d, s);
+        const float* Qp = h->s_q.as<float>();
+        if (!h->perm.empty()) {
+            HV_HIP(h->s_qp.ensure(sizeof(float) * n * d));
+            hv::launch_permute_rows(Qp, n, (int)d, h->d_perm.as<int>(), h->s_qp.as<float>(), s);
+            Qp = h->s_qp.as<float>();
+        }
+        HV_HIP(h->s_probes.ensure(sizeof(int) * n * P));
+        HV_HIP(h->s_pdist.ensure(sizeof(float) * n * P));
+        coarse_select(h, n, Qp, P, h->s_probes.as<int>(), h->s_pdist.as<float>());
+        HV_HIP(h->s_feat.ensure(sizeof(float) * n * hv::kMlpIn));
+        hv::launch_features(h->s_pdist.as<float>(), (int)P, Qp, (int)d, n, h->s_feat.as<float>(), s);
+        HV_CHECK_LAUNCH("features");
+        HV_HIP(hipMemcpyAsync(feat, h->s_feat.p, sizeof(float) * n * hv::kMlpIn, hipMemcpyDeviceToHost, s));
+        HV_HIP(hipStreamSynchronize(s));
+        h->last_n = 0;  // the probe buffers no longer describe a search
+    });
+}
+
+namespace {
+constexpr int kNormLen = 2 * hv::kMlpIn + 2 * hv::kMlpOut;
+
+void mlp_refresh_serving(hv_index_t* h) {
+    HV_HIP(h->m_w1b.ensure(sizeof(__bf16) * hv::kMlpIn * hv::kMlpHidden));
+    HV_HIP(h->m_w2b.ensure(sizeof(__bf16) * hv::kMlpHidden * hv::kMlpOut));
+    hv::launch_mlp_to_bf16(h->m_params.as<float>(), h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(), h->stream);
+    HV_CHECK_LAUNCH("mlp weights");
+    HV_HIP(hipStreamSynchronize(h->stream));
+}
+
+void mlp_upload_params(hv_index_t* h, const float* params) {
+    upload(h->m_params, params, sizeof(float) * hv::kMlpParams, h->stream);
+    HV_HIP(h->m_adam_m.ensure(sizeof(float) * hv::kMlpParams));
+    HV_HIP(h->m_adam_v.ensure(sizeof(float) * hv::kMlpParams));
+    hv::launch_fill_i32(h->m_adam_m.as<int>(), hv::kMlpParams, 0, h->stream);  // Adam moments restart
+    hv::launch_fill_i32(h->m_adam_v.as<int>(), hv::kMlpParams, 0, h->stream);
+    h->mlp_params = true;
+    mlp_refresh_serving(h);
+}
+}  // namespace
+
+int hv_mlp_set_norm(hv_index_t* h, const float* norm) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(norm != nullptr, HV_ERR_INVALID_ARG, "null norm");
+        for (int i = 0; i < hv::kMlpIn; ++i)
+            require(norm[hv::kMlpIn + i] > 0.f, HV_ERR_INVALID_ARG, "feature std must be > 0");
+        for (int o = 0; o < hv::kMlpOut; ++o)
+            require(norm[2 * hv::kMlpIn + hv::kMlpOut + o] > 0.f, HV_ERR_INVALID_ARG, "target std must be > 0");
+        upload(h->m_norm, norm, sizeof(float) * kNormLen, h->stream);
+        HV_HIP(hipStreamSynchronize(h->stream));
+        h->mlp_norm = true;
+    });
+}
+
+int hv_mlp_get_norm(const hv_index_t* h, float* norm) {
+    if (!h || !norm) return HV_ERR_INVALID_ARG;
+    if (!h->mlp_norm) return HV_ERR_STATE;
+    return hipMemcpy(norm, h->m_norm.p, sizeof(float) * kNormLen, hipMemcpyDeviceToHost) == hipSuccess
+               ? HV_OK
+               : HV_ERR_DEVICE;
+}
+
+int hv_mlp_init(hv_index_t* h, uint64_t seed) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        // He-uniform: U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)) from SplitMix64; biases zero
+        std::vector<float> p(hv::kMlpParams, 0.f);
+        uint64_t x = seed;
+        auto uni = [&]() {
+            x += 0x9e3779b97f4a7c15ull;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            z ^= z >> 31;
+            return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        };
+        const double a1 = std::sqrt(6.0 / hv::kMlpIn), a2 = std::sqrt(6.0 / hv::kMlpHidden);
+        const int w1 = hv::kMlpIn * hv::kMlpHidden, w2o = w1 + hv::kMlpHidden;
+        for (int i = 0; i < w1; ++i) p[i] = (float)((2.0 * uni() - 1.0) * a1);
+        for (int i = 0; i < hv::kMlpHidden * hv::kMlpOut; ++i) p[w2o + i] = (float)((2.0 * uni() - 1.0) * a2);
+        mlp_upload_params(h, p.data());
+    });
+}
+
+int hv_mlp_set_params(hv_index_t* h, const float* params) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(params != nullptr, HV_ERR_INVALID_ARG, "null params");
+        mlp_upload_params(h, params);
+    });
+}
+
+int hv_mlp_get_params(const hv_index_t* h, float* params) {
+    if (!h || !params) return HV_ERR_INVALID_ARG;
+    if (!h->mlp_params) return HV_ERR_STATE;
+    return hipMemcpy(params, h->m_params.p, sizeof(float) * hv::kMlpParams, hipMemcpyDeviceToHost) == hipSuccess
+               ? HV_OK
+               : HV_ERR_DEVICE;
+}
+
+namespace {
+// device training set and per-batch work buffers
+struct MlpWork {
+    DevBuf X, T, idx, Xn, H, Y, dY, dH, row_loss, loss;
+};
+
+void mlp_prepare(MlpWork& w, int64_t n, int B, const float* feat, const float* target, hipStream_t s) {
+    upload(w.X, feat, sizeof(float) * n * hv::kMlpIn, s);
+    upload(w.T, target, sizeof(float) * n * hv::kMlpOut, s);
+    HV_HIP(w.idx.ensure(sizeof(int) * n));
+    HV_HIP(w.Xn.ensure(sizeof(float) * B * hv::kMlpIn));
+    HV_HIP(w.H.ensure(sizeof(float) * B * hv::kMlpHidden));
+    HV_HIP(w.Y.ensure(sizeof(float) * B * hv::kMlpOut));
+    HV_HIP(w.dY.ensure(sizeof(float) * B * hv::kMlpOut));
+    HV_HIP(w.dH.ensure(sizeof(float) * B * hv::kMlpHidden));
+    HV_HIP(w.row_loss.ensure(sizeof(float) * B));
+    HV_HIP(w.loss.ensure(sizeof(float) * 4));
+}
+}  // namespace
+
+int hv_mlp_grad(hv_index_t* h, int64_t n, const float* feat, const float* target, float* grad, float* loss) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(h->mlp_params && h->mlp_norm, HV_ERR_STATE, "set the parameters and the standardization first");
+        require(n >= 1 && n <= (1 << 20) && feat && target && grad, HV_ERR_INVALID_ARG, "bad batch");
+        hipStream_t s = h->stream;
+        MlpWork w;
+        mlp_prepare(w, n, (int)n, feat, target, s);
+        std::vector<int> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        HV_HIP(hipMemcpyAsync(w.idx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        HV_HIP(h->m_grad.ensure(sizeof(float) * hv::kMlpParams));
+        hv::launch_fill_i32(w.loss.as<int>(), 1, 0, s);
+        hv::launch_mlp_step(w.X.as<float>(), w.T.as<float>(), w.idx.as<int>(), (int)n, h->m_params.as<float>(),
+                            h->m_norm.as<float>(), w.Xn.as<float>(), w.H.as<float>(), w.Y.as<float>(), w.dY.as<float>(),
+                            w.dH.as<float>(), w.row_loss.as<float>(), h->m_grad.as<float>(), w.loss.as<float>(), s);
+        HV_CHECK_LAUNCH("mlp grad");
+        HV_HIP(hipMemcpyAsync(grad, h->m_grad.p, sizeof(float) * hv::kMlpParams, hipMemcpyDeviceToHost, s));
+        float l = 0.f;
+        HV_HIP(hipMemcpyAsync(&l, w.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+        HV_HIP(hipStreamSynchronize(s));
+        if (loss) *loss = l;
+    });
+}
+
+int hv_mlp_train(hv_index_t* h, int64_t n, const float* feat, const float* target, int32_t epochs, int32_t batch,
+                 float lr, uint64_t seed, float* epoch_loss) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(h->mlp_params && h->mlp_norm, HV_ERR_STATE, "set the parameters and the standardization first");
+        require(n >= 1 && feat && target, HV_ERR_INVALID_ARG, "empty training set");
+        require(epochs >= 1 && batch >= 1 && batch <= 65536 && lr > 0.f, HV_ERR_INVALID_ARG, "bad schedule");
+        hipStream_t s = h->stream;
+        const int B = (int)std::min<int64_t>(batch, n);
+        MlpWork w;
+        mlp_prepare(w, n, B, feat, target, s);
+        HV_HIP(h->m_grad.ensure(sizeof(float) * hv::kMlpParams));
+        std::vector<int> perm(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        uint64_t x = seed;
+        auto next = [&]() {
+            x += 0x9e3779b97f4a7c15ull;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            return z ^ (z >> 31);
+        };
+        int step = 0;
+        for (int e = 0; e < epochs; ++e) {
+            for (int64_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[next() % (uint64_t)(i + 1)]);
+            HV_HIP(hipMemcpyAsync(w.idx.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+            hv::launch_fill_i32(w.loss.as<int>(), 1, 0, s);
+            int batches = 0;
+            for (int64_t b0 = 0; b0 + B <= n; b0 += B) {  // full batches only
+                hv::launch_mlp_step(w.X.as<float>(), w.T.as<float>(), w.idx.as<int>() + b0, B, h->m_params.as<float>(),
+                                    h->m_norm.as<float>(), w.Xn.as<float>(), w.H.as<float>(), w.Y.as<float>(),
+                                    w.dY.as<float>(), w.dH.as<float>(), w.row_loss.as<float>(), h->m_grad.as<float>(),
+                                    w.loss.as<float>(), s);
+                hv::launch_adam(h->m_params.as<float>(), h->m_grad.as<float>(), h->m_adam_m.as<float>(),
+                                h->m_adam_v.as<float>(), lr, 0.9f, 0.999f, 1e-8f, ++step, s);
+                ++batches;
+            }
+            HV_CHECK_LAUNCH("mlp train");
+            float l = 0.f;
+            HV_HIP(hipMemcpyAsync(&l, w.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+            HV_HIP(hipStreamSynchronize(s));
+            if (epoch_loss) epoch_loss[e] = l / (float)std::max(1, batches);
+        }
+        mlp_refresh_serving(h);
+    });
+}
+
+int hv_mlp_predict(hv_index_t* h, int64_t n, const float* feat, float* out) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(h->mlp_params && h->mlp_norm, HV_ERR_STATE, "set the parameters and the standardization first");
+        require(n >= 0 && (n == 0 || (feat && out)), HV_ERR_INVALID_ARG, "bad buffers");
+        if (n == 0) return;
+        hipStream_t s = h->stream;
+        upload(h->s_feat, feat, sizeof(float) * n * hv::kMlpIn, s);
+        HV_HIP(h->s_pred.ensure(sizeof(float) * n * 2));
+        hv::launch_mlp_predict(h->s_feat.as<float>(), n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
+                               h->m_params.as<float>(), h->m_norm.as<float>(), h->s_pred.as<float>(), s);
+        HV_CHECK_LAUNCH("mlp predict");
+        HV_HIP(hipMemcpyAsync(out, h->s_pred.p, sizeof(float) * n * 2, hipMemcpyDeviceToHost, s));
+        HV_HIP(hipStreamSynchronize(s));
+    });
 }
 
 int hv_device_count(void) {

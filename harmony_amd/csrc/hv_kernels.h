@@ -46,11 +46,12 @@ void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq
                          hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
-void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int* bucket_cnt, hipStream_t s);
+// qn (optional): per-query probe counts (ranks at or past qn[q] are skipped)
+void launch_count_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, int* bucket_cnt, hipStream_t s);
 // exclusive scan of n ints into out (n+1 entries, out[n] = total); tmp >= scan_tmp_ints(n)
 int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s);
-void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* bucket_off, int* fill,
+void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, const int* bucket_off, int* fill,
                           int* pairs, int* pair_slot, hipStream_t s);
 // queue: 3 x 8 ints (per-XCD length, head, base) + nlist ints (first item of each list rank)
 // vector range [vmin, vmax) of each list (vmin a multiple of the scan's vector tile)
@@ -74,8 +75,8 @@ void launch_select_survivors(const unsigned long long* surv, const unsigned int*
 int exact_segments(int64_t max_list);
 void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
                         const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
-                        int nprobe, int nseg, const ChunkPlan& plan, int prune, const float* bound, int k,
-                        unsigned long long* runs, float* D, int64_t* I, hipStream_t s);
+                        int nprobe, const int* qn, int nseg, const ChunkPlan& plan, int prune, const float* bound,
+                        int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s);
 
 // --- BF16 precision form (hv_bf16.hip)
 int bf16_ksteps(int d);  // 32-dim MFMA steps per vector (dpad = 32 * ksteps)
@@ -86,8 +87,8 @@ void launch_build_resid(const float* x, int d, int dpad, const int* order, const
 // rows in list order: by_pos[p] = rows[order[p]]
 void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, float* by_pos, hipStream_t s);
 // BF16 (q - c) rows of dpad dims for every (query, probe rank) slot
-void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, int64_t nq, int nprobe, int d,
-                       int dpad, __bf16* out, hipStream_t s);
+void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
+                       int nprobe, int d, int dpad, __bf16* out, hipStream_t s);
 void launch_bf16_scan(int grid, const __bf16* qres, int d, const float* pdist, int nprobe,
                       const __bf16* rcodes, const int64_t* roff, const float* rnorm, const int64_t* list_off,
                       const float* bound, const int* pair_slot, const ScanItem* items, const int* queue, int vmax,
@@ -100,6 +101,32 @@ void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned in
                          float* bound, hipStream_t s);
 void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
                    unsigned long long* surv, const unsigned int* cnt, int64_t nq, int cap, hipStream_t s);
+
+// --- learned probe/bound predictor (hv_learn.hip)
+constexpr int kNearest = 64;                                            // nearest-centroid distances
+constexpr int kMlpIn = kNearest + 1, kMlpHidden = 128, kMlpOut = 2;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpOut + kMlpOut;
+// norm: fmean [65], fstd [65], tmean [2], tstd [2]
+void launch_features(const float* pdist, int P, const float* Qp, int d, int64_t n, float* feat, hipStream_t s);
+void launch_mlp_step(const float* X, const float* T, const int* idx, int B, float* P, const float* norm, float* Xn,
+                     float* H, float* Y, float* dY, float* dH, float* row_loss, float* G, float* loss_acc,
+                     hipStream_t s);
+void launch_adam(float* P, const float* G, float* M, float* V, float lr, float beta1, float beta2, float eps, int t,
+                 hipStream_t s);
+void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s);
+void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
+                        const float* norm, float* out, hipStream_t s);
+void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
+                      const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
+                      float* bound, float* pred, hipStream_t s);
+
+// queries whose k-th result is missing or above its seeded bound -> list / cnt
+void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* seed, int64_t n, int* list, int* cnt,
+                          hipStream_t s);
+// dst[list[i]] = value, i < n
+void launch_scatter_const(int* dst, const int* list, int64_t n, int value, hipStream_t s);
+void launch_scatter_results(const float* Dsub, const int64_t* Isub, const int* list, int64_t nsub, int k, float* D,
+                            int64_t* I, hipStream_t s);
 
 // --- FP64 ground truth
 void launch_gt_f64(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,

@@ -1,0 +1,319 @@
+// Learned probe/bound predictor: a 65 -> 128 (ReLU) -> 2 MLP over a query's distances to its 64
+// nearest centroids and its squared norm, predicting log1p(k-th neighbour distance) and
+// log(nprobe needed for the target recall).
+//
+// Features (raw): f_i = log1p(i-th smallest coarse distance), i < 64 (the last one repeated when
+// there are fewer lists), f_64 = log1p(|q|^2). The network works on standardized features
+// xn = (f - fmean) / fstd and standardized targets; outputs are mapped back with the target
+// mean/std. Training keeps FP32 master weights (forward, backward and Adam below, all FP32);
+// serving uses BF16 copies of the weight matrices with FP32 accumulation.
+//
+// Parameter layout (flat FP32): W1 [65][128] (input-major), b1 [128], W2 [128][2], b2 [2].
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "hv_kernels.h"
+
+namespace hv {
+
+constexpr int kIn = kMlpIn, kHid = kMlpHidden, kOut = kMlpOut;
+constexpr int kOffB1 = kIn * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kOut;
+static_assert(kOffB2 + kOut == kMlpParams, "parameter layout");
+
+// ---------------------------------------------------------------------------------------
+// Features from the sorted coarse distances (pdist [n][P], ascending) and the queries.
+__global__ void k_features(const float* __restrict__ pdist, int P, const float* __restrict__ Qp, int d, int64_t n,
+                           float* __restrict__ feat) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= n) return;
+    const int lane = threadIdx.x & 63;
+    float s = 0.f;
+    for (int i = lane; i < d; i += 64) {
+        const float v = Qp[q * d + i];
+        s = __builtin_fmaf(v, v, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    const int c = min(lane, P - 1);
+    feat[q * kIn + lane] = log1pf(fmaxf(pdist[q * P + c], 0.f));
+    if (lane == 0) feat[q * kIn + kNearest] = log1pf(s);
+}
+
+void launch_features(const float* pdist, int P, const float* Qp, int d, int64_t n, float* feat, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, pdist, P, Qp, d, n, feat);
+}
+
+// ---------------------------------------------------------------------------------------
+// Training. A batch is rows idx[0..B) of X [N][65] / T [N][2] (raw features / raw targets).
+// Forward: one workgroup of 128 threads per row; thread j owns hidden unit j.
+__global__ __launch_bounds__(128) void k_mlp_fwd(const float* __restrict__ X, const int* __restrict__ idx,
+                                                 const float* __restrict__ P, const float* __restrict__ norm,
+                                                 float* __restrict__ Xn, float* __restrict__ H,
+                                                 float* __restrict__ Y) {
+    __shared__ float xs[kIn];
+    __shared__ float hs[kHid];
+    const int b = blockIdx.x, j = threadIdx.x;
+    const int64_t r = idx[b];
+    const float* fmean = norm;
+    const float* fstd = norm + kIn;
+    if (j < kIn) {
+        const float v = (X[r * kIn + j] - fmean[j]) / fstd[j];
+        xs[j] = v;
+        Xn[(int64_t)b * kIn + j] = v;
+    }
+    __syncthreads();
+    float h = P[kOffB1 + j];
+    for (int i = 0; i < kIn; ++i) h = __builtin_fmaf(xs[i], P[i * kHid + j], h);
+    h = fmaxf(h, 0.f);
+    hs[j] = h;
+    H[(int64_t)b * kHid + j] = h;
+    __syncthreads();
+    if (j < kOut) {
+        float y = P[kOffB2 + j];
+        for (int u = 0; u < kHid; ++u) y = __builtin_fmaf(hs[u], P[kOffW2 + u * kOut + j], y);
+        Y[(int64_t)b * kOut + j] = y;
+    }
+}
+
+// Output and hidden gradients of L = 1/(2B) sum_b sum_o (y_bo - t_bo)^2 (standardized targets),
+// and each row's loss term.
+__global__ __launch_bounds__(128) void k_mlp_bwd_rows(const float* __restrict__ T, const int* __restrict__ idx,
+                                                      int B, const float* __restrict__ P,
+                                                      const float* __restrict__ norm, const float* __restrict__ H,
+                                                      const float* __restrict__ Y, float* __restrict__ dY,
+                                                      float* __restrict__ dH, float* __restrict__ row_loss) {
+    __shared__ float dys[kOut];
+    const int b = blockIdx.x, j = threadIdx.x;
+    const float* tmean = norm + 2 * kIn;
+    const float* tstd = tmean + kOut;
+    if (j < kOut) {
+        const float t = (T[(int64_t)idx[b] * kOut + j] - tmean[j]) / tstd[j];
+        const float e = Y[(int64_t)b * kOut + j] - t;
+        dys[j] = e / (float)B;
+        dY[(int64_t)b * kOut + j] = e / (float)B;
+    }
+    __syncthreads();
+    if (j == 0) {
+        const float e0 = dys[0] * (float)B, e1 = dys[1] * (float)B;
+        row_loss[b] = 0.5f * (e0 * e0 + e1 * e1) / (float)B;
+    }
+    float g = 0.f;
+    for (int o = 0; o < kOut; ++o) g = __builtin_fmaf(dys[o], P[kOffW2 + j * kOut + o], g);
+    dH[(int64_t)b * kHid + j] = H[(int64_t)b * kHid + j] > 0.f ? g : 0.f;
+}
+
+// Parameter gradients: one thread per parameter, summing its batch terms in row order.
+__global__ void k_mlp_bwd_params(int B, const float* __restrict__ Xn, const float* __restrict__ H,
+                                 const float* __restrict__ dY, const float* __restrict__ dH, float* __restrict__ G) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float g = 0.f;
+    if (p < kOffB1) {
+        const int i = p / kHid, j = p % kHid;
+        for (int b = 0; b < B; ++b) g = __builtin_fmaf(Xn[(int64_t)b * kIn + i], dH[(int64_t)b * kHid + j], g);
+    } else if (p < kOffW2) {
+        const int j = p - kOffB1;
+        for (int b = 0; b < B; ++b) g += dH[(int64_t)b * kHid + j];
+    } else if (p < kOffB2) {
+        const int j = (p - kOffW2) / kOut, o = (p - kOffW2) % kOut;
+        for (int b = 0; b < B; ++b) g = __builtin_fmaf(H[(int64_t)b * kHid + j], dY[(int64_t)b * kOut + o], g);
+    } else {
+        const int o = p - kOffB2;
+        for (int b = 0; b < B; ++b) g += dY[(int64_t)b * kOut + o];
+    }
+    G[p] = g;
+}
+
+// Sum of the rows' losses (one workgroup, fixed order) added to *acc.
+__global__ __launch_bounds__(256) void k_sum_loss(const float* __restrict__ row_loss, int B, float* __restrict__ acc) {
+    __shared__ float s[256];
+    float v = 0.f;
+    for (int b = threadIdx.x; b < B; b += 256) v += row_loss[b];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += s[0];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; t is the 1-based step.
+__global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M, float* __restrict__ V,
+                       float lr, float beta1, float beta2, float eps, int t) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float g = G[p];
+    const float m = beta1 * M[p] + (1.f - beta1) * g;
+    const float v = beta2 * V[p] + (1.f - beta2) * g * g;
+    M[p] = m;
+    V[p] = v;
+    const float mh = m / (1.f - powf(beta1, (float)t));
+    const float vh = v / (1.f - powf(beta2, (float)t));
+    P[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+void launch_mlp_step(const float* X, const float* T, const int* idx, int B, float* P, const float* norm, float* Xn,
+                     float* H, float* Y, float* dY, float* dH, float* row_loss, float* G, float* loss_acc,
+                     hipStream_t s) {
+    if (B <= 0) return;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)B), dim3(kHid), 0, s, X, idx, P, norm, Xn, H, Y);
+    hipLaunchKernelGGL(k_mlp_bwd_rows, dim3((unsigned)B), dim3(kHid), 0, s, T, idx, B, P, norm, H, Y, dY, dH,
+                       row_loss);
+    hipLaunchKernelGGL(k_mlp_bwd_params, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, B, Xn, H, dY, dH, G);
+    if (loss_acc) hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, s, row_loss, B, loss_acc);
+}
+
+void launch_adam(float* P, const float* G, float* M, float* V, float lr, float beta1, float beta2, float eps, int t,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, P, G, M, V, lr, beta1, beta2, eps, t);
+}
+
+// ---------------------------------------------------------------------------------------
+// Serving weights: BF16 copies of W1 and W2 (biases stay FP32).
+__global__ void k_mlp_to_bf16(const float* __restrict__ P, __bf16* __restrict__ w1, __bf16* __restrict__ w2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kOffB1) w1[p] = (__bf16)P[p];
+    if (p < kHid * kOut) w2[p] = (__bf16)P[kOffW2 + p];
+}
+
+void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_to_bf16, dim3((kOffB1 + 255) / 256), dim3(256), 0, s, P, w1, w2);
+}
+
+// One wavefront per row: lane j owns hidden units j and j + 64; the two outputs are reduced
+// across the wavefront. Returns the outputs in raw target space (every lane gets them).
+__device__ __forceinline__ void mlp_eval_bf16(const float* __restrict__ f, int lane, const __bf16* __restrict__ w1,
+                                              const __bf16* __restrict__ w2, const float* __restrict__ P,
+                                              const float* __restrict__ norm, float& y0, float& y1) {
+    const float* fmean = norm;
+    const float* fstd = norm + kIn;
+    const float* tmean = norm + 2 * kIn;
+    const float* tstd = tmean + kOut;
+    float h0 = P[kOffB1 + lane], h1 = P[kOffB1 + lane + 64];
+    for (int i = 0; i < kIn; ++i) {
+        const float x = (f[i] - fmean[i]) / fstd[i];
+        h0 = __builtin_fmaf(x, (float)w1[i * kHid + lane], h0);
+        h1 = __builtin_fmaf(x, (float)w1[i * kHid + lane + 64], h1);
+    }
+    h0 = fmaxf(h0, 0.f);
+    h1 = fmaxf(h1, 0.f);
+    float a = h0 * (float)w2[lane * kOut] + h1 * (float)w2[(lane + 64) * kOut];
+    float c = h0 * (float)w2[lane * kOut + 1] + h1 * (float)w2[(lane + 64) * kOut + 1];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    y0 = (a + P[kOffB2]) * tstd[0] + tmean[0];
+    y1 = (c + P[kOffB2 + 1]) * tstd[1] + tmean[1];
+}
+
+__global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ X, int64_t n,
+                                                     const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
+                                                     const float* __restrict__ P, const float* __restrict__ norm,
+                                                     float* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= n) return;
+    const int lane = threadIdx.x & 63;
+    float y0, y1;
+    mlp_eval_bf16(X + q * kIn, lane, w1, w2, P, norm, y0, y1);
+    if (lane == 0) {
+        out[q * 2] = y0;
+        out[q * 2 + 1] = y1;
+    }
+}
+
+void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
+                        const float* norm, float* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mlp_predict, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, X, n, w1, w2, P, norm, out);
+}
+
+// Serving: per query, the probe count qn = clamp(ceil(exp(y1) * nprobe_scale), min_nprobe,
+// nprobe) and the seeded bound expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0).
+__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ feat, int64_t n,
+                                                   const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
+                                                   const float* __restrict__ P, const float* __restrict__ norm,
+                                                   float nprobe_scale, float bound_scale, int min_nprobe, int nprobe,
+                                                   int* __restrict__ qn, float* __restrict__ bound,
+                                                   float* __restrict__ pred) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= n) return;
+    const int lane = threadIdx.x & 63;
+    float y0, y1;
+    mlp_eval_bf16(feat + q * kIn, lane, w1, w2, P, norm, y0, y1);
+    if (lane == 0) {
+        const float np = expf(y1) * nprobe_scale;
+        const int c = np >= (float)nprobe ? nprobe : max(min_nprobe, (int)ceilf(np));
+        qn[q] = min(max(c, 1), nprobe);
+        bound[q] = bound_scale > 0.f ? fminf(expm1f(fmaxf(y0, 0.f)) * bound_scale, 3.402823466e38f) : 3.402823466e38f;
+        pred[q * 2] = y0;
+        pred[q * 2 + 1] = y1;
+    }
+}
+
+void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
+                      const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
+                      float* bound, float* pred, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, feat, n, w1, w2, P, norm,
+                       nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred);
+}
+
+}  // namespace hv
+
+namespace hv {
+
+// ---------------------------------------------------------------------------------------
+// Queries the seeded bound may have cut short: their k-th result is missing or lies above the
+// seed (then a true neighbour may have been pruned). They are redone on the plain path.
+__global__ void k_flag_fallback(const float* __restrict__ D, const int64_t* __restrict__ I, int k,
+                                const float* __restrict__ seed, int64_t n, int* __restrict__ list,
+                                int* __restrict__ cnt) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (I[q * k + k - 1] < 0 || D[q * k + k - 1] > seed[q]) list[atomicAdd(cnt, 1)] = (int)q;
+}
+
+void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* seed, int64_t n, int* list, int* cnt,
+                          hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_flag_fallback, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, I, k, seed, n, list,
+                       cnt);
+}
+
+// rows[i] of (Dsub, Isub) -> row list[i] of (D, I)
+__global__ void k_scatter_results(const float* __restrict__ Dsub, const int64_t* __restrict__ Isub,
+                                  const int* __restrict__ list, int64_t nsub, int k, float* __restrict__ D,
+                                  int64_t* __restrict__ I) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nsub * k) return;
+    const int64_t i = e / k, j = e % k;
+    const int64_t q = list[i];
+    D[q * k + j] = Dsub[e];
+    I[q * k + j] = Isub[e];
+}
+
+void launch_scatter_results(const float* Dsub, const int64_t* Isub, const int* list, int64_t nsub, int k, float* D,
+                            int64_t* I, hipStream_t s) {
+    if (nsub <= 0) return;
+    hipLaunchKernelGGL(k_scatter_results, dim3((unsigned)((nsub * k + 255) / 256)), dim3(256), 0, s, Dsub, Isub, list,
+                       nsub, k, D, I);
+}
+
+}  // namespace hv
+
+namespace hv {
+
+__global__ void k_scatter_const(int* __restrict__ dst, const int* __restrict__ list, int64_t n, int value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[list[i]] = value;
+}
+
+void launch_scatter_const(int* dst, const int* list, int64_t n, int value, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_const, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, list, n, value);
+}
+
+}  // namespace hv

@@ -48,39 +48,43 @@ __device__ __forceinline__ float blocked_l2_col(const float* qs, const float* __
 // Grouping: (query, probe rank) pairs bucketed by (list, rank). Within a list, queries that
 // rank the list first come first, so a query tile holds queries at similar distance from the
 // list and prunes (or survives) together.
-__global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int nprobe, int* __restrict__ cnt) {
+// qn (optional): per-query probe count; ranks at or past it are not searched
+__global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int nprobe, const int* __restrict__ qn,
+                              int* __restrict__ cnt) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
     const int j = (int)(e % nprobe);
+    if (qn && j >= qn[e / nprobe]) return;
     atomicAdd(&cnt[(int64_t)probes[e] * nprobe + j], 1);
 }
 
-void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int* bucket_cnt, hipStream_t s) {
+void launch_count_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, int* bucket_cnt, hipStream_t s) {
     const int64_t tot = nq * nprobe;
     if (tot <= 0) return;
-    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe,
+    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe, qn,
                        bucket_cnt);
 }
 
 // pairs[pos] = query, pair_slot[pos] = query * nprobe + rank (the probe slot, for the coarse
 // distance of the pair)
-__global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, int nprobe,
+__global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, int nprobe, const int* __restrict__ qn,
                                 const int* __restrict__ off, int* __restrict__ fill, int* __restrict__ pairs,
                                 int* __restrict__ pair_slot) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
     const int j = (int)(e % nprobe);
+    if (qn && j >= qn[e / nprobe]) return;
     const int64_t b = (int64_t)probes[e] * nprobe + j;
     const int pos = off[b] + atomicAdd(&fill[b], 1);
     pairs[pos] = (int)(e / nprobe);
     pair_slot[pos] = (int)e;
 }
 
-void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* bucket_off, int* fill, int* pairs,
-                          int* pair_slot, hipStream_t s) {
+void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, const int* bucket_off, int* fill,
+                          int* pairs, int* pair_slot, hipStream_t s) {
     const int64_t tot = nq * nprobe;
     if (tot <= 0) return;
-    hipLaunchKernelGGL(k_scatter_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe,
+    hipLaunchKernelGGL(k_scatter_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe, qn,
                        bucket_off, fill, pairs, pair_slot);
 }
 
@@ -740,8 +744,8 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
                                                      const int64_t* __restrict__ list_off,
                                                      const int* __restrict__ list_stride,
                                                      const uint32_t* __restrict__ ids, const int* __restrict__ probes,
-                                                     int nprobe, int nseg, ChunkPlan plan, int prune,
-                                                     const float* __restrict__ bound, int k,
+                                                     int nprobe, const int* __restrict__ qn, int nseg,
+                                                     ChunkPlan plan, int prune, const float* __restrict__ bound, int k,
                                                      unsigned long long* __restrict__ runs) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     // kept keys [kExactTop] | keys collected this segment [kExactSeg] | reduction output [kExactTop]
@@ -768,7 +772,8 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
         __syncthreads();
         const int l = probes[q * nprobe + j];
         const int64_t lb = list_off[l];
-        const int n = (int)min(list_off[l + 1] - lb, (long long)(seg + 1) * kExactSpan);
+        // a rank past the query's probe count contributes an empty run
+        const int n = (qn && j >= qn[q]) ? 0 : (int)min(list_off[l + 1] - lb, (long long)(seg + 1) * kExactSpan);
         const float* X = codes + code_off[l];
         const int ps = list_stride[l];
         for (int t0 = seg * kExactSpan; t0 < n; t0 += blockDim.x) {
@@ -844,14 +849,14 @@ int exact_segments(int64_t max_list) { return (int)std::max<int64_t>(1, (max_lis
 
 void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
                         const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
-                        int nprobe, int nseg, const ChunkPlan& plan, int prune, const float* bound, int k,
-                        unsigned long long* runs, float* D, int64_t* I, hipStream_t s) {
+                        int nprobe, const int* qn, int nseg, const ChunkPlan& plan, int prune, const float* bound,
+                        int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s) {
     if (nql <= 0) return;
     const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
     const int64_t pairs = (int64_t)nql * nprobe * nseg;
     hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, 65536)), dim3(256), lds, s, qlist, nql,
-                       Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nseg, plan, prune, bound, k,
-                       runs);
+                       Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, qn, nseg, plan, prune, bound,
+                       k, runs);
     hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql, 65536)), dim3(256), 0, s, qlist, nql, nprobe * nseg,
                        k, runs, D, I);
 }

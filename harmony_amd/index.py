@@ -117,14 +117,28 @@ class Index:
         self._push()
 
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
-                  survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None):
+                  survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None,
+                  learned=None, learned_nprobe_scale=None, learned_bound_scale=None, learned_min_nprobe=None):
         """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59).
 
         precision: HV_PRECISION_FP32 (default) or HV_PRECISION_BF16 (MFMA filter + FP32 distances
-        of the survivors; same results). None keeps the current setting."""
+        of the survivors; same results). None keeps the current setting.
+
+        learned (None keeps the current settings): per-query probe count and pruning bound from
+        the trained predictor (mlp_*): probes = ceil(predicted nprobe * learned_nprobe_scale),
+        capped at nprobe; bound = predicted k-th distance * learned_bound_scale (<= 0: none).
+        Queries whose k-th result lands above that bound are redone on the plain path."""
         p = self._params
         if precision is not None:
             p.precision = int(precision)
+        if learned is not None:
+            p.learned = 1 if learned else 0
+        if learned_nprobe_scale is not None:
+            p.learned_nprobe_scale = float(learned_nprobe_scale)
+        if learned_bound_scale is not None:
+            p.learned_bound_scale = float(learned_bound_scale)
+        if learned_min_nprobe is not None:
+            p.learned_min_nprobe = int(learned_min_nprobe)
         p.refine_bound = 1 if refine_bound else 0
         p.tile_shape = int(tile_shape)
         p.tile_shape_first = int(tile_shape_first)
@@ -215,8 +229,73 @@ class Index:
         return out
 
     def last_probes(self, n):
-        out = np.empty((n, min(self.nprobe, self.nlist)), dtype=np.int64)
+        """Probe lists of the last search ([n][stride]: nprobe, or >= 64 deep with the predictor)."""
+        stride = int(lib().hv_last_probe_stride(self._h)) or min(self.nprobe, self.nlist)
+        out = np.empty((n, stride), dtype=np.int64)
         check(self._h, lib().hv_get_last_probes(self._h, _ptr(out)))
+        return out
+
+    def last_nprobe(self, n):
+        """Per-query probe counts of the last search."""
+        out = np.empty(n, dtype=np.int32)
+        check(self._h, lib().hv_get_last_nprobe(self._h, _ptr(out)))
+        return out
+
+    # -- learned probe/bound predictor
+    def query_features(self, queries):
+        """[n][65]: log1p of the 64 nearest squared centroid distances, ascending; log1p(|q|^2)."""
+        q = _f32(queries, self.d)
+        out = np.empty((q.shape[0], _lib.HV_MLP_IN), dtype=np.float32)
+        check(self._h, lib().hv_query_features(self._h, q.shape[0], _ptr(q), _ptr(out)))
+        return out
+
+    def mlp_set_norm(self, fmean, fstd, tmean, tstd):
+        norm = np.concatenate([np.asarray(a, np.float32).ravel() for a in (fmean, fstd, tmean, tstd)])
+        assert norm.size == 2 * _lib.HV_MLP_IN + 2 * _lib.HV_MLP_OUT
+        check(self._h, lib().hv_mlp_set_norm(self._h, _ptr(norm)))
+
+    def mlp_get_norm(self):
+        out = np.empty(2 * _lib.HV_MLP_IN + 2 * _lib.HV_MLP_OUT, dtype=np.float32)
+        check(self._h, lib().hv_mlp_get_norm(self._h, _ptr(out)))
+        i, o = _lib.HV_MLP_IN, _lib.HV_MLP_OUT
+        return out[:i], out[i:2 * i], out[2 * i:2 * i + o], out[2 * i + o:]
+
+    def mlp_init(self, seed=0):
+        check(self._h, lib().hv_mlp_init(self._h, int(seed)))
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        assert p.size == _lib.HV_MLP_PARAMS
+        check(self._h, lib().hv_mlp_set_params(self._h, _ptr(p)))
+
+    def mlp_get_params(self):
+        out = np.empty(_lib.HV_MLP_PARAMS, dtype=np.float32)
+        check(self._h, lib().hv_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def mlp_grad(self, feat, target):
+        """(loss, gradient) of 1/(2n) sum (y - t)^2 over the standardized batch."""
+        x = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.empty(_lib.HV_MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        check(self._h, lib().hv_mlp_grad(self._h, x.shape[0], _ptr(x), _ptr(t), _ptr(g), ctypes.byref(loss)))
+        return float(loss.value), g
+
+    def mlp_train(self, feat, target, epochs=20, batch=1024, lr=1e-3, seed=0):
+        """Adam on the GPU; returns each epoch's mean minibatch loss."""
+        x = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        losses = np.empty(int(epochs), dtype=np.float32)
+        check(self._h, lib().hv_mlp_train(self._h, x.shape[0], _ptr(x), _ptr(t), int(epochs), int(batch),
+                                          float(lr), int(seed), _ptr(losses)))
+        return losses
+
+    def mlp_predict(self, feat):
+        """[n][2] raw targets: log1p(k-th distance), log(nprobe)."""
+        x = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.empty((x.shape[0], 2), dtype=np.float32)
+        check(self._h, lib().hv_mlp_predict(self._h, x.shape[0], _ptr(x), _ptr(out)))
         return out
 
     def last_bounds(self, n):

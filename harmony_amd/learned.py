@@ -1,0 +1,112 @@
+"""Learned probe/bound predictor: traces from the plain search, GPU training, calibration.
+
+The predictor (a 65 -> 128 -> 2 MLP run by the library's own HIP kernels) reads a query's
+distances to its 64 nearest centroids and its norm, and predicts the k-th neighbour distance and
+the number of lists needed to reach a target recall. Serving uses the prediction to trim each
+query's probe list and to seed its pruning bound; a query whose k-th result lands above the
+seeded bound is redone on the plain path. No counterpart in the reference, which probes a fixed
+nprobe for every query and takes its bound from a warm-up scan.
+
+Labels come from the plain search itself: a search over every list gives each query's exact
+top-k (the plain path's arithmetic), hence the k-th distance; the minimal nprobe is the smallest
+p such that at least ceil(target * k) of those neighbours lie in the query's p nearest lists
+(a search over those p lists returns all of them: they are the best candidates it sees).
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import HV_PRECISION_BF16
+
+
+def id_to_list(index):
+    """list number of every vector id."""
+    off = index.list_offsets()
+    ids = index.list_ids()
+    out = np.empty(int(ids.max()) + 1 if ids.size else 0, dtype=np.int32)
+    for l in range(len(off) - 1):
+        out[ids[off[l]:off[l + 1]]] = l
+    return out
+
+
+def collect_traces(index, queries, k, target_recall=0.95, chunk=10000):
+    """(features [n][65], targets [n][2], exact ids [n][k]) of the plain search: targets are
+    log1p(k-th distance) and log(minimal nprobe for target_recall)."""
+    q = np.ascontiguousarray(queries, dtype=np.float32)
+    n = q.shape[0]
+    nlist = index.nlist
+    saved = (index.nprobe, index._params.learned, index._params.precision)
+    list_of = id_to_list(index)
+    need = int(math.ceil(target_recall * k))
+    feats = np.empty((n, _lib.HV_MLP_IN), np.float32)
+    targets = np.empty((n, 2), np.float32)
+    exact = np.empty((n, k), np.int64)
+    try:
+        index.nprobe = nlist
+        index.preSearch(learned=False, precision=HV_PRECISION_BF16)
+        for c0 in range(0, n, chunk):
+            qc = q[c0:c0 + chunk]
+            m = qc.shape[0]
+            feats[c0:c0 + m] = index.query_features(qc)
+            D, I = index.search(qc, k)
+            probes = index.last_probes(m)[:, :nlist]
+            rank = np.empty((m, nlist), np.int32)
+            np.put_along_axis(rank, probes, np.arange(nlist, dtype=np.int32)[None, :].repeat(m, 0), axis=1)
+            valid = I >= 0
+            lists = list_of[np.where(valid, I, 0)]
+            r = np.where(valid, np.take_along_axis(rank, lists, axis=1), nlist)
+            r.sort(axis=1)
+            nmin = np.minimum(r[:, need - 1] + 1, nlist)
+            targets[c0:c0 + m, 0] = np.log1p(np.maximum(D[:, k - 1], 0.0))
+            targets[c0:c0 + m, 1] = np.log(nmin.astype(np.float64))
+            exact[c0:c0 + m] = I
+    finally:
+        index.nprobe = saved[0]
+        index.preSearch(learned=bool(saved[1]), precision=saved[2])
+    return feats, targets, exact
+
+
+def standardization(feats, targets):
+    """(fmean, fstd, tmean, tstd) of a training set (std floored so constant columns pass)."""
+    f = np.asarray(feats, np.float64)
+    t = np.asarray(targets, np.float64)
+    return (f.mean(0), np.maximum(f.std(0), 1e-3), t.mean(0), np.maximum(t.std(0), 1e-3))
+
+
+def train(index, feats, targets, epochs=30, batch=1024, lr=2e-3, seed=0):
+    """Fresh network on the index, trained on the GPU; returns per-epoch losses."""
+    index.mlp_set_norm(*standardization(feats, targets))
+    index.mlp_init(seed)
+    return index.mlp_train(feats, targets, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+
+
+def recall_per_query(I, exact):
+    k = exact.shape[1]
+    hit = np.zeros(I.shape[0], np.float64)
+    for row in range(I.shape[0]):
+        hit[row] = np.intersect1d(I[row], exact[row][exact[row] >= 0]).size
+    return hit / k
+
+
+def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.95, margin=0.01,
+              bound_quantile=0.98, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0)):
+    """Serving scales from held-out queries: the bound scale lets bound_quantile of the queries
+    keep their seeded bound (the rest fall back), the nprobe scale is the smallest of `scales`
+    whose mean recall against the exact ids reaches target_recall + margin. Leaves the index in
+    learned mode with those scales; returns (nprobe_scale, bound_scale, recall, mean probes)."""
+    pred = index.mlp_predict(feats)
+    ratio = np.expm1(targets[:, 0].astype(np.float64)) / np.maximum(np.expm1(pred[:, 0].astype(np.float64)), 1e-30)
+    bound_scale = float(max(1.0, np.quantile(ratio, bound_quantile)))
+    index.nprobe = nprobe
+    best = None
+    for s in scales:
+        index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale)
+        _, I = index.search(queries, k)
+        rec = float(recall_per_query(I, exact).mean())
+        probes = float(index.last_nprobe(queries.shape[0]).mean())
+        best = (s, bound_scale, rec, probes)
+        if rec >= target_recall + margin:
+            break
+    index.preSearch(learned=True, learned_nprobe_scale=best[0], learned_bound_scale=best[1])
+    return best

@@ -80,6 +80,11 @@ typedef struct {
     int32_t tile_shape;       /* list-scan tile shape of the remaining-lists phase: 0 = automatic,
                                  s > 0 = compiled shape s-1 */
     int32_t tile_shape_first; /* same, for the nearest-list phase */
+    /* learned probe/bound predictor (no counterpart in the reference; needs a trained model) */
+    int32_t learned;              /* 1: per-query nprobe and pruning bound from the predictor */
+    float learned_nprobe_scale;   /* probes = ceil(predicted nprobe * scale), at most nprobe */
+    float learned_bound_scale;    /* seeded bound = predicted k-th distance * scale (<= 0: none) */
+    int32_t learned_min_nprobe;   /* lower limit of the per-query probe count */
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */
@@ -100,6 +105,7 @@ typedef struct {
     int64_t work_items;      /* scan work items (list x query tile) */
     double ms_main;          /* the dominant kernel: the scan of the remaining probed lists */
     int64_t main_pairs;      /* (query, vector) pairs of that kernel (unpruned) */
+    int64_t fallback_queries;/* learned search: queries redone on the plain path */
 } hv_stats;
 
 /* ---- lifetime (Index::Index, Index.h:15-25) ---- */
@@ -157,6 +163,38 @@ int hv_get_list_ids(const hv_index_t* h, int64_t* out);      /* ntotal, list ord
 int hv_get_list_codes(const hv_index_t* h, float* out);      /* ntotal*d, list order, caller dims */
 int hv_get_last_probes(const hv_index_t* h, int64_t* out);   /* n*nprobe of the last search */
 int hv_get_last_bounds(const hv_index_t* h, float* out);     /* n bounds of the last search */
+/* per-query probe counts of the last search (nprobe each, unless the predictor trimmed them),
+ * and the probe-list stride of hv_get_last_probes (nprobe, or at least 64 with the predictor) */
+int hv_get_last_nprobe(const hv_index_t* h, int32_t* out);
+int64_t hv_last_probe_stride(const hv_index_t* h);
+
+/* ---- learned probe/bound predictor (no counterpart in the reference; the reference always
+ * probes nprobe lists and takes its bound from a warm-up scan).
+ * A 65 -> 128 (ReLU) -> 2 MLP. Features of a query (HV_MLP_IN): log1p of its squared
+ * distances to its 64 nearest centroids, ascending, then log1p(|q|^2). Targets (raw, HV_MLP_OUT):
+ * log1p(k-th neighbour distance), log(nprobe needed for the target recall). The network maps
+ * standardized features to standardized targets (hv_mlp_set_norm). Parameters, flat FP32
+ * (HV_MLP_PARAMS): W1 [65][128], b1 [128], W2 [128][2], b2 [2]. Training (FP32, Adam) and
+ * gradients run on the GPU; prediction and serving use BF16 weights with FP32 accumulation.
+ * All buffers are host memory. */
+#define HV_MLP_IN 65
+#define HV_MLP_HIDDEN 128
+#define HV_MLP_OUT 2
+#define HV_MLP_PARAMS (HV_MLP_IN * HV_MLP_HIDDEN + HV_MLP_HIDDEN + HV_MLP_HIDDEN * HV_MLP_OUT + HV_MLP_OUT)
+int hv_query_features(hv_index_t* h, int64_t n, const float* q, float* feat);  /* feat [n][HV_MLP_IN] */
+/* norm: feature mean [65], feature std [65], target mean [2], target std [2] */
+int hv_mlp_set_norm(hv_index_t* h, const float* norm);
+int hv_mlp_get_norm(const hv_index_t* h, float* norm);
+int hv_mlp_init(hv_index_t* h, uint64_t seed); /* He-uniform weights, zero biases */
+int hv_mlp_set_params(hv_index_t* h, const float* params);
+int hv_mlp_get_params(const hv_index_t* h, float* params);
+/* loss 1/(2n) sum (y - t)^2 over standardized outputs/targets, and its gradient (HV_MLP_PARAMS) */
+int hv_mlp_grad(hv_index_t* h, int64_t n, const float* feat, const float* target, float* grad, float* loss);
+/* Adam (lr, beta 0.9 / 0.999, eps 1e-8) over epochs of shuffled minibatches; epoch_loss
+ * [epochs] (may be NULL) gets each epoch's mean minibatch loss */
+int hv_mlp_train(hv_index_t* h, int64_t n, const float* feat, const float* target, int32_t epochs, int32_t batch,
+                 float lr, uint64_t seed, float* epoch_loss);
+int hv_mlp_predict(hv_index_t* h, int64_t n, const float* feat, float* out); /* out [n][2], raw targets */
 
 /* ---- device memory plumbing for callers without a GPU framework ---- */
 int hv_device_count(void);

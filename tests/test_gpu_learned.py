@@ -1,0 +1,194 @@
+"""GPU tests of the learned probe/bound predictor: the library's forward, gradient, Adam and BF16
+serving kernels against the CPU oracle (oracle/mlp_oracle.py), the query features against the
+oracle's coarse distances, the learned search's results against the oracle at each query's
+probe count, and what the trained network gains over the plain path on the 1M workload."""
+import sys
+
+import numpy as np
+import pytest
+
+from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP32, datagen, learned
+from harmony_amd.index import Index
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test learned]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_model(seed):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.15, M.NPARAMS).astype(np.float32)
+    norm = (rng.normal(5, 1, M.IN).astype(np.float32), rng.uniform(0.5, 2.0, M.IN).astype(np.float32),
+            rng.normal(3, 1, M.OUT).astype(np.float32), rng.uniform(0.5, 2.0, M.OUT).astype(np.float32))
+    return p, norm
+
+
+def rand_batch(seed, n):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(5, 2, (n, M.IN)).astype(np.float32), rng.normal(3, 1.5, (n, M.OUT)).astype(np.float32))
+
+
+@pytest.fixture(scope="module")
+def w1():
+    say("workload 1: 10000 x 128, nlist 100")
+    xb, xq = datagen.make(1)
+    g = Index(128, 100, nprobe=10)
+    g.train(xb, niter=10, seed=0, max_points_per_centroid=256)
+    g.add(xb)
+    return xb, xq, g
+
+
+@pytest.mark.parametrize("seed,n", [(0, 1), (1, 64), (2, 300)])
+def test_gradient_matches_oracle(w1, seed, n):
+    _, _, g = w1
+    p, norm = rand_model(seed)
+    X, T = rand_batch(seed + 10, n)
+    g.mlp_set_norm(*norm)
+    g.mlp_set_params(p)
+    loss, grad = g.mlp_grad(X, T)
+    normd = tuple(a.astype(np.float64) for a in norm)
+    oloss, ograd = M.loss_grad(p.astype(np.float64), normd, X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - oloss) <= 1e-5 * max(1.0, oloss)
+    scale = np.abs(ograd).max()
+    np.testing.assert_allclose(grad, ograd, rtol=1e-4, atol=1e-6 * scale)
+
+
+def test_predict_bf16_matches_oracle(w1):
+    _, _, g = w1
+    p, norm = rand_model(4)
+    X, _ = rand_batch(5, 513)
+    g.mlp_set_norm(*norm)
+    g.mlp_set_params(p)
+    out = g.mlp_predict(X)
+    normd = tuple(a.astype(np.float64) for a in norm)
+    ref = M.predict_raw_bf16(p, normd, X)
+    np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-4)
+    # the BF16 serving weights differ from FP32 ones: the FP32 prediction is measurably different
+    assert np.abs(M.predict_raw(p, normd, X) - ref).max() > 1e-4
+
+
+def test_adam_step_and_training_match_oracle(w1):
+    _, _, g = w1
+    p, norm = rand_model(6)
+    normd = tuple(a.astype(np.float64) for a in norm)
+    X, T = rand_batch(7, 256)
+    # one step over the whole set: the oracle's Adam step from the library's own gradient
+    g.mlp_set_norm(*norm)
+    g.mlp_set_params(p)
+    _, grad = g.mlp_grad(X, T)
+    g.mlp_train(X, T, epochs=1, batch=256, lr=1e-3, seed=3)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    gd = grad.astype(np.float64)
+    m, v = (1 - b1) * gd, (1 - b2) * gd * gd
+    step = p.astype(np.float64) - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    np.testing.assert_allclose(g.mlp_get_params(), step, rtol=1e-5, atol=1e-7)
+    # several epochs of minibatches: the loss trajectory of the oracle's schedule
+    g.mlp_set_params(p)
+    losses = g.mlp_train(X, T, epochs=6, batch=32, lr=3e-3, seed=5)
+    _, olosses = M.train_adam(p.astype(np.float64), normd, X.astype(np.float64), T.astype(np.float64), epochs=6,
+                              batch=32, lr=3e-3, seed=5)
+    np.testing.assert_allclose(losses, olosses, rtol=2e-3)
+    assert losses[-1] < 0.7 * losses[0]
+
+
+def test_query_features_match_oracle(w1):
+    _, xq, g = w1
+    f = g.query_features(xq)
+    _, pd = O.coarse(xq, g.centroids(), 64)
+    ref = M.features_from_coarse(pd, xq)
+    np.testing.assert_allclose(f, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+
+
+@pytest.fixture(scope="module")
+def w1_model(w1):
+    xb, xq, g = w1
+    say("w1: traces of 3000 training queries")
+    _, xt = datagen.make(1, nq=3000, query_set=1)
+    feats, targets, exact = learned.collect_traces(g, xt, k=10)
+    losses = learned.train(g, feats, targets, epochs=30, batch=256, lr=3e-3, seed=0)
+    say("w1 predictor loss %.4f -> %.4f" % (losses[0], losses[-1]))
+    assert losses[-1] < losses[0]
+    return feats, targets, exact
+
+
+@pytest.mark.parametrize("precision", [HV_PRECISION_FP32, HV_PRECISION_BF16])
+def test_learned_without_trimming_equals_plain(w1, w1_model, precision):
+    """Every query keeps all nprobe lists; the seeded bound and the fallback leave the results
+    exactly the plain path's."""
+    xb, xq, g = w1
+    g.nprobe = 10
+    g.preSearch(learned=False, precision=precision, dim_blocks=4)
+    D0, I0 = g.search(xq, 10)
+    for bscale in (1.0, 1e-6, 0.0):  # predicted bound, a bound far too low (all fall back), none
+        g.preSearch(learned=True, learned_nprobe_scale=1e6, learned_bound_scale=bscale)
+        D1, I1, st = g.search(xq, 10, return_stats=True)
+        np.testing.assert_array_equal(I1, I0)
+        np.testing.assert_array_equal(D1, D0)
+        assert np.all(g.last_nprobe(xq.shape[0]) == 10)
+        if bscale == 1e-6:
+            assert st["fallback_queries"] == xq.shape[0]
+        if bscale == 0.0:
+            assert st["fallback_queries"] == 0
+    g.preSearch(learned=False, precision=HV_PRECISION_FP32)
+
+
+@pytest.mark.parametrize("precision", [HV_PRECISION_FP32, HV_PRECISION_BF16])
+def test_learned_per_query_probes_match_oracle(w1, w1_model, precision):
+    """Trimmed probe lists: each query's results are the oracle's at that query's probe count."""
+    xb, xq, g = w1
+    o = O.IndexData.from_gpu_index(g)
+    g.nprobe = 10
+    g.preSearch(learned=True, learned_nprobe_scale=1.0, learned_bound_scale=1.2, precision=precision,
+                dim_blocks=4)
+    D, I, st = g.search(xq, 10, return_stats=True)
+    qn = g.last_nprobe(xq.shape[0])
+    say("w1 learned probes: mean %.2f, min %d, max %d, fallbacks %d" % (qn.mean(), qn.min(), qn.max(),
+                                                                          st["fallback_queries"]))
+    assert qn.min() >= 1 and qn.max() <= 10 and qn.mean() < 10
+    for p in np.unique(qn):
+        sel = np.nonzero(qn == p)[0]
+        oD, oI, _, _ = O.search(o, xq[sel], 10, nprobe=int(p), dim_blocks=4)
+        np.testing.assert_array_equal(I[sel], oI)
+        np.testing.assert_array_equal(D[sel], oD)
+    g.preSearch(learned=False, precision=HV_PRECISION_FP32)
+
+
+def test_learned_gain_on_1m(tmp_path):
+    """What the trained network gains over the method without it: on held-out queries of the
+    1M workload, recall@100 >= 0.95 against the exact neighbours with far fewer candidates than
+    the plain path at nprobe 16."""
+    say("workload 2: 1,000,000 x 128, nlist 1024")
+    xb, xq = datagen.make(2)
+    g = Index(128, 1024, nprobe=16)
+    g.train(xb, niter=10, seed=0, max_points_per_centroid=256)
+    g.add(xb)
+    del xb
+    say("w2: traces of 50,000 training queries and the 10,000 held-out ones")
+    _, xt = datagen.make(2, nb=1, nq=50000, query_set=1)
+    feats, targets, _ = learned.collect_traces(g, xt, k=100)
+    vf, vt, vexact = learned.collect_traces(g, xq, k=100)
+    losses = learned.train(g, feats, targets, epochs=20, batch=1024, lr=2e-3, seed=0)
+    say("w2 predictor loss %.4f -> %.4f" % (losses[0], losses[-1]))
+    # calibrate on the first half of the held-out set, judge on the second half
+    h = xq.shape[0] // 2
+    s, bs, rec_cal, probes_cal = learned.calibrate(g, vf[:h], vt[:h], xq[:h], vexact[:h], nprobe=16, k=100)
+    say("w2 calibration: nprobe scale %.2f, bound scale %.3f, recall %.4f, probes %.2f" % (s, bs, rec_cal,
+                                                                                         probes_cal))
+    g.preSearch(learned=False, precision=HV_PRECISION_BF16)
+    g.nprobe = 16
+    _, I0, st0 = g.search(xq[h:], 100, return_stats=True)
+    g.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bs)
+    _, I1, st1 = g.search(xq[h:], 100, return_stats=True)
+    r0 = learned.recall_per_query(I0, vexact[h:]).mean()
+    r1 = learned.recall_per_query(I1, vexact[h:]).mean()
+    say("w2 held-out: plain recall %.4f candidates %d ms %.2f | learned recall %.4f candidates %d ms %.2f "
+        "fallbacks %d" % (r0, st0["candidates"], st0["ms_total"], r1, st1["candidates"], st1["ms_total"],
+                          st1["fallback_queries"]))
+    assert r1 >= 0.95
+    assert st1["candidates"] < 0.7 * st0["candidates"]
+    g.preSearch(learned=False)
