@@ -175,7 +175,7 @@ struct hv_index {
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats, s_pdist, s_pair_slot, s_aux, s_qres, s_runs,
-        s_qn, s_feat, s_pred, s_seed, s_fb, s_fbq, s_fbD, s_fbI, s_save;
+        s_qn, s_feat, s_pred, s_seed, s_fb, s_save;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
 };
@@ -481,13 +481,15 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     float* bound = h->s_bound.as<float>();
     float* bound_ref = h->s_bound_ref.as<float>();
-    if (learned) {
+    // learned, BF16: no bound for the nearest lists (their k-th upper bound gives the safe bound
+    // below, then the seed lowers it); learned, FP32: the seed from the start. No warm-up.
+    if (learned && !bf16) {
         HV_HIP(hipMemcpyAsync(bound_ref, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     } else if (external) {
         HV_HIP(hipMemcpyAsync(bound_ref, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    } else if (wl > 0 && !bf16) {
+    } else if (wl > 0 && !bf16 && !learned) {
         scan_phase(0, (int)wl, (int)wls, 0, 1, stats + 4);
         hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, s);
     } else {
@@ -502,6 +504,18 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
     //    the remaining probed lists
     const int prune = B > 1 ? 1 : 0;
+    // learned: in the BF16 form the bound after the nearest lists is safe (an upper bound of the
+    // k-th distance over the probed lists): keep it for the exact redo, then lower the bound to
+    // the seed. The FP32 form ran the nearest lists with the seed already (its redo has no bound).
+    float* bound_safe = nullptr;
+    auto apply_seed = [&]() {
+        if (!learned || !bf16) return;
+        HV_HIP(h->s_save.ensure(sizeof(float) * n));
+        bound_safe = h->s_save.as<float>();
+        HV_HIP(hipMemcpyAsync(bound_safe, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        hv::launch_min_f32(bound, seed, n, s);
+        HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    };
     if (bf16) {
         // nearest list: vectors below cap go to their own slot unfiltered; the k-th smallest upper
         // bound then lowers the bound and the pairs whose lower bound exceeds it are dropped
@@ -511,6 +525,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
         hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
         HV_CHECK_LAUNCH("compact");
+        apply_seed();
         HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
@@ -521,10 +536,11 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("rerank");
     } else {
         int r_next = 0;
-        if (h->sp.refine_bound && nprobe > 1) {
+        if ((h->sp.refine_bound || learned) && nprobe > 1) {
             scan_phase(0, 1, INT32_MAX, prune, 0, stats);
             hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
             HV_CHECK_LAUNCH("compact");
+            apply_seed();
             r_next = 1;
         }
         scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
@@ -543,10 +559,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(hipStreamSynchronize(s));
     if (novf > 0) {
         const int nseg = hv::exact_segments(h->max_list);
-        HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * novf * nprobe * nseg * k));
+        HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * novf * nprobe_cap * nseg * k));
         hv::launch_query_exact(ovf_list, novf, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
                                h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), probes,
-                               (int)nprobe, qn, nseg, plan, prune, bound, (int)k, h->s_runs.as<unsigned long long>(), D,
+                               (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune, bound, (int)k, h->s_runs.as<unsigned long long>(), D,
                                I, s);
         HV_CHECK_LAUNCH("exact fallback");
     }
@@ -580,53 +596,34 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         }
     }
     if (learned) {
-        // queries whose k-th result is missing or above the seeded bound (a neighbour may have
-        // been pruned) are redone on the plain path: all nprobe lists, warm-up bound
+        // queries whose k-th result is missing or above the bound the seed set (a neighbour may
+        // have been pruned) are redone exactly over the same probe lists, with the safe bound
         HV_HIP(h->s_fb.ensure(sizeof(int) * (n + 1)));
         int* fb_cnt = h->s_fb.as<int>();
         int* fb_list = fb_cnt + 1;
         hv::launch_fill_i32(fb_cnt, 1, 0, s);
-        hv::launch_flag_fallback(D, I, (int)k, seed, n, fb_list, fb_cnt, s);
+        hv::launch_flag_fallback(D, I, (int)k, bound, n, fb_list, fb_cnt, s);
         int nfb = 0;
         HV_HIP(hipMemcpyAsync(&nfb, fb_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
         HV_HIP(hipStreamSynchronize(s));
         if (st) st->fallback_queries += nfb;
         if (nfb > 0) {
-            HV_HIP(h->s_fbq.ensure(sizeof(float) * nfb * d));
-            HV_HIP(h->s_fbD.ensure(sizeof(float) * nfb * k));
-            HV_HIP(h->s_fbI.ensure(sizeof(int64_t) * nfb * k));
-            hv::launch_gather_rows(q_dev, nfb, (int)d, fb_list, h->s_fbq.as<float>(), s);
-            // keep this search's probe lists and bounds for introspection
-            const size_t pbytes = sizeof(int) * n * nprobe, bbytes = sizeof(float) * n;
-            HV_HIP(h->s_save.ensure(pbytes + bbytes));
-            HV_HIP(hipMemcpyAsync(h->s_save.p, probes, pbytes, hipMemcpyDeviceToDevice, s));
-            HV_HIP(hipMemcpyAsync(static_cast<char*>(h->s_save.p) + pbytes, bound_ref, bbytes,
-                                  hipMemcpyDeviceToDevice, s));
-            const hv_search_params keep = h->sp;
-            const double total_before = st ? st->ms_total : 0.0;  // this call's wall time covers the redo
-            h->sp.learned = 0;
-            h->sp.bound_mode = HV_BOUND_WARMUP;
-            try {
-                do_search(h, nfb, h->s_fbq.as<float>(), k, h->s_fbD.as<float>(), h->s_fbI.as<int64_t>(), st);
-            } catch (...) {
-                h->sp = keep;
-                throw;
+            if (!bound_safe) {  // FP32 form, or no nearest-list phase ran: no bound
+                HV_HIP(h->s_save.ensure(sizeof(float) * n));
+                bound_safe = h->s_save.as<float>();
+                int fm;
+                const float fmax = FLT_MAX;
+                std::memcpy(&fm, &fmax, 4);
+                hv::launch_fill_i32(reinterpret_cast<int*>(bound_safe), n, fm, s);
             }
-            h->sp = keep;
-            if (st) st->ms_total = total_before;
-            hv::launch_scatter_results(h->s_fbD.as<float>(), h->s_fbI.as<int64_t>(), fb_list, nfb, (int)k, D, I, s);
-            hv::launch_scatter_const(h->s_qn.as<int>(), fb_list, nfb, (int)nprobe_cap, s);  // they probed nprobe
-            HV_HIP(h->s_probes.ensure(pbytes));
-            HV_HIP(h->s_bound_ref.ensure(bbytes));
-            HV_HIP(hipMemcpyAsync(h->s_probes.p, h->s_save.p, pbytes, hipMemcpyDeviceToDevice, s));
-            HV_HIP(hipMemcpyAsync(h->s_bound_ref.p, static_cast<char*>(h->s_save.p) + pbytes, bbytes,
-                                  hipMemcpyDeviceToDevice, s));
+            const int nseg = hv::exact_segments(h->max_list);
+            HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * nfb * nprobe_cap * nseg * k));
+            hv::launch_query_exact(fb_list, nfb, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
+                                   h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(),
+                                   probes, (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune, bound_safe, (int)k,
+                                   h->s_runs.as<unsigned long long>(), D, I, s);
             HV_CHECK_LAUNCH("fallback");
             HV_HIP(hipStreamSynchronize(s));
-            h->last_n = n;
-            h->last_nprobe = nprobe_cap;
-            h->last_stride = nprobe;
-            h->last_learned = true;
         }
     }
     if (st) st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -684,7 +681,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats,
                       &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres, &h->s_runs, &h->s_qn,
-                      &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_fbq, &h->s_fbD, &h->s_fbI, &h->s_save, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
+                      &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);

@@ -70,13 +70,14 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap,
                              int k, float* D, int64_t* I, int* overflow_list, int* overflow_cnt,
                              unsigned long long* stats, hipStream_t s);
-// exact path for the nql overflowed queries of qlist, lists cut into exact_segments(max list
-// length) segments; runs >= nql * nprobe * nseg * k keys of scratch
+// exact path for the nql overflowed queries of qlist over their probe ranks [0, nranks) (and
+// below qn[q] when given; probe rows are nprobe wide), lists cut into exact_segments(max list
+// length) segments; runs >= nql * nranks * nseg * k keys of scratch
 int exact_segments(int64_t max_list);
 void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
                         const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
-                        int nprobe, const int* qn, int nseg, const ChunkPlan& plan, int prune, const float* bound,
-                        int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s);
+                        int nprobe, int nranks, const int* qn, int nseg, const ChunkPlan& plan, int prune,
+                        const float* bound, int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s);
 
 // --- BF16 precision form (hv_bf16.hip)
 int bf16_ksteps(int d);  // 32-dim MFMA steps per vector (dpad = 32 * ksteps)
@@ -120,13 +121,11 @@ void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf
                       const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
                       float* bound, float* pred, hipStream_t s);
 
-// queries whose k-th result is missing or above its seeded bound -> list / cnt
+// queries whose k-th result is missing or above the bound the search used -> list / cnt
 void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* seed, int64_t n, int* list, int* cnt,
                           hipStream_t s);
-// dst[list[i]] = value, i < n
-void launch_scatter_const(int* dst, const int* list, int64_t n, int value, hipStream_t s);
-void launch_scatter_results(const float* Dsub, const int64_t* Isub, const int* list, int64_t nsub, int k, float* D,
-                            int64_t* I, hipStream_t s);
+// b[i] = min(b[i], c[i])
+void launch_min_f32(float* b, const float* c, int64_t n, hipStream_t s);
 
 // --- FP64 ground truth
 void launch_gt_f64(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,

@@ -267,7 +267,7 @@ namespace hv {
 
 // ---------------------------------------------------------------------------------------
 // Queries the seeded bound may have cut short: their k-th result is missing or lies above the
-// seed (then a true neighbour may have been pruned). They are redone on the plain path.
+// bound the search used (then a true neighbour may have been pruned). They are redone exactly.
 __global__ void k_flag_fallback(const float* __restrict__ D, const int64_t* __restrict__ I, int k,
                                 const float* __restrict__ seed, int64_t n, int* __restrict__ list,
                                 int* __restrict__ cnt) {
@@ -283,37 +283,20 @@ void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* 
                        cnt);
 }
 
-// rows[i] of (Dsub, Isub) -> row list[i] of (D, I)
-__global__ void k_scatter_results(const float* __restrict__ Dsub, const int64_t* __restrict__ Isub,
-                                  const int* __restrict__ list, int64_t nsub, int k, float* __restrict__ D,
-                                  int64_t* __restrict__ I) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nsub * k) return;
-    const int64_t i = e / k, j = e % k;
-    const int64_t q = list[i];
-    D[q * k + j] = Dsub[e];
-    I[q * k + j] = Isub[e];
-}
-
-void launch_scatter_results(const float* Dsub, const int64_t* Isub, const int* list, int64_t nsub, int k, float* D,
-                            int64_t* I, hipStream_t s) {
-    if (nsub <= 0) return;
-    hipLaunchKernelGGL(k_scatter_results, dim3((unsigned)((nsub * k + 255) / 256)), dim3(256), 0, s, Dsub, Isub, list,
-                       nsub, k, D, I);
-}
-
 }  // namespace hv
+
 
 namespace hv {
 
-__global__ void k_scatter_const(int* __restrict__ dst, const int* __restrict__ list, int64_t n, int value) {
+// b[i] = min(b[i], c[i])
+__global__ void k_min_f32(float* __restrict__ b, const float* __restrict__ c, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[list[i]] = value;
+    if (i < n) b[i] = fminf(b[i], c[i]);
 }
 
-void launch_scatter_const(int* dst, const int* list, int64_t n, int value, hipStream_t s) {
+void launch_min_f32(float* b, const float* c, int64_t n, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_scatter_const, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, list, n, value);
+    hipLaunchKernelGGL(k_min_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, c, n);
 }
 
 }  // namespace hv

@@ -744,7 +744,7 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
                                                      const int64_t* __restrict__ list_off,
                                                      const int* __restrict__ list_stride,
                                                      const uint32_t* __restrict__ ids, const int* __restrict__ probes,
-                                                     int nprobe, const int* __restrict__ qn, int nseg,
+                                                     int nprobe, int nranks, const int* __restrict__ qn, int nseg,
                                                      ChunkPlan plan, int prune, const float* __restrict__ bound, int k,
                                                      unsigned long long* __restrict__ runs) {
     extern __shared__ __align__(16) unsigned char s_raw[];
@@ -757,11 +757,21 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
     __shared__ unsigned long long s_sel;
     __shared__ int s_cnt, s_top;
     __shared__ unsigned long long s_kth;
-    const int64_t per_q = (int64_t)nprobe * nseg;
+    const int64_t per_q = (int64_t)nranks * nseg;  // ranks [0, nranks) of rows nprobe wide
     for (int64_t w = blockIdx.x; w < (int64_t)nql * per_q; w += gridDim.x) {
         const int64_t q = qlist[w / per_q];
         const int j = (int)(w % per_q) / nseg, seg = (int)(w % nseg);
         const float b = bound[q];
+        {
+            // an empty segment (rank past the query's probe count, or past the list's end)
+            // contributes an empty run
+            const int l0 = probes[q * nprobe + j];
+            const int64_t len = list_off[l0 + 1] - list_off[l0];
+            if ((qn && j >= qn[q]) || (int64_t)seg * kExactSpan >= len) {
+                for (int i = threadIdx.x; i < k; i += blockDim.x) runs[w * k + i] = HV_KEY_PAD;
+                continue;
+            }
+        }
         __syncthreads();
         for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Qp[q * d + i];
         if (threadIdx.x == 0) {
@@ -772,8 +782,7 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
         __syncthreads();
         const int l = probes[q * nprobe + j];
         const int64_t lb = list_off[l];
-        // a rank past the query's probe count contributes an empty run
-        const int n = (qn && j >= qn[q]) ? 0 : (int)min(list_off[l + 1] - lb, (long long)(seg + 1) * kExactSpan);
+        const int n = (int)min(list_off[l + 1] - lb, (long long)(seg + 1) * kExactSpan);
         const float* X = codes + code_off[l];
         const int ps = list_stride[l];
         for (int t0 = seg * kExactSpan; t0 < n; t0 += blockDim.x) {
@@ -849,15 +858,15 @@ int exact_segments(int64_t max_list) { return (int)std::max<int64_t>(1, (max_lis
 
 void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
                         const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
-                        int nprobe, const int* qn, int nseg, const ChunkPlan& plan, int prune, const float* bound,
-                        int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s) {
+                        int nprobe, int nranks, const int* qn, int nseg, const ChunkPlan& plan, int prune,
+                        const float* bound, int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s) {
     if (nql <= 0) return;
     const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
-    const int64_t pairs = (int64_t)nql * nprobe * nseg;
+    const int64_t pairs = (int64_t)nql * nranks * nseg;
     hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, 65536)), dim3(256), lds, s, qlist, nql,
-                       Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, qn, nseg, plan, prune, bound,
-                       k, runs);
-    hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql, 65536)), dim3(256), 0, s, qlist, nql, nprobe * nseg,
+                       Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nranks, qn, nseg, plan, prune,
+                       bound, k, runs);
+    hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql, 65536)), dim3(256), 0, s, qlist, nql, nranks * nseg,
                        k, runs, D, I);
 }
 

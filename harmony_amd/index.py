@@ -127,7 +127,8 @@ class Index:
         learned (None keeps the current settings): per-query probe count and pruning bound from
         the trained predictor (mlp_*): probes = ceil(predicted nprobe * learned_nprobe_scale),
         capped at nprobe; bound = predicted k-th distance * learned_bound_scale (<= 0: none).
-        Queries whose k-th result lands above that bound are redone on the plain path."""
+        Queries whose k-th result lands above the bound the search used are redone exactly over
+        the same lists with the bound of their nearest list (no seed)."""
         p = self._params
         if precision is not None:
             p.precision = int(precision)

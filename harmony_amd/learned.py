@@ -4,8 +4,10 @@ The predictor (a 65 -> 128 -> 2 MLP run by the library's own HIP kernels) reads 
 distances to its 64 nearest centroids and its norm, and predicts the k-th neighbour distance and
 the number of lists needed to reach a target recall. Serving uses the prediction to trim each
 query's probe list and to seed its pruning bound; a query whose k-th result lands above the
-seeded bound is redone on the plain path. No counterpart in the reference, which probes a fixed
-nprobe for every query and takes its bound from a warm-up scan.
+seeded bound is redone exactly over the same lists without the seed (the plain path's
+arithmetic), and a predicted nprobe at or above the cap falls back to the plain nprobe. No
+counterpart in the reference, which probes a fixed nprobe for every query and takes its bound
+from a warm-up scan.
 
 Labels come from the plain search itself: a search over every list gives each query's exact
 top-k (the plain path's arithmetic), hence the k-th distance; the minimal nprobe is the smallest
@@ -90,7 +92,7 @@ def recall_per_query(I, exact):
 
 
 def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.95, margin=0.01,
-              bound_quantile=0.98, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0)):
+              bound_quantile=0.999, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0)):
     """Serving scales from held-out queries: the bound scale lets bound_quantile of the queries
     keep their seeded bound (the rest fall back), the nprobe scale is the smallest of `scales`
     whose mean recall against the exact ids reaches target_recall + margin. Leaves the index in

@@ -105,7 +105,7 @@ typedef struct {
     int64_t work_items;      /* scan work items (list x query tile) */
     double ms_main;          /* the dominant kernel: the scan of the remaining probed lists */
     int64_t main_pairs;      /* (query, vector) pairs of that kernel (unpruned) */
-    int64_t fallback_queries;/* learned search: queries redone on the plain path */
+    int64_t fallback_queries;/* learned search: queries whose seeded bound was too low, redone exactly */
 } hv_stats;
 
 /* ---- lifetime (Index::Index, Index.h:15-25) ---- */
