@@ -6,10 +6,16 @@ coarse quantizer -> bound -> grouped list scan -> per-query top-k. The workload 
 10,000 queries; the operating point is the smallest nprobe of {16, 64, 256} whose recall@100
 against the FP64 ground truth is >= 0.95.
 
-Forms measured at that nprobe (each returns the same ids and distances):
-  plain-fp32  warm-up bound, FP32 blocked partial sums with pruning on every candidate
-  bf16        BF16 MFMA filter with a certified error bound, exact FP32 re-rank of survivors
-`value` is the fastest form's QPS; every form's QPS is under config.forms.
+Forms measured at that nprobe:
+  plain-fp32    warm-up bound, FP32 blocked partial sums with pruning on every candidate
+  bf16          BF16 MFMA filter with a certified error bound, exact FP32 re-rank of survivors
+                (same ids and distances as plain-fp32)
+  learned-bf16  the bf16 form with each query's probe count trimmed and its bound seeded by the
+                learned predictor (65 -> 128 -> 2 MLP trained on the GPU on traces of 200,000
+                training queries; serving scales calibrated on 10,000 separate validation
+                queries; nprobe capped at the operating point); recall judged on the 10,000
+                evaluation queries against the FP64 ground truth
+`value` is the learned-bf16 QPS (the headline form); every form's QPS is under config.forms.
 
 Prints one JSON line. `roofline` is for the headline form's dominant kernel (the scan of the
 remaining probed lists): algorithmic FLOPs per (query, vector) pair (2d multiply-adds for the
@@ -76,6 +82,8 @@ def main():
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--train-queries", type=int, default=200000)
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build targets a single MI355X (the path shards by "
@@ -83,7 +91,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from harmony_amd import datagen, metrics
+    from harmony_amd import datagen, learned, metrics
     from harmony_amd.index import DeviceArray, Index
 
     k, d, nlist, B = 100, 128, 1024, 4
@@ -125,8 +133,9 @@ def main():
 
     from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP32
 
-    def measure(precision):
-        index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B, precision=precision)
+    def measure(precision, use_learned=False):
+        index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B, precision=precision,
+                        learned=use_learned)
         for _ in range(args.warmup):
             index.search_device(q_dev, nq, k, D_dev, I_dev)
         times, sts = [], []
@@ -141,27 +150,54 @@ def main():
 
     forms = {}
     ids = {}
-    for name, prec, flops_per_dim, peak, pipe, kern in (
-            ("plain-fp32", HV_PRECISION_FP32, 3.0, PEAK_FP32_TFLOPS, "fp32-valu (direct-difference sub+fma)",
+    runs = [("plain-fp32", HV_PRECISION_FP32, False, 3.0, PEAK_FP32_TFLOPS, "fp32-valu (direct-difference sub+fma)",
              "k_list_scan"),
-            ("bf16", HV_PRECISION_BF16, 2.0, PEAK_BF16_TFLOPS, "bf16 mfma (v_mfma_f32_16x16x32_bf16)",
-             "k_bf16_scan")):
-        ms, st, I = measure(prec)
+            ("bf16", HV_PRECISION_BF16, False, 2.0, PEAK_BF16_TFLOPS, "bf16 mfma (v_mfma_f32_16x16x32_bf16)",
+             "k_bf16_scan")]
+    learned_info = None
+    if not args.no_learned:
+        t0 = time.perf_counter()
+        _, xt = datagen.make(2, nb=1, nq=args.train_queries, query_set=1)
+        _, xv = datagen.make(2, nb=1, nq=10000, query_set=2)
+        feats, targets, _ = learned.collect_traces(index, xt, k=k)
+        vf, vt, vexact = learned.collect_traces(index, xv, k=k)
+        del xt
+        t1 = time.perf_counter()
+        losses = learned.train(index, feats, targets, epochs=20, batch=1024, lr=2e-3, seed=0)
+        t2 = time.perf_counter()
+        s_np, s_b, rec_v, probes_v = learned.calibrate(index, vf, vt, xv, vexact, nprobe=nprobe, k=k)
+        learned_info = {"train_queries": int(feats.shape[0]), "trace_s": round(t1 - t0, 2),
+                        "train_s": round(t2 - t1, 2), "epochs": 20, "loss_first": float(losses[0]),
+                        "loss_last": float(losses[-1]), "nprobe_scale": s_np, "bound_scale": round(s_b, 4),
+                        "validation_recall": round(rec_v, 5), "validation_mean_probes": round(probes_v, 3)}
+        log("predictor: traces %.1fs, training %.1fs (loss %.4f -> %.4f), calibration: nprobe scale %.2f, "
+            "bound scale %.3f, validation recall %.4f, mean probes %.2f"
+            % (t1 - t0, t2 - t1, losses[0], losses[-1], s_np, s_b, rec_v, probes_v))
+        runs.append(("learned-bf16", HV_PRECISION_BF16, True, 2.0, PEAK_BF16_TFLOPS,
+                     "bf16 mfma (v_mfma_f32_16x16x32_bf16)", "k_bf16_scan"))
+    for name, prec, use_learned, flops_per_dim, peak, pipe, kern in runs:
+        ms, st, I = measure(prec, use_learned)
         ids[name] = I
         flops = flops_per_dim * d * st["main_pairs"]
         ach = flops / (st["ms_main"] / 1000.0) / 1e12 if st["ms_main"] > 0 else 0.0
+        rec = metrics.recall(I, GI) if not args.no_recall else None
         forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 4),
+                       "recall_at_100": rec,
                        "stage_ms": {kk: round(st[kk], 4) for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan",
                                                                     "ms_select", "ms_main")},
                        "candidates": int(st["candidates"]), "main_pairs": int(st["main_pairs"]),
                        "survivors": int(st["survivors"]), "overflow_queries": int(st["overflow_queries"]),
+                       "fallback_queries": int(st["fallback_queries"]),
                        "roofline": {"bound": "mfma", "pipe": pipe, "kernel": kern, "achieved": round(ach, 3),
                                     "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 5),
                                     "algorithmic_flops_per_launch": flops, "kernel_ms": round(st["ms_main"], 4)}}
-        log("%s: %.3f ms/step, %.0f QPS, main kernel %.3f ms (%.1f TFLOP/s)"
-            % (name, ms, nq / (ms / 1000.0), st["ms_main"], ach))
+        if use_learned:
+            forms[name]["mean_probes"] = round(float(index.last_nprobe(nq).mean()), 3)
+        log("%s: %.3f ms/step, %.0f QPS, recall %s, main kernel %.3f ms (%.1f TFLOP/s)"
+            % (name, ms, nq / (ms / 1000.0), rec, st["ms_main"], ach))
+    index.preSearch(learned=False)
     same = bool(np.array_equal(ids["plain-fp32"], ids["bf16"]))
-    best = max(forms, key=lambda f: forms[f]["qps"])
+    best = "learned-bf16" if "learned-bf16" in forms else "bf16"
     ms = forms[best]["ms_per_step"]
     qps = forms[best]["qps"]
 
@@ -190,9 +226,10 @@ def main():
                                "B=4 dimension blocks, pruning on",
                    "form": best,
                    "nprobe": nprobe, "nb": int(xb.shape[0]), "nq": nq,
-                   "recall_at_100": recall_at.get(nprobe), "recall_sweep": recall_at,
+                   "recall_at_100": forms[best]["recall_at_100"], "recall_sweep_plain": recall_at,
                    "forms": forms,
-                   "forms_identical_results": same,
+                   "bf16_identical_to_plain": same,
+                   "predictor": learned_info,
                    "speedup_vs_plain_gpu": round(qps / forms["plain-fp32"]["qps"], 3)},
         "roofline": roof,
         "cpu_baseline": base,
