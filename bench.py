@@ -17,10 +17,13 @@ Forms measured at that nprobe:
                 evaluation queries against the FP64 ground truth
 `value` is the learned-bf16 QPS (the headline form); every form's QPS is under config.forms.
 
-Prints one JSON line. `roofline` is for the headline form's dominant kernel (the scan of the
-remaining probed lists): algorithmic FLOPs per (query, vector) pair (2d multiply-adds for the
-BF16 dot product, 3d for the FP32 subtract + multiply-add) times the kernel's unpruned pair
-count, divided by the kernel's duration measured with HIP events on the library's stream.
+Prints one JSON line. `roofline` is for the headline form's dominant kernel, the longer of
+  - the scan of the remaining probed lists: algorithmic FLOPs per (query, vector) pair (2d
+    multiply-adds for the BF16 dot product, 3d for the FP32 subtract + multiply-add) times the
+    kernel's unpruned pair count, against the MFMA (BF16) or FP32 peak;
+  - the exact re-rank of the BF16 forms: algorithmic bytes per re-ranked pair (the FP32 row,
+    4d, plus its id, 4, and the key read and written, 16) against the HBM peak;
+each divided by the kernel's duration measured with HIP events on the library's stream.
 `cpu_baseline` is the oracle's restatement of the reference CPU scan (built with the
 reference's Release flags), timed on this host on a bounded query sample.
 """
@@ -39,6 +42,28 @@ METRIC = "QPS at recall@100≥0.95, 1M×128 fp32: learned+BF16 vs plain GPU vs C
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summaries under
+    profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        kern, val = None, None
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith("traffic_kernel:"):
+                    kern = line.split(":", 1)[1].strip()
+                elif line.startswith("traffic_bytes_per_launch:"):
+                    val = int(line.split(":", 1)[1].strip())
+        if kern == kernel and val:
+            return {"bytes_per_launch": val, "source": "profiles/" + name}
+    return None
 
 
 def log(*a):
@@ -84,6 +109,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--train-queries", type=int, default=200000)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run 3 device-to-device copies of this size (a copy kernel of known traffic, "
+                         "for scaling HBM counters in a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build targets a single MI355X (the path shards by "
@@ -92,7 +120,16 @@ def main():
         sys.exit(2)
 
     from harmony_amd import datagen, learned, metrics
+    from harmony_amd._lib import lib
     from harmony_amd.index import DeviceArray, Index
+
+    if args.copy_calibration_mb:
+        nbytes = args.copy_calibration_mb << 20
+        src, dst = DeviceArray(nbytes), DeviceArray(nbytes)
+        for _ in range(3):
+            assert lib().hv_memcpy_dtod(dst.ptr, src.ptr, nbytes) == 0
+        log("copy calibration: 3 device-to-device copies of %d MiB" % args.copy_calibration_mb)
+        del src, dst
 
     k, d, nlist, B = 100, 128, 1024, 4
     t0 = time.perf_counter()
@@ -180,6 +217,15 @@ def main():
         ids[name] = I
         flops = flops_per_dim * d * st["main_pairs"]
         ach = flops / (st["ms_main"] / 1000.0) / 1e12 if st["ms_main"] > 0 else 0.0
+        roof = {"bound": "mfma", "pipe": pipe, "kernel": kern, "achieved": round(ach, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(ach / peak, 5), "algorithmic_flops_per_launch": flops,
+                "kernel_ms": round(st["ms_main"], 4)}
+        if st["ms_rerank"] > st["ms_main"]:
+            rbytes = (4.0 * d + 20.0) * st["rerank_pairs"]
+            gbs = rbytes / (st["ms_rerank"] / 1000.0) / 1e9
+            roof = {"bound": "hbm", "kernel": "k_rerank", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 5), "algorithmic_bytes_per_launch": rbytes,
+                    "kernel_ms": round(st["ms_rerank"], 4), "pairs": int(st["rerank_pairs"])}
         rec = metrics.recall(I, GI) if not args.no_recall else None
         forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 4),
                        "recall_at_100": rec,
@@ -188,16 +234,25 @@ def main():
                        "candidates": int(st["candidates"]), "main_pairs": int(st["main_pairs"]),
                        "survivors": int(st["survivors"]), "overflow_queries": int(st["overflow_queries"]),
                        "fallback_queries": int(st["fallback_queries"]),
-                       "roofline": {"bound": "mfma", "pipe": pipe, "kernel": kern, "achieved": round(ach, 3),
-                                    "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 5),
-                                    "algorithmic_flops_per_launch": flops, "kernel_ms": round(st["ms_main"], 4)}}
+                       "rerank_ms": round(st["ms_rerank"], 4),
+                       "roofline": roof}
         if use_learned:
             forms[name]["mean_probes"] = round(float(index.last_nprobe(nq).mean()), 3)
-        log("%s: %.3f ms/step, %.0f QPS, recall %s, main kernel %.3f ms (%.1f TFLOP/s)"
-            % (name, ms, nq / (ms / 1000.0), rec, st["ms_main"], ach))
-    index.preSearch(learned=False)
+        log("%s: %.3f ms/step, %.0f QPS, recall %s, main scan %.3f ms (%.1f TFLOP/s), re-rank %.3f ms; "
+            "dominant %s %.1f%% of peak" % (name, ms, nq / (ms / 1000.0), rec, st["ms_main"], ach, st["ms_rerank"],
+                                           roof["kernel"], 100 * roof["frac"]))
     same = bool(np.array_equal(ids["plain-fp32"], ids["bf16"]))
     best = "learned-bf16" if "learned-bf16" in forms else "bf16"
+    # secondary figure: the host-buffer C-ABI call (queries and results crossing PCIe), headline form
+    index.preSearch(learned=best == "learned-bf16", precision=HV_PRECISION_BF16)
+    index.search(xq, k)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        index.search(xq, k)
+    h2h_ms = 1000.0 * (time.perf_counter() - t0) / 3
+    forms[best]["host_to_host_qps"] = round(nq / (h2h_ms / 1000.0), 1)
+    log("%s host-to-host (hv_search, PCIe included): %.3f ms, %.0f QPS" % (best, h2h_ms, nq / (h2h_ms / 1000.0)))
+    index.preSearch(learned=False)
     ms = forms[best]["ms_per_step"]
     qps = forms[best]["qps"]
 
@@ -208,7 +263,10 @@ def main():
         log("cpu baseline %.1f qps (%s) in %.1fs" % (base["value"], base["sample"], time.perf_counter() - t0))
 
     roof = dict(forms[best]["roofline"])
-    roof["traffic"] = None
+    tr = profiled_traffic(roof["kernel"])
+    roof["traffic"] = tr["bytes_per_launch"] if tr else None
+    if tr:
+        roof["traffic_source"] = tr["source"]
     out = {
         "metric": METRIC,
         "value": qps,

@@ -61,7 +61,8 @@ class Stats(ctypes.Structure):
                                                "ms_scan", "ms_select", "ms_download")] + \
                [(n, ctypes.c_int64) for n in ("candidates", "block_evals", "block_evals_max", "survivors",
                                               "overflow_queries", "work_items")] + \
-               [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64), ("fallback_queries", ctypes.c_int64)]
+               [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64), ("fallback_queries", ctypes.c_int64),
+                ("ms_rerank", ctypes.c_double), ("rerank_pairs", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -115,6 +116,7 @@ SIGNATURES = {
     "hv_device_free": (ctypes.c_int, [_P]),
     "hv_memcpy_htod": (ctypes.c_int, [_P, _P, _I64]),
     "hv_memcpy_dtoh": (ctypes.c_int, [_P, _P, _I64]),
+    "hv_memcpy_dtod": (ctypes.c_int, [_P, _P, _I64]),
 }
 
 _lib = None

@@ -384,97 +384,133 @@ int bf16_grid_size(int device) {
 // ---------------------------------------------------------------------------------------
 // After the BF16 scan of each query's nearest list: the k-th smallest upper bound bounds the
 // k-th exact distance, so the query's bound drops to it when tighter; the stored pairs whose
-// lower bound exceeds the new bound cannot be results and are dropped (stable in-place
-// compaction). One wavefront per query; the k-th upper bound by a 4-pass radix select (8-bit
-// digits of the FP32 bits, which order non-negative floats) over a 256-bin LDS histogram.
-// An overflowed query (count > cap) keeps its count: its bound still tightens (the k-th over the
-// stored subset is >= the k-th over all), and the exact path redoes it.
+// lower bound exceeds the new bound cannot be results and are dropped (in-place compaction, order
+// not kept). One workgroup per query holds the query's upper bounds and keys in registers
+// (kCompactPer per thread, so buffers up to 256 * kCompactPer pairs), finds the k-th upper bound
+// by a radix select (8-bit digits of the FP32 bits, which order non-negative floats) over a
+// 256-bin LDS histogram, starting at the first byte where the smallest and largest values differ
+// (all values share the bytes above it). An overflowed query (count > cap) keeps its count: its
+// bound still tightens (the k-th over the stored subset is >= the k-th over all), and the exact
+// path redoes it.
+constexpr int kCompactPer = kMaxSelectRow / 256;
+
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
                                                       const float* __restrict__ aux,
                                                       unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
                                                       float* __restrict__ bound) {
-    __shared__ unsigned int s_hist[4][256];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t q = (int64_t)blockIdx.x * 4 + w;
-    if (q >= nq) return;  // whole wavefront: no block-level barrier below
-    unsigned int* __restrict__ hist = s_hist[w];
+    __shared__ unsigned int s_hist[256];
+    __shared__ unsigned int s_red[2][4];
+    __shared__ unsigned int s_sel[2];
+    __shared__ unsigned int s_out;
+    const int64_t q = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     float b = bound[q];
     unsigned long long* __restrict__ buf = surv + q * cap;
-    const unsigned int* __restrict__ ub = reinterpret_cast<const unsigned int*>(aux + q * cap);
+    const unsigned int* __restrict__ ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
+    unsigned int u[kCompactPer];
+    unsigned long long key[kCompactPer];
+#pragma unroll
+    for (int i = 0; i < kCompactPer; ++i) {
+        const int j = t + 256 * i;
+        unsigned int v = 0xffffffffu;
+        key[i] = HV_KEY_PAD;
+        if (j < n) {
+            v = ubits[j];
+            v = (int)v < 0 ? 0u : v;  // -0.0 orders as 0
+            key[i] = buf[j];
+        }
+        u[i] = v;
+    }
     if (n >= k) {
-        unsigned int prefix = 0u, mask = 0u;
+        unsigned int lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+        for (int i = 0; i < kCompactPer; ++i)
+            if (t + 256 * i < n) {
+                lo = min(lo, u[i]);
+                hi = max(hi, u[i]);
+            }
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, (unsigned int)__shfl_xor((int)lo, o, 64));
+            hi = max(hi, (unsigned int)__shfl_xor((int)hi, o, 64));
+        }
+        if (lane == 0) {
+            s_red[0][w] = lo;
+            s_red[1][w] = hi;
+        }
+        __syncthreads();
+        lo = min(min(s_red[0][0], s_red[0][1]), min(s_red[0][2], s_red[0][3]));
+        hi = max(max(s_red[1][0], s_red[1][1]), max(s_red[1][2], s_red[1][3]));
+        const unsigned int diff = lo ^ hi;
+        const int top = diff ? (31 - __builtin_clz(diff)) / 8 * 8 : -8;  // shift of the first differing byte
+        unsigned int mask = top >= 24 ? 0u : (0xffffffffu << (top + 8));
+        unsigned int prefix = lo & mask;
         int kk = k;  // rank (1-based) still to find inside the current prefix
-        for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int shift = top; shift >= 0; shift -= 8) {
+            s_hist[t] = 0u;
+            __syncthreads();
 #pragma unroll
-            for (int t = 0; t < 4; ++t) hist[lane * 4 + t] = 0u;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            for (int j = lane; j < n; j += 64) {
-                unsigned int u = ub[j];
-                u = (int)u < 0 ? 0u : u;  // -0.0 orders as 0
-                if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            unsigned int h[4], sum = 0u;
+            for (int i = 0; i < kCompactPer; ++i)
+                if (t + 256 * i < n && (u[i] & mask) == prefix) atomicAdd(&s_hist[(u[i] >> shift) & 255u], 1u);
+            __syncthreads();
+            if (w == 0) {
+                unsigned int h[4], sum = 0u;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                h[t] = hist[lane * 4 + t];
-                sum += h[t];
-            }
-            unsigned int incl = sum;  // inclusive prefix over lanes
+                for (int e = 0; e < 4; ++e) {
+                    h[e] = s_hist[lane * 4 + e];
+                    sum += h[e];
+                }
+                unsigned int incl = sum;
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
-                if (lane >= o) incl += y;
-            }
-            const unsigned int excl = incl - sum;
-            // the lane whose bins hold rank kk
-            const bool mine = excl < (unsigned int)kk && (unsigned int)kk <= incl;
-            int digit = 0;
-            unsigned int below = excl;
-            if (mine) {
-                unsigned int run = excl;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+                    if (lane >= o) incl += y;
+                }
+                const unsigned int excl = incl - sum;
+                if (excl < (unsigned int)kk && (unsigned int)kk <= incl) {
+                    unsigned int run = excl;
+                    int digit = lane * 4 + 3;
+                    unsigned int below = run + h[0] + h[1] + h[2];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    if (run + h[t] >= (unsigned int)kk) {
-                        digit = lane * 4 + t;
-                        below = run;
-                        break;
+                    for (int e = 0; e < 4; ++e) {
+                        if (run + h[e] >= (unsigned int)kk) {
+                            digit = lane * 4 + e;
+                            below = run;
+                            break;
+                        }
+                        run += h[e];
                     }
-                    run += h[t];
+                    s_sel[0] = (unsigned int)digit;
+                    s_sel[1] = below;
                 }
             }
-            const unsigned long long who = __ballot(mine);
-            const int src = __builtin_ctzll(who);
-            digit = __shfl(digit, src, 64);
-            below = (unsigned int)__shfl((int)below, src, 64);
-            kk -= (int)below;
-            prefix |= (unsigned int)digit << shift;
+            __syncthreads();
+            kk -= (int)s_sel[1];
+            prefix |= s_sel[0] << shift;
             mask |= 255u << shift;
         }
         b = fminf(b, __uint_as_float(prefix));
     }
     if (c > (unsigned int)cap) {
-        if (lane == 0) bound[q] = b;
+        if (t == 0) bound[q] = b;
         return;
     }
-    // stable in-place compaction of the pairs with lb <= b: each round reads its 64 keys before
-    // writing, and writes land at or before the round's first position
-    int out = 0;
-    for (int j0 = 0; j0 < n; j0 += 64) {
-        const int j = j0 + lane;
-        const unsigned long long key = j < n ? buf[j] : HV_KEY_PAD;
-        const bool keep = j < n && hv_key_dist(key) <= b;
+    // every key is in registers: rewrite the buffer with the pairs whose lower bound is <= b
+    if (t == 0) s_out = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kCompactPer; ++i) {
+        const bool keep = t + 256 * i < n && hv_key_dist(key[i]) <= b;
         const unsigned long long m = __ballot(keep);
-        const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
-        if (keep) buf[out + before] = key;
-        out += __builtin_popcountll(m);
+        unsigned int base = 0u;
+        if (lane == 0 && m) base = atomicAdd(&s_out, (unsigned int)__builtin_popcountll(m));
+        base = (unsigned int)__shfl((int)base, 0, 64);
+        if (keep) buf[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key[i];
     }
-    if (lane == 0) {
-        cnt[q] = (unsigned int)out;
+    __syncthreads();
+    if (t == 0) {
+        cnt[q] = s_out;
         bound[q] = b;
     }
 }
@@ -482,8 +518,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
 void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s) {
     if (nq <= 0) return;
-    hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, aux, cnt, nq, cap, k,
-                       bound);
+    hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -145,7 +145,7 @@ struct hv_index {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     int scan_grid[hv::kNumShapes] = {};
 
     std::vector<int> perm;  // empty = identity
@@ -531,9 +531,11 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
         if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
         // exact FP32 distances of the survivors (the plain path's arithmetic)
+        HV_HIP(hipEventRecord(h->ev[8], s));
         hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
                           (int)cap, s);
         HV_CHECK_LAUNCH("rerank");
+        HV_HIP(hipEventRecord(h->ev[9], s));
     } else {
         int r_next = 0;
         if ((h->sp.refine_bound || learned) && nprobe > 1) {
@@ -593,6 +595,12 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             HV_HIP(hipEventElapsedTime(&mm, h->ev[6], h->ev[7]));
             st->ms_main += mm;
             st->main_pairs += (int64_t)hs[8];
+        }
+        if (bf16) {
+            float mr = 0.f;
+            HV_HIP(hipEventElapsedTime(&mr, h->ev[8], h->ev[9]));
+            st->ms_rerank += mr;
+            st->rerank_pairs += (int64_t)hs[2];  // the select sees exactly the re-ranked pairs
         }
     }
     if (learned) {
@@ -1450,6 +1458,10 @@ int hv_memcpy_htod(void* dst, const void* src, int64_t bytes) {
 }
 int hv_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
     return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess ? HV_OK : HV_ERR_DEVICE;
+}
+
+int hv_memcpy_dtod(void* dst, const void* src, int64_t bytes) {
+    return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice) == hipSuccess ? HV_OK : HV_ERR_DEVICE;
 }
 
 }  // extern "C"

@@ -106,6 +106,8 @@ typedef struct {
     double ms_main;          /* the dominant kernel: the scan of the remaining probed lists */
     int64_t main_pairs;      /* (query, vector) pairs of that kernel (unpruned) */
     int64_t fallback_queries;/* learned search: queries whose seeded bound was too low, redone exactly */
+    double ms_rerank;        /* BF16 form: the exact FP32 re-rank kernel */
+    int64_t rerank_pairs;    /* pairs it re-ranked */
 } hv_stats;
 
 /* ---- lifetime (Index::Index, Index.h:15-25) ---- */
@@ -202,6 +204,7 @@ int hv_device_malloc(void** ptr, int64_t bytes);
 int hv_device_free(void* ptr);
 int hv_memcpy_htod(void* dst, const void* src, int64_t bytes);
 int hv_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
+int hv_memcpy_dtod(void* dst, const void* src, int64_t bytes);
 
 #ifdef __cplusplus
 }
