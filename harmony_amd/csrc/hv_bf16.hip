@@ -522,82 +522,163 @@ void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned in
 }
 
 // ---------------------------------------------------------------------------------------
-// Exact re-rank: every stored pair's key (bound, position) becomes (exact FP32 distance, id), the
-// distance being the plain path's arithmetic (blocked FMA chains in dimension order) on the FP32
-// rows kept in list order. One wavefront per query, one pair per lane; each staging chunk
-// (<= 32 dims) of the row is loaded in one batch, then chained. V4: d and every chunk start are
-// multiples of 4.
+// Flat exact re-rank. A query's stored pairs are cut into work items of 64 (one pair per lane),
+// so a query with many survivors spreads over many wavefronts instead of holding one. Each
+// pair's exact FP32 distance (the plain path's blocked FMA chains in dimension order) is kept
+// only when it does not exceed the query's bound: the bound is the k-th smallest upper bound of
+// the nearest list (or lower), so it is at least the k-th exact distance and no top-k member
+// is dropped. Kept keys (distance, id) are appended per query for the final select.
+//
+// k_rerank_items: one thread per query; lists overflowed queries, clears the kept counters,
+// counts the re-ranked pairs (stats[2]) and reserves its items with one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __restrict__ cnt, int64_t nq, int cap,
+                                                      int2* __restrict__ items, unsigned int* __restrict__ n_items,
+                                                      unsigned int* __restrict__ kept_cnt,
+                                                      int* __restrict__ overflow_list, int* __restrict__ overflow_cnt,
+                                                      unsigned long long* __restrict__ stats) {
+    __shared__ unsigned int s_wsum[4];
+    __shared__ unsigned int s_pairs[4];
+    __shared__ unsigned int s_base;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 256 + t;
+    unsigned int c = 0u;
+    if (q < nq) {
+        c = cnt[q];
+        kept_cnt[q] = 0u;
+        if (c > (unsigned int)cap) {
+            overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
+            c = 0u;
+        }
+    }
+    const unsigned int nch = (c + 63u) / 64u;
+    unsigned int incl = nch, pc = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+        if (lane >= o) incl += y;
+        pc += (unsigned int)__shfl_xor((int)pc, o, 64);
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    if (lane == 0) s_pairs[w] = pc;
+    __syncthreads();
+    if (t == 0) {
+        const unsigned int tot = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        s_base = tot ? atomicAdd(n_items, tot) : 0u;
+        const unsigned long long np = (unsigned long long)s_pairs[0] + s_pairs[1] + s_pairs[2] + s_pairs[3];
+        if (np) atomicAdd(&stats[2], np);
+    }
+    __syncthreads();
+    unsigned int base = s_base + incl - nch;
+    for (int i = 0; i < w; ++i) base += s_wsum[i];
+    for (unsigned int i = 0; i < nch; ++i) items[base + i] = make_int2((int)q, (int)(i * 64u));
+}
+
+// One staging chunk (<= kChunkDims dims from lo) of a row: V4 loads 16 bytes at a time (a
+// float4 starting at or past d only covers dims past the chunk, as d % 4 == 0).
 template <bool V4>
-__global__ __launch_bounds__(256) void k_rerank(const float* __restrict__ Qp, int d,
-                                                const float* __restrict__ rows_by_pos,
-                                                const uint32_t* __restrict__ ids, ChunkPlan plan,
-                                                unsigned long long* __restrict__ surv,
-                                                const unsigned int* __restrict__ cnt, int64_t nq, int cap) {
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= nq) return;
+__device__ __forceinline__ void load_chunk(const float* __restrict__ v, int lo, int d, float (&r)[kChunkDims]) {
+    if (V4) {
+#pragma unroll
+        for (int t = 0; t < kChunkDims / 4; ++t) {
+            const float4 x = *reinterpret_cast<const float4*>(v + min(lo + 4 * t, d - 4));
+            r[4 * t] = x.x;
+            r[4 * t + 1] = x.y;
+            r[4 * t + 2] = x.z;
+            r[4 * t + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < kChunkDims; ++t) r[t] = v[min(lo + t, d - 1)];
+    }
+}
+
+// p += (q - x)^2 over the chunk's len dims, in dimension order
+template <bool V4>
+__device__ __forceinline__ void chain_chunk(const float* __restrict__ qv, int lo, int len, int d,
+                                            const float (&x)[kChunkDims], float& p) {
+    float qr[kChunkDims];
+    load_chunk<V4>(qv, lo, d, qr);
+#pragma unroll
+    for (int t = 0; t < kChunkDims; ++t) {
+        if (t < len) {
+            const float u = qr[t] - x[t];
+            p = __builtin_fmaf(u, u, p);
+        }
+    }
+}
+
+template <bool V4>
+__global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Qp, int d,
+                                                     const float* __restrict__ rows_by_pos,
+                                                     const uint32_t* __restrict__ ids, ChunkPlan plan,
+                                                     const unsigned long long* __restrict__ surv,
+                                                     const unsigned int* __restrict__ cnt, int cap,
+                                                     const int2* __restrict__ items,
+                                                     const unsigned int* __restrict__ n_items,
+                                                     const float* __restrict__ bound,
+                                                     unsigned long long* __restrict__ kept,
+                                                     unsigned int* __restrict__ kept_cnt) {
     const int lane = threadIdx.x & 63;
-    const unsigned int c = cnt[q];
-    if (c > (unsigned int)cap) return;  // overflowed: the exact path handles the query
-    const float* __restrict__ qv = Qp + q * d;
-    unsigned long long* __restrict__ buf = surv + q * cap;
-    for (int j = lane; j < (int)c; j += 64) {
-        const uint32_t pos = (uint32_t)(buf[j] & 0xffffffffu);
-        const uint32_t id = ids[pos];
+    const int nw = (int)gridDim.x * 4;
+    const int ni = (int)*n_items;
+    for (int it = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); it < ni; it += nw) {
+        const int2 item = items[it];
+        const int q = __builtin_amdgcn_readfirstlane(item.x);
+        const int j0 = __builtin_amdgcn_readfirstlane(item.y);
+        const int c = min((int)cnt[q] - j0, 64);
+        const bool valid = lane < c;
+        const uint32_t pos = valid ? (uint32_t)(surv[(int64_t)q * cap + j0 + lane] & 0xffffffffu) : 0u;
         const float* __restrict__ xv = rows_by_pos + (int64_t)pos * d;
+        const float* __restrict__ qv = Qp + (int64_t)q * d;
+        float cur[kChunkDims], nxt[kChunkDims];
+        load_chunk<V4>(xv, plan.lo[0], d, cur);
         float dis = 0.f, p = 0.f;
-        for (int ch = 0; ch < plan.n; ++ch) {
-            const int lo = plan.lo[ch], len = plan.len[ch];
-            float xr[kChunkDims], qr[kChunkDims];
-            if (V4) {
-                // a float4 starting at or past d only covers dims past the chunk (d % 4 == 0)
-#pragma unroll
-                for (int t = 0; t < kChunkDims / 4; ++t) {
-                    const int i = min(lo + 4 * t, d - 4);
-                    const float4 xx = *reinterpret_cast<const float4*>(xv + i);
-                    const float4 qq = *reinterpret_cast<const float4*>(qv + i);
-                    xr[4 * t] = xx.x;
-                    xr[4 * t + 1] = xx.y;
-                    xr[4 * t + 2] = xx.z;
-                    xr[4 * t + 3] = xx.w;
-                    qr[4 * t] = qq.x;
-                    qr[4 * t + 1] = qq.y;
-                    qr[4 * t + 2] = qq.z;
-                    qr[4 * t + 3] = qq.w;
-                }
-            } else {
-#pragma unroll
-                for (int t = 0; t < kChunkDims; ++t) {
-                    const int i = min(lo + t, d - 1);
-                    xr[t] = xv[i];
-                    qr[t] = qv[i];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < kChunkDims; ++t) {
-                if (t < len) {
-                    const float u = qr[t] - xr[t];
-                    p = __builtin_fmaf(u, u, p);
-                }
-            }
+        // the next chunk's loads are in flight during each chain (the last chunk is peeled so
+        // that every loop iteration issues them)
+        const int last = plan.n - 1;
+        for (int ch = 0; ch < last; ++ch) {
+            load_chunk<V4>(xv, plan.lo[ch + 1], d, nxt);
+            chain_chunk<V4>(qv, plan.lo[ch], plan.len[ch], d, cur, p);
             if (plan.ends_block[ch]) {
                 dis += p;
                 p = 0.f;
             }
+#pragma unroll
+            for (int t = 0; t < kChunkDims; ++t) cur[t] = nxt[t];
         }
-        buf[j] = hv_key(dis, id);
+        chain_chunk<V4>(qv, plan.lo[last], plan.len[last], d, cur, p);
+        dis += p;  // the last chunk closes the last block
+        const bool keep = valid && dis <= bound[q];
+        const unsigned long long m = __ballot(keep);
+        if (m) {
+            unsigned int b0 = 0u;
+            if (lane == 0) b0 = atomicAdd(&kept_cnt[q], (unsigned int)__builtin_popcountll(m));
+            b0 = (unsigned int)__shfl((int)b0, 0, 64);
+            if (keep)
+                kept[(int64_t)q * cap + b0 + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = hv_key(dis, ids[pos]);
+        }
     }
 }
 
-void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
-                   unsigned long long* surv, const unsigned int* cnt, int64_t nq, int cap, hipStream_t s) {
+void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,
+                          const ChunkPlan& plan, const unsigned long long* surv, const unsigned int* cnt, int64_t nq,
+                          int cap, int k, const float* bound, int2* items, unsigned int* n_items,
+                          unsigned long long* kept, unsigned int* kept_cnt, float* D, int64_t* I, int* overflow_list,
+                          int* overflow_cnt, unsigned long long* stats, hipStream_t s) {
     if (nq <= 0) return;
     bool v4 = d >= 4 && (d & 3) == 0;
     for (int ch = 0; ch < plan.n; ++ch) v4 = v4 && (plan.lo[ch] & 3) == 0;
-    const dim3 grid((unsigned)((nq + 3) / 4));
+    hipLaunchKernelGGL(k_rerank_items, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, cnt, nq, cap, items,
+                       n_items, kept_cnt, overflow_list, overflow_cnt, stats);
+    const int64_t max_items = nq * ((cap + 63) / 64);
+    const dim3 grid((unsigned)std::min<int64_t>((max_items + 3) / 4, kRerankGrid));
     if (v4)
-        hipLaunchKernelGGL(k_rerank<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, nq, cap);
+        hipLaunchKernelGGL(k_rerank_flat<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, cap,
+                           items, n_items, bound, kept, kept_cnt);
     else
-        hipLaunchKernelGGL(k_rerank<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, nq, cap);
+        hipLaunchKernelGGL(k_rerank_flat<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, cap,
+                           items, n_items, bound, kept, kept_cnt);
+    launch_select_survivors(kept, kept_cnt, nq, cap, k, D, I, nullptr, nullptr, nullptr, s);
 }
 
 }  // namespace hv

@@ -1,0 +1,306 @@
+// Coarse quantizer on the matrix cores with an exact boundary re-check.
+//
+// The P nearest centroids of each query must come out exactly as the FP32 direct-difference
+// chain orders them (acc = fma(q-c, q-c, acc) over the dimensions in order, ties to the lower
+// centroid id): the plain path's probe lists and their distances are compared bit for bit.
+//
+// 1. k_coarse_mfma: d~(q, c) = |q|^2 + |c|^2 - 2 q.c for every (query, centroid), the dot
+//    products on v_mfma_f32_16x16x4f32 (64 x 64 tiles, 32-dim LDS chunks).
+// 2. k_coarse_select, one workgroup per query: with err = kappa (|q|^2 + |c|^2) bounding
+//    |d~ - chain| (host: kappa from d), the P-th smallest upper bound tau = P-th of (d~ + err)
+//    bounds the P-th chain distance, so the P nearest all have d~ - err <= tau. Only those
+//    candidates get the exact chain; their (distance, id) keys are sorted and the first P kept.
+//    A query with more than kCoarseCand candidates is listed for k_coarse_exact_row (all
+//    centroids, exact), which strides over the list on the device.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hv_device.h"
+#include "hv_kernels.h"
+
+namespace hv {
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+constexpr int kCoarseCand = 1024;  // exact re-checks per query in k_coarse_select
+
+// squared norms of rows (any summation order: kappa covers it)
+__global__ void k_row_norms(const float* __restrict__ X, int64_t n, int d, float* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const int lane = threadIdx.x & 63;
+    float s = 0.f;
+    for (int i = lane; i < d; i += 64) {
+        const float v = X[r * d + i];
+        s = __builtin_fmaf(v, v, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[r] = s;
+}
+
+void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_row_norms, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, X, n, d, out);
+}
+
+// 64 queries x 64 centroids per workgroup; wavefront w computes queries 16w..16w+15 against the
+// 64 centroids (four 16 x 16 MFMA tiles). Lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[4(l/16)+i][l%16].
+__global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q, int64_t nq,
+                                                     const float* __restrict__ C, int nc, int d,
+                                                     const float* __restrict__ qn, const float* __restrict__ cn,
+                                                     float* __restrict__ out) {
+    constexpr int KC = 32, LS = KC + 1;  // odd row stride: the 16 rows a K-step reads hit 16 banks
+    __shared__ float Qs[64 * LS];
+    __shared__ float Cs[64 * LS];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int64_t q0 = (int64_t)blockIdx.y * 64;
+    const int c0 = blockIdx.x * 64;
+    floatx4 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < d; k0 += KC) {
+        const int kn = min(KC, d - k0);
+        for (int e = t; e < 64 * KC; e += 256) {
+            const int r = e / KC, kk = e % KC;
+            float v = 0.f, u = 0.f;
+            if (kk < kn) {
+                if (q0 + r < nq) v = Q[(q0 + r) * d + k0 + kk];
+                if (c0 + r < nc) u = C[(int64_t)(c0 + r) * d + k0 + kk];
+            }
+            Qs[r * LS + kk] = v;
+            Cs[r * LS + kk] = u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < KC / 4; ++s) {
+            const float a = Qs[(16 * w + (l & 15)) * LS + 4 * s + (l >> 4)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float b = Cs[(16 * j + (l & 15)) * LS + 4 * s + (l >> 4)];
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = c0 + 16 * j + (l & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t q = q0 + 16 * w + 4 * (l >> 4) + i;
+            if (q < nq && c < nc) out[q * nc + c] = qn[q] + cn[c] - 2.f * acc[j][i];
+        }
+    }
+}
+
+// The exact chain of one (query, centroid) pair: q from LDS, the centroid row from memory.
+__device__ __forceinline__ float coarse_chain(const float* __restrict__ qs, const float* __restrict__ crow, int d) {
+    float acc = 0.f;
+    int i = 0;
+    if ((d & 3) == 0) {
+        for (; i + 4 <= d; i += 4) {
+            const float4 c4 = *reinterpret_cast<const float4*>(crow + i);
+            float t = qs[i] - c4.x;
+            acc = __builtin_fmaf(t, t, acc);
+            t = qs[i + 1] - c4.y;
+            acc = __builtin_fmaf(t, t, acc);
+            t = qs[i + 2] - c4.z;
+            acc = __builtin_fmaf(t, t, acc);
+            t = qs[i + 3] - c4.w;
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    for (; i < d; ++i) {
+        const float t = qs[i] - crow[i];
+        acc = __builtin_fmaf(t, t, acc);
+    }
+    return acc;
+}
+
+constexpr int kSelPer = kMaxSelectRow / 256;  // centroids per thread (nlist <= 8192)
+
+__global__ __launch_bounds__(256) void k_coarse_select(const float* __restrict__ approx, const float* __restrict__ Q,
+                                                       const float* __restrict__ C, int nc, int d,
+                                                       const float* __restrict__ qn, const float* __restrict__ cn,
+                                                       float kappa, int P, int* __restrict__ probes,
+                                                       float* __restrict__ pdist, int* __restrict__ redo_list,
+                                                       int* __restrict__ redo_cnt) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(s_raw);          // [kCoarseCand]
+    float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * kCoarseCand);  // [d]
+    __shared__ unsigned int s_hist[256];
+    __shared__ unsigned int s_sel[2];
+    __shared__ unsigned int s_red[2][4];
+    __shared__ int s_cnt;
+    const int64_t q = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const float* __restrict__ row = approx + q * nc;
+    const float qq = qn[q];
+    // upper bounds (FP32 bits of non-negative values order as unsigned ints)
+    unsigned int ub[kSelPer];
+    float lbv[kSelPer];
+    unsigned int lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+    for (int i = 0; i < kSelPer; ++i) {
+        const int c = t + 256 * i;
+        ub[i] = 0xffffffffu;
+        lbv[i] = 3.402823466e38f;
+        if (c < nc) {
+            const float dt = row[c];
+            const float err = kappa * (qq + cn[c]);
+            const float u = fmaxf(dt + err, 0.f);
+            lbv[i] = dt - err;
+            ub[i] = __float_as_uint(u);
+            if (ub[i] == 0x80000000u) ub[i] = 0u;
+            lo = min(lo, ub[i]);
+            hi = max(hi, ub[i]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (unsigned int)__shfl_xor((int)lo, o, 64));
+        hi = max(hi, (unsigned int)__shfl_xor((int)hi, o, 64));
+    }
+    if (lane == 0) {
+        s_red[0][w] = lo;
+        s_red[1][w] = hi;
+    }
+    for (int i = t; i < d; i += 256) qs[i] = Q[q * d + i];
+    __syncthreads();
+    lo = min(min(s_red[0][0], s_red[0][1]), min(s_red[0][2], s_red[0][3]));
+    hi = max(max(s_red[1][0], s_red[1][1]), max(s_red[1][2], s_red[1][3]));
+    // tau = P-th smallest upper bound (radix select from the first byte where lo and hi differ)
+    const unsigned int diff = lo ^ hi;
+    const int top = diff ? (31 - __builtin_clz(diff)) / 8 * 8 : -8;
+    unsigned int mask = top >= 24 ? 0u : (0xffffffffu << (top + 8));
+    unsigned int prefix = lo & mask;
+    int kk = P;
+    for (int shift = top; shift >= 0; shift -= 8) {
+        s_hist[t] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSelPer; ++i)
+            if (t + 256 * i < nc && (ub[i] & mask) == prefix) atomicAdd(&s_hist[(ub[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (w == 0) {
+            unsigned int h[4], sum = 0u;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                h[e] = s_hist[lane * 4 + e];
+                sum += h[e];
+            }
+            unsigned int incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const unsigned int excl = incl - sum;
+            if (excl < (unsigned int)kk && (unsigned int)kk <= incl) {
+                unsigned int run = excl;
+                int digit = lane * 4 + 3;
+                unsigned int below = run + h[0] + h[1] + h[2];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (run + h[e] >= (unsigned int)kk) {
+                        digit = lane * 4 + e;
+                        below = run;
+                        break;
+                    }
+                    run += h[e];
+                }
+                s_sel[0] = (unsigned int)digit;
+                s_sel[1] = below;
+            }
+        }
+        __syncthreads();
+        kk -= (int)s_sel[1];
+        prefix |= s_sel[0] << shift;
+        mask |= 255u << shift;
+    }
+    const float tau = __uint_as_float(prefix);
+    // the candidates: lower bound <= tau; their exact chain distances as (distance, id) keys
+    if (t == 0) s_cnt = 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSelPer; ++i) {
+        const int c = t + 256 * i;
+        if (c < nc && lbv[i] <= tau) {
+            const int pos = atomicAdd(&s_cnt, 1);
+            if (pos < kCoarseCand) keys[pos] = (unsigned long long)(unsigned int)c;
+        }
+    }
+    __syncthreads();
+    const int m = s_cnt;
+    if (m > kCoarseCand) {  // too wide a band: the exact kernel redoes this query
+        if (t == 0) redo_list[atomicAdd(redo_cnt, 1)] = (int)q;
+        return;
+    }
+    for (int e = t; e < m; e += 256) {
+        const int c = (int)keys[e];
+        keys[e] = hv_key(coarse_chain(qs, C + (int64_t)c * d, d), (uint32_t)c);
+    }
+    const int p2 = hv_next_pow2(max(m, 2));
+    for (int e = m + t; e < p2; e += 256) keys[e] = HV_KEY_PAD;
+    __syncthreads();
+    block_bitonic_sort(keys, p2);
+    for (int r = t; r < P; r += 256) {
+        probes[q * P + r] = (int)hv_key_id(keys[r]);
+        pdist[q * P + r] = hv_key_dist(keys[r]);
+    }
+}
+
+// Fallback: every centroid's exact chain, LDS sort, first P (one workgroup per listed query).
+__global__ __launch_bounds__(256) void k_coarse_exact_row(const int* __restrict__ redo_list,
+                                                          const int* __restrict__ redo_cnt,
+                                                          const float* __restrict__ Q, const float* __restrict__ C,
+                                                          int nc, int d, int P, int* __restrict__ probes,
+                                                          float* __restrict__ pdist) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    const int p2 = hv_next_pow2(max(nc, 2));
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(s_raw);  // [p2]
+    float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * p2);
+    const int cnt = *redo_cnt;
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+        const int64_t q = redo_list[w];
+        __syncthreads();
+        for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Q[q * d + i];
+        __syncthreads();
+        for (int c = threadIdx.x; c < p2; c += blockDim.x)
+            keys[c] = c < nc ? hv_key(coarse_chain(qs, C + (int64_t)c * d, d), (uint32_t)c) : HV_KEY_PAD;
+        __syncthreads();
+        block_bitonic_sort(keys, p2);
+        for (int r = threadIdx.x; r < P; r += blockDim.x) {
+            probes[q * P + r] = (int)hv_key_id(keys[r]);
+            pdist[q * P + r] = hv_key_dist(keys[r]);
+        }
+    }
+}
+
+float coarse_kappa(int d) {
+    // |d~ - chain| <= kappa (|q|^2 + |c|^2): the MFMA dot product (gamma_2d |q||c|), the FP32
+    // norms (gamma_d each), the chain's own error against the real distance (2 gamma_2d of
+    // 2(|q|^2 + |c|^2)), the three roundings of d~; 2% slack for the rounding of this bound.
+    const double u = 1.0 / 16777216.0;
+    return (float)(1.02 * (12.0 * d + 8.0) * u);
+}
+
+void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, const float* qn,
+                        const float* cn, float* approx, hipStream_t s) {
+    if (nq <= 0) return;
+    const dim3 grid((unsigned)((nc + 63) / 64), (unsigned)((nq + 63) / 64));
+    hipLaunchKernelGGL(k_coarse_mfma, grid, dim3(256), 0, s, Q, nq, C, nc, d, qn, cn, approx);
+}
+
+void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
+                          const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
+                          int* redo_list, int* redo_cnt, hipStream_t s) {
+    if (nq <= 0) return;
+    const size_t lds = sizeof(unsigned long long) * kCoarseCand + sizeof(float) * ((d + 3) & ~3);
+    hipLaunchKernelGGL(k_coarse_select, dim3((unsigned)nq), dim3(256), lds, s, approx, Q, C, nc, d, qn, cn, kappa, P,
+                       probes, pdist, redo_list, redo_cnt);
+    const size_t lds2 = sizeof(unsigned long long) * hv_next_pow2(nc > 2 ? nc : 2) + sizeof(float) * ((d + 3) & ~3);
+    hipLaunchKernelGGL(k_coarse_exact_row, dim3((unsigned)std::min<int64_t>(nq, 256)), dim3(256), lds2, s, redo_list,
+                       redo_cnt, Q, C, nc, d, P, probes, pdist);
+}
+
+}  // namespace hv

@@ -174,8 +174,8 @@ struct hv_index {
 
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
-        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_stats, s_pdist, s_pair_slot, s_aux, s_qres, s_runs,
-        s_qn, s_feat, s_pred, s_seed, s_fb, s_save;
+        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_kept_cnt, s_stats, s_pdist, s_pair_slot,
+        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_save, s_qnorm, s_cnorm, s_redo;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
 };
@@ -290,13 +290,35 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
     h->added = true;
 }
 
-// Coarse quantizer: full query x centroid distance rows (in slices of <= 256 MB) and the P
-// nearest lists of each query, ascending (Index.cpp:967-997).
+// Coarse quantizer: the P nearest lists of each query, ascending by the FP32 chain distance,
+// ties to the lower id (Index.cpp:967-997). Up to P = 512 and nlist = 4096 on the matrix cores
+// with an exact re-check of the candidates near the P-th (hv_coarse.hip); otherwise full exact
+// distance rows and a select. Rows in slices of <= 256 MB.
 void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist) {
     const int64_t d = h->d, nlist = h->nlist;
     hipStream_t s = h->stream;
     const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (nlist * 4)));
     HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
+    if (P <= 512 && nlist <= 4096) {
+        HV_HIP(h->s_qnorm.ensure(sizeof(float) * n));
+        HV_HIP(h->s_cnorm.ensure(sizeof(float) * nlist));
+        HV_HIP(h->s_redo.ensure(sizeof(int) * (per + 1)));
+        hv::launch_row_norms(h->d_centroids.as<float>(), nlist, (int)d, h->s_cnorm.as<float>(), s);
+        hv::launch_row_norms(Qp, n, (int)d, h->s_qnorm.as<float>(), s);
+        const float kappa = hv::coarse_kappa((int)d);
+        int* redo_cnt = h->s_redo.as<int>();
+        for (int64_t r0 = 0; r0 < n; r0 += per) {
+            const int64_t rows = std::min(per, n - r0);
+            hv::launch_coarse_mfma(Qp + r0 * d, rows, h->d_centroids.as<float>(), (int)nlist, (int)d,
+                                   h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), h->s_dist.as<float>(), s);
+            hv::launch_fill_i32(redo_cnt, 1, 0, s);
+            hv::launch_coarse_select(h->s_dist.as<float>(), Qp + r0 * d, h->d_centroids.as<float>(), (int)nlist,
+                                     (int)d, h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), kappa, rows, (int)P,
+                                     probes + r0 * P, pdist + r0 * P, redo_cnt + 1, redo_cnt, s);
+        }
+        HV_CHECK_LAUNCH("coarse");
+        return;
+    }
     for (int64_t r0 = 0; r0 < n; r0 += per) {
         const int64_t rows = std::min(per, n - r0);
         hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d, h->s_dist.as<float>(),
@@ -530,12 +552,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
         if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
-        // exact FP32 distances of the survivors (the plain path's arithmetic)
-        HV_HIP(hipEventRecord(h->ev[8], s));
-        hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
-                          (int)cap, s);
-        HV_CHECK_LAUNCH("rerank");
-        HV_HIP(hipEventRecord(h->ev[9], s));
     } else {
         int r_next = 0;
         if ((h->sp.refine_bound || learned) && nprobe > 1) {
@@ -548,13 +564,28 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
     }
     HV_HIP(hipEventRecord(h->ev[4], s));
-    // 6. top-k selection (+ exact path for overflowed buffers)
-    HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 1)));
+    // 6. top-k selection (+ exact path for overflowed buffers). The BF16 form re-ranks its
+    //    survivors (the plain path's FP32 arithmetic) and selects in the same kernel.
+    HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
     int* ovf_cnt = h->s_overflow.as<int>();
-    int* ovf_list = ovf_cnt + 1;
-    hv::launch_fill_i32(ovf_cnt, 1, 0, s);
-    hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, ovf_list, ovf_cnt, stats, s);
-    HV_CHECK_LAUNCH("select");
+    unsigned int* n_items = reinterpret_cast<unsigned int*>(ovf_cnt + 1);
+    int* ovf_list = ovf_cnt + 2;
+    hv::launch_fill_i32(ovf_cnt, 2, 0, s);
+    if (bf16) {
+        HV_HIP(hipEventRecord(h->ev[8], s));
+        HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
+        HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
+        HV_HIP(h->s_kept_cnt.ensure(sizeof(unsigned int) * n));
+        hv::launch_rerank_select(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
+                                 (int)cap, (int)k, bound, h->s_ritems.as<int2>(), n_items,
+                                 h->s_kept.as<unsigned long long>(), h->s_kept_cnt.as<unsigned int>(), D, I, ovf_list,
+                                 ovf_cnt, stats, s);
+        HV_CHECK_LAUNCH("rerank + select");
+        HV_HIP(hipEventRecord(h->ev[9], s));
+    } else {
+        hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, ovf_list, ovf_cnt, stats, s);
+        HV_CHECK_LAUNCH("select");
+    }
     // the overflowed queries (rare) take the exact path, spread over (query, probed list) pairs
     int novf = 0;
     HV_HIP(hipMemcpyAsync(&novf, ovf_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -687,9 +718,10 @@ void hv_destroy(hv_index_t* h) {
     DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
-                      &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_stats,
-                      &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres, &h->s_runs, &h->s_qn,
-                      &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
+                      &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
+                      &h->s_kept, &h->s_kept_cnt, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
+                      &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->s_qnorm,
+                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);

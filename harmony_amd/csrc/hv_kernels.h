@@ -100,8 +100,19 @@ void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_
                          unsigned int* cnt, hipStream_t s);
 void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s);
-void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
-                   unsigned long long* surv, const unsigned int* cnt, int64_t nq, int cap, hipStream_t s);
+// re-rank + final top-k of the BF16 form: work items of 64 stored pairs (items: nq * ceil(cap/64)
+// entries, n_items zeroed by the caller), exact keys at or below bound[q] kept per query
+// (kept: nq * cap keys), then the select over the kept keys. Overflowed queries are listed for
+// the exact path (their results are left as padding); stats[2] counts the re-ranked pairs.
+constexpr int kRerankGrid = 2048;
+void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,
+                          const ChunkPlan& plan, const unsigned long long* surv, const unsigned int* cnt, int64_t nq,
+                          int cap, int k, const float* bound, int2* items, unsigned int* n_items,
+                          unsigned long long* kept, unsigned int* kept_cnt, float* D, int64_t* I, int* overflow_list,
+                          int* overflow_cnt, unsigned long long* stats, hipStream_t s);
+// the block select alone (queries with more than kWaveSortMax survivors)
+void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
+                         float* D, int64_t* I, hipStream_t s);
 
 // --- learned probe/bound predictor (hv_learn.hip)
 constexpr int kNearest = 64;                                            // nearest-centroid distances
@@ -126,6 +137,16 @@ void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* 
                           hipStream_t s);
 // b[i] = min(b[i], c[i])
 void launch_min_f32(float* b, const float* c, int64_t n, hipStream_t s);
+
+// --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
+void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);
+float coarse_kappa(int d);
+void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, const float* qn,
+                        const float* cn, float* approx, hipStream_t s);
+// P nearest per query, exact (distance, id) order; redo_cnt must be 0 on entry
+void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
+                          const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
+                          int* redo_list, int* redo_cnt, hipStream_t s);
 
 // --- FP64 ground truth
 void launch_gt_f64(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,

@@ -167,7 +167,8 @@ void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq
 // ---------------------------------------------------------------------------------------
 // Final top-k of the surviving candidates: the first k keys in (distance, id) order, padded
 // with (FLT_MAX, -1) (heap_init + sort_result, heap.hpp:13-90, merge of Index.cpp:1327-1343).
-// Queries whose buffer overflowed are listed for the exact per-query kernel instead.
+// Queries whose buffer overflowed are listed for the exact per-query kernel instead (a caller
+// whose counts never exceed cap may pass null lists; null stats skip the pair count).
 __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* __restrict__ surv,
                                                      const unsigned int* __restrict__ surv_cnt, int64_t nq, int cap,
                                                      int k, float* __restrict__ D, int64_t* __restrict__ I,
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* _
         if (lane0) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
         return;
     }
-    if (lane0 && n) atomicAdd(&stats[2], (unsigned long long)n);
+    if (stats && lane0 && n) atomicAdd(&stats[2], (unsigned long long)n);
     const int m = max((int)n, k);
     if (m > kWaveSortMax) return;  // k_select_block
     const unsigned long long* __restrict__ src = surv + q * cap;
@@ -209,6 +210,13 @@ __global__ __launch_bounds__(256) void k_select_block(const unsigned long long* 
         D[q * k + i] = hv_key_dist(s_keys[i]);
         I[q * k + i] = hv_key_id(s_keys[i]);
     }
+}
+
+void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
+                         float* D, int64_t* I, hipStream_t s) {
+    if (nq <= 0 || (cap <= kWaveSortMax && k <= kWaveSortMax)) return;
+    const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
+    hipLaunchKernelGGL(k_select_block, dim3((unsigned)nq), dim3(256), lds, s, surv, surv_cnt, cap, k, D, I);
 }
 
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
