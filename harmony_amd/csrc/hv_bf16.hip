@@ -393,6 +393,8 @@ int bf16_grid_size(int device) {
 // bound still tightens (the k-th over the stored subset is >= the k-th over all), and the exact
 // path redoes it.
 constexpr int kCompactPer = kMaxSelectRow / 256;
+constexpr int kCompactWaveMax = 2048;  // one wavefront per query up to here,
+constexpr int kCompactWaveBig = 4096;  // then with 64 values per lane; larger: the workgroup kernel
 
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
                                                       const float* __restrict__ aux,
@@ -406,6 +408,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
+    if (n <= kCompactWaveBig) return;  // k_bf16_compact_wave(_big)
     float b = bound[q];
     unsigned long long* __restrict__ buf = surv + q * cap;
     const unsigned int* __restrict__ ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
@@ -515,10 +518,108 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     }
 }
 
+// The same with one wavefront per query for buffers of at most kCompactWaveMax pairs (upper
+// bounds in registers, the k-th from wave_kth_u32, keys streamed through registers after it).
+
+template <int E>
+__device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restrict__ buf,
+                                                    const unsigned int* __restrict__ ubits, int n, unsigned int c,
+                                                    int cap, int k, float b, unsigned int* __restrict__ cnt_q,
+                                                    float* __restrict__ bound_q, unsigned int* s_tmp) {
+    const int lane = threadIdx.x & 63;
+    if (n >= k) {
+        unsigned int u[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = e * 64 + lane;
+            unsigned int x = 0xffffffffu;
+            if (j < n) {
+                x = ubits[j];
+                x = (int)x < 0 ? 0u : x;  // -0.0 orders as 0
+            }
+            u[e] = x;
+        }
+        b = fminf(b, __uint_as_float(wave_kth_u32<E>(u, k, s_tmp)));
+    }
+    if (c > (unsigned int)cap) {
+        if (lane == 0) *bound_q = b;
+        return;
+    }
+    // the keys pass through registers H per lane at a time: every key of a part is loaded before
+    // any of its kept keys is stored, and stores land at or below the key's own position
+    constexpr int H = E > 32 ? 32 : E;
+    unsigned int out = 0u;
+#pragma unroll
+    for (int h0 = 0; h0 < E; h0 += H) {
+        unsigned long long key[H];
+#pragma unroll
+        for (int e = 0; e < H; ++e) {
+            const int j = (h0 + e) * 64 + lane;
+            key[e] = j < n ? buf[j] : HV_KEY_PAD;
+        }
+#pragma unroll
+        for (int e = 0; e < H; ++e) {
+            const bool keep = (h0 + e) * 64 + lane < n && hv_key_dist(key[e]) <= b;
+            const unsigned long long m = __ballot(keep);
+            if (keep) buf[out + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key[e];
+            out += (unsigned int)__builtin_popcountll(m);
+        }
+    }
+    if (lane == 0) {
+        *cnt_q = out;
+        *bound_q = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* __restrict__ surv,
+                                                           const float* __restrict__ aux,
+                                                           unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
+                                                           float* __restrict__ bound) {
+    __shared__ unsigned int s_tmp[4][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= nq) return;
+    const unsigned int c = cnt[q];
+    const int n = (int)min(c, (unsigned int)cap);
+    if (n > kCompactWaveMax) return;  // k_bf16_compact_wave_big, k_bf16_compact
+    unsigned long long* buf = surv + q * cap;
+    const unsigned int* ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
+    const float b = bound[q];
+    if (n <= 256)
+        bf16_compact_wave_e<4>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+    else if (n <= 512)
+        bf16_compact_wave_e<8>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+    else if (n <= 1024)
+        bf16_compact_wave_e<16>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+    else
+        bf16_compact_wave_e<32>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+}
+
+// buffers of kCompactWaveMax < n <= kCompactWaveBig pairs (64 upper bounds per lane)
+__global__ __launch_bounds__(256) void k_bf16_compact_wave_big(unsigned long long* __restrict__ surv,
+                                                               const float* __restrict__ aux,
+                                                               unsigned int* __restrict__ cnt, int64_t nq, int cap,
+                                                               int k, float* __restrict__ bound) {
+    __shared__ unsigned int s_tmp[4][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= nq) return;
+    const unsigned int c = cnt[q];
+    const int n = (int)min(c, (unsigned int)cap);
+    if (n <= kCompactWaveMax || n > kCompactWaveBig) return;
+    bf16_compact_wave_e<kCompactWaveBig / 64>(surv + q * cap, reinterpret_cast<const unsigned int*>(aux + q * cap), n,
+                                              c, cap, k, bound[q], cnt + q, bound + q, s_tmp[w]);
+}
+
 void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s) {
     if (nq <= 0) return;
-    hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
+    const dim3 wgrid((unsigned)((nq + 3) / 4));
+    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
+    if (cap > kCompactWaveMax)
+        hipLaunchKernelGGL(k_bf16_compact_wave_big, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
+    if (cap > kCompactWaveBig)
+        hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -546,7 +647,7 @@ __global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __rest
         c = cnt[q];
         kept_cnt[q] = 0u;
         if (c > (unsigned int)cap) {
-            overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
+            if (overflow_list) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
             c = 0u;
         }
     }

@@ -6,7 +6,8 @@
 //
 // 1. k_coarse_mfma: d~(q, c) = |q|^2 + |c|^2 - 2 q.c for every (query, centroid), the dot
 //    products on v_mfma_f32_16x16x4f32 (64 x 64 tiles, 32-dim LDS chunks).
-// 2. k_coarse_select, one workgroup per query: with err = kappa (|q|^2 + |c|^2) bounding
+// 2. k_coarse_select_wave (one wavefront per query; k_coarse_select, one workgroup per query,
+//    for wide rows or large P): with err = kappa (|q|^2 + |c|^2) bounding
 //    |d~ - chain| (host: kappa from d), the P-th smallest upper bound tau = P-th of (d~ + err)
 //    bounds the P-th chain distance, so the P nearest all have d~ - err <= tau. Only those
 //    candidates get the exact chain; their (distance, id) keys are sorted and the first P kept.
@@ -93,26 +94,52 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
     }
 }
 
-// The exact chain of one (query, centroid) pair: q from LDS, the centroid row from memory.
-__device__ __forceinline__ float coarse_chain(const float* __restrict__ qs, const float* __restrict__ crow, int d) {
-    float acc = 0.f;
-    int i = 0;
-    if ((d & 3) == 0) {
-        for (; i + 4 <= d; i += 4) {
-            const float4 c4 = *reinterpret_cast<const float4*>(crow + i);
-            float t = qs[i] - c4.x;
-            acc = __builtin_fmaf(t, t, acc);
-            t = qs[i + 1] - c4.y;
-            acc = __builtin_fmaf(t, t, acc);
-            t = qs[i + 2] - c4.z;
-            acc = __builtin_fmaf(t, t, acc);
-            t = qs[i + 3] - c4.w;
-            acc = __builtin_fmaf(t, t, acc);
+// The exact chain of one (query, centroid) pair, acc = fma(q-c, q-c, acc) over the dimensions in
+// order. The centroid row is loaded 32 dims at a time (16-byte loads when d % 4 == 0), the next
+// 32 in flight while the current ones are chained; q is read where it lies (LDS, or memory with a
+// wavefront-uniform address).
+constexpr int kChainDims = 32;
+
+__device__ __forceinline__ void coarse_load(const float* __restrict__ row, int lo, int d, bool v4,
+                                            float (&r)[kChainDims]) {
+    if (v4) {
+#pragma unroll
+        for (int t = 0; t < kChainDims / 4; ++t) {
+            const float4 x = *reinterpret_cast<const float4*>(row + min(lo + 4 * t, d - 4));
+            r[4 * t] = x.x;
+            r[4 * t + 1] = x.y;
+            r[4 * t + 2] = x.z;
+            r[4 * t + 3] = x.w;
         }
+    } else {
+#pragma unroll
+        for (int t = 0; t < kChainDims; ++t) r[t] = row[min(lo + t, d - 1)];
     }
-    for (; i < d; ++i) {
-        const float t = qs[i] - crow[i];
-        acc = __builtin_fmaf(t, t, acc);
+}
+
+__device__ __forceinline__ float coarse_chain(const float* __restrict__ qs, const float* __restrict__ crow, int d) {
+    const bool v4 = (d & 3) == 0;
+    const int last = (d - 1) / kChainDims;
+    float cur[kChainDims], nxt[kChainDims];
+    coarse_load(crow, 0, d, v4, cur);
+    float acc = 0.f;
+    for (int ch = 0; ch < last; ++ch) {
+        coarse_load(crow, (ch + 1) * kChainDims, d, v4, nxt);
+#pragma unroll
+        for (int t = 0; t < kChainDims; ++t) {
+            const float u = qs[ch * kChainDims + t] - cur[t];
+            acc = __builtin_fmaf(u, u, acc);
+        }
+#pragma unroll
+        for (int t = 0; t < kChainDims; ++t) cur[t] = nxt[t];
+    }
+    const int lo = last * kChainDims, len = d - lo;
+#pragma unroll
+    for (int t = 0; t < kChainDims; ++t) {
+        if (t < len) {
+            const float u = qs[lo + t] - cur[t];
+            acc = __builtin_fmaf(u, u, acc);
+        }
     }
     return acc;
 }
@@ -249,6 +276,99 @@ __global__ __launch_bounds__(256) void k_coarse_select(const float* __restrict__
     }
 }
 
+// The same selection with one wavefront per query (P <= kWaveP, nc <= 64 * E): the upper bounds
+// live in registers, tau comes from wave_kth_u32, the candidates' exact keys (at most
+// kWaveCand, else the query is listed for k_coarse_exact_row) are sorted in registers.
+constexpr int kWaveP = 256;
+constexpr int kWaveCand = 512;
+
+template <int EC>
+__device__ __forceinline__ void coarse_finish_wave(const float* __restrict__ qv, const float* __restrict__ C, int d,
+                                                   const unsigned int* __restrict__ cand, int m, int P,
+                                                   int* __restrict__ probes, float* __restrict__ pdist) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long v[EC];
+#pragma unroll
+    for (int e = 0; e < EC; ++e) {
+        const int j = e * 64 + lane;
+        v[e] = HV_KEY_PAD;
+        if (j < m) {
+            const unsigned int c = cand[j];
+            v[e] = hv_key(coarse_chain(qv, C + (int64_t)c * d, d), c);
+        }
+    }
+    wave_bitonic_sort<EC>(v);
+#pragma unroll
+    for (int e = 0; e < EC; ++e) {
+        const int r = lane * EC + e;
+        if (r < P) {
+            probes[r] = (int)hv_key_id(v[e]);
+            pdist[r] = hv_key_dist(v[e]);
+        }
+    }
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restrict__ approx,
+                                                            const float* __restrict__ Q,
+                                                            const float* __restrict__ C, int nc, int d,
+                                                            const float* __restrict__ qn,
+                                                            const float* __restrict__ cn, float kappa, int64_t nq,
+                                                            int P, int* __restrict__ probes,
+                                                            float* __restrict__ pdist, int* __restrict__ redo_list,
+                                                            int* __restrict__ redo_cnt) {
+    __shared__ unsigned int s_cand[4][kWaveCand];
+    __shared__ unsigned int s_tmp[4][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= nq) return;
+    const float* __restrict__ row = approx + q * nc;
+    const float qq = qn[q];
+    unsigned int ub[E];
+    float lbv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int c = e * 64 + lane;
+        ub[e] = 0xffffffffu;
+        lbv[e] = 3.402823466e38f;
+        if (c < nc) {
+            const float dt = row[c];
+            const float err = kappa * (qq + cn[c]);
+            lbv[e] = dt - err;
+            const unsigned int u = __float_as_uint(dt + err);
+            ub[e] = (int)u < 0 ? 0u : u;  // negative upper bounds (and -0) order as 0
+        }
+    }
+    const float tau = __uint_as_float(wave_kth_u32<E>(ub, P, s_tmp[w]));
+    int m = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int c = e * 64 + lane;
+        const bool in = c < nc && lbv[e] <= tau;
+        const unsigned long long b = __ballot(in);
+        const int slot = m + (int)__builtin_popcountll(b & ((1ull << lane) - 1ull));
+        if (in && slot < kWaveCand) s_cand[w][slot] = (unsigned int)c;
+        m += (int)__builtin_popcountll(b);
+    }
+    if (m > kWaveCand) {  // too wide a band: the exact kernel redoes this query
+        if (lane == 0) redo_list[atomicAdd(redo_cnt, 1)] = (int)q;
+        return;
+    }
+    __threadfence_block();  // the candidates other lanes stored are read below
+    const float* __restrict__ qv = Q + q * d;
+    int* pq = probes + q * P;
+    float* dq = pdist + q * P;
+    if (m <= 64)
+        coarse_finish_wave<1>(qv, C, d, s_cand[w], m, P, pq, dq);
+    else if (m <= 128)
+        coarse_finish_wave<2>(qv, C, d, s_cand[w], m, P, pq, dq);
+    else if (m <= 256)
+        coarse_finish_wave<4>(qv, C, d, s_cand[w], m, P, pq, dq);
+    else
+        coarse_finish_wave<8>(qv, C, d, s_cand[w], m, P, pq, dq);
+}
+
 // Fallback: every centroid's exact chain, LDS sort, first P (one workgroup per listed query).
 __global__ __launch_bounds__(256) void k_coarse_exact_row(const int* __restrict__ redo_list,
                                                           const int* __restrict__ redo_cnt,
@@ -295,9 +415,21 @@ void launch_coarse_select(const float* approx, const float* Q, const float* C, i
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
                           int* redo_list, int* redo_cnt, hipStream_t s) {
     if (nq <= 0) return;
-    const size_t lds = sizeof(unsigned long long) * kCoarseCand + sizeof(float) * ((d + 3) & ~3);
-    hipLaunchKernelGGL(k_coarse_select, dim3((unsigned)nq), dim3(256), lds, s, approx, Q, C, nc, d, qn, cn, kappa, P,
-                       probes, pdist, redo_list, redo_cnt);
+    const dim3 wgrid((unsigned)((nq + 3) / 4));
+    if (P <= kWaveP && nc <= 512)
+        hipLaunchKernelGGL(k_coarse_select_wave<8>, wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, nq, P,
+                           probes, pdist, redo_list, redo_cnt);
+    else if (P <= kWaveP && nc <= 1024)
+        hipLaunchKernelGGL(k_coarse_select_wave<16>, wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, nq, P,
+                           probes, pdist, redo_list, redo_cnt);
+    else if (P <= kWaveP && nc <= 2048)
+        hipLaunchKernelGGL(k_coarse_select_wave<32>, wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, nq, P,
+                           probes, pdist, redo_list, redo_cnt);
+    else {
+        const size_t lds = sizeof(unsigned long long) * kCoarseCand + sizeof(float) * ((d + 3) & ~3);
+        hipLaunchKernelGGL(k_coarse_select, dim3((unsigned)nq), dim3(256), lds, s, approx, Q, C, nc, d, qn, cn, kappa,
+                           P, probes, pdist, redo_list, redo_cnt);
+    }
     const size_t lds2 = sizeof(unsigned long long) * hv_next_pow2(nc > 2 ? nc : 2) + sizeof(float) * ((d + 3) & ~3);
     hipLaunchKernelGGL(k_coarse_exact_row, dim3((unsigned)std::min<int64_t>(nq, 256)), dim3(256), lds2, s, redo_list,
                        redo_cnt, Q, C, nc, d, P, probes, pdist);

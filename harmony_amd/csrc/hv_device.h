@@ -88,6 +88,83 @@ __device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
     }
 }
 
+// Sum over the wavefront of a per-lane count below 128, as a wave-uniform value: one ballot per
+// bit of the count, no cross-lane data movement.
+__device__ __forceinline__ int wave_sum_small(unsigned int x) {
+    int s = 0;
+#pragma unroll
+    for (int bit = 0; bit < 7; ++bit) s += __builtin_popcountll(__ballot((x >> bit) & 1u)) << bit;
+    return s;
+}
+
+// The r-th smallest (1-based, 1 <= r <= 64*E, E <= 64) of the 64*E unsigned values a wavefront
+// holds E per lane (callers pad with 0xffffffff, which then counts among the values). Exact: a
+// search over the value range keeps a bracket (lo, hi] with count(<= lo) < r <= count(<= hi)
+// until at most 64 values lie inside it; those are gathered (s_tmp: 64 words of LDS for this
+// wavefront alone) and sorted across the wavefront. The probe alternates between interpolation
+// on the counts and bisection, so it never needs more than twice the bisection's steps. All 64
+// lanes must call it.
+template <int E>
+__device__ __forceinline__ unsigned int wave_kth_u32(const unsigned int (&v)[E], int r, unsigned int* s_tmp) {
+    static_assert(E <= 64, "per-lane counts must stay below 128");
+    const int lane = threadIdx.x & 63;
+    // range of the values other than the padding (which would stretch it to 2^32)
+    unsigned int mn = 0xffffffffu, mx = 0u, npad = 0u;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        mn = min(mn, v[e]);
+        mx = v[e] == 0xffffffffu ? mx : max(mx, v[e]);
+        npad += v[e] == 0xffffffffu ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (unsigned int)__shfl_xor((int)mn, o, 64));
+        mx = max(mx, (unsigned int)__shfl_xor((int)mx, o, 64));
+    }
+    const int c_valid = 64 * E - wave_sum_small(npad);
+    if (r > c_valid) return 0xffffffffu;
+    long long lo = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mn) - 1;
+    long long hi = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mx);
+    int c_lo = 0, c_hi = c_valid;
+    bool interp = true;
+    while (c_hi - c_lo > 64 && hi - lo > 1) {
+        long long mid;
+        if (interp) {
+            const float f = (float)(r - c_lo) / (float)(c_hi - c_lo);
+            mid = lo + (long long)(f * (float)(hi - lo));
+            mid = mid <= lo ? lo + 1 : (mid >= hi ? hi - 1 : mid);
+        } else {
+            mid = lo + ((hi - lo) >> 1);
+        }
+        interp = !interp;
+        const unsigned int t = (unsigned int)mid;  // lo < mid < hi, so 0 <= mid < 2^32
+        unsigned int cl = 0u;
+#pragma unroll
+        for (int e = 0; e < E; ++e) cl += v[e] <= t ? 1u : 0u;
+        const int c = wave_sum_small(cl);
+        if (c >= r) {
+            hi = mid;
+            c_hi = c;
+        } else {
+            lo = mid;
+            c_lo = c;
+        }
+    }
+    if (hi - lo <= 1) return (unsigned int)hi;  // every value in the bracket equals hi
+    int base = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool in = (long long)v[e] > lo && (long long)v[e] <= hi;
+        const unsigned long long m = __ballot(in);
+        if (in) s_tmp[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = v[e];
+        base += __builtin_popcountll(m);
+    }
+    __threadfence_block();  // the words other lanes stored are read below
+    unsigned long long x[1] = {lane < base ? (unsigned long long)s_tmp[lane] : ~0ull};
+    wave_bitonic_sort<1>(x);
+    return (unsigned int)__shfl((int)(unsigned int)x[0], r - c_lo - 1, 64);
+}
+
 // Sorts the n (<= 2048) keys produced by key_of(j), j in [0, n), with one wavefront and hands
 // every element of the sorted order to emit(rank, key); ranks >= n hold HV_KEY_PAD. All 64 lanes
 // of the wavefront must call it with the same n.

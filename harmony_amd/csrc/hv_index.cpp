@@ -114,6 +114,12 @@ void kmeans_pick(int64_t n, int64_t nlist, int64_t max_pts, uint64_t seed, std::
 // Longest vector range of one scan work item: long lists are split so that no workgroup holds
 // a whole large list while the others run dry.
 constexpr int kVecSplit = 1024;
+// The BF16 scan's items: its per-item setup is small (query residuals from memory), so shorter
+// ranges spread long nearest lists over more workgroups.
+constexpr int kVecSplitBf16 = 256;
+
+// Scratch of one batch of exact redos (per-(query, rank, segment) runs of k keys)
+constexpr int64_t kRedoRunBytes = 64ll << 20;
 
 // Dimension blocks of node.cpp:639-646 (B blocks of floor(d/B) dims, the last takes the
 // rest), each cut into staging chunks of at most kChunkDims dims.
@@ -407,17 +413,17 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_pair_slot.ensure(sizeof(int) * n * nprobe));
     // work items of one phase: at most (query tiles) x (vector ranges); query tiles hold >= 16
     // queries, vector ranges >= kVecSplit vectors
-    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / kVecSplit + 1);
+    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / std::min(kVecSplit, kVecSplitBf16) + 1);
     require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
     // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
-    // (the remaining probed lists)
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 12));
+    // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 14));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
     hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
     hv::launch_fill_i32(h->s_fill.as<int>(), nb, 0, s);
-    hv::launch_fill_u64(stats, 12, 0ull, s);
+    hv::launch_fill_u64(stats, 14, 0ull, s);
     hv::launch_count_pairs(probes, n, (int)nprobe, qn, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
     hv::launch_scatter_pairs(probes, n, (int)nprobe, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
@@ -483,7 +489,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("pair residuals");
     }
     auto bf16_phase = [&](int r0, int r1, int vmin, int vmax, int mode, unsigned long long* st_slot) {
-        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, kVecSplit,
+        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, kVecSplitBf16,
                               h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
                               h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
@@ -492,7 +498,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_bf16_scan(h->bf16_grid, h->s_qres.as<__bf16>(), (int)d, pdist, (int)nprobe,
                              h->d_rcodes.as<__bf16>(), h->d_roff.as<int64_t>(), h->d_rnorm.as<float>(),
                              h->d_list_off.as<int64_t>(), h->s_bound.as<float>(), h->s_pair_slot.as<int>(),
-                             h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplit, mode, kappa1, kappa2, surv, aux,
+                             h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplitBf16, mode, kappa1, kappa2, surv, aux,
                              surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("bf16 scan");
         if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
@@ -534,9 +540,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (!learned || !bf16) return;
         HV_HIP(h->s_save.ensure(sizeof(float) * n));
         bound_safe = h->s_save.as<float>();
-        HV_HIP(hipMemcpyAsync(bound_safe, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-        hv::launch_min_f32(bound, seed, n, s);
-        HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        hv::launch_apply_seed(bound, seed, bound_safe, bound_ref, n, s);
     };
     if (bf16) {
         // nearest list: vectors below cap go to their own slot unfiltered; the k-th smallest upper
@@ -547,8 +551,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
         hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
         HV_CHECK_LAUNCH("compact");
-        apply_seed();
-        HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        if (learned)
+            apply_seed();
+        else
+            HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
         if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
@@ -564,13 +570,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
     }
     HV_HIP(hipEventRecord(h->ev[4], s));
-    // 6. top-k selection (+ exact path for overflowed buffers). The BF16 form re-ranks its
-    //    survivors (the plain path's FP32 arithmetic) and selects in the same kernel.
+    // 6. top-k selection. The BF16 form re-ranks its survivors (the plain path's FP32 arithmetic)
+    //    and selects in the same pass.
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
-    int* ovf_cnt = h->s_overflow.as<int>();
-    unsigned int* n_items = reinterpret_cast<unsigned int*>(ovf_cnt + 1);
-    int* ovf_list = ovf_cnt + 2;
-    hv::launch_fill_i32(ovf_cnt, 2, 0, s);
+    int* redo_cnt = h->s_overflow.as<int>();
+    unsigned int* n_items = reinterpret_cast<unsigned int*>(redo_cnt + 1);
+    int* redo_list = redo_cnt + 2;
+    hv::launch_fill_i32(redo_cnt, 2, 0, s);
     if (bf16) {
         HV_HIP(hipEventRecord(h->ev[8], s));
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
@@ -578,31 +584,45 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_kept_cnt.ensure(sizeof(unsigned int) * n));
         hv::launch_rerank_select(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
                                  (int)cap, (int)k, bound, h->s_ritems.as<int2>(), n_items,
-                                 h->s_kept.as<unsigned long long>(), h->s_kept_cnt.as<unsigned int>(), D, I, ovf_list,
-                                 ovf_cnt, stats, s);
+                                 h->s_kept.as<unsigned long long>(), h->s_kept_cnt.as<unsigned int>(), D, I, nullptr,
+                                 nullptr, stats, s);
         HV_CHECK_LAUNCH("rerank + select");
         HV_HIP(hipEventRecord(h->ev[9], s));
     } else {
-        hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, ovf_list, ovf_cnt, stats, s);
+        hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, nullptr, nullptr, stats, s);
         HV_CHECK_LAUNCH("select");
     }
-    // the overflowed queries (rare) take the exact path, spread over (query, probed list) pairs
-    int novf = 0;
-    HV_HIP(hipMemcpyAsync(&novf, ovf_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-    HV_HIP(hipStreamSynchronize(s));
-    if (novf > 0) {
-        const int nseg = hv::exact_segments(h->max_list);
-        HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * novf * nprobe_cap * nseg * k));
-        hv::launch_query_exact(ovf_list, novf, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
-                               h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), probes,
-                               (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune, bound, (int)k, h->s_runs.as<unsigned long long>(), D,
-                               I, s);
-        HV_CHECK_LAUNCH("exact fallback");
-    }
+    // 7. the exact path redoes the queries whose survivor buffer overflowed (rare) and, in learned
+    //    mode, those whose k-th result is missing or above the seeded bound (a neighbour may have
+    //    been pruned), over the same probe lists, with the safe bound in learned mode (none in the
+    //    FP32 form). The list and its count stay on the device: the kernels take the first batch
+    //    by themselves, and the host adds batches only when the final count exceeds one.
+    hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound, learned ? 1 : 0, n, redo_list, redo_cnt, stats + 12,
+                         s);
+    const float* redo_bound = learned ? bound_safe : bound;
+    const int nseg = hv::exact_segments(h->max_list);
+    const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
+    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
+    HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
+    auto redo = [&](int q0) {
+        hv::launch_query_exact(redo_list, redo_cnt, q0, batch, Qp, (int)d, h->d_codes.as<float>(),
+                               h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(),
+                               h->d_ids.as<uint32_t>(), probes, (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune,
+                               redo_bound, (int)k, h->s_runs.as<unsigned long long>(), D, I, s);
+    };
+    redo(0);
+    HV_CHECK_LAUNCH("exact redo");
     HV_HIP(hipEventRecord(h->ev[5], s));
-    unsigned long long hs[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long hs[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int nredo = 0;
     HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HV_HIP(hipMemcpyAsync(&nredo, redo_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
+    if (nredo > batch) {
+        for (int q0 = batch; q0 < nredo; q0 += batch) redo(q0);
+        HV_CHECK_LAUNCH("exact redo");
+        HV_HIP(hipStreamSynchronize(s));
+    }
     h->last_n = n;
     h->last_nprobe = nprobe_cap;
     h->last_stride = nprobe;
@@ -620,7 +640,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         st->block_evals += (int64_t)(hs[1] + hs[9]);
         st->block_evals_max += cand * B;
         st->survivors += (int64_t)hs[2];
-        st->overflow_queries += novf;
+        st->overflow_queries += (int64_t)hs[12];
+        st->fallback_queries += (int64_t)hs[13];
         if (main_recorded) {
             float mm = 0.f;
             HV_HIP(hipEventElapsedTime(&mm, h->ev[6], h->ev[7]));
@@ -632,37 +653,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             HV_HIP(hipEventElapsedTime(&mr, h->ev[8], h->ev[9]));
             st->ms_rerank += mr;
             st->rerank_pairs += (int64_t)hs[2];  // the select sees exactly the re-ranked pairs
-        }
-    }
-    if (learned) {
-        // queries whose k-th result is missing or above the bound the seed set (a neighbour may
-        // have been pruned) are redone exactly over the same probe lists, with the safe bound
-        HV_HIP(h->s_fb.ensure(sizeof(int) * (n + 1)));
-        int* fb_cnt = h->s_fb.as<int>();
-        int* fb_list = fb_cnt + 1;
-        hv::launch_fill_i32(fb_cnt, 1, 0, s);
-        hv::launch_flag_fallback(D, I, (int)k, bound, n, fb_list, fb_cnt, s);
-        int nfb = 0;
-        HV_HIP(hipMemcpyAsync(&nfb, fb_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-        HV_HIP(hipStreamSynchronize(s));
-        if (st) st->fallback_queries += nfb;
-        if (nfb > 0) {
-            if (!bound_safe) {  // FP32 form, or no nearest-list phase ran: no bound
-                HV_HIP(h->s_save.ensure(sizeof(float) * n));
-                bound_safe = h->s_save.as<float>();
-                int fm;
-                const float fmax = FLT_MAX;
-                std::memcpy(&fm, &fmax, 4);
-                hv::launch_fill_i32(reinterpret_cast<int*>(bound_safe), n, fm, s);
-            }
-            const int nseg = hv::exact_segments(h->max_list);
-            HV_HIP(h->s_runs.ensure(sizeof(unsigned long long) * nfb * nprobe_cap * nseg * k));
-            hv::launch_query_exact(fb_list, nfb, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
-                                   h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(),
-                                   probes, (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune, bound_safe, (int)k,
-                                   h->s_runs.as<unsigned long long>(), D, I, s);
-            HV_CHECK_LAUNCH("fallback");
-            HV_HIP(hipStreamSynchronize(s));
         }
     }
     if (st) st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -70,14 +70,18 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap,
                              int k, float* D, int64_t* I, int* overflow_list, int* overflow_cnt,
                              unsigned long long* stats, hipStream_t s);
-// exact path for the nql overflowed queries of qlist over their probe ranks [0, nranks) (and
-// below qn[q] when given; probe rows are nprobe wide), lists cut into exact_segments(max list
-// length) segments; runs >= nql * nranks * nseg * k keys of scratch
+// exact path for the queries qlist[q0 .. q0 + min(*nql_dev - q0, nql_max)) (the count read on the
+// device) over their probe ranks [0, nranks) (and below qn[q] when given; probe rows are nprobe
+// wide), lists cut into exact_segments(max list length) segments: per-(query, rank, segment)
+// top-k runs appended per query (runs: nql_max * nranks * nseg * k keys, then nql_max ints of
+// counts), merged into D/I. bound may be null (none).
+constexpr int kExactGrid = 2048;
 int exact_segments(int64_t max_list);
-void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
-                        const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
-                        int nprobe, int nranks, const int* qn, int nseg, const ChunkPlan& plan, int prune,
-                        const float* bound, int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s);
+void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_max, const float* Qp, int d,
+                        const float* codes, const int64_t* code_off, const int64_t* list_off, const int* list_stride,
+                        const uint32_t* ids, const int* probes, int nprobe, int nranks, const int* qn, int nseg,
+                        const ChunkPlan& plan, int prune, const float* bound, int k, unsigned long long* runs, float* D,
+                        int64_t* I, hipStream_t s);
 
 // --- BF16 precision form (hv_bf16.hip)
 int bf16_ksteps(int d);  // 32-dim MFMA steps per vector (dpad = 32 * ksteps)
@@ -110,6 +114,11 @@ void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, cons
                           int cap, int k, const float* bound, int2* items, unsigned int* n_items,
                           unsigned long long* kept, unsigned int* kept_cnt, float* D, int64_t* I, int* overflow_list,
                           int* overflow_cnt, unsigned long long* stats, hipStream_t s);
+// the queries the exact path redoes -> list / list_cnt (both zeroed by the caller): overflowed
+// survivor buffers (cnt[q] > cap; counts[0]) and, with check_bound, queries whose k-th result is
+// missing or above bound[q] (counts[1])
+void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
+                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, hipStream_t s);
 // the block select alone (queries with more than kWaveSortMax survivors)
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
                          float* D, int64_t* I, hipStream_t s);
@@ -132,11 +141,8 @@ void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf
                       const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
                       float* bound, float* pred, hipStream_t s);
 
-// queries whose k-th result is missing or above the bound the search used -> list / cnt
-void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* seed, int64_t n, int* list, int* cnt,
-                          hipStream_t s);
-// b[i] = min(b[i], c[i])
-void launch_min_f32(float* b, const float* c, int64_t n, hipStream_t s);
+// learned seed after the nearest lists: safe = bound; bound = ref = min(bound, seed)
+void launch_apply_seed(float* bound, const float* seed, float* safe, float* ref, int64_t n, hipStream_t s);
 
 // --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
 void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);

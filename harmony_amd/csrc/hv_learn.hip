@@ -261,42 +261,23 @@ void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf
                        nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred);
 }
 
-}  // namespace hv
 
-namespace hv {
-
-// ---------------------------------------------------------------------------------------
-// Queries the seeded bound may have cut short: their k-th result is missing or lies above the
-// bound the search used (then a true neighbour may have been pruned). They are redone exactly.
-__global__ void k_flag_fallback(const float* __restrict__ D, const int64_t* __restrict__ I, int k,
-                                const float* __restrict__ seed, int64_t n, int* __restrict__ list,
-                                int* __restrict__ cnt) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    if (I[q * k + k - 1] < 0 || D[q * k + k - 1] > seed[q]) list[atomicAdd(cnt, 1)] = (int)q;
-}
-
-void launch_flag_fallback(const float* D, const int64_t* I, int k, const float* seed, int64_t n, int* list, int* cnt,
-                          hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_flag_fallback, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, I, k, seed, n, list,
-                       cnt);
-}
-
-}  // namespace hv
-
-
-namespace hv {
-
-// b[i] = min(b[i], c[i])
-__global__ void k_min_f32(float* __restrict__ b, const float* __restrict__ c, int64_t n) {
+// The learned seed applied to the bound after the nearest lists: safe[i] = bound[i] (kept for the
+// exact redo), then bound[i] = ref[i] = min(bound[i], seed[i]).
+__global__ void k_apply_seed(float* __restrict__ bound, const float* __restrict__ seed, float* __restrict__ safe,
+                             float* __restrict__ ref, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = fminf(b[i], c[i]);
+    if (i >= n) return;
+    const float b = bound[i];
+    safe[i] = b;
+    const float m = fminf(b, seed[i]);
+    bound[i] = m;
+    ref[i] = m;
 }
 
-void launch_min_f32(float* b, const float* c, int64_t n, hipStream_t s) {
+void launch_apply_seed(float* bound, const float* seed, float* safe, float* ref, int64_t n, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_min_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, c, n);
+    hipLaunchKernelGGL(k_apply_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bound, seed, safe, ref, n);
 }
 
 }  // namespace hv

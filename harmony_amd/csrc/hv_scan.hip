@@ -635,16 +635,16 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 }
 
 // ---------------------------------------------------------------------------------------
-// Exact path for queries whose candidate buffer overflowed. k_exact_lists: one workgroup per
-// (overflowed query, probed list, segment of kExactSpan vectors), one candidate per thread;
-// candidates that pass the bound and
-// beat the current k-th key are collected in LDS (up to kExactSeg per segment of the list), the
-// k-th smallest key of the collected set is found by a block-wide radix select (8 passes of 8
-// bits over the 64-bit keys, which are unique), and only the keys up to it are kept. It writes
-// its k best keys (sorted, padded). k_exact_merge: one workgroup per query reduces the
-// nprobe * nseg runs to the final top-k the same way. Same arithmetic, pruning and
-// (distance <= bound) filter as the scan, so the result is the same.
-constexpr int kExactTop = 1024, kExactSeg = 4096, kExactSpan = 1024;
+// Exact path for the queries the search redoes (overflowed candidate buffers, learned-mode
+// fallbacks). k_exact_lists: one workgroup per (query, probed list, segment of kExactSpan
+// vectors), one candidate per thread; candidates that pass the bound and beat the current k-th
+// key are collected in LDS (up to kExactSeg per segment of the list), the k-th smallest key of
+// the collected set is found by a block-wide radix select (8 passes of 8 bits over the 64-bit
+// keys, which are unique), and only the keys up to it are kept. It appends its (at most k) best
+// keys to the query's run buffer (an atomic count per query, order free). k_exact_merge: one
+// workgroup per query reduces the appended keys to the final top-k the same way and sorts them.
+// Same arithmetic, pruning and (distance <= bound) filter as the scan, so the result is the same.
+constexpr int kExactTop = 1024, kExactSeg = 4096, kExactSpan = 256;
 
 // k-th smallest (1-based) of the na keys of a and the nb keys of b (block-wide; every thread
 // gets it)
@@ -737,7 +737,8 @@ __device__ void topk_reduce(unsigned long long* s, const unsigned long long* s_c
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qlist, int nql,
+__global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qlist,
+                                                     const int* __restrict__ nql_dev, int q0, int nql_max,
                                                      const float* __restrict__ Qp, int d,
                                                      const float* __restrict__ codes,
                                                      const int64_t* __restrict__ code_off,
@@ -746,7 +747,8 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
                                                      const uint32_t* __restrict__ ids, const int* __restrict__ probes,
                                                      int nprobe, int nranks, const int* __restrict__ qn, int nseg,
                                                      ChunkPlan plan, int prune, const float* __restrict__ bound, int k,
-                                                     unsigned long long* __restrict__ runs) {
+                                                     unsigned long long* __restrict__ runs,
+                                                     int* __restrict__ run_cnt) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     // kept keys [kExactTop] | keys collected this segment [kExactSeg] | reduction output [kExactTop]
     unsigned long long* s = reinterpret_cast<unsigned long long*>(s_raw);
@@ -755,22 +757,20 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
     float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * (2 * kExactTop + kExactSeg));
     __shared__ unsigned int s_hist[256];
     __shared__ unsigned long long s_sel;
-    __shared__ int s_cnt, s_top;
+    __shared__ int s_cnt, s_top, s_base;
     __shared__ unsigned long long s_kth;
     const int64_t per_q = (int64_t)nranks * nseg;  // ranks [0, nranks) of rows nprobe wide
+    const int nql = min(*nql_dev - q0, nql_max);
     for (int64_t w = blockIdx.x; w < (int64_t)nql * per_q; w += gridDim.x) {
-        const int64_t q = qlist[w / per_q];
+        const int64_t q = qlist[q0 + w / per_q];
         const int j = (int)(w % per_q) / nseg, seg = (int)(w % nseg);
-        const float b = bound[q];
+        const float b = bound ? bound[q] : 3.402823466e38f;
         {
             // an empty segment (rank past the query's probe count, or past the list's end)
             // contributes an empty run
             const int l0 = probes[q * nprobe + j];
             const int64_t len = list_off[l0 + 1] - list_off[l0];
-            if ((qn && j >= qn[q]) || (int64_t)seg * kExactSpan >= len) {
-                for (int i = threadIdx.x; i < k; i += blockDim.x) runs[w * k + i] = HV_KEY_PAD;
-                continue;
-            }
+            if ((qn && j >= qn[q]) || (int64_t)seg * kExactSpan >= len) continue;
         }
         __syncthreads();
         for (int i = threadIdx.x; i < d; i += blockDim.x) qs[i] = Qp[q * d + i];
@@ -800,19 +800,21 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
             if (s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= n)
                 topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
         }
-        // sort the kept keys (padded to a power of two) and write the run
+        // append the kept keys (at most k) to the query's runs
         const int top = s_top;
-        const int p2 = hv_next_pow2(max(k, 2));
-        for (int i = top + threadIdx.x; i < p2; i += blockDim.x) s[i] = HV_KEY_PAD;
+        const int64_t wq = w / per_q;
+        if (threadIdx.x == 0) s_base = top ? atomicAdd(&run_cnt[wq], top) : 0;
         __syncthreads();
-        block_bitonic_sort(s, p2);
-        for (int i = threadIdx.x; i < k; i += blockDim.x) runs[w * k + i] = s[i];
+        unsigned long long* dst = runs + wq * per_q * k + s_base;
+        for (int i = threadIdx.x; i < top; i += blockDim.x) dst[i] = s[i];
     }
 }
 
-__global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qlist, int nql, int nruns, int k,
+__global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qlist, const int* __restrict__ nql_dev,
+                                                     int q0, int nql_max, int nruns, int k,
                                                      const unsigned long long* __restrict__ runs,
-                                                     float* __restrict__ D, int64_t* __restrict__ I) {
+                                                     const int* __restrict__ run_cnt, float* __restrict__ D,
+                                                     int64_t* __restrict__ I) {
     __shared__ unsigned long long s[2 * kExactTop + kExactSeg];
     unsigned long long* s_col = s + kExactTop;
     unsigned long long* s_out = s_col + kExactSeg;
@@ -820,10 +822,11 @@ __global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qli
     __shared__ unsigned long long s_sel;
     __shared__ int s_cnt, s_top;
     __shared__ unsigned long long s_kth;
+    const int nql = min(*nql_dev - q0, nql_max);
     for (int w = blockIdx.x; w < nql; w += gridDim.x) {
-        const int64_t q = qlist[w];
+        const int64_t q = qlist[q0 + w];
         const unsigned long long* __restrict__ src = runs + (int64_t)w * nruns * k;
-        const int total = nruns * k;
+        const int total = run_cnt[w];
         __syncthreads();
         if (threadIdx.x == 0) {
             s_cnt = 0;
@@ -856,18 +859,21 @@ __global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qli
 
 int exact_segments(int64_t max_list) { return (int)std::max<int64_t>(1, (max_list + kExactSpan - 1) / kExactSpan); }
 
-void launch_query_exact(const int* qlist, int nql, const float* Qp, int d, const float* codes, const int64_t* code_off,
-                        const int64_t* list_off, const int* list_stride, const uint32_t* ids, const int* probes,
-                        int nprobe, int nranks, const int* qn, int nseg, const ChunkPlan& plan, int prune,
-                        const float* bound, int k, unsigned long long* runs, float* D, int64_t* I, hipStream_t s) {
-    if (nql <= 0) return;
+void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_max, const float* Qp, int d,
+                        const float* codes, const int64_t* code_off, const int64_t* list_off, const int* list_stride,
+                        const uint32_t* ids, const int* probes, int nprobe, int nranks, const int* qn, int nseg,
+                        const ChunkPlan& plan, int prune, const float* bound, int k, unsigned long long* runs, float* D,
+                        int64_t* I, hipStream_t s) {
+    if (nql_max <= 0) return;
     const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
-    const int64_t pairs = (int64_t)nql * nranks * nseg;
-    hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, 65536)), dim3(256), lds, s, qlist, nql,
-                       Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nranks, qn, nseg, plan, prune,
-                       bound, k, runs);
-    hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql, 65536)), dim3(256), 0, s, qlist, nql, nranks * nseg,
-                       k, runs, D, I);
+    const int64_t pairs = (int64_t)nql_max * nranks * nseg;
+    int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * nranks * nseg * k);
+    launch_fill_i32(run_cnt, nql_max, 0, s);
+    hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, kExactGrid)), dim3(256), lds, s, qlist,
+                       nql_dev, q0, nql_max, Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nranks,
+                       qn, nseg, plan, prune, bound, k, runs, run_cnt);
+    hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql_max, kExactGrid)), dim3(256), 0, s, qlist, nql_dev,
+                       q0, nql_max, nranks * nseg, k, runs, run_cnt, D, I);
 }
 
 }  // namespace hv

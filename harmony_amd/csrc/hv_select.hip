@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* _
     const unsigned int n = surv_cnt[q];
     const bool lane0 = (threadIdx.x & 63) == 0;
     if (n > (unsigned int)cap) {
-        if (lane0) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
+        if (overflow_list && lane0) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
         return;
     }
     if (stats && lane0 && n) atomicAdd(&stats[2], (unsigned long long)n);
@@ -210,6 +210,37 @@ __global__ __launch_bounds__(256) void k_select_block(const unsigned long long* 
         D[q * k + i] = hv_key_dist(s_keys[i]);
         I[q * k + i] = hv_key_id(s_keys[i]);
     }
+}
+
+__global__ __launch_bounds__(256) void k_redo_list(const unsigned int* __restrict__ cnt, int cap,
+                                                   const float* __restrict__ D, const int64_t* __restrict__ I, int k,
+                                                   const float* __restrict__ bound, int check_bound, int64_t n,
+                                                   int* __restrict__ list, int* __restrict__ list_cnt,
+                                                   unsigned long long* __restrict__ counts) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool ovf = false, redo = false;
+    if (q < n) {
+        ovf = cnt[q] > (unsigned int)cap;
+        redo = ovf || (check_bound && (I[q * k + k - 1] < 0 || D[q * k + k - 1] > bound[q]));
+    }
+    const unsigned long long m = __ballot(redo), mo = __ballot(ovf);
+    if (!m) return;
+    int base = 0;
+    if (lane == 0) {
+        base = atomicAdd(list_cnt, (int)__builtin_popcountll(m));
+        if (mo) atomicAdd(&counts[0], (unsigned long long)__builtin_popcountll(mo));
+        if (m != mo) atomicAdd(&counts[1], (unsigned long long)__builtin_popcountll(m & ~mo));
+    }
+    base = __shfl(base, 0, 64);
+    if (redo) list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
+}
+
+void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
+                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_redo_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cnt, cap, D, I, k, bound,
+                       check_bound, n, list, list_cnt, counts);
 }
 
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,

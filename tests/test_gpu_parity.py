@@ -86,6 +86,24 @@ def test_overflow_falls_back_to_exact_path(w1):
     np.testing.assert_array_equal(D, oD)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_overflow_redo_in_several_batches(w1, precision):
+    """Every query overflows a 1024-pair buffer at k = 1024 over all 100 lists (no warm-up bound):
+    the exact redo runs in batches (its scratch holds about 80 such queries), the later ones
+    added by the host."""
+    xb, xq, g, o = w1
+    _, q = datagen.make(1, nb=1, nq=200)
+    g.nprobe = 100
+    g.preSearch(warmup_list=0, dim_blocks=4, survivor_cap=1024, precision=precision)
+    D, I, st = g.search(q, 1024, return_stats=True)
+    g.preSearch(warmup_list=10, dim_blocks=4, precision=0)
+    g.nprobe = 10
+    assert st["overflow_queries"] == q.shape[0]
+    oD, oI, _, _ = O.search(o, q, 1024, nprobe=100, dim_blocks=4)
+    np.testing.assert_array_equal(I, oI)
+    np.testing.assert_array_equal(D, oD)
+
+
 def test_external_bound(w1):
     xb, xq, g, o = w1
     g.nprobe = 10
