@@ -628,13 +628,13 @@ void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned in
 // pair's exact FP32 distance (the plain path's blocked FMA chains in dimension order) is kept
 // only when it does not exceed the query's bound: the bound is the k-th smallest upper bound of
 // the nearest list (or lower), so it is at least the k-th exact distance and no top-k member
-// is dropped. Kept keys (distance, id) are appended per query for the final select.
+// is dropped. Each pair's slot of the kept buffer gets its key (distance, id), or padding when
+// dropped; the final select compacts and sorts them.
 //
-// k_rerank_items: one thread per query; lists overflowed queries, clears the kept counters,
-// counts the re-ranked pairs (stats[2]) and reserves its items with one atomic per workgroup.
+// k_rerank_items: one thread per query; lists overflowed queries, counts the re-ranked pairs
+// (stats[2]) and reserves its items with one atomic per workgroup.
 __global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __restrict__ cnt, int64_t nq, int cap,
                                                       int2* __restrict__ items, unsigned int* __restrict__ n_items,
-                                                      unsigned int* __restrict__ kept_cnt,
                                                       int* __restrict__ overflow_list, int* __restrict__ overflow_cnt,
                                                       unsigned long long* __restrict__ stats) {
     __shared__ unsigned int s_wsum[4];
@@ -645,7 +645,6 @@ __global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __rest
     unsigned int c = 0u;
     if (q < nq) {
         c = cnt[q];
-        kept_cnt[q] = 0u;
         if (c > (unsigned int)cap) {
             if (overflow_list) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
             c = 0u;
@@ -671,7 +670,8 @@ __global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __rest
     __syncthreads();
     unsigned int base = s_base + incl - nch;
     for (int i = 0; i < w; ++i) base += s_wsum[i];
-    for (unsigned int i = 0; i < nch; ++i) items[base + i] = make_int2((int)q, (int)(i * 64u));
+    for (unsigned int i = 0; i < nch; ++i)
+        items[base + i] = make_int2((int)q, (int)(i * 64u | min(64u, c - i * 64u) << 16));
 }
 
 // One staging chunk (<= kChunkDims dims from lo) of a row: V4 loads 16 bytes at a time (a
@@ -712,31 +712,46 @@ template <bool V4>
 __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Qp, int d,
                                                      const float* __restrict__ rows_by_pos,
                                                      const uint32_t* __restrict__ ids, ChunkPlan plan,
-                                                     const unsigned long long* __restrict__ surv,
-                                                     const unsigned int* __restrict__ cnt, int cap,
+                                                     const unsigned long long* __restrict__ surv, int cap,
                                                      const int2* __restrict__ items,
                                                      const unsigned int* __restrict__ n_items,
                                                      const float* __restrict__ bound,
-                                                     unsigned long long* __restrict__ kept,
-                                                     unsigned int* __restrict__ kept_cnt) {
+                                                     unsigned long long* __restrict__ kept) {
     const int lane = threadIdx.x & 63;
     const int nw = (int)gridDim.x * 4;
     const int ni = (int)*n_items;
-    for (int it = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); it < ni; it += nw) {
-        const int2 item = items[it];
-        const int q = __builtin_amdgcn_readfirstlane(item.x);
-        const int j0 = __builtin_amdgcn_readfirstlane(item.y);
-        const int c = min((int)cnt[q] - j0, 64);
+    int it = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (it >= ni) return;
+    // an item: (query, first pair | pairs << 16). The next item's descriptor, positions and bound
+    // are loaded while the current one is chained.
+    int2 item = items[it];
+    int q = __builtin_amdgcn_readfirstlane(item.x);
+    int j0 = __builtin_amdgcn_readfirstlane(item.y & 0xffff), c = __builtin_amdgcn_readfirstlane(item.y >> 16);
+    uint32_t pos = lane < c ? (uint32_t)(surv[(int64_t)q * cap + j0 + lane] & 0xffffffffu) : 0u;
+    float b = bound[q];
+    const int last = plan.n - 1;
+    while (true) {
         const bool valid = lane < c;
-        const uint32_t pos = valid ? (uint32_t)(surv[(int64_t)q * cap + j0 + lane] & 0xffffffffu) : 0u;
+        const uint32_t id = ids[pos];
         const float* __restrict__ xv = rows_by_pos + (int64_t)pos * d;
         const float* __restrict__ qv = Qp + (int64_t)q * d;
         float cur[kChunkDims], nxt[kChunkDims];
         load_chunk<V4>(xv, plan.lo[0], d, cur);
+        const int itn = it + nw;
+        int qn = q, j0n = 0, cn = 0;
+        uint32_t posn = 0u;
+        float bn = 0.f;
+        if (itn < ni) {
+            const int2 nitem = items[itn];
+            qn = __builtin_amdgcn_readfirstlane(nitem.x);
+            j0n = __builtin_amdgcn_readfirstlane(nitem.y & 0xffff);
+            cn = __builtin_amdgcn_readfirstlane(nitem.y >> 16);
+            posn = lane < cn ? (uint32_t)(surv[(int64_t)qn * cap + j0n + lane] & 0xffffffffu) : 0u;
+            bn = bound[qn];
+        }
         float dis = 0.f, p = 0.f;
         // the next chunk's loads are in flight during each chain (the last chunk is peeled so
         // that every loop iteration issues them)
-        const int last = plan.n - 1;
         for (int ch = 0; ch < last; ++ch) {
             load_chunk<V4>(xv, plan.lo[ch + 1], d, nxt);
             chain_chunk<V4>(qv, plan.lo[ch], plan.len[ch], d, cur, p);
@@ -749,37 +764,138 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
         }
         chain_chunk<V4>(qv, plan.lo[last], plan.len[last], d, cur, p);
         dis += p;  // the last chunk closes the last block
-        const bool keep = valid && dis <= bound[q];
-        const unsigned long long m = __ballot(keep);
-        if (m) {
-            unsigned int b0 = 0u;
-            if (lane == 0) b0 = atomicAdd(&kept_cnt[q], (unsigned int)__builtin_popcountll(m));
-            b0 = (unsigned int)__shfl((int)b0, 0, 64);
-            if (keep)
-                kept[(int64_t)q * cap + b0 + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = hv_key(dis, ids[pos]);
-        }
+        if (valid) kept[(int64_t)q * cap + j0 + lane] = dis <= b ? hv_key(dis, id) : HV_KEY_PAD;
+        if (itn >= ni) break;
+        it = itn;
+        q = qn;
+        j0 = j0n;
+        c = cn;
+        pos = posn;
+        b = bn;
     }
+}
+
+// Final top-k of a re-ranked buffer (kept keys in place, padding for dropped pairs): one
+// wavefront per query compacts the kept keys into LDS (ballot prefix), sorts them in registers and
+// emits the first k. Buffers of up to kSelLds pairs are loaded at once; longer ones are read in
+// 64-key steps, and a query whose kept keys do not fit kSelLds is left to the workgroup select
+// (done[q] = 0).
+constexpr int kSelLds = 1024;
+
+// sort the n (<= kSelLds) keys of s_keys (padded), emit the first k
+__device__ __forceinline__ void sort_emit_lds(const unsigned long long* s_keys, int n, int k, float* __restrict__ Dq,
+                                              int64_t* __restrict__ Iq) {
+    const int m = max(n, k);
+    auto key_of = [&](int j) { return j < n ? s_keys[j] : HV_KEY_PAD; };
+    auto emit = [&](int r, unsigned long long key) {
+        if (r < k) {
+            Dq[r] = hv_key_dist(key);
+            Iq[r] = hv_key_id(key);
+        }
+    };
+    if (m <= 64)
+        wave_sort_emit_e<1>(m, key_of, emit);
+    else if (m <= 128)
+        wave_sort_emit_e<2>(m, key_of, emit);
+    else if (m <= 256)
+        wave_sort_emit_e<4>(m, key_of, emit);
+    else if (m <= 512)
+        wave_sort_emit_e<8>(m, key_of, emit);
+    else
+        wave_sort_emit_e<16>(m, key_of, emit);
+}
+
+template <int E>
+__device__ __forceinline__ void select_kept_e(const unsigned long long* __restrict__ buf, int c, int k,
+                                              unsigned long long* s_keys, float* __restrict__ Dq,
+                                              int64_t* __restrict__ Iq) {
+    static_assert(64 * E <= kSelLds, "kept keys must fit the LDS row");
+    const int lane = threadIdx.x & 63;
+    unsigned long long v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = e * 64 + lane;
+        v[e] = j < c ? buf[j] : HV_KEY_PAD;
+    }
+    int m = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool keep = v[e] != HV_KEY_PAD;
+        const unsigned long long b = __ballot(keep);
+        if (keep) s_keys[m + __builtin_popcountll(b & ((1ull << lane) - 1ull))] = v[e];
+        m += (int)__builtin_popcountll(b);
+    }
+    __threadfence_block();  // the keys other lanes stored are read below
+    sort_emit_lds(s_keys, m, k, Dq, Iq);
+}
+
+__global__ __launch_bounds__(256) void k_select_kept(const unsigned long long* __restrict__ kept,
+                                                     const unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
+                                                     float* __restrict__ D, int64_t* __restrict__ I,
+                                                     unsigned int* __restrict__ done) {
+    __shared__ unsigned long long s_keys[4][kSelLds];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= nq) return;
+    const unsigned int cu = cnt[q];
+    if (cu > (unsigned int)cap) return;  // overflowed: the exact path redoes the query
+    const int c = (int)cu;
+    const unsigned long long* __restrict__ buf = kept + q * cap;
+    float* Dq = D + q * k;
+    int64_t* Iq = I + q * k;
+    if (c <= 64)
+        select_kept_e<1>(buf, c, k, s_keys[w], Dq, Iq);
+    else if (c <= 128)
+        select_kept_e<2>(buf, c, k, s_keys[w], Dq, Iq);
+    else if (c <= 256)
+        select_kept_e<4>(buf, c, k, s_keys[w], Dq, Iq);
+    else if (c <= 512)
+        select_kept_e<8>(buf, c, k, s_keys[w], Dq, Iq);
+    else if (c <= 1024)
+        select_kept_e<16>(buf, c, k, s_keys[w], Dq, Iq);
+    else {
+        int m = 0;
+        for (int j0 = 0; j0 < c; j0 += 64) {
+            const int j = j0 + lane;
+            const unsigned long long key = j < c ? buf[j] : HV_KEY_PAD;
+            const bool keep = key != HV_KEY_PAD;
+            const unsigned long long b = __ballot(keep);
+            const int slot = m + (int)__builtin_popcountll(b & ((1ull << lane) - 1ull));
+            if (keep && slot < kSelLds) s_keys[w][slot] = key;
+            m += (int)__builtin_popcountll(b);
+        }
+        if (m > kSelLds) {
+            if (lane == 0) done[q] = 0u;  // the workgroup select sorts the whole buffer
+            return;
+        }
+        __threadfence_block();
+        sort_emit_lds(s_keys[w], m, k, Dq, Iq);
+    }
+    if (lane == 0) done[q] = 1u;
 }
 
 void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,
                           const ChunkPlan& plan, const unsigned long long* surv, const unsigned int* cnt, int64_t nq,
                           int cap, int k, const float* bound, int2* items, unsigned int* n_items,
-                          unsigned long long* kept, unsigned int* kept_cnt, float* D, int64_t* I, int* overflow_list,
+                          unsigned long long* kept, unsigned int* done, float* D, int64_t* I, int* overflow_list,
                           int* overflow_cnt, unsigned long long* stats, hipStream_t s) {
     if (nq <= 0) return;
     bool v4 = d >= 4 && (d & 3) == 0;
     for (int ch = 0; ch < plan.n; ++ch) v4 = v4 && (plan.lo[ch] & 3) == 0;
     hipLaunchKernelGGL(k_rerank_items, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, cnt, nq, cap, items,
-                       n_items, kept_cnt, overflow_list, overflow_cnt, stats);
+                       n_items, overflow_list, overflow_cnt, stats);
     const int64_t max_items = nq * ((cap + 63) / 64);
     const dim3 grid((unsigned)std::min<int64_t>((max_items + 3) / 4, kRerankGrid));
     if (v4)
-        hipLaunchKernelGGL(k_rerank_flat<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, cap,
-                           items, n_items, bound, kept, kept_cnt);
+        hipLaunchKernelGGL(k_rerank_flat<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
+                           n_items, bound, kept);
     else
-        hipLaunchKernelGGL(k_rerank_flat<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cnt, cap,
-                           items, n_items, bound, kept, kept_cnt);
-    launch_select_survivors(kept, kept_cnt, nq, cap, k, D, I, nullptr, nullptr, nullptr, s);
+        hipLaunchKernelGGL(k_rerank_flat<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
+                           n_items, bound, kept);
+    hipLaunchKernelGGL(k_select_kept, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, kept, cnt, nq, cap, k, D, I,
+                       done);
+    if (cap > kSelLds) launch_select_block(kept, cnt, nq, cap, k, D, I, done, s);
 }
 
 }  // namespace hv

@@ -283,19 +283,14 @@ constexpr int kWaveP = 256;
 constexpr int kWaveCand = 512;
 
 template <int EC>
-__device__ __forceinline__ void coarse_finish_wave(const float* __restrict__ qv, const float* __restrict__ C, int d,
-                                                   const unsigned int* __restrict__ cand, int m, int P,
+__device__ __forceinline__ void coarse_finish_wave(const unsigned long long* __restrict__ keys, int m, int P,
                                                    int* __restrict__ probes, float* __restrict__ pdist) {
     const int lane = threadIdx.x & 63;
     unsigned long long v[EC];
 #pragma unroll
     for (int e = 0; e < EC; ++e) {
         const int j = e * 64 + lane;
-        v[e] = HV_KEY_PAD;
-        if (j < m) {
-            const unsigned int c = cand[j];
-            v[e] = hv_key(coarse_chain(qv, C + (int64_t)c * d, d), c);
-        }
+        v[e] = j < m ? keys[j] : HV_KEY_PAD;
     }
     wave_bitonic_sort<EC>(v);
 #pragma unroll
@@ -318,6 +313,7 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
                                                             float* __restrict__ pdist, int* __restrict__ redo_list,
                                                             int* __restrict__ redo_cnt) {
     __shared__ unsigned int s_cand[4][kWaveCand];
+    __shared__ unsigned long long s_ckey[4][kWaveCand];
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -356,17 +352,24 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
         return;
     }
     __threadfence_block();  // the candidates other lanes stored are read below
+    // exact keys of the candidates, one chain at a time per lane
     const float* __restrict__ qv = Q + q * d;
+#pragma unroll 1
+    for (int j = lane; j < m; j += 64) {
+        const unsigned int c = s_cand[w][j];
+        s_ckey[w][j] = hv_key(coarse_chain(qv, C + (int64_t)c * d, d), c);
+    }
+    __threadfence_block();
     int* pq = probes + q * P;
     float* dq = pdist + q * P;
     if (m <= 64)
-        coarse_finish_wave<1>(qv, C, d, s_cand[w], m, P, pq, dq);
+        coarse_finish_wave<1>(s_ckey[w], m, P, pq, dq);
     else if (m <= 128)
-        coarse_finish_wave<2>(qv, C, d, s_cand[w], m, P, pq, dq);
+        coarse_finish_wave<2>(s_ckey[w], m, P, pq, dq);
     else if (m <= 256)
-        coarse_finish_wave<4>(qv, C, d, s_cand[w], m, P, pq, dq);
+        coarse_finish_wave<4>(s_ckey[w], m, P, pq, dq);
     else
-        coarse_finish_wave<8>(qv, C, d, s_cand[w], m, P, pq, dq);
+        coarse_finish_wave<8>(s_ckey[w], m, P, pq, dq);
 }
 
 // Fallback: every centroid's exact chain, LDS sort, first P (one workgroup per listed query).

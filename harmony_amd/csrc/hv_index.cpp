@@ -180,7 +180,7 @@ struct hv_index {
 
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
-        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_kept_cnt, s_stats, s_pdist, s_pair_slot,
+        s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_done, s_stats, s_pdist, s_pair_slot,
         s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_save, s_qnorm, s_cnorm, s_redo;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
@@ -581,10 +581,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(hipEventRecord(h->ev[8], s));
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
-        HV_HIP(h->s_kept_cnt.ensure(sizeof(unsigned int) * n));
+        HV_HIP(h->s_done.ensure(sizeof(unsigned int) * n));
         hv::launch_rerank_select(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
                                  (int)cap, (int)k, bound, h->s_ritems.as<int2>(), n_items,
-                                 h->s_kept.as<unsigned long long>(), h->s_kept_cnt.as<unsigned int>(), D, I, nullptr,
+                                 h->s_kept.as<unsigned long long>(), h->s_done.as<unsigned int>(), D, I, nullptr,
                                  nullptr, stats, s);
         HV_CHECK_LAUNCH("rerank + select");
         HV_HIP(hipEventRecord(h->ev[9], s));
@@ -709,7 +709,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
-                      &h->s_kept, &h->s_kept_cnt, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
+                      &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
                       &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->s_qnorm,
                       &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
     for (DevBuf* b : bufs) b->release();

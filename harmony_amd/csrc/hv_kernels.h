@@ -105,23 +105,25 @@ void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_
 void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s);
 // re-rank + final top-k of the BF16 form: work items of 64 stored pairs (items: nq * ceil(cap/64)
-// entries, n_items zeroed by the caller), exact keys at or below bound[q] kept per query
-// (kept: nq * cap keys), then the select over the kept keys. Overflowed queries are listed for
-// the exact path (their results are left as padding); stats[2] counts the re-ranked pairs.
+// entries, n_items zeroed by the caller), each pair's exact key written to its slot of kept
+// (nq * cap keys; padding when above bound[q]), then the select over the kept keys (done: nq
+// flags of scratch). Overflowed queries are listed for the exact path when overflow_list is
+// given (their results are left as they were); stats[2] counts the re-ranked pairs.
 constexpr int kRerankGrid = 2048;
 void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,
                           const ChunkPlan& plan, const unsigned long long* surv, const unsigned int* cnt, int64_t nq,
                           int cap, int k, const float* bound, int2* items, unsigned int* n_items,
-                          unsigned long long* kept, unsigned int* kept_cnt, float* D, int64_t* I, int* overflow_list,
+                          unsigned long long* kept, unsigned int* done, float* D, int64_t* I, int* overflow_list,
                           int* overflow_cnt, unsigned long long* stats, hipStream_t s);
 // the queries the exact path redoes -> list / list_cnt (both zeroed by the caller): overflowed
 // survivor buffers (cnt[q] > cap; counts[0]) and, with check_bound, queries whose k-th result is
 // missing or above bound[q] (counts[1])
 void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
                       int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, hipStream_t s);
-// the block select alone (queries with more than kWaveSortMax survivors)
+// the workgroup select alone: queries whose done flag is 0 (with done), else those with more than
+// kWaveSortMax survivors
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
-                         float* D, int64_t* I, hipStream_t s);
+                         float* D, int64_t* I, const unsigned int* done, hipStream_t s);
 
 // --- learned probe/bound predictor (hv_learn.hip)
 constexpr int kNearest = 64;                                            // nearest-centroid distances
