@@ -4,6 +4,6 @@ The product is libharmony_amd.so (C-ABI in include/harmony_ivf.h): host C++ plus
 HIP kernels. This package holds the sources, the in-tree build, and a Python mirror of the
 reference's harmony::Index class over the C-ABI.
 """
-from ._lib import HV_PRECISION_BF16, HV_PRECISION_FP32, HipExtensionMissing, HvError, lib  # noqa: F401
+from ._lib import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32, HipExtensionMissing, HvError, lib  # noqa: F401
 from .index import DeviceArray, Index  # noqa: F401
 from . import datagen, learned, metrics  # noqa: F401

@@ -25,6 +25,7 @@ HV_BOUND_WARMUP = 0
 HV_BOUND_EXTERNAL = 1
 HV_PRECISION_FP32 = 0
 HV_PRECISION_BF16 = 1
+HV_PRECISION_FP8 = 2
 HV_MLP_IN = 65
 HV_MLP_HIDDEN = 128
 HV_MLP_OUT = 2
@@ -53,7 +54,7 @@ class SearchParams(ctypes.Structure):
                 ("refine_bound", ctypes.c_int32), ("survivor_cap", ctypes.c_int64), ("tile_shape", ctypes.c_int32),
                 ("tile_shape_first", ctypes.c_int32), ("learned", ctypes.c_int32),
                 ("learned_nprobe_scale", ctypes.c_float), ("learned_bound_scale", ctypes.c_float),
-                ("learned_min_nprobe", ctypes.c_int32)]
+                ("learned_min_nprobe", ctypes.c_int32), ("rerank_r", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

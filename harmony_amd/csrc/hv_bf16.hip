@@ -29,6 +29,43 @@ namespace hv {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
+// Operand types of the filter scan: 8 residual values per lane and K-step, one 16x16x32 MFMA.
+// BF16 codes are the residuals as they are; FP8 (e4m3) codes are scaled by a power of two per
+// list (vectors) and per (query, list) slot (queries), undone on the FP32 product.
+struct ElemBf16 {
+    typedef __bf16 elem;
+    typedef bf16x8 vec;
+    static constexpr bool kScaled = false;
+    static __device__ __forceinline__ floatx4 mma(vec a, vec b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+struct ElemFp8 {
+    typedef uint8_t elem;
+    typedef long vec;
+    static constexpr bool kScaled = true;
+    static __device__ __forceinline__ floatx4 mma(vec a, vec b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+    }
+};
+
+// 8 floats -> 8 e4m3 bytes (round to nearest even, the converter's format is the MFMA's)
+__device__ __forceinline__ long pack_fp8x8(const float (&v)[8]) {
+    unsigned int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0u, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+    unsigned int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0u, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+    return (long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Power-of-two scale that brings a largest magnitude m into [64, 128) (e4m3 holds up to 448),
+// as its inverse: codes are value * 2^e, products are multiplied by 2^-e.
+__device__ __forceinline__ float fp8_inv_scale(float m) {
+    if (!(m > 0.f) || !(m < 3.0e38f)) return 1.f;
+    const int e = min(100, max(-100, 6 - ilogbf(m)));
+    return ldexpf(1.f, -e);
+}
+
 // ---------------------------------------------------------------------------------------
 // Index build: BF16 residual codes and FP32 residual norms of every list (one workgroup per
 // list), and the FP32 rows in list order for the re-rank.
@@ -118,6 +155,94 @@ void launch_pair_resid(const float* Qp, const float* centroids, const int* probe
                        nq * nprobe, nprobe, d, dpad, reinterpret_cast<bf16x8*>(out));
 }
 
+// FP8 codes of every list from the rows in list order: one workgroup per list finds the largest
+// residual magnitude, then writes the scaled e4m3 codes in the BF16 codes' order (one byte per
+// value) and the list's inverse scale.
+__global__ __launch_bounds__(256) void k_build_resid_fp8(const float* __restrict__ rows, int d, int dpad,
+                                                         const int64_t* __restrict__ list_off,
+                                                         const float* __restrict__ centroids,
+                                                         const int64_t* __restrict__ roff,
+                                                         uint8_t* __restrict__ codes, float* __restrict__ inv_scale) {
+    __shared__ float s_max[4];
+    const int l = blockIdx.x;
+    const int64_t b = list_off[l];
+    const int n = (int)(list_off[l + 1] - b);
+    const int n16 = (n + 15) / 16 * 16;
+    const float* __restrict__ c = centroids + (int64_t)l * d;
+    float m = 0.f;
+    for (int64_t e = threadIdx.x; e < (int64_t)n * d; e += blockDim.x) {
+        const int v = (int)(e / d), k = (int)(e - (int64_t)v * d);
+        m = fmaxf(m, fabsf(rows[(b + v) * d + k] - c[k]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    const float inv = fp8_inv_scale(fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3])));
+    const float sc = 1.f / inv;  // a power of two
+    if (threadIdx.x == 0) inv_scale[l] = inv;
+    uint8_t* __restrict__ out = codes + roff[l];
+    const int nks = dpad / 32;
+    // one thread per 8 consecutive dims of a vector (one packed code word)
+    const int g8 = dpad / 8;
+    for (int64_t e = threadIdx.x; e < (int64_t)n16 * g8; e += blockDim.x) {
+        const int v = (int)(e / g8), k0 = (int)(e - (int64_t)v * g8) * 8;
+        float r[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int k = k0 + t;
+            r[t] = (v < n && k < d) ? (rows[(b + v) * d + k] - c[k]) * sc : 0.f;
+        }
+        const int64_t o = (((int64_t)(v >> 4) * nks + (k0 >> 5)) * 64 + ((k0 & 31) >> 3) * 16 + (v & 15)) << 3;
+        *reinterpret_cast<long*>(out + o) = pack_fp8x8(r);
+    }
+}
+
+void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int64_t* list_off,
+                            const float* centroids, const int64_t* roff, int nlist, uint8_t* codes, float* inv_scale,
+                            hipStream_t s) {
+    if (nlist <= 0) return;
+    hipLaunchKernelGGL(k_build_resid_fp8, dim3((unsigned)nlist), dim3(256), 0, s, rows_by_pos, d, dpad, list_off,
+                       centroids, roff, codes, inv_scale);
+}
+
+// FP8 query residuals of every (query, probe rank) slot: one wavefront per slot (lane g holds
+// dims 8g .. 8g+7 of dpad <= 512), scaled by the slot's power of two.
+__global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict__ Qp,
+                                                        const float* __restrict__ centroids,
+                                                        const int* __restrict__ probes, const int* __restrict__ qn,
+                                                        int64_t nslots, int nprobe, int d, int dpad,
+                                                        long* __restrict__ out, float* __restrict__ inv_scale) {
+    const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (slot >= nslots) return;
+    if (qn && (int)(slot % nprobe) >= qn[slot / nprobe]) return;  // rank not searched
+    const float* __restrict__ qv = Qp + (slot / nprobe) * d;
+    const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
+    const int k0 = lane * 8;
+    float r[8], m = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int k = k0 + t;
+        r[t] = k < d ? qv[k] - cv[k] : 0.f;
+        m = fmaxf(m, fabsf(r[t]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    const float inv = fp8_inv_scale(m);
+    const float sc = 1.f / inv;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) r[t] *= sc;
+    if (k0 < dpad) out[slot * (dpad / 8) + lane] = pack_fp8x8(r);
+    if (lane == 0) inv_scale[slot] = inv;
+}
+
+void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
+                           int nprobe, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s) {
+    const int64_t nslots = nq * nprobe;
+    if (nslots <= 0) return;
+    hipLaunchKernelGGL(k_pair_resid_fp8, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, s, Qp, centroids, probes,
+                       qn, nslots, nprobe, d, dpad, reinterpret_cast<long*>(out), inv_scale);
+}
+
 // ---------------------------------------------------------------------------------------
 // The BF16 filter scan. Same work items as the FP32 scan (list, <= 64 queries, vector range),
 // assigned statically: workgroup b takes the items j, j + G/8, j + 2G/8, ... of the queue of its
@@ -132,10 +257,12 @@ void launch_pair_resid(const float* Qp, const float* centroids, const int* probe
 //   mode 1: the query's nearest list (probe rank 0 only): every pair of vector v is written to
 //           slot v (no atomics; the caller limits the vectors to v < cap and sets the counts with
 //           k_first_counts), key = lb, aux = ub.
-template <int NKS>
-__global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qres,
+template <int NKS, typename E>
+__global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __restrict__ qres,
+                                                   const float* __restrict__ qscale,
                                                    const float* __restrict__ pdist, int nprobe,
-                                                   const __bf16* __restrict__ rcodes,
+                                                   const typename E::elem* __restrict__ rcodes,
+                                                   const float* __restrict__ lscale,
                                                    const int64_t* __restrict__ roff,
                                                    const float* __restrict__ rnorm,
                                                    const int64_t* __restrict__ list_off,
@@ -152,6 +279,8 @@ __global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qr
     __shared__ __align__(16) float s_nrq[64];
     __shared__ __align__(16) float s_srq[64];
     __shared__ __align__(16) float s_bound[64];
+    __shared__ __align__(16) float s_qsc[64];
+    typedef typename E::vec vec;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, col16 = lane & 15;
     const int xcd = blockIdx.x % kNumXcd;
@@ -164,20 +293,23 @@ __global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qr
         const int nQ = item.pair_end - item.pair_begin;
         if (tid < 64) {
             int q = 0, slot = 0;
-            float nr = 0.f, b = -1.f;  // rows past nQ: bound -1 keeps nothing
+            float nr = 0.f, b = -1.f, qsc = 0.f;  // rows past nQ: bound -1 keeps nothing
             if (tid < nQ) {
                 slot = pair_slot[item.pair_begin + tid];
                 q = slot / nprobe;
                 nr = pdist[slot];
                 b = mode == 1 ? 0.f : bound[q];
+                if (E::kScaled) qsc = qscale[slot];
             }
             s_q[tid] = q;
             s_slot[tid] = slot;
             s_nrq[tid] = nr;
             s_srq[tid] = __builtin_amdgcn_sqrtf(nr);
             s_bound[tid] = b;
+            s_qsc[tid] = qsc;
         }
         const int l = item.list;
+        const float lsc = E::kScaled ? lscale[l] : 1.f;
         const int64_t lbeg = list_off[l];
         const int nl = (int)min(list_off[l + 1] - lbeg, (long long)vmax);
         const int vend = min(nl, item.vec_begin + vsplit);
@@ -193,21 +325,21 @@ __global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qr
             const int nqb = min(QBM, QB - qb0);
             // A operands: query row (qb0 + j)*16 + col16, dims s*32 + g*8 .. +8. Rows past nQ read
             // slot s_slot[0]; their bound is -1.
-            bf16x8 a[QBM][NKS];
+            vec a[QBM][NKS];
 #pragma unroll
             for (int j = 0; j < QBM; ++j) {
                 const int qrow = (qb0 + j) * 16 + col16;
-                const bf16x8* __restrict__ qa = reinterpret_cast<const bf16x8*>(qres) +
+                const vec* __restrict__ qa = reinterpret_cast<const vec*>(qres) +
                                                 (int64_t)s_slot[qrow < nQ ? qrow : 0] * (NKS * 4) + g;
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) a[j][s] = qa[s * 4];
             }
-            const bf16x8* __restrict__ R = reinterpret_cast<const bf16x8*>(rcodes + roff[l]) + lane;
+            const vec* __restrict__ R = reinterpret_cast<const vec*>(rcodes + roff[l]) + lane;
             const int vstep = 16 * wpg;
             int v0 = item.vec_begin + vsub * 16;
             // software pipeline: the codes and norm of the next block are in flight while the
             // current block is computed
-            bf16x8 bn[NKS];
+            vec bn[NKS];
             float nxn = 0.f;
             if (kPrefetch && v0 < vend) {
                 const int64_t blk = (int64_t)(v0 >> 4) * NKS * 64;
@@ -216,7 +348,7 @@ __global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qr
                 nxn = rnorm[lbeg + min(v0 + col16, nl - 1)];
             }
             for (; v0 < vend; v0 += vstep) {
-                bf16x8 bc[NKS];
+                vec bc[NKS];
                 float nx;
                 const int v1 = kPrefetch ? v0 + vstep : v0;
                 if (kPrefetch) {
@@ -242,7 +374,7 @@ __global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qr
                     if (j < nqb) {
 #pragma unroll
                         for (int s = 0; s < NKS; ++s)
-                            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j][s], bc[s], acc[j], 0, 0, 0);
+                            acc[j] = E::mma(a[j][s], bc[s], acc[j]);
                     }
                 }
                 const int v = v0 + col16;
@@ -263,12 +395,20 @@ __global__ __launch_bounds__(256) void k_bf16_scan(const __bf16* __restrict__ qr
                     const float nrq[4] = {nr4.x, nr4.y, nr4.z, nr4.w};
                     const float srq[4] = {sr4.x, sr4.y, sr4.z, sr4.w};
                     const float bq[4] = {b4.x, b4.y, b4.z, b4.w};
+                    float dot[4] = {acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
+                    if (E::kScaled) {
+                        const float4 q4 = *reinterpret_cast<const float4*>(&s_qsc[r0]);
+                        dot[0] *= lsc * q4.x;
+                        dot[1] *= lsc * q4.y;
+                        dot[2] *= lsc * q4.z;
+                        dot[3] *= lsc * q4.w;
+                    }
                     float lbv[4], ubv[4];
                     bool kp[4];
                     bool anyk = false;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const float dt = nrq[i] + nx - 2.f * acc[j][i];
+                        const float dt = nrq[i] + nx - 2.f * dot[i];
                         const float err = kappa1 * srq[i] * sx + kappa2 * (nrq[i] + nx);
                         lbv[i] = fmaxf(dt - err, 0.f);
                         ubv[i] = fmaxf(dt + err, 0.f);
@@ -338,44 +478,53 @@ void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_
 
 int bf16_ksteps(int d) { return (d + 31) / 32; }
 
-void launch_bf16_scan(int grid, const __bf16* qres, int d, const float* pdist, int nprobe,
-                      const __bf16* rcodes, const int64_t* roff, const float* rnorm, const int64_t* list_off,
-                      const float* bound, const int* pair_slot, const ScanItem* items, const int* queue, int vmax,
-                      int vsplit, int mode, float kappa1, float kappa2, unsigned long long* surv, float* aux,
-                      unsigned int* surv_cnt, int cap, unsigned long long* stats, hipStream_t s) {
-#define HV_BF16_CASE(K)                                                                                         \
-    case K:                                                                                                     \
-        hipLaunchKernelGGL(k_bf16_scan<K>, dim3((unsigned)grid), dim3(256), 0, s, qres, pdist, nprobe, rcodes, roff, \
-                           rnorm, list_off, bound, pair_slot, items, queue, vmax, vsplit, mode, kappa1, kappa2, surv,  \
-                           aux, surv_cnt, cap, stats);                                                               \
+void launch_mfma_scan(int grid, int elem, const void* qres, const float* qscale, int d, const float* pdist,
+                      int nprobe, const void* rcodes, const float* lscale, const int64_t* roff, const float* rnorm,
+                      const int64_t* list_off, const float* bound, const int* pair_slot, const ScanItem* items,
+                      const int* queue, int vmax, int vsplit, int mode, float kappa1, float kappa2,
+                      unsigned long long* surv, float* aux, unsigned int* surv_cnt, int cap,
+                      unsigned long long* stats, hipStream_t s) {
+#define HV_SCAN_LAUNCH(K, E)                                                                                    \
+    hipLaunchKernelGGL((k_mfma_scan<K, E>), dim3((unsigned)grid), dim3(256), 0, s,                              \
+                       reinterpret_cast<const E::elem*>(qres), qscale, pdist, nprobe,                          \
+                       reinterpret_cast<const E::elem*>(rcodes), lscale, roff, rnorm, list_off, bound, pair_slot, \
+                       items, queue, vmax, vsplit, mode, kappa1, kappa2, surv, aux, surv_cnt, cap, stats)
+#define HV_SCAN_CASE(K)                \
+    case K:                            \
+        if (elem == 1)                 \
+            HV_SCAN_LAUNCH(K, ElemFp8);  \
+        else                           \
+            HV_SCAN_LAUNCH(K, ElemBf16); \
         break;
     switch (bf16_ksteps(d)) {
-        HV_BF16_CASE(1)
-        HV_BF16_CASE(2)
-        HV_BF16_CASE(3)
-        HV_BF16_CASE(4)
-        HV_BF16_CASE(5)
-        HV_BF16_CASE(6)
-        HV_BF16_CASE(7)
-        HV_BF16_CASE(8)
-        HV_BF16_CASE(9)
-        HV_BF16_CASE(10)
-        HV_BF16_CASE(11)
-        HV_BF16_CASE(12)
-        HV_BF16_CASE(13)
-        HV_BF16_CASE(14)
-        HV_BF16_CASE(15)
-        HV_BF16_CASE(16)
+        HV_SCAN_CASE(1)
+        HV_SCAN_CASE(2)
+        HV_SCAN_CASE(3)
+        HV_SCAN_CASE(4)
+        HV_SCAN_CASE(5)
+        HV_SCAN_CASE(6)
+        HV_SCAN_CASE(7)
+        HV_SCAN_CASE(8)
+        HV_SCAN_CASE(9)
+        HV_SCAN_CASE(10)
+        HV_SCAN_CASE(11)
+        HV_SCAN_CASE(12)
+        HV_SCAN_CASE(13)
+        HV_SCAN_CASE(14)
+        HV_SCAN_CASE(15)
+        HV_SCAN_CASE(16)
         default: break;
     }
-#undef HV_BF16_CASE
+#undef HV_SCAN_CASE
+#undef HV_SCAN_LAUNCH
 }
 
 int bf16_grid_size(int device) {
     int ncu = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
         ncu = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bf16_scan<4>, 256, 0) != hipSuccess || per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mfma_scan<4, ElemBf16>, 256, 0) != hipSuccess ||
+        per_cu <= 0)
         per_cu = 4;
     // the static item assignment deals the items of each XCD queue over grid / 8 workgroups
     return std::max(kNumXcd, ncu * per_cu / kNumXcd * kNumXcd);
@@ -622,6 +771,77 @@ void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned in
         hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
 }
 
+// FP8 shortlist: each query keeps its kr smallest estimates (keys' distance bits; every key tied
+// with the kr-th stays too), compacted in place. One wavefront per query, buffers up to
+// kCompactWaveBig pairs (the FP8 form's survivor buffers are no larger); overflowed queries are
+// left to the exact path.
+template <int E>
+__device__ __forceinline__ void shortlist_wave_e(unsigned long long* __restrict__ buf, int n, int kr,
+                                                 unsigned int* __restrict__ cnt_q, unsigned int* s_tmp) {
+    const int lane = threadIdx.x & 63;
+    unsigned int kth;
+    {
+        unsigned int u[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = e * 64 + lane;
+            u[e] = j < n ? (unsigned int)(buf[j] >> 32) : 0xffffffffu;
+        }
+        kth = wave_kth_u32<E>(u, kr, s_tmp);
+    }
+    constexpr int H = E > 32 ? 32 : E;
+    unsigned int out = 0u;
+#pragma unroll
+    for (int h0 = 0; h0 < E; h0 += H) {
+        unsigned long long key[H];
+#pragma unroll
+        for (int e = 0; e < H; ++e) {
+            const int j = (h0 + e) * 64 + lane;
+            key[e] = j < n ? buf[j] : HV_KEY_PAD;
+        }
+#pragma unroll
+        for (int e = 0; e < H; ++e) {
+            const bool keep = (h0 + e) * 64 + lane < n && (unsigned int)(key[e] >> 32) <= kth;
+            const unsigned long long m = __ballot(keep);
+            if (keep) buf[out + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key[e];
+            out += (unsigned int)__builtin_popcountll(m);
+        }
+    }
+    if (lane == 0) *cnt_q = out;
+}
+
+template <bool BIG>
+__global__ __launch_bounds__(256) void k_shortlist(unsigned long long* __restrict__ surv,
+                                                   unsigned int* __restrict__ cnt, int64_t nq, int cap, int kr) {
+    __shared__ unsigned int s_tmp[4][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= nq) return;
+    const unsigned int c = cnt[q];
+    if (c > (unsigned int)cap || c <= (unsigned int)kr) return;
+    const int n = (int)c;
+    unsigned long long* buf = surv + q * cap;
+    if (BIG) {
+        if (n > kCompactWaveMax && n <= kCompactWaveBig)
+            shortlist_wave_e<kCompactWaveBig / 64>(buf, n, kr, cnt + q, s_tmp[w]);
+    } else if (n <= 256) {
+        shortlist_wave_e<4>(buf, n, kr, cnt + q, s_tmp[w]);
+    } else if (n <= 512) {
+        shortlist_wave_e<8>(buf, n, kr, cnt + q, s_tmp[w]);
+    } else if (n <= 1024) {
+        shortlist_wave_e<16>(buf, n, kr, cnt + q, s_tmp[w]);
+    } else if (n <= kCompactWaveMax) {
+        shortlist_wave_e<32>(buf, n, kr, cnt + q, s_tmp[w]);
+    }
+}
+
+void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, hipStream_t s) {
+    if (nq <= 0) return;
+    const dim3 grid((unsigned)((nq + 3) / 4));
+    hipLaunchKernelGGL(k_shortlist<false>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr);
+    if (cap > kCompactWaveMax) hipLaunchKernelGGL(k_shortlist<true>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr);
+}
+
 // ---------------------------------------------------------------------------------------
 // Flat exact re-rank. A query's stored pairs are cut into work items of 64 (one pair per lane),
 // so a query with many survivors spreads over many wavefronts instead of holding one. Each
@@ -728,7 +948,7 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
     int q = __builtin_amdgcn_readfirstlane(item.x);
     int j0 = __builtin_amdgcn_readfirstlane(item.y & 0xffff), c = __builtin_amdgcn_readfirstlane(item.y >> 16);
     uint32_t pos = lane < c ? (uint32_t)(surv[(int64_t)q * cap + j0 + lane] & 0xffffffffu) : 0u;
-    float b = bound[q];
+    float b = bound ? bound[q] : 3.402823466e38f;
     const int last = plan.n - 1;
     while (true) {
         const bool valid = lane < c;
@@ -747,7 +967,7 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
             j0n = __builtin_amdgcn_readfirstlane(nitem.y & 0xffff);
             cn = __builtin_amdgcn_readfirstlane(nitem.y >> 16);
             posn = lane < cn ? (uint32_t)(surv[(int64_t)qn * cap + j0n + lane] & 0xffffffffu) : 0u;
-            bn = bound[qn];
+            bn = bound ? bound[qn] : 3.402823466e38f;
         }
         float dis = 0.f, p = 0.f;
         // the next chunk's loads are in flight during each chain (the last chunk is peeled so

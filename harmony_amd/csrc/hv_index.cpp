@@ -169,6 +169,11 @@ struct hv_index {
     // BF16 residual codes per list (MFMA operand order), their FP32 norms, and the FP32 rows in list order
     DevBuf d_rcodes, d_roff, d_rnorm, d_rows;
     int dpad = 0, bf16_grid = 0;
+    // FP8 (e4m3) codes per list in the same order and their per-list inverse scales, built on the
+    // first FP8-form search after an add
+    DevBuf d_rcodes8, d_lscale8;
+    bool fp8_built = false;
+    int64_t rcode_elems = 0;  // code values of all lists (both code arrays)
 
     hv_search_params sp{};
     std::vector<float> ext_bounds;
@@ -181,7 +186,7 @@ struct hv_index {
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_done, s_stats, s_pdist, s_pair_slot,
-        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_save, s_qnorm, s_cnorm, s_redo;
+        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_save, s_qnorm, s_cnorm, s_redo, s_qscale;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
 };
@@ -279,6 +284,7 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
     for (int64_t l = 0; l < nlist; ++l) roff[l + 1] = roff[l] + (int64_t)h->list_stride[l] * h->dpad;
     upload(h->d_roff, roff.data(), sizeof(int64_t) * (nlist + 1), h->stream);
     HV_HIP(h->d_rcodes.ensure(sizeof(__bf16) * std::max<int64_t>(1, roff[nlist])));
+    h->rcode_elems = roff[nlist];
     HV_HIP(h->d_rnorm.ensure(sizeof(float) * std::max<int64_t>(1, n)));
     HV_HIP(h->d_rows.ensure(sizeof(float) * std::max<int64_t>(1, n * d)));
     if (n) {
@@ -291,6 +297,7 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
     HV_HIP(hipStreamSynchronize(h->stream));
     order_buf.release();
     h->ntotal = n;
+    h->fp8_built = false;
     h->max_list = 0;
     for (int64_t l = 0; l < nlist; ++l) h->max_list = std::max<int64_t>(h->max_list, off[l + 1] - off[l]);
     h->added = true;
@@ -355,18 +362,25 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     if (learned)
         require(h->mlp_params && h->mlp_norm, HV_ERR_STATE,
                 "learned search needs a predictor (hv_mlp_init/train or set_params, and hv_mlp_set_norm)");
+    const bool fp8 = h->sp.precision == HV_PRECISION_FP8;
     const int64_t nprobe_cap = nprobe;
     if (learned) nprobe = std::min<int64_t>(nlist, std::max<int64_t>(nprobe, hv::kNearest));
     int64_t wl = learned ? 0 : std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
     int64_t wls = h->sp.warmup_list_size == 0 ? k : h->sp.warmup_list_size;  // query.cpp:328-333
     if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
         require(wl * wls <= hv::kMaxWarmup, HV_ERR_CAPACITY, "warmup_list * warmup_list_size > 8192");
-    const bool external = !learned && h->sp.bound_mode == HV_BOUND_EXTERNAL;
+    const bool external = !learned && !fp8 && h->sp.bound_mode == HV_BOUND_EXTERNAL;
     if (external)
         require((int64_t)h->ext_bounds.size() == n, HV_ERR_STATE, "external bounds not set for this batch");
+    // FP8 form: the shortlist of the kr smallest estimates per query is re-ranked exactly
+    const int64_t kr = fp8 ? k * std::max(1, h->sp.rerank_r) : k;
     int64_t cap = h->sp.survivor_cap;
-    if (cap == 0) cap = std::min<int64_t>(8192, std::max<int64_t>(4096, hv_next_pow2((int)(4 * k))));
+    if (cap == 0) cap = std::min<int64_t>(fp8 ? 4096 : 8192, std::max<int64_t>(4096, hv_next_pow2((int)(4 * k))));
     require(cap >= k && cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap must be in [k, 8192]");
+    if (fp8) {
+        require(cap <= 4096, HV_ERR_INVALID_ARG, "survivor_cap must be <= 4096 in the FP8 form");
+        require(kr <= cap, HV_ERR_CAPACITY, "k * rerank_r must not exceed the survivor buffer (4096)");
+    }
     require((int64_t)n * nprobe < (1ll << 31), HV_ERR_CAPACITY, "n * nprobe >= 2^31");
 
     hipStream_t s = h->stream;
@@ -470,37 +484,59 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
     };
     // BF16 precision form: one phase of the MFMA filter scan over probe ranks [r0, r1)
+    // MFMA filter forms: BF16 (certified error bound, exact results) and FP8 (estimates only, a
+    // shortlist of k * rerank_r per query re-ranked exactly)
     const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
+    const bool mfma = bf16 || fp8;
     float* aux = nullptr;
     float kappa1 = 0.f, kappa2 = 0.f;
-    if (bf16) {
+    if (mfma) {
         HV_HIP(h->s_aux.ensure(sizeof(float) * n * cap));
         aux = h->s_aux.as<float>();
+        if (h->bf16_grid == 0) h->bf16_grid = hv::bf16_grid_size(h->device);
+    }
+    if (bf16) {
         // error bound of the BF16 estimate (see hv_bf16.hip): BF16 unit roundoff 2^-8 on both
         // residuals, FP32 accumulation over d terms, FP32 norms, and the FP32 chain of the exact path
         const double ubf = std::ldexp(1.0, -8), u = std::ldexp(1.0, -24);
         const double gd = d * u / (1.0 - d * u);
         kappa1 = (float)(1.01 * 2.0 * (2.0 * (ubf + u) + (ubf + u) * (ubf + u) + gd * (1.0 + ubf) * (1.0 + ubf)));
         kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
-        if (h->bf16_grid == 0) h->bf16_grid = hv::bf16_grid_size(h->device);
         HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
         hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)d, h->dpad,
                               h->s_qres.as<__bf16>(), s);
         HV_CHECK_LAUNCH("pair residuals");
     }
-    auto bf16_phase = [&](int r0, int r1, int vmin, int vmax, int mode, unsigned long long* st_slot) {
+    if (fp8) {
+        if (!h->fp8_built) {
+            HV_HIP(h->d_rcodes8.ensure(std::max<int64_t>(1, h->rcode_elems)));
+            HV_HIP(h->d_lscale8.ensure(sizeof(float) * nlist));
+            hv::launch_build_resid_fp8(h->d_rows.as<float>(), (int)d, h->dpad, h->d_list_off.as<int64_t>(),
+                                       h->d_centroids.as<float>(), h->d_roff.as<int64_t>(), (int)nlist,
+                                       h->d_rcodes8.as<uint8_t>(), h->d_lscale8.as<float>(), s);
+            HV_CHECK_LAUNCH("fp8 codes");
+            h->fp8_built = true;
+        }
+        HV_HIP(h->s_qres.ensure(n * nprobe * h->dpad));
+        HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
+        hv::launch_pair_resid_fp8(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)d, h->dpad,
+                                  h->s_qres.as<uint8_t>(), h->s_qscale.as<float>(), s);
+        HV_CHECK_LAUNCH("pair residuals");
+    }
+    auto mfma_phase = [&](int r0, int r1, int vmin, int vmax, int mode, unsigned long long* st_slot) {
         hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, kVecSplitBf16,
                               h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
                               h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
         main_recorded |= main_phase;
         if (main_phase) HV_HIP(hipEventRecord(h->ev[6], s));
-        hv::launch_bf16_scan(h->bf16_grid, h->s_qres.as<__bf16>(), (int)d, pdist, (int)nprobe,
-                             h->d_rcodes.as<__bf16>(), h->d_roff.as<int64_t>(), h->d_rnorm.as<float>(),
-                             h->d_list_off.as<int64_t>(), h->s_bound.as<float>(), h->s_pair_slot.as<int>(),
-                             h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplitBf16, mode, kappa1, kappa2, surv, aux,
-                             surv_cnt, (int)cap, st_slot, s);
-        HV_CHECK_LAUNCH("bf16 scan");
+        hv::launch_mfma_scan(h->bf16_grid, fp8 ? 1 : 0, h->s_qres.p, fp8 ? h->s_qscale.as<float>() : nullptr, (int)d,
+                             pdist, (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
+                             fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
+                             h->d_rnorm.as<float>(), h->d_list_off.as<int64_t>(), h->s_bound.as<float>(),
+                             h->s_pair_slot.as<int>(), h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplitBf16,
+                             mode, kappa1, kappa2, surv, aux, surv_cnt, (int)cap, st_slot, s);
+        HV_CHECK_LAUNCH("mfma scan");
         if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
     };
     // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
@@ -510,14 +546,15 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     float* bound = h->s_bound.as<float>();
     float* bound_ref = h->s_bound_ref.as<float>();
     // learned, BF16: no bound for the nearest lists (their k-th upper bound gives the safe bound
-    // below, then the seed lowers it); learned, FP32: the seed from the start. No warm-up.
-    if (learned && !bf16) {
+    // below, then the seed lowers it); learned, FP32: the seed from the start. No warm-up. The
+    // FP8 form starts without a bound and filters on estimates only.
+    if (learned && !mfma) {
         HV_HIP(hipMemcpyAsync(bound_ref, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     } else if (external) {
         HV_HIP(hipMemcpyAsync(bound_ref, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    } else if (wl > 0 && !bf16 && !learned) {
+    } else if (wl > 0 && !mfma && !learned) {
         scan_phase(0, (int)wl, (int)wls, 0, 1, stats + 4);
         hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, s);
     } else {
@@ -542,22 +579,27 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         bound_safe = h->s_save.as<float>();
         hv::launch_apply_seed(bound, seed, bound_safe, bound_ref, n, s);
     };
-    if (bf16) {
+    if (mfma) {
         // nearest list: vectors below cap go to their own slot unfiltered; the k-th smallest upper
-        // bound then lowers the bound and the pairs whose lower bound exceeds it are dropped
-        // (vector ranges start on 16-vector blocks)
+        // bound (FP8: the kr-th smallest estimate) then lowers the bound and the pairs whose lower
+        // bound exceeds it are dropped (vector ranges start on 16-vector blocks)
         const int vdirect = (int)(cap / 16 * 16);
-        if (vdirect > 0) bf16_phase(0, 1, 0, vdirect, 1, stats);
+        if (vdirect > 0) mfma_phase(0, 1, 0, vdirect, 1, stats);
         hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
-        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
+        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, s);
         HV_CHECK_LAUNCH("compact");
-        if (learned)
+        if (learned && bf16)
             apply_seed();
         else
             HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
-        if (h->max_list > vdirect) bf16_phase(0, 1, vdirect, INT32_MAX, 0, stats);
-        if (nprobe > 1) bf16_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
+        if (h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
+        if (nprobe > 1) mfma_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
+        // FP8: the kr smallest estimates of each query go on to the exact re-rank
+        if (fp8) {
+            hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, s);
+            HV_CHECK_LAUNCH("shortlist");
+        }
     } else {
         int r_next = 0;
         if ((h->sp.refine_bound || learned) && nprobe > 1) {
@@ -577,13 +619,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     unsigned int* n_items = reinterpret_cast<unsigned int*>(redo_cnt + 1);
     int* redo_list = redo_cnt + 2;
     hv::launch_fill_i32(redo_cnt, 2, 0, s);
-    if (bf16) {
+    if (mfma) {
         HV_HIP(hipEventRecord(h->ev[8], s));
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
         HV_HIP(h->s_done.ensure(sizeof(unsigned int) * n));
         hv::launch_rerank_select(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
-                                 (int)cap, (int)k, bound, h->s_ritems.as<int2>(), n_items,
+                                 (int)cap, (int)k, fp8 ? nullptr : bound, h->s_ritems.as<int2>(), n_items,
                                  h->s_kept.as<unsigned long long>(), h->s_done.as<unsigned int>(), D, I, nullptr,
                                  nullptr, stats, s);
         HV_CHECK_LAUNCH("rerank + select");
@@ -597,9 +639,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     //    been pruned), over the same probe lists, with the safe bound in learned mode (none in the
     //    FP32 form). The list and its count stay on the device: the kernels take the first batch
     //    by themselves, and the host adds batches only when the final count exceeds one.
-    hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound, learned ? 1 : 0, n, redo_list, redo_cnt, stats + 12,
-                         s);
-    const float* redo_bound = learned ? bound_safe : bound;
+    hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound, learned && !fp8 ? 1 : 0, n, redo_list, redo_cnt,
+                         stats + 12, s);
+    const float* redo_bound = fp8 ? nullptr : (learned ? bound_safe : bound);
     const int nseg = hv::exact_segments(h->max_list);
     const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
     const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
@@ -648,7 +690,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             st->ms_main += mm;
             st->main_pairs += (int64_t)hs[8];
         }
-        if (bf16) {
+        if (mfma) {
             float mr = 0.f;
             HV_HIP(hipEventElapsedTime(&mr, h->ev[8], h->ev[9]));
             st->ms_rerank += mr;
@@ -711,7 +753,8 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
                       &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->s_qnorm,
-                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows};
+                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows, &h->s_qscale,
+                      &h->d_rcodes8, &h->d_lscale8};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1024,6 +1067,7 @@ void hv_default_search_params(hv_search_params* p) {
     p->warmup_list = 10;
     p->warmup_list_size = 0;
     p->precision = HV_PRECISION_FP32;
+    p->rerank_r = 4;
     p->refine_bound = 1;
     p->survivor_cap = 0;
     p->learned = 0;
@@ -1042,8 +1086,10 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
         require(p->bound_mode == HV_BOUND_WARMUP || p->bound_mode == HV_BOUND_EXTERNAL, HV_ERR_INVALID_ARG,
                 "invalid bound_mode");
         require(p->warmup_list >= 0 && p->warmup_list_size >= 0, HV_ERR_INVALID_ARG, "negative warm-up size");
-        require(p->precision == HV_PRECISION_FP32 || p->precision == HV_PRECISION_BF16, HV_ERR_INVALID_ARG,
-                "unsupported precision");
+        require(p->precision == HV_PRECISION_FP32 || p->precision == HV_PRECISION_BF16 ||
+                    p->precision == HV_PRECISION_FP8,
+                HV_ERR_INVALID_ARG, "unsupported precision");
+        require(p->rerank_r >= 1, HV_ERR_INVALID_ARG, "rerank_r must be >= 1");
         require(p->survivor_cap >= 0 && p->survivor_cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap > 8192");
         require(p->tile_shape >= 0 && p->tile_shape <= hv::kNumShapes && p->tile_shape_first >= 0 &&
                     p->tile_shape_first <= hv::kNumShapes,

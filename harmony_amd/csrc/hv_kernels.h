@@ -94,20 +94,33 @@ void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, f
 // BF16 (q - c) rows of dpad dims for every (query, probe rank) slot
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
                        int nprobe, int d, int dpad, __bf16* out, hipStream_t s);
-void launch_bf16_scan(int grid, const __bf16* qres, int d, const float* pdist, int nprobe,
-                      const __bf16* rcodes, const int64_t* roff, const float* rnorm, const int64_t* list_off,
-                      const float* bound, const int* pair_slot, const ScanItem* items, const int* queue, int vmax,
-                      int vsplit, int mode, float kappa1, float kappa2, unsigned long long* surv, float* aux,
-                      unsigned int* surv_cnt, int cap, unsigned long long* stats, hipStream_t s);
+// FP8 (e4m3) codes of every list from the rows in list order, scaled per list by a power of two
+// (inverse in inv_scale[l]), in the BF16 codes' order, one byte per value
+void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int64_t* list_off,
+                            const float* centroids, const int64_t* roff, int nlist, uint8_t* codes, float* inv_scale,
+                            hipStream_t s);
+// FP8 (q - c) rows of every slot, scaled per slot (inverse in inv_scale[slot])
+void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
+                           int nprobe, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s);
+// the MFMA filter scan: elem 0 = BF16 codes (qscale, lscale unused), 1 = FP8 codes with the
+// per-slot and per-list inverse scales
+void launch_mfma_scan(int grid, int elem, const void* qres, const float* qscale, int d, const float* pdist,
+                      int nprobe, const void* rcodes, const float* lscale, const int64_t* roff, const float* rnorm,
+                      const int64_t* list_off, const float* bound, const int* pair_slot, const ScanItem* items,
+                      const int* queue, int vmax, int vsplit, int mode, float kappa1, float kappa2,
+                      unsigned long long* surv, float* aux, unsigned int* surv_cnt, int cap,
+                      unsigned long long* stats, hipStream_t s);
+// FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place
+void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, hipStream_t s);
 // counts after the direct-slot scan of the nearest lists (vectors v < cap)
 void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
                          unsigned int* cnt, hipStream_t s);
 void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s);
-// re-rank + final top-k of the BF16 form: work items of 64 stored pairs (items: nq * ceil(cap/64)
+// re-rank + final top-k of the MFMA forms: work items of 64 stored pairs (items: nq * ceil(cap/64)
 // entries, n_items zeroed by the caller), each pair's exact key written to its slot of kept
-// (nq * cap keys; padding when above bound[q]), then the select over the kept keys (done: nq
-// flags of scratch). Overflowed queries are listed for the exact path when overflow_list is
+// (nq * cap keys; padding when above bound[q], no filter when bound is null), then the select
+// over the kept keys (done: nq flags of scratch). Overflowed queries are listed for the exact path when overflow_list is
 // given (their results are left as they were); stats[2] counts the re-ranked pairs.
 constexpr int kRerankGrid = 2048;
 void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,

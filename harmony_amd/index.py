@@ -118,11 +118,13 @@ class Index:
 
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
                   survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None,
-                  learned=None, learned_nprobe_scale=None, learned_bound_scale=None, learned_min_nprobe=None):
+                  learned=None, learned_nprobe_scale=None, learned_bound_scale=None, learned_min_nprobe=None,
+                  rerank_r=None):
         """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59).
 
-        precision: HV_PRECISION_FP32 (default) or HV_PRECISION_BF16 (MFMA filter + FP32 distances
-        of the survivors; same results). None keeps the current setting.
+        precision: HV_PRECISION_FP32 (default), HV_PRECISION_BF16 (MFMA filter + FP32 distances
+        of the survivors; same results) or HV_PRECISION_FP8 (FP8 MFMA estimates, the k * rerank_r
+        smallest per query re-ranked in FP32; approximate). None keeps the current setting.
 
         learned (None keeps the current settings): per-query probe count and pruning bound from
         the trained predictor (mlp_*): probes = ceil(predicted nprobe * learned_nprobe_scale),
@@ -140,6 +142,8 @@ class Index:
             p.learned_bound_scale = float(learned_bound_scale)
         if learned_min_nprobe is not None:
             p.learned_min_nprobe = int(learned_min_nprobe)
+        if rerank_r is not None:
+            p.rerank_r = int(rerank_r)
         p.refine_bound = 1 if refine_bound else 0
         p.tile_shape = int(tile_shape)
         p.tile_shape_first = int(tile_shape_first)

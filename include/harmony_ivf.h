@@ -52,6 +52,10 @@ extern "C" {
 #define HV_PRECISION_FP32 0 /* FP32 blocked partial sums on every candidate (reference arithmetic) */
 #define HV_PRECISION_BF16 1 /* BF16 MFMA filter with a certified error bound, then the FP32 distance
                                of each survivor: same results as HV_PRECISION_FP32 */
+#define HV_PRECISION_FP8 2  /* FP8 (e4m3, power-of-two scale per list and per query slot) MFMA
+                               estimates; the k * rerank_r smallest per query are re-ranked with the
+                               FP32 distance (approximate: recall below 1 is possible, the returned
+                               distances are exact) */
 
 typedef struct hv_index hv_index_t;
 
@@ -85,6 +89,8 @@ typedef struct {
     float learned_nprobe_scale;   /* probes = ceil(predicted nprobe * scale), at most nprobe */
     float learned_bound_scale;    /* seeded bound = predicted k-th distance * scale (<= 0: none) */
     int32_t learned_min_nprobe;   /* lower limit of the per-query probe count */
+    /* FP8 form (HV_PRECISION_FP8): shortlist width factor r, k * r <= 4096 (default 4) */
+    int32_t rerank_r;
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */
