@@ -54,7 +54,8 @@ class SearchParams(ctypes.Structure):
                 ("refine_bound", ctypes.c_int32), ("survivor_cap", ctypes.c_int64), ("tile_shape", ctypes.c_int32),
                 ("tile_shape_first", ctypes.c_int32), ("learned", ctypes.c_int32),
                 ("learned_nprobe_scale", ctypes.c_float), ("learned_bound_scale", ctypes.c_float),
-                ("learned_min_nprobe", ctypes.c_int32), ("rerank_r", ctypes.c_int32)]
+                ("learned_min_nprobe", ctypes.c_int32), ("rerank_r", ctypes.c_int32),
+                ("graph", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -87,6 +88,7 @@ SIGNATURES = {
     "hv_set_search_params": (ctypes.c_int, [_P, ctypes.POINTER(SearchParams)]),
     "hv_get_search_params": (ctypes.c_int, [_P, ctypes.POINTER(SearchParams)]),
     "hv_set_external_bounds": (ctypes.c_int, [_P, _I64, _P]),
+    "hv_set_query_nprobe": (ctypes.c_int, [_P, _I64, _P]),
     "hv_search": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_search_device": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_ground_truth_f64": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P]),

@@ -19,15 +19,23 @@
 #include "hv_device.h"
 #include "hv_kernels.h"
 
+#include <atomic>
+#include <functional>
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// Bumped by every device allocation and free of the search buffers: a captured search graph
+// holds their addresses and is replayed only while this has not moved.
+std::atomic<uint64_t> g_alloc_gen{0};
 
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
+        g_alloc_gen.fetch_add(1);
         if (p) {
             hipError_t e = hipFree(p);
             if (e != hipSuccess) return e;
@@ -41,12 +49,64 @@ struct DevBuf {
     }
     void release() {
         if (p) (void)hipFree(p);
+        g_alloc_gen.fetch_add(1);
         p = nullptr;
         cap = 0;
     }
     template <typename T>
     T* as() const {
         return static_cast<T*>(p);
+    }
+};
+
+// Pinned host staging (a graph's host-to-device copy reads it at replay time)
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        g_alloc_gen.fetch_add(1);
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+        if (e == hipSuccess) cap = std::max<size_t>(bytes, 256);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// What the host does after a search's device work: read the counters, add redo batches, report.
+struct SearchPost {
+    int batch = 1;
+    bool main_recorded = false, mfma = false, per_query = false;
+    int B = 1;
+    int64_t nprobe = 0, nprobe_cap = 0;
+    unsigned long long* stats = nullptr;
+    int* redo_cnt = nullptr;
+    std::function<void(int)> redo;
+};
+
+// A search captured as a HIP graph, replayed while the signature and the buffers match
+struct SearchGraph {
+    std::vector<unsigned char> sig;
+    uint64_t gen = 0;
+    hipGraphExec_t exec = nullptr;
+    SearchPost post;
+    std::vector<unsigned char> pending_sig;  // an eager search waiting for a second identical call
+    uint64_t pending_gen = 0;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        sig.clear();
+        pending_sig.clear();
     }
 };
 
@@ -177,6 +237,9 @@ struct hv_index {
 
     hv_search_params sp{};
     std::vector<float> ext_bounds;
+    std::vector<int32_t> query_nprobe;  // per-query probe counts (empty: nprobe for all)
+    PinnedBuf qn_host;                  // the clamped counts of the current search
+    SearchGraph graph;
 
     // learned probe/bound predictor: FP32 parameters and Adam moments, BF16 serving weights,
     // feature/target standardization
@@ -341,6 +404,63 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
     HV_CHECK_LAUNCH("coarse");
 }
 
+// After a search's device work: read the counters and the redo count (the one host sync), add
+// redo batches beyond the first, and fill the report (phase times only when the phases were
+// recorded, i.e. not from a captured graph).
+void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, bool timed,
+                   std::chrono::steady_clock::time_point t0) {
+    hipStream_t s = h->stream;
+    unsigned long long hs[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int nredo = 0;
+    HV_HIP(hipMemcpyAsync(hs, p.stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HV_HIP(hipMemcpyAsync(&nredo, p.redo_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+    HV_HIP(hipStreamSynchronize(s));
+    if (nredo > p.batch) {
+        for (int q0 = p.batch; q0 < nredo; q0 += p.batch) p.redo(q0);
+        HV_CHECK_LAUNCH("exact redo");
+        HV_HIP(hipStreamSynchronize(s));
+    }
+    h->last_n = n;
+    h->last_nprobe = p.nprobe_cap;
+    h->last_stride = p.nprobe;
+    h->last_learned = p.per_query;
+    if (st) {
+        if (timed) {
+            float ms[5];
+            for (int i = 0; i < 5; ++i) HV_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+            st->ms_coarse += ms[0];
+            st->ms_group += ms[1];
+            st->ms_warmup += ms[2];
+            st->ms_scan += ms[3];
+            st->ms_select += ms[4];
+            if (p.main_recorded) {
+                float mm = 0.f;
+                HV_HIP(hipEventElapsedTime(&mm, h->ev[6], h->ev[7]));
+                st->ms_main += mm;
+            }
+            if (p.mfma) {
+                float mr = 0.f;
+                HV_HIP(hipEventElapsedTime(&mr, h->ev[8], h->ev[9]));
+                st->ms_rerank += mr;
+            }
+        } else {
+            float mg = 0.f;  // the whole device work of the graph
+            HV_HIP(hipEventElapsedTime(&mg, h->ev[0], h->ev[5]));
+            st->ms_scan += mg;
+        }
+        const int64_t cand = (int64_t)(hs[0] + hs[8]);
+        st->candidates += cand;
+        st->block_evals += (int64_t)(hs[1] + hs[9]);
+        st->block_evals_max += cand * p.B;
+        st->survivors += (int64_t)hs[2];
+        st->overflow_queries += (int64_t)hs[12];
+        st->fallback_queries += (int64_t)hs[13];
+        if (p.main_recorded) st->main_pairs += (int64_t)hs[8];
+        if (p.mfma) st->rerank_pairs += (int64_t)hs[2];  // the select sees exactly the re-ranked pairs
+        st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
 void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D, int64_t* I, hv_stats* st) {
     require(h->trained && h->added, HV_ERR_STATE, "search before train/add");
     require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
@@ -359,6 +479,11 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // learned predictor: per-query probe counts (at most nprobe) and seeded bounds; the probe
     // lists are then selected at least 64 deep (its features are the 64 nearest distances)
     const bool learned = h->sp.learned != 0;
+    const bool uneven = !h->query_nprobe.empty();
+    if (uneven) {
+        require((int64_t)h->query_nprobe.size() == n, HV_ERR_STATE, "per-query nprobe not set for this batch");
+        require(!learned, HV_ERR_INVALID_ARG, "per-query nprobe and the learned predictor are exclusive");
+    }
     if (learned)
         require(h->mlp_params && h->mlp_norm, HV_ERR_STATE,
                 "learned search needs a predictor (hv_mlp_init/train or set_params, and hv_mlp_set_norm)");
@@ -385,7 +510,56 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
 
     hipStream_t s = h->stream;
     auto t0 = std::chrono::steady_clock::now();
+    // per-query probe counts, clamped, in pinned memory (a captured search copies them from there)
+    auto stage_qn = [&]() {
+        HV_HIP(h->qn_host.ensure(sizeof(int) * n));
+        int* qh = static_cast<int*>(h->qn_host.p);
+        for (int64_t i = 0; i < n; ++i) qh[i] = (int)std::min<int64_t>(nprobe, std::max<int64_t>(1, h->query_nprobe[i]));
+    };
+    // graph mode: the device work of a search is captured on the second call with the same
+    // signature (the first one sizes every buffer) and replayed from then on
+    const bool graph_ok = h->sp.graph != 0 && !external;
+    std::vector<unsigned char> sig;
+    if (graph_ok) {
+        auto put = [&](const void* v, size_t b) {
+            const unsigned char* c = static_cast<const unsigned char*>(v);
+            sig.insert(sig.end(), c, c + b);
+        };
+        const int64_t shape[6] = {n, k, h->ntotal, h->max_list, (int64_t)uneven, (int64_t)h->fp8_built};
+        const void* ptrs[3] = {q_dev, D, I};
+        put(shape, sizeof(shape));
+        put(ptrs, sizeof(ptrs));
+        put(&h->sp, sizeof(h->sp));
+        if (h->graph.exec && h->graph.sig == sig && h->graph.gen == g_alloc_gen.load()) {
+            if (uneven) stage_qn();
+            HV_HIP(hipEventRecord(h->ev[0], s));
+            HV_HIP(hipGraphLaunch(h->graph.exec, s));
+            HV_HIP(hipEventRecord(h->ev[5], s));
+            finish_search(h, h->graph.post, n, st, false, t0);
+            return;
+        }
+    }
+    const bool capture = graph_ok && h->graph.pending_sig == sig && h->graph.pending_gen == g_alloc_gen.load();
+    auto rec = [&](int i) {
+        if (!capture) HV_HIP(hipEventRecord(h->ev[i], s));
+    };
     HV_HIP(hipEventRecord(h->ev[0], s));
+    if (capture) {
+        h->graph.reset();
+        HV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    }
+    SearchPost post;
+    // an error thrown while capturing ends the capture (the stream stays usable)
+    struct CaptureGuard {
+        hipStream_t s;
+        bool active;
+        ~CaptureGuard() {
+            if (!active) return;
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+        }
+    } guard{s, capture};
 
     // 1. queries in index dim order
     const float* Qp = q_dev;
@@ -415,8 +589,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("predictor");
         qn = h->s_qn.as<int>();
         seed = h->s_seed.as<float>();
+    } else if (uneven) {
+        stage_qn();
+        HV_HIP(h->s_qn.ensure(sizeof(int) * n));
+        HV_HIP(hipMemcpyAsync(h->s_qn.p, h->qn_host.p, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        qn = h->s_qn.as<int>();
     }
-    HV_HIP(hipEventRecord(h->ev[1], s));
+    rec(1);
     // 3. grouping by (list, rank)
     const int64_t nb = nlist * nprobe;
     HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
@@ -443,7 +622,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     hv::launch_scatter_pairs(probes, n, (int)nprobe, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
                              h->s_pairs.as<int>(), h->s_pair_slot.as<int>(), s);
     HV_CHECK_LAUNCH("grouping");
-    HV_HIP(hipEventRecord(h->ev[2], s));
+    rec(2);
 
     int* queue = h->s_queue.as<int>();
     // survivor buffer: room for the warm-up set too
@@ -474,14 +653,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                               h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
         main_recorded |= main_phase;
-        if (main_phase) HV_HIP(hipEventRecord(h->ev[6], s));
+        if (main_phase) rec(6);
         hv::launch_list_scan(shape, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
                              h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), plan,
                              prune, h->s_bound.as<float>(), h->s_pairs.as<int>(), h->s_items.as<hv::ScanItem>(),
                              queue, vmax, vsplit, accept_all, surv, surv_cnt, (int)cap, st_slot, h->scan_grid[shape],
                              s);
         HV_CHECK_LAUNCH("list scan");
-        if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
+        if (main_phase) rec(7);
     };
     // BF16 precision form: one phase of the MFMA filter scan over probe ranks [r0, r1)
     // MFMA filter forms: BF16 (certified error bound, exact results) and FP8 (estimates only, a
@@ -529,7 +708,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                               h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
         main_recorded |= main_phase;
-        if (main_phase) HV_HIP(hipEventRecord(h->ev[6], s));
+        if (main_phase) rec(6);
         hv::launch_mfma_scan(h->bf16_grid, fp8 ? 1 : 0, h->s_qres.p, fp8 ? h->s_qscale.as<float>() : nullptr, (int)d,
                              pdist, (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
                              fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
@@ -537,7 +716,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              h->s_pair_slot.as<int>(), h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplitBf16,
                              mode, kappa1, kappa2, surv, aux, surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("mfma scan");
-        if (main_phase) HV_HIP(hipEventRecord(h->ev[7], s));
+        if (main_phase) rec(7);
     };
     // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
     //    warm-up scan: it scans each query's nearest list unpruned and takes the bound from there.
@@ -565,7 +744,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_fill_i32(reinterpret_cast<int*>(bound_ref), n, fm, s);
     }
     HV_CHECK_LAUNCH("bound");
-    HV_HIP(hipEventRecord(h->ev[3], s));
+    rec(3);
     // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
     //    the remaining probed lists
     const int prune = B > 1 ? 1 : 0;
@@ -611,7 +790,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         }
         scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
     }
-    HV_HIP(hipEventRecord(h->ev[4], s));
+    rec(4);
     // 6. top-k selection. The BF16 form re-ranks its survivors (the plain path's FP32 arithmetic)
     //    and selects in the same pass.
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
@@ -620,7 +799,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     int* redo_list = redo_cnt + 2;
     hv::launch_fill_i32(redo_cnt, 2, 0, s);
     if (mfma) {
-        HV_HIP(hipEventRecord(h->ev[8], s));
+        rec(8);
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
         HV_HIP(h->s_done.ensure(sizeof(unsigned int) * n));
@@ -629,7 +808,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                                  h->s_kept.as<unsigned long long>(), h->s_done.as<unsigned int>(), D, I, nullptr,
                                  nullptr, stats, s);
         HV_CHECK_LAUNCH("rerank + select");
-        HV_HIP(hipEventRecord(h->ev[9], s));
+        rec(9);
     } else {
         hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, nullptr, nullptr, stats, s);
         HV_CHECK_LAUNCH("select");
@@ -646,58 +825,44 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
     const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
     HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
-    auto redo = [&](int q0) {
+    post.batch = batch;
+    post.main_recorded = main_recorded;
+    post.mfma = mfma;
+    post.per_query = learned || uneven;
+    post.B = B;
+    post.nprobe = nprobe;
+    post.nprobe_cap = nprobe_cap;
+    post.stats = stats;
+    post.redo_cnt = redo_cnt;
+    unsigned long long* runs = h->s_runs.as<unsigned long long>();
+    post.redo = [=](int q0) {
         hv::launch_query_exact(redo_list, redo_cnt, q0, batch, Qp, (int)d, h->d_codes.as<float>(),
                                h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(),
                                h->d_ids.as<uint32_t>(), probes, (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune,
-                               redo_bound, (int)k, h->s_runs.as<unsigned long long>(), D, I, s);
+                               redo_bound, (int)k, runs, D, I, s);
     };
-    redo(0);
+    post.redo(0);
     HV_CHECK_LAUNCH("exact redo");
+    if (capture) {
+        hipGraph_t g = nullptr;
+        guard.active = false;
+        HV_HIP(hipStreamEndCapture(s, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HV_HIP(ie);
+        h->graph.exec = ex;
+        h->graph.sig = sig;
+        h->graph.gen = g_alloc_gen.load();
+        h->graph.post = post;
+        HV_HIP(hipGraphLaunch(ex, s));
+    }
     HV_HIP(hipEventRecord(h->ev[5], s));
-    unsigned long long hs[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int nredo = 0;
-    HV_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
-    HV_HIP(hipMemcpyAsync(&nredo, redo_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-    HV_HIP(hipStreamSynchronize(s));
-    if (nredo > batch) {
-        for (int q0 = batch; q0 < nredo; q0 += batch) redo(q0);
-        HV_CHECK_LAUNCH("exact redo");
-        HV_HIP(hipStreamSynchronize(s));
+    finish_search(h, capture ? h->graph.post : post, n, st, !capture, t0);
+    if (graph_ok && !capture) {
+        h->graph.pending_sig = sig;
+        h->graph.pending_gen = g_alloc_gen.load();
     }
-    h->last_n = n;
-    h->last_nprobe = nprobe_cap;
-    h->last_stride = nprobe;
-    h->last_learned = learned;
-    if (st) {
-        float ms[5];
-        for (int i = 0; i < 5; ++i) HV_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
-        st->ms_coarse += ms[0];
-        st->ms_group += ms[1];
-        st->ms_warmup += ms[2];
-        st->ms_scan += ms[3];
-        st->ms_select += ms[4];
-        const int64_t cand = (int64_t)(hs[0] + hs[8]);
-        st->candidates += cand;
-        st->block_evals += (int64_t)(hs[1] + hs[9]);
-        st->block_evals_max += cand * B;
-        st->survivors += (int64_t)hs[2];
-        st->overflow_queries += (int64_t)hs[12];
-        st->fallback_queries += (int64_t)hs[13];
-        if (main_recorded) {
-            float mm = 0.f;
-            HV_HIP(hipEventElapsedTime(&mm, h->ev[6], h->ev[7]));
-            st->ms_main += mm;
-            st->main_pairs += (int64_t)hs[8];
-        }
-        if (mfma) {
-            float mr = 0.f;
-            HV_HIP(hipEventElapsedTime(&mr, h->ev[8], h->ev[9]));
-            st->ms_rerank += mr;
-            st->rerank_pairs += (int64_t)hs[2];  // the select sees exactly the re-ranked pairs
-        }
-    }
-    if (st) st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // namespace
@@ -747,6 +912,8 @@ hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_leve
 void hv_destroy(hv_index_t* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->graph.reset();
+    h->qn_host.release();
     DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
@@ -1112,6 +1279,14 @@ int hv_set_external_bounds(hv_index_t* h, int64_t n, const float* bounds) {
     return guarded(h, [&] {
         require(n >= 0 && (bounds || n == 0), HV_ERR_INVALID_ARG, "bad bounds");
         h->ext_bounds.assign(bounds, bounds + n);
+    });
+}
+
+int hv_set_query_nprobe(hv_index_t* h, int64_t n, const int32_t* nprobe) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(n >= 0 && (nprobe || n == 0), HV_ERR_INVALID_ARG, "bad per-query nprobe");
+        h->query_nprobe.assign(nprobe, nprobe + n);
     });
 }
 

@@ -119,7 +119,7 @@ class Index:
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
                   survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None,
                   learned=None, learned_nprobe_scale=None, learned_bound_scale=None, learned_min_nprobe=None,
-                  rerank_r=None):
+                  rerank_r=None, graph=None):
         """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59).
 
         precision: HV_PRECISION_FP32 (default), HV_PRECISION_BF16 (MFMA filter + FP32 distances
@@ -144,6 +144,8 @@ class Index:
             p.learned_min_nprobe = int(learned_min_nprobe)
         if rerank_r is not None:
             p.rerank_r = int(rerank_r)
+        if graph is not None:
+            p.graph = 1 if graph else 0
         p.refine_bound = 1 if refine_bound else 0
         p.tile_shape = int(tile_shape)
         p.tile_shape_first = int(tile_shape_first)
@@ -158,6 +160,12 @@ class Index:
     def set_external_bounds(self, bounds):
         b = np.ascontiguousarray(bounds, dtype=np.float32)
         check(self._h, lib().hv_set_external_bounds(self._h, b.size, _ptr(b)))
+
+    def set_query_nprobe(self, nprobe_per_query):
+        """Per-query probe counts for the next searches (None clears): query i scans its first
+        nprobe_per_query[i] lists (clamped to [1, nprobe])."""
+        a = np.zeros(0, np.int32) if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.int32)
+        check(self._h, lib().hv_set_query_nprobe(self._h, a.size, _ptr(a)))
 
     # -- build
     def train(self, x, niter=10, seed=0, max_points_per_centroid=256):

@@ -91,6 +91,10 @@ typedef struct {
     int32_t learned_min_nprobe;   /* lower limit of the per-query probe count */
     /* FP8 form (HV_PRECISION_FP8): shortlist width factor r, k * r <= 4096 (default 4) */
     int32_t rerank_r;
+    /* 1: capture the device work of a search as a HIP graph on the second call with the same
+     * shape, buffers and parameters, and replay it from then on (one launch per search; phase
+     * times are then reported as one figure in ms_scan). Not with HV_BOUND_EXTERNAL. */
+    int32_t graph;
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */
@@ -137,6 +141,11 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p);
 int hv_get_search_params(const hv_index_t* h, hv_search_params* p);
 /* per-query external bound for HV_BOUND_EXTERNAL (heapTops, Index.h:60) */
 int hv_set_external_bounds(hv_index_t* h, int64_t n, const float* bounds);
+/* per-query probe counts for the next searches of n queries (each clamped to [1, nprobe]; the
+ * search still selects nprobe lists per query and scans the first nprobe_q of them); n = 0
+ * clears. No counterpart in the reference (one nprobe for every query, Index.h:117); not
+ * combined with the learned predictor, which sets its own counts. */
+int hv_set_query_nprobe(hv_index_t* h, int64_t n, const int32_t* nprobe);
 /* host buffers: q [n][d] f32 row-major, D [n][k] f32, I [n][k] int64 */
 int hv_search(hv_index_t* h, int64_t n, const float* q, int64_t k, float* D, int64_t* I, hv_stats* st);
 /* device buffers (already in HBM): same layouts */

@@ -104,6 +104,64 @@ def test_overflow_redo_in_several_batches(w1, precision):
     np.testing.assert_array_equal(D, oD)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_uneven_nprobe_matches_oracle_per_query(w1, precision):
+    """Per-query probe counts: each query's results are the oracle's at its own nprobe."""
+    xb, xq, g, o = w1
+    npq = np.random.default_rng(3).integers(1, 11, size=xq.shape[0]).astype(np.int32)
+    g.nprobe = 10
+    g.preSearch(pruning=True, dim_blocks=4, precision=precision)
+    g.set_query_nprobe(npq)
+    D, I = g.search(xq, 10)
+    got = g.last_nprobe(xq.shape[0])
+    g.set_query_nprobe(None)
+    g.preSearch(pruning=True, dim_blocks=4, precision=0)
+    np.testing.assert_array_equal(got, npq)
+    for p in np.unique(npq):
+        rows = np.where(npq == p)[0]
+        oD, oI, _, _ = O.search(o, xq[rows], 10, nprobe=int(p), dim_blocks=4)
+        np.testing.assert_array_equal(I[rows], oI)
+        np.testing.assert_array_equal(D[rows], oD)
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_graph_replays_match_eager(w1, precision):
+    """Graph mode: the first call sizes the buffers, the second captures, later ones replay.
+    Every call (with eager searches of another shape in between) returns the eager results."""
+    xb, xq, g, o = w1
+    g.nprobe = 10
+    g.preSearch(pruning=True, dim_blocks=4, precision=precision, graph=False)
+    eD, eI = g.search(xq, 10)
+    for it in range(5):
+        g.preSearch(pruning=True, dim_blocks=4, precision=precision, graph=True)
+        D, I = g.search(xq, 10)
+        np.testing.assert_array_equal(I, eI)
+        np.testing.assert_array_equal(D, eD)
+        g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False)
+        sD, sI = g.search(xq[:7], 5)
+        oD, oI, _, _ = O.search(o, xq[:7], 5, nprobe=10, dim_blocks=4)
+        np.testing.assert_array_equal(sI, oI)
+    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False)
+
+
+def test_graph_replay_follows_per_query_nprobe(w1):
+    xb, xq, g, o = w1
+    g.nprobe = 10
+    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=True)
+    rng = np.random.default_rng(5)
+    for it in range(4):
+        npq = rng.integers(1, 11, size=xq.shape[0]).astype(np.int32)
+        g.set_query_nprobe(npq)
+        D, I = g.search(xq, 10)
+        for p in np.unique(npq):
+            rows = np.where(npq == p)[0]
+            oD, oI, _, _ = O.search(o, xq[rows], 10, nprobe=int(p), dim_blocks=4)
+            np.testing.assert_array_equal(I[rows], oI)
+            np.testing.assert_array_equal(D[rows], oD)
+    g.set_query_nprobe(None)
+    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False)
+
+
 def test_external_bound(w1):
     xb, xq, g, o = w1
     g.nprobe = 10
