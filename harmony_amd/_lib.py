@@ -55,7 +55,7 @@ class SearchParams(ctypes.Structure):
                 ("tile_shape_first", ctypes.c_int32), ("learned", ctypes.c_int32),
                 ("learned_nprobe_scale", ctypes.c_float), ("learned_bound_scale", ctypes.c_float),
                 ("learned_min_nprobe", ctypes.c_int32), ("rerank_r", ctypes.c_int32),
-                ("graph", ctypes.c_int32)]
+                ("graph", ctypes.c_int32), ("scan_vsplit", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

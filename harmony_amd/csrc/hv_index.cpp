@@ -606,7 +606,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_pair_slot.ensure(sizeof(int) * n * nprobe));
     // work items of one phase: at most (query tiles) x (vector ranges); query tiles hold >= 16
     // queries, vector ranges >= kVecSplit vectors
-    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / std::min(kVecSplit, kVecSplitBf16) + 1);
+    const int vsplit_mfma = h->sp.scan_vsplit > 0 ? h->sp.scan_vsplit : kVecSplitBf16;
+    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / std::min(kVecSplit, vsplit_mfma) + 1);
     require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
@@ -703,7 +704,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("pair residuals");
     }
     auto mfma_phase = [&](int r0, int r1, int vmin, int vmax, int mode, unsigned long long* st_slot) {
-        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, kVecSplitBf16,
+        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, vsplit_mfma,
                               h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
                               h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
@@ -713,7 +714,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              pdist, (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
                              fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
                              h->d_rnorm.as<float>(), h->d_list_off.as<int64_t>(), h->s_bound.as<float>(),
-                             h->s_pair_slot.as<int>(), h->s_items.as<hv::ScanItem>(), queue, vmax, kVecSplitBf16,
+                             h->s_pair_slot.as<int>(), h->s_items.as<hv::ScanItem>(), queue, vmax, vsplit_mfma,
                              mode, kappa1, kappa2, surv, aux, surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("mfma scan");
         if (main_phase) rec(7);
@@ -1257,6 +1258,8 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
                     p->precision == HV_PRECISION_FP8,
                 HV_ERR_INVALID_ARG, "unsupported precision");
         require(p->rerank_r >= 1, HV_ERR_INVALID_ARG, "rerank_r must be >= 1");
+        require(p->scan_vsplit >= 0 && p->scan_vsplit % 16 == 0 && p->scan_vsplit <= (1 << 20), HV_ERR_INVALID_ARG,
+                "scan_vsplit must be 0 or a positive multiple of 16");
         require(p->survivor_cap >= 0 && p->survivor_cap <= 8192, HV_ERR_INVALID_ARG, "survivor_cap > 8192");
         require(p->tile_shape >= 0 && p->tile_shape <= hv::kNumShapes && p->tile_shape_first >= 0 &&
                     p->tile_shape_first <= hv::kNumShapes,

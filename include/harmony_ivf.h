@@ -95,6 +95,8 @@ typedef struct {
      * shape, buffers and parameters, and replay it from then on (one launch per search; phase
      * times are then reported as one figure in ms_scan). Not with HV_BOUND_EXTERNAL. */
     int32_t graph;
+    /* vectors per work item of the MFMA filter scans (a multiple of 16; 0 = 256) */
+    int32_t scan_vsplit;
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */

@@ -162,6 +162,22 @@ def test_graph_replay_follows_per_query_nprobe(w1):
     g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_keeps_results(w1, precision):
+    from harmony_amd.autotune import autotune
+    xb, xq, g, o = w1
+    g.nprobe = 10
+    g.preSearch(pruning=True, dim_blocks=4, precision=precision)
+    best, trials = autotune(g, xq, 10, reps=1, log=say)
+    assert len(trials) >= 3 and best["ms"] == min(trials.values())
+    D, I = g.search(xq, 10)
+    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False, tile_shape=0, tile_shape_first=0,
+                scan_vsplit=0)
+    oD, oI, _, _ = O.search(o, xq, 10, nprobe=10, dim_blocks=4)
+    np.testing.assert_array_equal(I, oI)
+    np.testing.assert_array_equal(D, oD)
+
+
 def test_external_bound(w1):
     xb, xq, g, o = w1
     g.nprobe = 10
