@@ -6,16 +6,22 @@ coarse quantizer -> bound -> grouped list scan -> per-query top-k. The workload 
 10,000 queries; the operating point is the smallest nprobe of {16, 64, 256} whose recall@100
 against the FP64 ground truth is >= 0.95.
 
-Forms measured at that nprobe:
+Forms measured at that nprobe (each with the tile shapes, MFMA item length and graph mode its
+autotuner picks on the same batch; none of these change results):
   plain-fp32    warm-up bound, FP32 blocked partial sums with pruning on every candidate
   bf16          BF16 MFMA filter with a certified error bound, exact FP32 re-rank of survivors
                 (same ids and distances as plain-fp32)
+  fp8           FP8 e4m3 MFMA estimates (power-of-two scales per list and per query slot), the
+                k * r smallest per query re-ranked in FP32; r is the smallest of 1, 2, 4 whose
+                recall@100 reaches 0.95 (approximate: recall below the plain path's is possible)
   learned-bf16  the bf16 form with each query's probe count trimmed and its bound seeded by the
                 learned predictor (65 -> 128 -> 2 MLP trained on the GPU on traces of 200,000
                 training queries; serving scales calibrated on 10,000 separate validation
                 queries; nprobe capped at the operating point); recall judged on the 10,000
                 evaluation queries against the FP64 ground truth
-`value` is the learned-bf16 QPS (the headline form); every form's QPS is under config.forms.
+`value` is the learned-bf16 QPS (the headline form); every form's QPS, recall@100 and distance
+error against the FP64 ground truth (over the returned ids the ground truth also holds) are
+under config.forms.
 
 Prints one JSON line. `roofline` is for the headline form's dominant kernel, the longer of
   - the scan of the remaining probed lists: algorithmic FLOPs per (query, vector) pair (2d
@@ -109,6 +115,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--train-queries", type=int, default=200000)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-fp8", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this size (a copy kernel of known traffic, "
                          "for scaling HBM counters in a profiler run)")
@@ -168,11 +176,18 @@ def main():
         nprobe = sweep[0]
     index.nprobe = nprobe
 
-    from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP32
+    from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32
 
-    def measure(precision, use_learned=False):
+    from harmony_amd.autotune import autotune
+
+    def measure(precision, use_learned=False, rerank_r=4):
         index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B, precision=precision,
-                        learned=use_learned)
+                        learned=use_learned, rerank_r=rerank_r, graph=False, tile_shape=0, tile_shape_first=0,
+                        scan_vsplit=0)
+        tuned = None
+        if not args.no_autotune:
+            tuned, _ = autotune(index, xq, k, sample=nq, reps=3)
+            log("autotuned: %s" % tuned)
         for _ in range(args.warmup):
             index.search_device(q_dev, nq, k, D_dev, I_dev)
         times, sts = [], []
@@ -183,14 +198,49 @@ def main():
             sts.append(st)
         ms = 1000.0 * float(np.mean(times))
         avg = {kk: float(np.mean([st[kk] for st in sts])) for kk in sts[0]}
-        return ms, avg, I_dev.to_host(np.int64, (nq, k))
+        if tuned and tuned.get("graph"):
+            # graph replays report the whole device work as one figure: take the kernel timings
+            # from an eager call with the same tuned shapes
+            index.preSearch(graph=False)
+            est = index.search_device(q_dev, nq, k, D_dev, I_dev)
+            for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan", "ms_select", "ms_main", "ms_rerank"):
+                avg[kk] = est[kk]
+            index.preSearch(graph=True)
+        return ms, avg, I_dev.to_host(np.int64, (nq, k)), D_dev.to_host(np.float32, (nq, k)), tuned
+
+    def dist_error(D, I):
+        """max relative |D - D_fp64| over the returned ids that the FP64 ground truth holds"""
+        if args.no_recall:
+            return None
+        worst = 0.0
+        for i in range(0, nq, 7):
+            m = dict(zip(GI[i].tolist(), GD[i].tolist()))
+            for dist, j in zip(D[i], I[i]):
+                ref = m.get(int(j))
+                if ref is not None and ref > 0:
+                    worst = max(worst, abs(float(dist) - ref) / ref)
+        return worst
 
     forms = {}
     ids = {}
     runs = [("plain-fp32", HV_PRECISION_FP32, False, 3.0, PEAK_FP32_TFLOPS, "fp32-valu (direct-difference sub+fma)",
              "k_list_scan"),
             ("bf16", HV_PRECISION_BF16, False, 2.0, PEAK_BF16_TFLOPS, "bf16 mfma (v_mfma_f32_16x16x32_bf16)",
-             "k_bf16_scan")]
+             "k_mfma_scan")]
+    fp8_r = None
+    if not args.no_fp8:
+        fp8_r = 4
+        if not args.no_recall:
+            for r in (1, 2, 4):
+                index.preSearch(precision=HV_PRECISION_FP8, rerank_r=r, learned=False)
+                index.search_device(q_dev, nq, k, D_dev, I_dev)
+                rr = metrics.recall(I_dev.to_host(np.int64, (nq, k)), GI)
+                log("fp8 r %d recall@100 %.4f" % (r, rr))
+                if rr >= 0.95:
+                    fp8_r = r
+                    break
+        runs.append(("fp8", HV_PRECISION_FP8, False, 2.0, 5000.0, "fp8 mfma (v_mfma_f32_16x16x32_fp8_fp8)",
+                     "k_mfma_scan"))
     learned_info = None
     if not args.no_learned:
         t0 = time.perf_counter()
@@ -211,9 +261,9 @@ def main():
             "bound scale %.3f, validation recall %.4f, mean probes %.2f"
             % (t1 - t0, t2 - t1, losses[0], losses[-1], s_np, s_b, rec_v, probes_v))
         runs.append(("learned-bf16", HV_PRECISION_BF16, True, 2.0, PEAK_BF16_TFLOPS,
-                     "bf16 mfma (v_mfma_f32_16x16x32_bf16)", "k_bf16_scan"))
+                     "bf16 mfma (v_mfma_f32_16x16x32_bf16)", "k_mfma_scan"))
     for name, prec, use_learned, flops_per_dim, peak, pipe, kern in runs:
-        ms, st, I = measure(prec, use_learned)
+        ms, st, I, Dh, tuned = measure(prec, use_learned, fp8_r if prec == HV_PRECISION_FP8 else 4)
         ids[name] = I
         flops = flops_per_dim * d * st["main_pairs"]
         ach = flops / (st["ms_main"] / 1000.0) / 1e12 if st["ms_main"] > 0 else 0.0
@@ -223,12 +273,13 @@ def main():
         if st["ms_rerank"] > st["ms_main"]:
             rbytes = (4.0 * d + 20.0) * st["rerank_pairs"]
             gbs = rbytes / (st["ms_rerank"] / 1000.0) / 1e9
-            roof = {"bound": "hbm", "kernel": "k_rerank", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+            roof = {"bound": "hbm", "kernel": "k_rerank_flat", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 5), "algorithmic_bytes_per_launch": rbytes,
                     "kernel_ms": round(st["ms_rerank"], 4), "pairs": int(st["rerank_pairs"])}
         rec = metrics.recall(I, GI) if not args.no_recall else None
         forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 4),
-                       "recall_at_100": rec,
+                       "recall_at_100": rec, "max_rel_dist_error_vs_fp64": dist_error(Dh, I),
+                       "autotuned": tuned,
                        "stage_ms": {kk: round(st[kk], 4) for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan",
                                                                     "ms_select", "ms_main")},
                        "candidates": int(st["candidates"]), "main_pairs": int(st["main_pairs"]),
@@ -238,13 +289,15 @@ def main():
                        "roofline": roof}
         if use_learned:
             forms[name]["mean_probes"] = round(float(index.last_nprobe(nq).mean()), 3)
+        if prec == HV_PRECISION_FP8:
+            forms[name]["rerank_r"] = fp8_r
         log("%s: %.3f ms/step, %.0f QPS, recall %s, main scan %.3f ms (%.1f TFLOP/s), re-rank %.3f ms; "
             "dominant %s %.1f%% of peak" % (name, ms, nq / (ms / 1000.0), rec, st["ms_main"], ach, st["ms_rerank"],
                                            roof["kernel"], 100 * roof["frac"]))
     same = bool(np.array_equal(ids["plain-fp32"], ids["bf16"]))
     best = "learned-bf16" if "learned-bf16" in forms else "bf16"
     # secondary figure: the host-buffer C-ABI call (queries and results crossing PCIe), headline form
-    index.preSearch(learned=best == "learned-bf16", precision=HV_PRECISION_BF16)
+    index.preSearch(learned=best == "learned-bf16", precision=HV_PRECISION_BF16, graph=False)
     index.search(xq, k)
     t0 = time.perf_counter()
     for _ in range(3):
@@ -278,7 +331,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16+f32" if best == "bf16" else "f32",
+        "dtype": "bf16+f32",
         "data": "synthetic",
         "config": {"workload": "int-mixture-1m: 1,000,000 x 128 fp32 base, 10,000 queries, nlist 1024, k 100, "
                                "B=4 dimension blocks, pruning on",
