@@ -205,42 +205,59 @@ void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int
                        centroids, roff, codes, inv_scale);
 }
 
-// FP8 query residuals of every (query, probe rank) slot: one wavefront per slot (lane g holds
-// dims 8g .. 8g+7 of dpad <= 512), scaled by the slot's power of two.
+// FP8 query residuals of every (query, probe rank) slot: a group of G lanes per slot (G = the
+// slot's 8-dim words rounded up to a power of two, <= 64; lane g holds dims 8g .. 8g+7), scaled
+// by the slot's power of two.
+template <int G>
 __global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict__ Qp,
                                                         const float* __restrict__ centroids,
                                                         const int* __restrict__ probes, const int* __restrict__ qn,
                                                         int64_t nslots, int nprobe, int d, int dpad,
                                                         long* __restrict__ out, float* __restrict__ inv_scale) {
-    const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (slot >= nslots) return;
-    if (qn && (int)(slot % nprobe) >= qn[slot / nprobe]) return;  // rank not searched
-    const float* __restrict__ qv = Qp + (slot / nprobe) * d;
-    const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
-    const int k0 = lane * 8;
+    const int64_t slot = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int g = threadIdx.x % G;
+    const bool live = slot < nslots && !(qn && (int)(slot % nprobe) >= qn[slot / nprobe]);
     float r[8], m = 0.f;
+    const int k0 = g * 8;
+    if (live) {
+        const float* __restrict__ qv = Qp + (slot / nprobe) * d;
+        const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int k = k0 + t;
-        r[t] = k < d ? qv[k] - cv[k] : 0.f;
-        m = fmaxf(m, fabsf(r[t]));
+        for (int t = 0; t < 8; ++t) {
+            const int k = k0 + t;
+            r[t] = k < d ? qv[k] - cv[k] : 0.f;
+            m = fmaxf(m, fabsf(r[t]));
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    for (int o = G / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));  // within the group
+    if (!live) return;
     const float inv = fp8_inv_scale(m);
     const float sc = 1.f / inv;
 #pragma unroll
     for (int t = 0; t < 8; ++t) r[t] *= sc;
-    if (k0 < dpad) out[slot * (dpad / 8) + lane] = pack_fp8x8(r);
-    if (lane == 0) inv_scale[slot] = inv;
+    if (k0 < dpad) out[slot * (dpad / 8) + g] = pack_fp8x8(r);
+    if (g == 0) inv_scale[slot] = inv;
 }
 
 void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
                            int nprobe, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s) {
     const int64_t nslots = nq * nprobe;
     if (nslots <= 0) return;
-    hipLaunchKernelGGL(k_pair_resid_fp8, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, s, Qp, centroids, probes,
-                       qn, nslots, nprobe, d, dpad, reinterpret_cast<long*>(out), inv_scale);
+    const int words = dpad / 8;  // <= 64
+#define HV_RESID8(G)                                                                                           \
+    hipLaunchKernelGGL(k_pair_resid_fp8<G>, dim3((unsigned)((nslots + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
+                       Qp, centroids, probes, qn, nslots, nprobe, d, dpad, reinterpret_cast<long*>(out), inv_scale)
+    if (words <= 4)
+        HV_RESID8(4);
+    else if (words <= 8)
+        HV_RESID8(8);
+    else if (words <= 16)
+        HV_RESID8(16);
+    else if (words <= 32)
+        HV_RESID8(32);
+    else
+        HV_RESID8(64);
+#undef HV_RESID8
 }
 
 // ---------------------------------------------------------------------------------------
@@ -772,12 +789,13 @@ void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned in
 }
 
 // FP8 shortlist: each query keeps its kr smallest estimates (keys' distance bits; every key tied
-// with the kr-th stays too), compacted in place. One wavefront per query, buffers up to
-// kCompactWaveBig pairs (the FP8 form's survivor buffers are no larger); overflowed queries are
-// left to the exact path.
+// with the kr-th stays too), compacted in place, and its filter bound drops to the kr-th
+// estimate. One wavefront per query, buffers up to kCompactWaveBig pairs (the FP8 form's
+// survivor buffers are no larger); overflowed queries are left to the exact path.
 template <int E>
 __device__ __forceinline__ void shortlist_wave_e(unsigned long long* __restrict__ buf, int n, int kr,
-                                                 unsigned int* __restrict__ cnt_q, unsigned int* s_tmp) {
+                                                 unsigned int* __restrict__ cnt_q, float* __restrict__ bound_q,
+                                                 unsigned int* s_tmp) {
     const int lane = threadIdx.x & 63;
     unsigned int kth;
     {
@@ -807,12 +825,16 @@ __device__ __forceinline__ void shortlist_wave_e(unsigned long long* __restrict_
             out += (unsigned int)__builtin_popcountll(m);
         }
     }
-    if (lane == 0) *cnt_q = out;
+    if (lane == 0) {
+        *cnt_q = out;
+        *bound_q = fminf(*bound_q, __uint_as_float(kth));
+    }
 }
 
 template <bool BIG>
 __global__ __launch_bounds__(256) void k_shortlist(unsigned long long* __restrict__ surv,
-                                                   unsigned int* __restrict__ cnt, int64_t nq, int cap, int kr) {
+                                                   unsigned int* __restrict__ cnt, int64_t nq, int cap, int kr,
+                                                   float* __restrict__ bound) {
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -823,23 +845,25 @@ __global__ __launch_bounds__(256) void k_shortlist(unsigned long long* __restric
     unsigned long long* buf = surv + q * cap;
     if (BIG) {
         if (n > kCompactWaveMax && n <= kCompactWaveBig)
-            shortlist_wave_e<kCompactWaveBig / 64>(buf, n, kr, cnt + q, s_tmp[w]);
+            shortlist_wave_e<kCompactWaveBig / 64>(buf, n, kr, cnt + q, bound + q, s_tmp[w]);
     } else if (n <= 256) {
-        shortlist_wave_e<4>(buf, n, kr, cnt + q, s_tmp[w]);
+        shortlist_wave_e<4>(buf, n, kr, cnt + q, bound + q, s_tmp[w]);
     } else if (n <= 512) {
-        shortlist_wave_e<8>(buf, n, kr, cnt + q, s_tmp[w]);
+        shortlist_wave_e<8>(buf, n, kr, cnt + q, bound + q, s_tmp[w]);
     } else if (n <= 1024) {
-        shortlist_wave_e<16>(buf, n, kr, cnt + q, s_tmp[w]);
+        shortlist_wave_e<16>(buf, n, kr, cnt + q, bound + q, s_tmp[w]);
     } else if (n <= kCompactWaveMax) {
-        shortlist_wave_e<32>(buf, n, kr, cnt + q, s_tmp[w]);
+        shortlist_wave_e<32>(buf, n, kr, cnt + q, bound + q, s_tmp[w]);
     }
 }
 
-void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, hipStream_t s) {
+void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, float* bound,
+                      hipStream_t s) {
     if (nq <= 0) return;
     const dim3 grid((unsigned)((nq + 3) / 4));
-    hipLaunchKernelGGL(k_shortlist<false>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr);
-    if (cap > kCompactWaveMax) hipLaunchKernelGGL(k_shortlist<true>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr);
+    hipLaunchKernelGGL(k_shortlist<false>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr, bound);
+    if (cap > kCompactWaveMax)
+        hipLaunchKernelGGL(k_shortlist<true>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr, bound);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1095,16 +1119,19 @@ __global__ __launch_bounds__(256) void k_select_kept(const unsigned long long* _
     if (lane == 0) done[q] = 1u;
 }
 
-void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,
-                          const ChunkPlan& plan, const unsigned long long* surv, const unsigned int* cnt, int64_t nq,
-                          int cap, int k, const float* bound, int2* items, unsigned int* n_items,
-                          unsigned long long* kept, unsigned int* done, float* D, int64_t* I, int* overflow_list,
-                          int* overflow_cnt, unsigned long long* stats, hipStream_t s) {
+void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, int2* items, unsigned int* n_items,
+                         int* overflow_list, int* overflow_cnt, unsigned long long* stats, hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_rerank_items, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, cnt, nq, cap, items,
+                       n_items, overflow_list, overflow_cnt, stats);
+}
+
+void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
+                   const unsigned long long* surv, int64_t nq, int cap, const float* bound, const int2* items,
+                   const unsigned int* n_items, unsigned long long* kept, hipStream_t s) {
     if (nq <= 0) return;
     bool v4 = d >= 4 && (d & 3) == 0;
     for (int ch = 0; ch < plan.n; ++ch) v4 = v4 && (plan.lo[ch] & 3) == 0;
-    hipLaunchKernelGGL(k_rerank_items, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, cnt, nq, cap, items,
-                       n_items, overflow_list, overflow_cnt, stats);
     const int64_t max_items = nq * ((cap + 63) / 64);
     const dim3 grid((unsigned)std::min<int64_t>((max_items + 3) / 4, kRerankGrid));
     if (v4)
@@ -1113,6 +1140,11 @@ void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, cons
     else
         hipLaunchKernelGGL(k_rerank_flat<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
                            n_items, bound, kept);
+}
+
+void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt, int64_t nq, int cap, int k, float* D,
+                        int64_t* I, unsigned int* done, hipStream_t s) {
+    if (nq <= 0) return;
     hipLaunchKernelGGL(k_select_kept, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, kept, cnt, nq, cap, k, D, I,
                        done);
     if (cap > kSelLds) launch_select_block(kept, cnt, nq, cap, k, D, I, done, s);

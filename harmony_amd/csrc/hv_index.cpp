@@ -708,8 +708,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                               h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
                               h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
+        if (main_phase && !main_recorded) rec(6);  // events 6-7 span the main phases
         main_recorded |= main_phase;
-        if (main_phase) rec(6);
         hv::launch_mfma_scan(h->bf16_grid, fp8 ? 1 : 0, h->s_qres.p, fp8 ? h->s_qscale.as<float>() : nullptr, (int)d,
                              pdist, (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
                              fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
@@ -774,10 +774,20 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
-        if (nprobe > 1) mfma_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
-        // FP8: the kr smallest estimates of each query go on to the exact re-rank
-        if (fp8) {
-            hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, s);
+        if (!fp8) {
+            if (nprobe > 1) mfma_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
+        } else {
+            // FP8: the estimates carry no error bound, so the filter is the kr-th smallest
+            // estimate so far; the probe ranks go in groups of growing width ([1, 4), [4, 16),
+            // ...), each followed by the shortlist, which keeps the kr smallest and lowers the
+            // filter. The last shortlist is what the exact re-rank sees.
+            for (int64_t r0 = 1; r0 < nprobe;) {
+                const int64_t r1 = std::min<int64_t>(nprobe, r0 * 4);
+                mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, stats + 8);
+                hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
+                r0 = r1;
+            }
+            if (nprobe == 1) hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
             HV_CHECK_LAUNCH("shortlist");
         }
     } else {
@@ -800,16 +810,17 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     int* redo_list = redo_cnt + 2;
     hv::launch_fill_i32(redo_cnt, 2, 0, s);
     if (mfma) {
-        rec(8);
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
         HV_HIP(h->s_done.ensure(sizeof(unsigned int) * n));
-        hv::launch_rerank_select(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, surv_cnt, n,
-                                 (int)cap, (int)k, fp8 ? nullptr : bound, h->s_ritems.as<int2>(), n_items,
-                                 h->s_kept.as<unsigned long long>(), h->s_done.as<unsigned int>(), D, I, nullptr,
-                                 nullptr, stats, s);
-        HV_CHECK_LAUNCH("rerank + select");
+        hv::launch_rerank_items(surv_cnt, n, (int)cap, h->s_ritems.as<int2>(), n_items, nullptr, nullptr, stats, s);
+        rec(8);  // events 8-9 bracket the re-rank kernel alone
+        hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, n, (int)cap,
+                          fp8 ? nullptr : bound, h->s_ritems.as<int2>(), n_items, h->s_kept.as<unsigned long long>(), s);
         rec(9);
+        hv::launch_select_kept(h->s_kept.as<unsigned long long>(), surv_cnt, n, (int)cap, (int)k, D, I,
+                               h->s_done.as<unsigned int>(), s);
+        HV_CHECK_LAUNCH("rerank + select");
     } else {
         hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, nullptr, nullptr, stats, s);
         HV_CHECK_LAUNCH("select");

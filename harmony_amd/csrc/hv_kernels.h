@@ -110,24 +110,31 @@ void launch_mfma_scan(int grid, int elem, const void* qres, const float* qscale,
                       const int* queue, int vmax, int vsplit, int mode, float kappa1, float kappa2,
                       unsigned long long* surv, float* aux, unsigned int* surv_cnt, int cap,
                       unsigned long long* stats, hipStream_t s);
-// FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place
-void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, hipStream_t s);
+// FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place;
+// bound[q] drops to the kr-th estimate
+void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, float* bound,
+                      hipStream_t s);
 // counts after the direct-slot scan of the nearest lists (vectors v < cap)
 void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
                          unsigned int* cnt, hipStream_t s);
 void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s);
-// re-rank + final top-k of the MFMA forms: work items of 64 stored pairs (items: nq * ceil(cap/64)
-// entries, n_items zeroed by the caller), each pair's exact key written to its slot of kept
-// (nq * cap keys; padding when above bound[q], no filter when bound is null), then the select
-// over the kept keys (done: nq flags of scratch). Overflowed queries are listed for the exact path when overflow_list is
-// given (their results are left as they were); stats[2] counts the re-ranked pairs.
+// re-rank + final top-k of the MFMA forms, in three launches:
+//  launch_rerank_items: work items of 64 stored pairs (items: nq * ceil(cap/64) entries, n_items
+//    zeroed by the caller); overflowed queries listed when overflow_list is given; stats[2]
+//    counts the pairs;
+//  launch_rerank: each pair's exact key written to its slot of kept (nq * cap keys; padding when
+//    above bound[q], no filter when bound is null);
+//  launch_select_kept: the first k kept keys of every non-overflowed query into D/I (done: nq
+//    flags of scratch).
 constexpr int kRerankGrid = 2048;
-void launch_rerank_select(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids,
-                          const ChunkPlan& plan, const unsigned long long* surv, const unsigned int* cnt, int64_t nq,
-                          int cap, int k, const float* bound, int2* items, unsigned int* n_items,
-                          unsigned long long* kept, unsigned int* done, float* D, int64_t* I, int* overflow_list,
-                          int* overflow_cnt, unsigned long long* stats, hipStream_t s);
+void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, int2* items, unsigned int* n_items,
+                         int* overflow_list, int* overflow_cnt, unsigned long long* stats, hipStream_t s);
+void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
+                   const unsigned long long* surv, int64_t nq, int cap, const float* bound, const int2* items,
+                   const unsigned int* n_items, unsigned long long* kept, hipStream_t s);
+void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt, int64_t nq, int cap, int k, float* D,
+                        int64_t* I, unsigned int* done, hipStream_t s);
 // the queries the exact path redoes -> list / list_cnt (both zeroed by the caller): overflowed
 // survivor buffers (cnt[q] > cap; counts[0]) and, with check_bound, queries whose k-th result is
 // missing or above bound[q] (counts[1])

@@ -96,6 +96,15 @@ def make(workload, nb=None, nq=None, query_set=0):
     return base, queries
 
 
+def uneven_plan(workload, nq, seed_offset=5):
+    """Per-query probe counts round(exp(U[ln 8, ln 512])) and k in {10, 100} (stream S+5) of
+    the uneven-nprobe workload (zipf-96d-10m)."""
+    r = _rng(workload + seed_offset)
+    nprobe = np.rint(np.exp(r.uniform(np.log(8.0), np.log(512.0), size=nq))).astype(np.int32)
+    k = np.where(r.integers(0, 2, size=nq) == 0, 10, 100).astype(np.int32)
+    return nprobe, k
+
+
 def sha256(*arrays):
     h = hashlib.sha256()
     for a in arrays:
