@@ -466,6 +466,7 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                             if (slot < (unsigned int)cap) {
                                 const int64_t o = (int64_t)s_q[r0 + i] * cap + slot;
                                 surv[o] = hv_key(lbv[i], pos);
+                                aux[o] = ubv[i];
                             }
                         }
                     }
@@ -563,7 +564,7 @@ constexpr int kCompactWaveMax = 2048;  // one wavefront per query up to here,
 constexpr int kCompactWaveBig = 4096;  // then with 64 values per lane; larger: the workgroup kernel
 
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
-                                                      const float* __restrict__ aux,
+                                                      float* __restrict__ aux,
                                                       unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
                                                       float* __restrict__ bound) {
     __shared__ unsigned int s_hist[256];
@@ -578,6 +579,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     float b = bound[q];
     unsigned long long* __restrict__ buf = surv + q * cap;
     const unsigned int* __restrict__ ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
+    unsigned int* ubw = reinterpret_cast<unsigned int*>(aux + q * cap);
     unsigned int u[kCompactPer];
     unsigned long long key[kCompactPer];
 #pragma unroll
@@ -675,7 +677,11 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
         unsigned int base = 0u;
         if (lane == 0 && m) base = atomicAdd(&s_out, (unsigned int)__builtin_popcountll(m));
         base = (unsigned int)__shfl((int)base, 0, 64);
-        if (keep) buf[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key[i];
+        if (keep) {
+            const unsigned int o = base + __builtin_popcountll(m & ((1ull << lane) - 1ull));
+            buf[o] = key[i];
+            ubw[o] = u[i];  // upper bounds move with their keys
+        }
     }
     __syncthreads();
     if (t == 0) {
@@ -689,7 +695,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
 
 template <int E>
 __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restrict__ buf,
-                                                    const unsigned int* __restrict__ ubits, int n, unsigned int c,
+                                                    unsigned int* __restrict__ ubits, int n, unsigned int c,
                                                     int cap, int k, float b, unsigned int* __restrict__ cnt_q,
                                                     float* __restrict__ bound_q, unsigned int* s_tmp) {
     const int lane = threadIdx.x & 63;
@@ -718,16 +724,22 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
 #pragma unroll
     for (int h0 = 0; h0 < E; h0 += H) {
         unsigned long long key[H];
+        unsigned int ub[H];
 #pragma unroll
         for (int e = 0; e < H; ++e) {
             const int j = (h0 + e) * 64 + lane;
             key[e] = j < n ? buf[j] : HV_KEY_PAD;
+            ub[e] = j < n ? ubits[j] : 0u;
         }
 #pragma unroll
         for (int e = 0; e < H; ++e) {
             const bool keep = (h0 + e) * 64 + lane < n && hv_key_dist(key[e]) <= b;
             const unsigned long long m = __ballot(keep);
-            if (keep) buf[out + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key[e];
+            if (keep) {
+                const unsigned int o = out + __builtin_popcountll(m & ((1ull << lane) - 1ull));
+                buf[o] = key[e];
+                ubits[o] = ub[e];  // upper bounds move with their keys (later compactions read them)
+            }
             out += (unsigned int)__builtin_popcountll(m);
         }
     }
@@ -738,7 +750,7 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
 }
 
 __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* __restrict__ surv,
-                                                           const float* __restrict__ aux,
+                                                           float* __restrict__ aux,
                                                            unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
                                                            float* __restrict__ bound) {
     __shared__ unsigned int s_tmp[4][64];
@@ -749,7 +761,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
     const int n = (int)min(c, (unsigned int)cap);
     if (n > kCompactWaveMax) return;  // k_bf16_compact_wave_big, k_bf16_compact
     unsigned long long* buf = surv + q * cap;
-    const unsigned int* ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
+    unsigned int* ubits = reinterpret_cast<unsigned int*>(aux + q * cap);
     const float b = bound[q];
     if (n <= 256)
         bf16_compact_wave_e<4>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
@@ -763,7 +775,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
 
 // buffers of kCompactWaveMax < n <= kCompactWaveBig pairs (64 upper bounds per lane)
 __global__ __launch_bounds__(256) void k_bf16_compact_wave_big(unsigned long long* __restrict__ surv,
-                                                               const float* __restrict__ aux,
+                                                               float* __restrict__ aux,
                                                                unsigned int* __restrict__ cnt, int64_t nq, int cap,
                                                                int k, float* __restrict__ bound) {
     __shared__ unsigned int s_tmp[4][64];
@@ -773,11 +785,11 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave_big(unsigned long lon
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     if (n <= kCompactWaveMax || n > kCompactWaveBig) return;
-    bf16_compact_wave_e<kCompactWaveBig / 64>(surv + q * cap, reinterpret_cast<const unsigned int*>(aux + q * cap), n,
+    bf16_compact_wave_e<kCompactWaveBig / 64>(surv + q * cap, reinterpret_cast<unsigned int*>(aux + q * cap), n,
                                               c, cap, k, bound[q], cnt + q, bound + q, s_tmp[w]);
 }
 
-void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
+void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s) {
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));

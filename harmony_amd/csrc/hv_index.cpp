@@ -178,6 +178,10 @@ constexpr int kVecSplit = 1024;
 // ranges spread long nearest lists over more workgroups.
 constexpr int kVecSplitBf16 = 256;
 
+// The BF16 form scans the probe ranks after the nearest list in one phase up to this many probes,
+// beyond it in groups of growing width with the bound tightened in between
+constexpr int64_t kBf16SinglePhase = 16;
+
 // Scratch of one batch of exact redos (per-(query, rank, segment) runs of k keys)
 constexpr int64_t kRedoRunBytes = 64ll << 20;
 
@@ -774,8 +778,19 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
-        if (!fp8) {
+        if (!fp8 && nprobe <= kBf16SinglePhase) {
             if (nprobe > 1) mfma_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
+        } else if (!fp8) {
+            // many probes: the ranks go in groups of growing width, each followed by the
+            // compaction, which lowers the bound to the k-th smallest upper bound so far and drops
+            // the pairs whose lower bound exceeds it (upper bounds travel with the keys)
+            for (int64_t r0 = 1; r0 < nprobe;) {
+                const int64_t r1 = std::min<int64_t>(nprobe, r0 * 4);
+                mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, stats + 8);
+                hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
+                r0 = r1;
+            }
+            HV_CHECK_LAUNCH("compact");
         } else {
             // FP8: the estimates carry no error bound, so the filter is the kr-th smallest
             // estimate so far; the probe ranks go in groups of growing width ([1, 4), [4, 16),
@@ -830,8 +845,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     //    been pruned), over the same probe lists, with the safe bound in learned mode (none in the
     //    FP32 form). The list and its count stay on the device: the kernels take the first batch
     //    by themselves, and the host adds batches only when the final count exceeds one.
-    hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound, learned && !fp8 ? 1 : 0, n, redo_list, redo_cnt,
-                         stats + 12, s);
+    // (the check is against the bound the seed set, bound_ref: later compactions only tighten
+    //  the working bound towards the k-th distance itself)
+    hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
+                         redo_cnt, stats + 12, s);
     const float* redo_bound = fp8 ? nullptr : (learned ? bound_safe : bound);
     const int nseg = hv::exact_segments(h->max_list);
     const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query

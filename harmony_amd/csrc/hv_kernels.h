@@ -117,7 +117,7 @@ void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, i
 // counts after the direct-slot scan of the nearest lists (vectors v < cap)
 void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
                          unsigned int* cnt, hipStream_t s);
-void launch_bf16_compact(unsigned long long* surv, const float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
+void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
                          float* bound, hipStream_t s);
 // re-rank + final top-k of the MFMA forms, in three launches:
 //  launch_rerank_items: work items of 64 stored pairs (items: nq * ceil(cap/64) entries, n_items

@@ -93,17 +93,17 @@ def test_w1_bf16_overflow_falls_back_to_exact_path(w1):
 
 
 def test_w1_bf16_wide_buffers(w1):
-    """k = 600 over 40 lists with no usable bound (each nearest list holds fewer than k vectors):
-    thousands of kept keys per query, so the final select reads the buffer in steps and hands the
-    widest ones to the workgroup select."""
+    """k = 600 over 16 lists (one scan phase after the nearest list) with no usable bound (each
+    nearest list holds fewer than k vectors): over a thousand kept keys per query, so the final
+    select reads the buffer in steps and hands the widest ones to the workgroup select."""
     xb, xq, g, o = w1
-    g.nprobe = 40
+    g.nprobe = 16
     g.preSearch(warmup_list=0, dim_blocks=4, survivor_cap=8192, precision=HV_PRECISION_BF16)
     D, I, st = g.search(xq, 600, return_stats=True)
     g.preSearch(warmup_list=10, dim_blocks=4, precision=HV_PRECISION_FP32)
     g.nprobe = 10
     assert st["overflow_queries"] == 0 and st["survivors"] > 1024 * xq.shape[0]
-    oD, oI, _, _ = O.search(o, xq, 600, nprobe=40, dim_blocks=4)
+    oD, oI, _, _ = O.search(o, xq, 600, nprobe=16, dim_blocks=4)
     np.testing.assert_array_equal(I, oI)
     np.testing.assert_array_equal(D, oD)
 
