@@ -33,6 +33,11 @@ each divided by the kernel's duration measured with HIP events on the library's 
 `cpu_baseline` is the oracle's restatement of the reference CPU scan (built with the
 reference's Release flags), timed on this host on a bounded query sample.
 
+`--workload skewed-420d` measures the 1,000,000 x 420 float mixture with per-dimension scales
+exp(N(0,1)) (nlist 2048, nprobe 64, k 100, B = 8 blocks over the dimensions in descending
+variance order): plain-fp32, bf16 and fp8 (r = 2), autotuned, recall and distance error against
+the FP64 ground truth.
+
 `--workload zipf-96d-10m` instead measures the uneven workload: 10,000,000 x 96 unit vectors from
 a Zipf-weighted mixture (nlist 4096), 20,000 queries with per-query nprobe round(exp U[ln 8,
 ln 512]) and k in {10, 100} (two calls per step, one per k), each form in graph mode (one
@@ -105,6 +110,74 @@ def cpu_baseline(index, xq, nprobe, k, dim_blocks, budget_s):
     return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": "%d of %d queries, nprobe %d, k %d, B=%d pruning on, single process, %d OpenMP threads"
                       % (n, xq.shape[0], nprobe, k, dim_blocks, threads)}
+
+
+def bench_skewed(args):
+    """The skewed-420d workload: float data, d = 420, B = 8 in variance order."""
+    from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32, datagen, metrics
+    from harmony_amd.autotune import autotune
+    from harmony_amd.index import DeviceArray, Index
+    w = datagen.WORKLOADS[3]
+    d, nlist, B, k, nprobe = w["d"], w["nlist"], w["dim_blocks"], w["k"], w["nprobe"][0]
+    t0 = time.perf_counter()
+    xb, xq = datagen.make(3, nb=args.nb, nq=args.nq)
+    order = datagen.variance_order(xb)
+    log("skewed data %d x %d: %.1fs" % (xb.shape[0], d, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    index = Index(d, nlist, nprobe=nprobe, dim_order=order)
+    index.train(xb, niter=10, seed=0, max_points_per_centroid=256)
+    index.add(xb)
+    log("index build %.1fs" % (time.perf_counter() - t0))
+    nb = xb.shape[0]
+    del xb
+    nq = xq.shape[0]
+    q_dev, D_dev, I_dev = DeviceArray.from_host(xq), DeviceArray(nq * k * 4), DeviceArray(nq * k * 8)
+    t0 = time.perf_counter()
+    index.nprobe = nprobe
+    GD, GI = index.ground_truth_f64(xq, k)
+    log("fp64 ground truth %.1fs" % (time.perf_counter() - t0))
+    forms, ref = {}, None
+    for name, prec, r in (("plain-fp32", HV_PRECISION_FP32, 4), ("bf16", HV_PRECISION_BF16, 4),
+                          ("fp8", HV_PRECISION_FP8, 2)):
+        index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B, precision=prec,
+                        rerank_r=r, graph=False, tile_shape=0, tile_shape_first=0, scan_vsplit=0)
+        tuned = None if args.no_autotune else autotune(index, xq, k, sample=nq, reps=3)[0]
+        for _ in range(max(2, args.warmup)):
+            index.search_device(q_dev, nq, k, D_dev, I_dev)
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            index.search_device(q_dev, nq, k, D_dev, I_dev)
+            times.append(time.perf_counter() - t0)
+        ms = 1000.0 * float(np.mean(times))
+        I = I_dev.to_host(np.int64, (nq, k))
+        D = D_dev.to_host(np.float32, (nq, k))
+        if ref is None:
+            ref = I
+        worst = 0.0
+        for i in range(0, nq, 7):
+            m = dict(zip(GI[i].tolist(), GD[i].tolist()))
+            for dist, j in zip(D[i], I[i]):
+                g = m.get(int(j))
+                if g is not None and g > 0:
+                    worst = max(worst, abs(float(dist) - g) / g)
+        forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 3),
+                       "recall_at_100": metrics.recall(I, GI), "recall_vs_plain": metrics.recall(I, ref),
+                       "max_rel_dist_error_vs_fp64": worst, "autotuned": tuned}
+        log("%s: %.3f ms, %.0f QPS, recall@100 %.5f (vs plain %.5f), max rel distance error vs fp64 %.2e"
+            % (name, ms, nq / (ms / 1000.0), forms[name]["recall_at_100"], forms[name]["recall_vs_plain"], worst))
+    best = max(forms, key=lambda f: forms[f]["qps"] if forms[f]["recall_at_100"] >= 0.95 else 0)
+    out = {
+        "metric": "QPS at recall@100>=0.95, 1M x 420 fp32 skewed mixture, nprobe 64",
+        "value": forms[best]["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": forms[best]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32" if best != "fp8" else "fp8+f32",
+        "data": "synthetic",
+        "config": {"workload": "skewed-420d: %d x %d fp32, per-dimension scale exp(N(0,1)), nlist %d, %d queries, "
+                               "nprobe %d, k %d, B=%d in variance order" % (nb, d, nlist, nq, nprobe, k, B),
+                   "form": best, "forms": forms},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_uneven(args):
@@ -195,7 +268,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-fp8", action="store_true")
-    ap.add_argument("--workload", default="int-mixture-1m", choices=["int-mixture-1m", "zipf-96d-10m"])
+    ap.add_argument("--workload", default="int-mixture-1m", choices=["int-mixture-1m", "skewed-420d", "zipf-96d-10m"])
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this size (a copy kernel of known traffic, "
                          "for scaling HBM counters in a profiler run)")
@@ -207,6 +280,9 @@ def main():
         sys.exit(2)
     if args.workload == "zipf-96d-10m":
         bench_uneven(args)
+        return
+    if args.workload == "skewed-420d":
+        bench_skewed(args)
         return
 
     from harmony_amd import datagen, learned, metrics
