@@ -123,6 +123,32 @@ void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, f
                        by_pos);
 }
 
+// v[e] = q[k0 + e] - c[k0 + e] (0 past d); 16-byte loads when the 8 values are in range and
+// d % 4 == 0 (rows then start on 16-byte boundaries)
+__device__ __forceinline__ void load8_diff(const float* __restrict__ qv, const float* __restrict__ cv, int k0, int d,
+                                           float (&v)[8]) {
+    if ((d & 3) == 0 && k0 + 8 <= d) {
+        const float4 q0 = *reinterpret_cast<const float4*>(qv + k0);
+        const float4 q1 = *reinterpret_cast<const float4*>(qv + k0 + 4);
+        const float4 c0 = *reinterpret_cast<const float4*>(cv + k0);
+        const float4 c1 = *reinterpret_cast<const float4*>(cv + k0 + 4);
+        v[0] = q0.x - c0.x;
+        v[1] = q0.y - c0.y;
+        v[2] = q0.z - c0.z;
+        v[3] = q0.w - c0.w;
+        v[4] = q1.x - c1.x;
+        v[5] = q1.y - c1.y;
+        v[6] = q1.z - c1.z;
+        v[7] = q1.w - c1.w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = k0 + e;
+            v[e] = k < d ? qv[k] - cv[k] : 0.f;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Query residuals of every (query, probe rank) slot: BF16 (q - c) rows of dpad dims (zero past
 // d), one thread per 8 dims.
@@ -138,12 +164,11 @@ __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp
     const int k0 = (int)(t - slot * g8) * 8;
     const float* __restrict__ qv = Qp + (slot / nprobe) * d;
     const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
+    float v[8];
+    load8_diff(qv, cv, k0, d, v);
     bf16x8 r;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int k = k0 + e;
-        r[e] = (__bf16)(k < d ? qv[k] - cv[k] : 0.f);
-    }
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
     out[t] = r;
 }
 
@@ -217,17 +242,12 @@ __global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict_
     const int64_t slot = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
     const int g = threadIdx.x % G;
     const bool live = slot < nslots && !(qn && (int)(slot % nprobe) >= qn[slot / nprobe]);
-    float r[8], m = 0.f;
+    float r[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, m = 0.f;
     const int k0 = g * 8;
     if (live) {
-        const float* __restrict__ qv = Qp + (slot / nprobe) * d;
-        const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
+        load8_diff(Qp + (slot / nprobe) * d, centroids + (int64_t)probes[slot] * d, k0, d, r);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int k = k0 + t;
-            r[t] = k < d ? qv[k] - cv[k] : 0.f;
-            m = fmaxf(m, fabsf(r[t]));
-        }
+        for (int t = 0; t < 8; ++t) m = fmaxf(m, fabsf(r[t]));
     }
     for (int o = G / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));  // within the group
     if (!live) return;
