@@ -285,11 +285,14 @@ void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* p
 // assigned statically: workgroup b takes the items j, j + G/8, j + 2G/8, ... of the queue of its
 // XCD (b % 8), j = b / 8, G = grid size (the queues deal lists round-robin by size, so each
 // stride of items is similar work). The item's queries form up to 4 blocks of 16. A wavefront
-// holds the BF16 residuals of up to QBM query blocks in registers (NKS K-steps of 8 values per
-// lane and block) and streams 16-vector blocks, each loaded once and multiplied into every
-// block it holds; the wavefronts of the workgroup split the vector blocks (and the query blocks
-// when there are more than QBM). After the MFMAs each lane tests the 4 (query, vector) pairs of
-// each block it holds. Keys carry the vector's position in the list-ordered arrays, not its id.
+// holds the residuals of QBM query blocks in registers (NKS K-steps of 8 values per lane and
+// block) and streams 16-vector blocks (the next one in flight), multiplying each into every
+// block it holds; the wavefronts of the workgroup split the query blocks, and the vector blocks
+// when there are fewer query blocks than wavefronts. QBM = 1: holding more query blocks saves
+// code loads but the registers it takes cost more occupancy than that saves (measured at d = 128:
+// the remaining-lists phase 0.77 ms with 4, 0.64 with 2, 0.60 with 1). After the MFMAs each lane
+// tests the 4 (query, vector) pairs of each block it holds. Keys carry the vector's position in
+// the list-ordered arrays, not its id.
 //   mode 0: append the pairs with lb <= bound to the query's buffer, key = lb.
 //   mode 1: the query's nearest list (probe rank 0 only): every pair of vector v is written to
 //           slot v (no atomics; the caller limits the vectors to v < cap and sets the counts with
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                                                    unsigned long long* __restrict__ surv, float* __restrict__ aux,
                                                    unsigned int* __restrict__ surv_cnt, int cap,
                                                    unsigned long long* __restrict__ stats) {
-    constexpr int QBM = NKS <= 4 ? 4 : (NKS <= 8 ? 2 : 1);  // query blocks per wavefront
+    constexpr int QBM = 1;  // query blocks per wavefront
     constexpr bool kPrefetch = NKS <= 8;                     // larger K keeps occupancy instead
     __shared__ int s_q[64], s_slot[64];
     __shared__ __align__(16) float s_nrq[64];
