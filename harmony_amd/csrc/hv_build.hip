@@ -142,6 +142,24 @@ void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hip
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
+// The segments share one grid-stride loop each (a search clears ~7 small buffers at once).
+__global__ void k_fill_multi(FillSegs f) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = 0; j < f.nseg; ++j) {
+        int* p = f.p[j];
+        const int64_t n = f.n[j];
+        const int v = f.v[j];
+        for (int64_t i = g; i < n; i += stride) p[i] = v;
+    }
+}
+void launch_fill_multi(const FillSegs& f, hipStream_t s) {
+    int64_t mx = 0;
+    for (int j = 0; j < f.nseg; ++j) mx = std::max<int64_t>(mx, f.n[j]);
+    if (mx <= 0) return;
+    const int64_t blocks = std::min<int64_t>((mx + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_fill_multi, dim3((unsigned)blocks), dim3(256), 0, s, f);
+}
 
 // ---------------------------------------------------------------------------------------
 // k-means centroid update with FP64 sums (Clustering.cpp:93-152, sums in FP64 instead of the

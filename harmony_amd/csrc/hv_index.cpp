@@ -374,24 +374,34 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
 // ties to the lower id (Index.cpp:967-997). Up to P = 512 and nlist = 4096 on the matrix cores
 // with an exact re-check of the candidates near the P-th (hv_coarse.hip); otherwise full exact
 // distance rows and a select. Rows in slices of <= 256 MB.
-void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist) {
+int64_t coarse_rows_per_slice(const hv_index_t* h, int64_t n) {
+    return std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (h->nlist * 4)));
+}
+bool coarse_on_mfma(const hv_index_t* h, int64_t P) { return P <= 512 && h->nlist <= 4096; }
+// the MFMA path's re-check counter and list (slice rows + 1 ints)
+int* coarse_counter(hv_index_t* h, int64_t n) {
+    HV_HIP(h->s_redo.ensure(sizeof(int) * (coarse_rows_per_slice(h, n) + 1)));
+    return h->s_redo.as<int>();
+}
+// counter_zeroed: the caller has cleared the re-check counter (the first slice skips its fill)
+void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist,
+                   bool counter_zeroed = false) {
     const int64_t d = h->d, nlist = h->nlist;
     hipStream_t s = h->stream;
-    const int64_t per = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (nlist * 4)));
+    const int64_t per = coarse_rows_per_slice(h, n);
     HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
-    if (P <= 512 && nlist <= 4096) {
+    if (coarse_on_mfma(h, P)) {
         HV_HIP(h->s_qnorm.ensure(sizeof(float) * n));
         HV_HIP(h->s_cnorm.ensure(sizeof(float) * nlist));
-        HV_HIP(h->s_redo.ensure(sizeof(int) * (per + 1)));
+        int* redo_cnt = coarse_counter(h, n);
         hv::launch_row_norms(h->d_centroids.as<float>(), nlist, (int)d, h->s_cnorm.as<float>(), s);
         hv::launch_row_norms(Qp, n, (int)d, h->s_qnorm.as<float>(), s);
         const float kappa = hv::coarse_kappa((int)d);
-        int* redo_cnt = h->s_redo.as<int>();
         for (int64_t r0 = 0; r0 < n; r0 += per) {
             const int64_t rows = std::min(per, n - r0);
             hv::launch_coarse_mfma(Qp + r0 * d, rows, h->d_centroids.as<float>(), (int)nlist, (int)d,
                                    h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), h->s_dist.as<float>(), s);
-            hv::launch_fill_i32(redo_cnt, 1, 0, s);
+            if (r0 > 0 || !counter_zeroed) hv::launch_fill_i32(redo_cnt, 1, 0, s);
             hv::launch_coarse_select(h->s_dist.as<float>(), Qp + r0 * d, h->d_centroids.as<float>(), (int)nlist,
                                      (int)d, h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), kappa, rows, (int)P,
                                      probes + r0 * P, pdist + r0 * P, redo_cnt + 1, redo_cnt, s);
@@ -492,6 +502,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         require(h->mlp_params && h->mlp_norm, HV_ERR_STATE,
                 "learned search needs a predictor (hv_mlp_init/train or set_params, and hv_mlp_set_norm)");
     const bool fp8 = h->sp.precision == HV_PRECISION_FP8;
+    const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
+    const bool mfma = bf16 || fp8;
     const int64_t nprobe_cap = nprobe;
     if (learned) nprobe = std::min<int64_t>(nlist, std::max<int64_t>(nprobe, hv::kNearest));
     int64_t wl = learned ? 0 : std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
@@ -572,12 +584,48 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_permute_rows(q_dev, n, (int)d, h->d_perm.as<int>(), h->s_qp.as<float>(), s);
         Qp = h->s_qp.as<float>();
     }
+    // counters, bucket counts and (when no warm-up, seed or caller's bound applies) the bounds
+    // start cleared: one fill launch for all of them
+    const int64_t nb = nlist * nprobe;
+    const bool bound_fmax = !(learned && !mfma) && !external && !(wl > 0 && !mfma && !learned);
+    HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
+    HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
+    // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
+    // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 14));
+    HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
+    HV_HIP(h->s_bound.ensure(sizeof(float) * n));
+    HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
+    HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
+    {
+        hv::FillSegs f{};
+        auto add = [&](void* p, int64_t words, int v) {
+            f.p[f.nseg] = static_cast<int*>(p);
+            f.n[f.nseg] = words;
+            f.v[f.nseg] = v;
+            ++f.nseg;
+        };
+        add(h->s_bucket.p, nb, 0);
+        add(h->s_fill.p, nb, 0);
+        add(h->s_stats.p, 28, 0);
+        add(h->s_surv_cnt.p, n, 0);
+        add(h->s_overflow.p, 2, 0);  // redo count, re-rank item count
+        if (coarse_on_mfma(h, nprobe)) add(coarse_counter(h, n), 1, 0);
+        if (bound_fmax) {
+            int fm;
+            const float fmax = FLT_MAX;
+            std::memcpy(&fm, &fmax, 4);
+            add(h->s_bound.p, n, fm);
+            add(h->s_bound_ref.p, n, fm);
+        }
+        hv::launch_fill_multi(f, s);
+    }
     // 2. coarse: full distance rows + nprobe select (Index.cpp:967-997)
     HV_HIP(h->s_probes.ensure(sizeof(int) * n * nprobe));
     HV_HIP(h->s_pdist.ensure(sizeof(float) * n * nprobe));
     int* probes = h->s_probes.as<int>();
     float* pdist = h->s_pdist.as<float>();
-    coarse_select(h, n, Qp, nprobe, probes, pdist);
+    coarse_select(h, n, Qp, nprobe, probes, pdist, true);
     const int* qn = nullptr;
     float* seed = nullptr;
     if (learned) {
@@ -601,9 +649,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     rec(1);
     // 3. grouping by (list, rank)
-    const int64_t nb = nlist * nprobe;
-    HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
-    HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
     HV_HIP(h->s_bucket_off.ensure(sizeof(int) * (nb + 1)));
     HV_HIP(h->s_scan_tmp.ensure(sizeof(int) * hv::scan_tmp_ints(nb)));
     HV_HIP(h->s_pairs.ensure(sizeof(int) * n * nprobe));
@@ -615,13 +660,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
-    // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
-    // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 14));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
-    hv::launch_fill_i32(h->s_bucket.as<int>(), nb, 0, s);
-    hv::launch_fill_i32(h->s_fill.as<int>(), nb, 0, s);
-    hv::launch_fill_u64(stats, 14, 0ull, s);
     hv::launch_count_pairs(probes, n, (int)nprobe, qn, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
     hv::launch_scatter_pairs(probes, n, (int)nprobe, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
@@ -634,10 +673,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     if (!learned && h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
         cap = std::max<int64_t>(cap, hv_next_pow2((int)(wl * wls)));
     HV_HIP(h->s_surv.ensure(sizeof(unsigned long long) * n * cap));
-    HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
     unsigned long long* surv = h->s_surv.as<unsigned long long>();
     unsigned int* surv_cnt = h->s_surv_cnt.as<unsigned int>();
-    hv::launch_fill_i32(reinterpret_cast<int*>(surv_cnt), n, 0, s);
     // one scan phase over probe ranks [r0, r1), at most vmax vectors per list
     // Tile shape: the caller's, or by the expected number of queries per list in the phase
     // (few queries per list -> narrow query tiles, long vector tiles).
@@ -670,8 +707,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // BF16 precision form: one phase of the MFMA filter scan over probe ranks [r0, r1)
     // MFMA filter forms: BF16 (certified error bound, exact results) and FP8 (estimates only, a
     // shortlist of k * rerank_r per query re-ranked exactly)
-    const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
-    const bool mfma = bf16 || fp8;
     float* aux = nullptr;
     float kappa1 = 0.f, kappa2 = 0.f;
     if (mfma) {
@@ -725,8 +760,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     };
     // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
     //    warm-up scan: it scans each query's nearest list unpruned and takes the bound from there.
-    HV_HIP(h->s_bound.ensure(sizeof(float) * n));
-    HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     float* bound = h->s_bound.as<float>();
     float* bound_ref = h->s_bound_ref.as<float>();
     // learned, BF16: no bound for the nearest lists (their k-th upper bound gives the safe bound
@@ -741,13 +774,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     } else if (wl > 0 && !mfma && !learned) {
         scan_phase(0, (int)wl, (int)wls, 0, 1, stats + 4);
         hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, s);
-    } else {
-        int fm;
-        const float fmax = FLT_MAX;
-        std::memcpy(&fm, &fmax, 4);
-        hv::launch_fill_i32(reinterpret_cast<int*>(bound), n, fm, s);
-        hv::launch_fill_i32(reinterpret_cast<int*>(bound_ref), n, fm, s);
-    }
+    }  // otherwise both start at FLT_MAX (the fill above)
     HV_CHECK_LAUNCH("bound");
     rec(3);
     // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
@@ -819,11 +846,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     rec(4);
     // 6. top-k selection. The BF16 form re-ranks its survivors (the plain path's FP32 arithmetic)
     //    and selects in the same pass.
-    HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
-    int* redo_cnt = h->s_overflow.as<int>();
+    int* redo_cnt = h->s_overflow.as<int>();  // cleared with the counters above
     unsigned int* n_items = reinterpret_cast<unsigned int*>(redo_cnt + 1);
     int* redo_list = redo_cnt + 2;
-    hv::launch_fill_i32(redo_cnt, 2, 0, s);
     if (mfma) {
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));

@@ -46,6 +46,15 @@ void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq
                          hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+// up to kFillSegs fills of 32-bit words (pointer, word count, value) in one launch
+constexpr int kFillSegs = 8;
+struct FillSegs {
+    int* p[kFillSegs];
+    int64_t n[kFillSegs];
+    int v[kFillSegs];
+    int nseg;
+};
+void launch_fill_multi(const FillSegs& f, hipStream_t s);
 // qn (optional): per-query probe counts (ranks at or past qn[q] are skipped)
 void launch_count_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, int* bucket_cnt, hipStream_t s);
 // exclusive scan of n ints into out (n+1 entries, out[n] = total); tmp >= scan_tmp_ints(n)
