@@ -12,10 +12,20 @@
 //   element (v, k) -> (((v / 16) * NKS + k / 32) * 64 + ((k % 32) / 8) * 16 + v % 16) * 8 + k % 8
 // so one K-step of one block is one contiguous 1 KB load of the wavefront.
 //
-// Rounding makes the estimate d~ inexact; the kernel bounds the error of every pair by
-//   err = kappa1 * |rq| * |rx| + kappa2 * (|rq|^2 + |rx|^2)
-// (BF16 rounding of both residuals, FP32 accumulation, and the gap between the real distance
-// and the FP32 chain the exact path computes; the host derives kappa1/kappa2 from d). So
+// Rounding makes the estimate d~ = |rq|^2 + |rx|^2 - 2 dot~ inexact. With the rounding errors
+// eq = rq - bf16(rq), ex = rx - bf16(rx) (their norms measured, plus 2^-23 |r| for the FP32
+// subtraction that formed r), rq.rx - bf16(rq).bf16(rx) = eq.rx + bf16(rq).ex, so by
+// Cauchy-Schwarz
+//   |rq.rx - dot~| <= |eq| |rx| + (|rq| + |eq|) |ex| + g_d (|rq| + |eq|) (|rx| + |ex|)
+// (g_d: FP32 accumulation of d exact BF16 products, taken twice for the MFMA's order). Every
+// pair's error is bounded by
+//   err = kappa1 (|eq| |rx| + (|rq| + |eq|) |ex|) + kappa_g (|rq| + |eq|) (|rx| + |ex|)
+//         + kappa2 (|rq|^2 + |rx|^2)
+// with kappa1 = 2 * 1.01, kappa_g = 2 * 1.01 * 2 g_d and kappa2 covering the FP32 norms, the
+// FP32 evaluation of d~ and the gap between the real distance and the FP32 chain the exact path
+// computes (the host derives them from d). |ex| of every vector is stored with the codes, |eq|
+// of every (query, list) slot comes with the query residuals: measured errors are ~0.4 of the
+// worst case 2^-8 |r| on typical data, so the bound is ~2.6x tighter than a worst-case one. So
 // lb = d~ - err <= exact <= d~ + err = ub for every pair, and a pair can be dropped when
 // lb > bound without changing the result. Survivors are re-ranked with the exact FP32 distance
 // of the plain path, so the BF16 form returns the same ids and distances as the plain path.
@@ -67,14 +77,14 @@ __device__ __forceinline__ float fp8_inv_scale(float m) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Index build: BF16 residual codes and FP32 residual norms of every list (one workgroup per
-// list), and the FP32 rows in list order for the re-rank.
+// Index build: BF16 residual codes, FP32 residual norms and BF16 rounding-error norms of every
+// list (one workgroup per list), and the FP32 rows in list order for the re-rank.
 __global__ __launch_bounds__(256) void k_build_resid(const float* __restrict__ x, int d, int dpad,
                                                      const int* __restrict__ order,
                                                      const int64_t* __restrict__ list_off,
                                                      const float* __restrict__ centroids,
                                                      const int64_t* __restrict__ roff, __bf16* __restrict__ rcodes,
-                                                     float* __restrict__ rnorm) {
+                                                     float* __restrict__ rnorm, float* __restrict__ rerr) {
     const int l = blockIdx.x;
     const int64_t b = list_off[l];
     const int n = (int)(list_off[l + 1] - b);
@@ -91,21 +101,24 @@ __global__ __launch_bounds__(256) void k_build_resid(const float* __restrict__ x
     }
     for (int v = threadIdx.x; v < n; v += blockDim.x) {
         const float* __restrict__ xv = x + (int64_t)order[b + v] * d;
-        float acc = 0.f;
+        float acc = 0.f, ae = 0.f;
         for (int k = 0; k < d; ++k) {
             const float t = xv[k] - c[k];
             acc = __builtin_fmaf(t, t, acc);
+            const float e = t - (float)(__bf16)t;  // exact: t and its BF16 rounding share the exponent
+            ae = __builtin_fmaf(e, e, ae);
         }
         rnorm[b + v] = acc;
+        rerr[b + v] = __builtin_sqrtf(ae) + 0x1p-23f * __builtin_sqrtf(acc);
     }
 }
 
 void launch_build_resid(const float* x, int d, int dpad, const int* order, const int64_t* list_off,
                         const float* centroids, const int64_t* roff, int nlist, __bf16* rcodes, float* rnorm,
-                        hipStream_t s) {
+                        float* rerr, hipStream_t s) {
     if (nlist <= 0) return;
     hipLaunchKernelGGL(k_build_resid, dim3((unsigned)nlist), dim3(256), 0, s, x, d, dpad, order, list_off, centroids,
-                       roff, rcodes, rnorm);
+                       roff, rcodes, rnorm, rerr);
 }
 
 __global__ void k_gather_rows(const float* __restrict__ rows, int64_t n, int d, const int* __restrict__ order,
@@ -151,33 +164,56 @@ __device__ __forceinline__ void load8_diff(const float* __restrict__ qv, const f
 
 // ---------------------------------------------------------------------------------------
 // Query residuals of every (query, probe rank) slot: BF16 (q - c) rows of dpad dims (zero past
-// d), one thread per 8 dims.
+// d) and the norm of their rounding error (plus 2^-23 |q - c|, see the bound above). A group of
+// G lanes per slot (G = the slot's 8-dim words rounded up to a power of two, <= 64; lane g holds
+// dims 8g .. 8g+7).
+template <int G>
 __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp, const float* __restrict__ centroids,
                                                     const int* __restrict__ probes, const int* __restrict__ qn,
                                                     int64_t nslots, int nprobe, int d, int dpad,
-                                                    bf16x8* __restrict__ out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int g8 = dpad / 8;
-    if (t >= nslots * g8) return;
-    const int64_t slot = t / g8;
-    if (qn && (int)(slot % nprobe) >= qn[slot / nprobe]) return;  // rank not searched
-    const int k0 = (int)(t - slot * g8) * 8;
-    const float* __restrict__ qv = Qp + (slot / nprobe) * d;
-    const float* __restrict__ cv = centroids + (int64_t)probes[slot] * d;
-    float v[8];
-    load8_diff(qv, cv, k0, d, v);
+                                                    bf16x8* __restrict__ out, float* __restrict__ qerr) {
+    const int64_t slot = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int g = threadIdx.x % G;
+    const bool live = slot < nslots && !(qn && (int)(slot % nprobe) >= qn[slot / nprobe]);
+    const int k0 = g * 8;
+    float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, se = 0.f, sr = 0.f;
     bf16x8 r;
+    if (live) load8_diff(Qp + (slot / nprobe) * d, centroids + (int64_t)probes[slot] * d, k0, d, v);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
-    out[t] = r;
+    for (int e = 0; e < 8; ++e) {
+        r[e] = (__bf16)v[e];
+        const float er = v[e] - (float)r[e];
+        se = __builtin_fmaf(er, er, se);
+        sr = __builtin_fmaf(v[e], v[e], sr);
+    }
+    for (int o = G / 2; o > 0; o >>= 1) {  // within the group
+        se += __shfl_xor(se, o, 64);
+        sr += __shfl_xor(sr, o, 64);
+    }
+    if (!live) return;
+    if (k0 < dpad) out[slot * (dpad / 8) + g] = r;
+    if (g == 0) qerr[slot] = __builtin_sqrtf(se) + 0x1p-23f * __builtin_sqrtf(sr);
 }
 
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                       int nprobe, int d, int dpad, __bf16* out, hipStream_t s) {
-    const int64_t total = nq * nprobe * (dpad / 8);
-    if (total <= 0) return;
-    hipLaunchKernelGGL(k_pair_resid, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Qp, centroids, probes, qn,
-                       nq * nprobe, nprobe, d, dpad, reinterpret_cast<bf16x8*>(out));
+                       int nprobe, int d, int dpad, __bf16* out, float* qerr, hipStream_t s) {
+    const int64_t nslots = nq * nprobe;
+    if (nslots <= 0) return;
+    const int words = dpad / 8;  // <= 64
+#define HV_RESID(G)                                                                                          \
+    hipLaunchKernelGGL(k_pair_resid<G>, dim3((unsigned)((nslots + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
+                       Qp, centroids, probes, qn, nslots, nprobe, d, dpad, reinterpret_cast<bf16x8*>(out), qerr)
+    if (words <= 4)
+        HV_RESID(4);
+    else if (words <= 8)
+        HV_RESID(8);
+    else if (words <= 16)
+        HV_RESID(16);
+    else if (words <= 32)
+        HV_RESID(32);
+    else
+        HV_RESID(64);
+#undef HV_RESID
 }
 
 // FP8 codes of every list from the rows in list order: one workgroup per list finds the largest
@@ -297,19 +333,23 @@ void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* p
 //   mode 1: the query's nearest list (probe rank 0 only): every pair of vector v is written to
 //           slot v (no atomics; the caller limits the vectors to v < cap and sets the counts with
 //           k_first_counts), key = lb, aux = ub.
+// qaux: per slot, the FP8 inverse scale or the BF16 rounding-error norm |eq|; rerr: per vector,
+// |ex| (BF16 only).
 template <int NKS, typename E>
 __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __restrict__ qres,
-                                                   const float* __restrict__ qscale,
+                                                   const float* __restrict__ qaux,
                                                    const float* __restrict__ pdist, int nprobe,
                                                    const typename E::elem* __restrict__ rcodes,
                                                    const float* __restrict__ lscale,
                                                    const int64_t* __restrict__ roff,
                                                    const float* __restrict__ rnorm,
+                                                   const float* __restrict__ rerr,
                                                    const int64_t* __restrict__ list_off,
                                                    const float* __restrict__ bound,
                                                    const int* __restrict__ pair_slot,
                                                    const ScanItem* __restrict__ items, const int* __restrict__ queue,
-                                                   int vmax, int vsplit, int mode, float kappa1, float kappa2,
+                                                   int vmax, int vsplit, int mode, float kappa1, float kappa_g,
+                                                   float kappa2,
                                                    unsigned long long* __restrict__ surv, float* __restrict__ aux,
                                                    unsigned int* __restrict__ surv_cnt, int cap,
                                                    unsigned long long* __restrict__ stats) {
@@ -317,9 +357,9 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
     constexpr bool kPrefetch = NKS <= 8;                     // larger K keeps occupancy instead
     __shared__ int s_q[64], s_slot[64];
     __shared__ __align__(16) float s_nrq[64];
-    __shared__ __align__(16) float s_srq[64];
+    __shared__ __align__(16) float s_srq[64];  // BF16: |rq| + |eq|
     __shared__ __align__(16) float s_bound[64];
-    __shared__ __align__(16) float s_qsc[64];
+    __shared__ __align__(16) float s_qsc[64];  // FP8: the slot's inverse scale; BF16: |eq|
     typedef typename E::vec vec;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, col16 = lane & 15;
@@ -333,20 +373,20 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
         const int nQ = item.pair_end - item.pair_begin;
         if (tid < 64) {
             int q = 0, slot = 0;
-            float nr = 0.f, b = -1.f, qsc = 0.f;  // rows past nQ: bound -1 keeps nothing
+            float nr = 0.f, b = -1.f, qa = 0.f;  // rows past nQ: bound -1 keeps nothing
             if (tid < nQ) {
                 slot = pair_slot[item.pair_begin + tid];
                 q = slot / nprobe;
                 nr = pdist[slot];
                 b = mode == 1 ? 0.f : bound[q];
-                if (E::kScaled) qsc = qscale[slot];
+                qa = qaux[slot];
             }
             s_q[tid] = q;
             s_slot[tid] = slot;
             s_nrq[tid] = nr;
-            s_srq[tid] = __builtin_amdgcn_sqrtf(nr);
+            s_srq[tid] = __builtin_amdgcn_sqrtf(nr) + (E::kScaled ? 0.f : qa);
             s_bound[tid] = b;
-            s_qsc[tid] = qsc;
+            s_qsc[tid] = qa;
         }
         const int l = item.list;
         const float lsc = E::kScaled ? lscale[l] : 1.f;
@@ -380,32 +420,36 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
             // software pipeline: the codes and norm of the next block are in flight while the
             // current block is computed
             vec bn[NKS];
-            float nxn = 0.f;
+            float nxn = 0.f, exn = 0.f;
             if (kPrefetch && v0 < vend) {
                 const int64_t blk = (int64_t)(v0 >> 4) * NKS * 64;
 #pragma unroll
                 for (int s = 0; s < NKS; ++s) bn[s] = R[blk + s * 64];
                 nxn = rnorm[lbeg + min(v0 + col16, nl - 1)];
+                if (!E::kScaled) exn = rerr[lbeg + min(v0 + col16, nl - 1)];
             }
             for (; v0 < vend; v0 += vstep) {
                 vec bc[NKS];
-                float nx;
+                float nx, ex;
                 const int v1 = kPrefetch ? v0 + vstep : v0;
                 if (kPrefetch) {
 #pragma unroll
                     for (int s = 0; s < NKS; ++s) bc[s] = bn[s];
                     nx = nxn;
+                    ex = exn;
                 }
                 if (v1 < vend) {
                     const int64_t blk = (int64_t)(v1 >> 4) * NKS * 64;
 #pragma unroll
                     for (int s = 0; s < NKS; ++s) bn[s] = R[blk + s * 64];
                     nxn = rnorm[lbeg + min(v1 + col16, nl - 1)];
+                    if (!E::kScaled) exn = rerr[lbeg + min(v1 + col16, nl - 1)];
                 }
                 if (!kPrefetch) {
 #pragma unroll
                     for (int s = 0; s < NKS; ++s) bc[s] = bn[s];
                     nx = nxn;
+                    ex = exn;
                 }
                 floatx4 acc[QBM];
 #pragma unroll
@@ -419,7 +463,9 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                 }
                 const int v = v0 + col16;
                 const bool cvalid = v < vend;
-                const float sx = __builtin_amdgcn_sqrtf(nx);
+                // per-vector parts of the error bound (BF16)
+                const float sx = __builtin_amdgcn_sqrtf(nx);  // within 1 ulp (the 1.01 factor covers it)
+                const float kg_x = kappa_g * (sx + ex), k2_x = kappa2 * nx;
                 const uint32_t pos = (uint32_t)(lbeg + v);
                 pairs_done += 256 * nqb;
 #pragma unroll
@@ -430,18 +476,22 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                     // registers across the loop (occupancy)
                     asm volatile("" : "+v"(r0));
                     const float4 nr4 = *reinterpret_cast<const float4*>(&s_nrq[r0]);
-                    const float4 sr4 = *reinterpret_cast<const float4*>(&s_srq[r0]);
                     const float4 b4 = *reinterpret_cast<const float4*>(&s_bound[r0]);
+                    const float4 q4 = *reinterpret_cast<const float4*>(&s_qsc[r0]);
                     const float nrq[4] = {nr4.x, nr4.y, nr4.z, nr4.w};
-                    const float srq[4] = {sr4.x, sr4.y, sr4.z, sr4.w};
                     const float bq[4] = {b4.x, b4.y, b4.z, b4.w};
+                    const float qa[4] = {q4.x, q4.y, q4.z, q4.w};
                     float dot[4] = {acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
+                    float sqp[4] = {0.f, 0.f, 0.f, 0.f};
                     if (E::kScaled) {
-                        const float4 q4 = *reinterpret_cast<const float4*>(&s_qsc[r0]);
-                        dot[0] *= lsc * q4.x;
-                        dot[1] *= lsc * q4.y;
-                        dot[2] *= lsc * q4.z;
-                        dot[3] *= lsc * q4.w;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) dot[i] *= lsc * qa[i];
+                    } else {
+                        const float4 sr4 = *reinterpret_cast<const float4*>(&s_srq[r0]);
+                        sqp[0] = sr4.x;
+                        sqp[1] = sr4.y;
+                        sqp[2] = sr4.z;
+                        sqp[3] = sr4.w;
                     }
                     float lbv[4], ubv[4];
                     bool kp[4];
@@ -449,7 +499,10 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float dt = nrq[i] + nx - 2.f * dot[i];
-                        const float err = kappa1 * srq[i] * sx + kappa2 * (nrq[i] + nx);
+                        // FP8: estimates only (no bound)
+                        const float err = E::kScaled ? 0.f
+                                                     : kappa1 * __builtin_fmaf(qa[i], sx, sqp[i] * ex) +
+                                                           __builtin_fmaf(kg_x, sqp[i], __builtin_fmaf(kappa2, nrq[i], k2_x));
                         lbv[i] = fmaxf(dt - err, 0.f);
                         ubv[i] = fmaxf(dt + err, 0.f);
                         kp[i] = cvalid && bq[i] >= 0.f && (mode == 1 || lbv[i] <= bq[i]);
@@ -519,17 +572,18 @@ void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_
 
 int bf16_ksteps(int d) { return (d + 31) / 32; }
 
-void launch_mfma_scan(int grid, int elem, const void* qres, const float* qscale, int d, const float* pdist,
+void launch_mfma_scan(int grid, int elem, const void* qres, const float* qaux, int d, const float* pdist,
                       int nprobe, const void* rcodes, const float* lscale, const int64_t* roff, const float* rnorm,
-                      const int64_t* list_off, const float* bound, const int* pair_slot, const ScanItem* items,
-                      const int* queue, int vmax, int vsplit, int mode, float kappa1, float kappa2,
-                      unsigned long long* surv, float* aux, unsigned int* surv_cnt, int cap,
-                      unsigned long long* stats, hipStream_t s) {
-#define HV_SCAN_LAUNCH(K, E)                                                                                    \
-    hipLaunchKernelGGL((k_mfma_scan<K, E>), dim3((unsigned)grid), dim3(256), 0, s,                              \
-                       reinterpret_cast<const E::elem*>(qres), qscale, pdist, nprobe,                          \
-                       reinterpret_cast<const E::elem*>(rcodes), lscale, roff, rnorm, list_off, bound, pair_slot, \
-                       items, queue, vmax, vsplit, mode, kappa1, kappa2, surv, aux, surv_cnt, cap, stats)
+                      const float* rerr, const int64_t* list_off, const float* bound, const int* pair_slot,
+                      const ScanItem* items, const int* queue, int vmax, int vsplit, int mode, float kappa1,
+                      float kappa_g, float kappa2, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
+                      int cap, unsigned long long* stats, hipStream_t s) {
+#define HV_SCAN_LAUNCH(K, E)                                                                                     \
+    hipLaunchKernelGGL((k_mfma_scan<K, E>), dim3((unsigned)grid), dim3(256), 0, s,                               \
+                       reinterpret_cast<const E::elem*>(qres), qaux, pdist, nprobe,                             \
+                       reinterpret_cast<const E::elem*>(rcodes), lscale, roff, rnorm, rerr, list_off, bound,    \
+                       pair_slot, items, queue, vmax, vsplit, mode, kappa1, kappa_g, kappa2, surv, aux, surv_cnt, \
+                       cap, stats)
 #define HV_SCAN_CASE(K)                \
     case K:                            \
         if (elem == 1)                 \

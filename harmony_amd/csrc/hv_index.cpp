@@ -231,7 +231,7 @@ struct hv_index {
     std::vector<int> list_stride, rank_order;
     DevBuf d_codes, d_list_off, d_code_off, d_list_stride, d_ids, d_rank_order;
     // BF16 residual codes per list (MFMA operand order), their FP32 norms, and the FP32 rows in list order
-    DevBuf d_rcodes, d_roff, d_rnorm, d_rows;
+    DevBuf d_rcodes, d_roff, d_rnorm, d_rerr, d_rows;
     int dpad = 0, bf16_grid = 0;
     // FP8 (e4m3) codes per list in the same order and their per-list inverse scales, built on the
     // first FP8-form search after an add
@@ -353,11 +353,12 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
     HV_HIP(h->d_rcodes.ensure(sizeof(__bf16) * std::max<int64_t>(1, roff[nlist])));
     h->rcode_elems = roff[nlist];
     HV_HIP(h->d_rnorm.ensure(sizeof(float) * std::max<int64_t>(1, n)));
+    HV_HIP(h->d_rerr.ensure(sizeof(float) * std::max<int64_t>(1, n)));
     HV_HIP(h->d_rows.ensure(sizeof(float) * std::max<int64_t>(1, n * d)));
     if (n) {
         hv::launch_build_resid(xp, (int)d, h->dpad, order_buf.as<int>(), h->d_list_off.as<int64_t>(),
                                h->d_centroids.as<float>(), h->d_roff.as<int64_t>(), (int)nlist,
-                               h->d_rcodes.as<__bf16>(), h->d_rnorm.as<float>(), h->stream);
+                               h->d_rcodes.as<__bf16>(), h->d_rnorm.as<float>(), h->d_rerr.as<float>(), h->stream);
         hv::launch_gather_rows(xp, n, (int)d, order_buf.as<int>(), h->d_rows.as<float>(), h->stream);
     }
     HV_CHECK_LAUNCH("bf16 layout");
@@ -708,22 +709,26 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // MFMA filter forms: BF16 (certified error bound, exact results) and FP8 (estimates only, a
     // shortlist of k * rerank_r per query re-ranked exactly)
     float* aux = nullptr;
-    float kappa1 = 0.f, kappa2 = 0.f;
+    float kappa1 = 0.f, kappa_g = 0.f, kappa2 = 0.f;
     if (mfma) {
         HV_HIP(h->s_aux.ensure(sizeof(float) * n * cap));
         aux = h->s_aux.as<float>();
         if (h->bf16_grid == 0) h->bf16_grid = hv::bf16_grid_size(h->device);
     }
     if (bf16) {
-        // error bound of the BF16 estimate (see hv_bf16.hip): BF16 unit roundoff 2^-8 on both
-        // residuals, FP32 accumulation over d terms, FP32 norms, and the FP32 chain of the exact path
-        const double ubf = std::ldexp(1.0, -8), u = std::ldexp(1.0, -24);
+        // error bound of the BF16 estimate (hv_bf16.hip): the measured rounding-error norms of
+        // both residuals (factor 2 of the cross term), FP32 accumulation over d terms (twice
+        // g_d for the MFMA's order), FP32 norms, the FP32 evaluation of the estimate and the FP32
+        // chain of the exact path; 1 % on top for the FP32 arithmetic of the bound itself
+        const double u = std::ldexp(1.0, -24);
         const double gd = d * u / (1.0 - d * u);
-        kappa1 = (float)(1.01 * 2.0 * (2.0 * (ubf + u) + (ubf + u) * (ubf + u) + gd * (1.0 + ubf) * (1.0 + ubf)));
+        kappa1 = (float)(1.01 * 2.0);
+        kappa_g = (float)(1.01 * 2.0 * 2.0 * gd);
         kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
         HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
+        HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
         hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)d, h->dpad,
-                              h->s_qres.as<__bf16>(), s);
+                              h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), s);
         HV_CHECK_LAUNCH("pair residuals");
     }
     if (fp8) {
@@ -749,12 +754,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         const bool main_phase = st_slot == stats + 8;
         if (main_phase && !main_recorded) rec(6);  // events 6-7 span the main phases
         main_recorded |= main_phase;
-        hv::launch_mfma_scan(h->bf16_grid, fp8 ? 1 : 0, h->s_qres.p, fp8 ? h->s_qscale.as<float>() : nullptr, (int)d,
-                             pdist, (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
+        // per slot: FP8 inverse scales or BF16 rounding-error norms (s_qscale holds either)
+        hv::launch_mfma_scan(h->bf16_grid, fp8 ? 1 : 0, h->s_qres.p, h->s_qscale.as<float>(), (int)d, pdist,
+                             (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
                              fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
-                             h->d_rnorm.as<float>(), h->d_list_off.as<int64_t>(), h->s_bound.as<float>(),
-                             h->s_pair_slot.as<int>(), h->s_items.as<hv::ScanItem>(), queue, vmax, vsplit_mfma,
-                             mode, kappa1, kappa2, surv, aux, surv_cnt, (int)cap, st_slot, s);
+                             h->d_rnorm.as<float>(), fp8 ? nullptr : h->d_rerr.as<float>(),
+                             h->d_list_off.as<int64_t>(), h->s_bound.as<float>(), h->s_pair_slot.as<int>(),
+                             h->s_items.as<hv::ScanItem>(), queue, vmax, vsplit_mfma, mode, kappa1, kappa_g, kappa2,
+                             surv, aux, surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("mfma scan");
         if (main_phase) rec(7);
     };
@@ -974,7 +981,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
                       &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->s_qnorm,
-                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rows, &h->s_qscale,
+                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale,
                       &h->d_rcodes8, &h->d_lscale8};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)

@@ -95,14 +95,16 @@ void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_ma
 // --- BF16 precision form (hv_bf16.hip)
 int bf16_ksteps(int d);  // 32-dim MFMA steps per vector (dpad = 32 * ksteps)
 int bf16_grid_size(int device);
+// rerr[p]: norm of the BF16 rounding error of row p's residual (+ 2^-23 of its norm)
 void launch_build_resid(const float* x, int d, int dpad, const int* order, const int64_t* list_off,
                         const float* centroids, const int64_t* roff, int nlist, __bf16* rcodes, float* rnorm,
-                        hipStream_t s);
+                        float* rerr, hipStream_t s);
 // rows in list order: by_pos[p] = rows[order[p]]
 void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, float* by_pos, hipStream_t s);
-// BF16 (q - c) rows of dpad dims for every (query, probe rank) slot
+// BF16 (q - c) rows of dpad dims for every (query, probe rank) slot, and the norm of each
+// slot's rounding error (+ 2^-23 of its norm) in qerr
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                       int nprobe, int d, int dpad, __bf16* out, hipStream_t s);
+                       int nprobe, int d, int dpad, __bf16* out, float* qerr, hipStream_t s);
 // FP8 (e4m3) codes of every list from the rows in list order, scaled per list by a power of two
 // (inverse in inv_scale[l]), in the BF16 codes' order, one byte per value
 void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int64_t* list_off,
@@ -111,14 +113,15 @@ void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int
 // FP8 (q - c) rows of every slot, scaled per slot (inverse in inv_scale[slot])
 void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
                            int nprobe, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s);
-// the MFMA filter scan: elem 0 = BF16 codes (qscale, lscale unused), 1 = FP8 codes with the
-// per-slot and per-list inverse scales
-void launch_mfma_scan(int grid, int elem, const void* qres, const float* qscale, int d, const float* pdist,
+// the MFMA filter scan: elem 0 = BF16 codes (qaux = per-slot rounding-error norms, rerr =
+// per-vector ones, lscale unused), 1 = FP8 codes (qaux = per-slot inverse scales, lscale =
+// per-list ones, rerr and the kappas unused)
+void launch_mfma_scan(int grid, int elem, const void* qres, const float* qaux, int d, const float* pdist,
                       int nprobe, const void* rcodes, const float* lscale, const int64_t* roff, const float* rnorm,
-                      const int64_t* list_off, const float* bound, const int* pair_slot, const ScanItem* items,
-                      const int* queue, int vmax, int vsplit, int mode, float kappa1, float kappa2,
-                      unsigned long long* surv, float* aux, unsigned int* surv_cnt, int cap,
-                      unsigned long long* stats, hipStream_t s);
+                      const float* rerr, const int64_t* list_off, const float* bound, const int* pair_slot,
+                      const ScanItem* items, const int* queue, int vmax, int vsplit, int mode, float kappa1,
+                      float kappa_g, float kappa2, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
+                      int cap, unsigned long long* stats, hipStream_t s);
 // FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place;
 // bound[q] drops to the kr-th estimate
 void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, float* bound,

@@ -171,6 +171,14 @@ def test_bf16_float_data_bit_exact(B):
     oD, oI, _, _ = O.search(o, xq[:, perm], 20, nprobe=12, dim_blocks=B)
     np.testing.assert_array_equal(I, oI)
     np.testing.assert_array_equal(D, oD)
+    # the caller's bound at the exact 20th distance: the 20th pair survives only if its lower
+    # bound is certified
+    g.preSearch(dim_blocks=B, bound_mode=1, precision=HV_PRECISION_BF16)
+    g.set_external_bounds(oD[:, -1].copy())
+    D, I = g.search(xq, 20)
+    g.preSearch(dim_blocks=B, bound_mode=0, precision=HV_PRECISION_BF16)
+    np.testing.assert_array_equal(I, oI)
+    np.testing.assert_array_equal(D, oD)
 
 
 @pytest.mark.parametrize("wl", [4])
@@ -214,3 +222,19 @@ def test_w2_bf16_equals_fp32_full_batch(w2, nprobe):
     say("w2 nprobe=%d scan fp32 %.2f ms, bf16 %.2f ms; survivors/query fp32 %.0f bf16 %.0f; overflow %d"
         % (nprobe, s0["ms_scan"], s1["ms_scan"], s0["survivors"] / xq.shape[0], s1["survivors"] / xq.shape[0],
            s1["overflow_queries"]))
+
+
+def test_w2_bf16_tight_external_bound(w2):
+    """1M rows, 10000 queries: with the caller's bound at each query's exact 100th distance,
+    every pair at the boundary must keep a lower bound at or below its exact distance."""
+    xb, xq, g = w2
+    g.nprobe = 16
+    g.preSearch(dim_blocks=4, precision=HV_PRECISION_FP32)
+    D0, I0 = g.search(xq, 100)
+    g.preSearch(dim_blocks=4, bound_mode=1, precision=HV_PRECISION_BF16)
+    g.set_external_bounds(D0[:, -1].copy())
+    D1, I1, st = g.search(xq, 100, return_stats=True)
+    g.preSearch(dim_blocks=4, bound_mode=0, precision=HV_PRECISION_FP32)
+    np.testing.assert_array_equal(I1, I0)
+    np.testing.assert_array_equal(D1, D0)
+    say("w2 tight external bound: survivors/query %.1f" % (st["survivors"] / xq.shape[0]))
