@@ -965,8 +965,13 @@ void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, i
 // dropped; the final select compacts and sorts them.
 //
 // k_rerank_items: one thread per query; lists overflowed queries, counts the re-ranked pairs
-// (stats[2]) and reserves its items with one atomic per workgroup.
+// (stats[2]) and reserves its items with one atomic per workgroup. With the (list, rank)-sorted
+// pairs given, the queries are taken in the order of their rank-0 pairs, i.e. grouped by nearest
+// list: queries of one list share many survivor rows, and the re-rank below runs neighbouring
+// items on one XCD, so those rows are read from HBM once into its L2.
 __global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __restrict__ cnt, int64_t nq, int cap,
+                                                      const int* __restrict__ pairs, const int* __restrict__ pair_slot,
+                                                      const int* __restrict__ npairs, int nprobe,
                                                       int2* __restrict__ items, unsigned int* __restrict__ n_items,
                                                       int* __restrict__ overflow_list, int* __restrict__ overflow_cnt,
                                                       unsigned long long* __restrict__ stats) {
@@ -974,9 +979,15 @@ __global__ __launch_bounds__(256) void k_rerank_items(const unsigned int* __rest
     __shared__ unsigned int s_pairs[4];
     __shared__ unsigned int s_base;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int64_t q = (int64_t)blockIdx.x * 256 + t;
+    const int64_t e = (int64_t)blockIdx.x * 256 + t;
+    int64_t q = -1;
+    if (pairs) {
+        if (e < (int64_t)*npairs && pair_slot[e] % nprobe == 0) q = pairs[e];
+    } else if (e < nq) {
+        q = e;
+    }
     unsigned int c = 0u;
-    if (q < nq) {
+    if (q >= 0) {
         c = cnt[q];
         if (c > (unsigned int)cap) {
             if (overflow_list) overflow_list[atomicAdd(overflow_cnt, 1)] = (int)q;
@@ -1041,6 +1052,9 @@ __device__ __forceinline__ void chain_chunk(const float* __restrict__ qv, int lo
     }
 }
 
+// The items are dealt to the XCDs in contiguous eighths (workgroup b runs on XCD b % 8; the
+// grid is a multiple of 8), each XCD's wavefronts striding through its eighth, so items next to
+// each other (the queries of one nearest list) run together under one L2.
 template <bool V4>
 __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Qp, int d,
                                                      const float* __restrict__ rows_by_pos,
@@ -1051,9 +1065,12 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
                                                      const float* __restrict__ bound,
                                                      unsigned long long* __restrict__ kept) {
     const int lane = threadIdx.x & 63;
-    const int nw = (int)gridDim.x * 4;
-    const int ni = (int)*n_items;
-    int it = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = (int)blockIdx.x % kNumXcd;
+    const int nw = (int)gridDim.x / kNumXcd * 4;  // wavefronts per XCD
+    const int64_t nall = (int64_t)*n_items;
+    const int ni = (int)(nall * (xcd + 1) / kNumXcd);  // end of this XCD's items
+    int it = (int)(nall * xcd / kNumXcd) + (int)blockIdx.x / kNumXcd * 4 +
+             __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (it >= ni) return;
     // an item: (query, first pair | pairs << 16). The next item's descriptor, positions and bound
     // are loaded while the current one is chained.
@@ -1208,11 +1225,13 @@ __global__ __launch_bounds__(256) void k_select_kept(const unsigned long long* _
     if (lane == 0) done[q] = 1u;
 }
 
-void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, int2* items, unsigned int* n_items,
+void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, const int* pairs, const int* pair_slot,
+                         const int* npairs, int64_t max_pairs, int nprobe, int2* items, unsigned int* n_items,
                          int* overflow_list, int* overflow_cnt, unsigned long long* stats, hipStream_t s) {
-    if (nq <= 0) return;
-    hipLaunchKernelGGL(k_rerank_items, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, cnt, nq, cap, items,
-                       n_items, overflow_list, overflow_cnt, stats);
+    const int64_t threads = pairs ? max_pairs : nq;
+    if (nq <= 0 || threads <= 0) return;
+    hipLaunchKernelGGL(k_rerank_items, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, cnt, nq, cap, pairs,
+                       pair_slot, npairs, nprobe, items, n_items, overflow_list, overflow_cnt, stats);
 }
 
 void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
@@ -1222,7 +1241,8 @@ void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint3
     bool v4 = d >= 4 && (d & 3) == 0;
     for (int ch = 0; ch < plan.n; ++ch) v4 = v4 && (plan.lo[ch] & 3) == 0;
     const int64_t max_items = nq * ((cap + 63) / 64);
-    const dim3 grid((unsigned)std::min<int64_t>((max_items + 3) / 4, kRerankGrid));
+    const int64_t wgs = std::min<int64_t>((max_items + 3) / 4, kRerankGrid);
+    const dim3 grid((unsigned)((wgs + kNumXcd - 1) / kNumXcd * kNumXcd));  // every XCD gets workgroups
     if (v4)
         hipLaunchKernelGGL(k_rerank_flat<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
                            n_items, bound, kept);

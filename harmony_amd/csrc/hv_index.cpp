@@ -860,7 +860,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
         HV_HIP(h->s_done.ensure(sizeof(unsigned int) * n));
-        hv::launch_rerank_items(surv_cnt, n, (int)cap, h->s_ritems.as<int2>(), n_items, nullptr, nullptr, stats, s);
+        // queries in nearest-list order (their rank-0 pairs), so the re-rank reads shared rows once
+        hv::launch_rerank_items(surv_cnt, n, (int)cap, h->s_pairs.as<int>(), h->s_pair_slot.as<int>(),
+                                h->s_bucket_off.as<int>() + nb, n * nprobe, (int)nprobe, h->s_ritems.as<int2>(),
+                                n_items, nullptr, nullptr, stats, s);
         rec(8);  // events 8-9 bracket the re-rank kernel alone
         hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, n, (int)cap,
                           fp8 ? nullptr : bound, h->s_ritems.as<int2>(), n_items, h->s_kept.as<unsigned long long>(), s);

@@ -134,13 +134,15 @@ void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt
 // re-rank + final top-k of the MFMA forms, in three launches:
 //  launch_rerank_items: work items of 64 stored pairs (items: nq * ceil(cap/64) entries, n_items
 //    zeroed by the caller); overflowed queries listed when overflow_list is given; stats[2]
-//    counts the pairs;
+//    counts the pairs. Queries in index order, or, with the (list, rank)-sorted pairs and their
+//    slots given (npairs on the device, at most max_pairs), in the order of their rank-0 pairs;
 //  launch_rerank: each pair's exact key written to its slot of kept (nq * cap keys; padding when
 //    above bound[q], no filter when bound is null);
 //  launch_select_kept: the first k kept keys of every non-overflowed query into D/I (done: nq
 //    flags of scratch).
 constexpr int kRerankGrid = 2048;
-void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, int2* items, unsigned int* n_items,
+void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, const int* pairs, const int* pair_slot,
+                         const int* npairs, int64_t max_pairs, int nprobe, int2* items, unsigned int* n_items,
                          int* overflow_list, int* overflow_cnt, unsigned long long* stats, hipStream_t s);
 void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint32_t* ids, const ChunkPlan& plan,
                    const unsigned long long* surv, int64_t nq, int cap, const float* bound, const int2* items,
