@@ -253,7 +253,7 @@ struct hv_index {
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_done, s_stats, s_pdist, s_pair_slot,
-        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_save, s_qnorm, s_cnorm, s_redo, s_qscale;
+        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_qnorm, s_cnorm, s_redo, s_qscale;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
 };
@@ -588,7 +588,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // counters, bucket counts and (when no warm-up, seed or caller's bound applies) the bounds
     // start cleared: one fill launch for all of them
     const int64_t nb = nlist * nprobe;
-    const bool bound_fmax = !(learned && !mfma) && !external && !(wl > 0 && !mfma && !learned);
+    const bool learned_seed = learned && !fp8;  // the seed is the starting bound (FP32 and BF16)
+    const bool bound_fmax = !learned_seed && !external && !(wl > 0 && !mfma && !learned);
     HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
     HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
     // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
@@ -769,10 +770,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     //    warm-up scan: it scans each query's nearest list unpruned and takes the bound from there.
     float* bound = h->s_bound.as<float>();
     float* bound_ref = h->s_bound_ref.as<float>();
-    // learned, BF16: no bound for the nearest lists (their k-th upper bound gives the safe bound
-    // below, then the seed lowers it); learned, FP32: the seed from the start. No warm-up. The
-    // FP8 form starts without a bound and filters on estimates only.
-    if (learned && !mfma) {
+    // learned (FP32 and BF16): the seed from the start, no warm-up. The FP8 form starts without a
+    // bound and filters on estimates only.
+    if (learned_seed) {
         HV_HIP(hipMemcpyAsync(bound_ref, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     } else if (external) {
@@ -787,17 +787,21 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
     //    the remaining probed lists
     const int prune = B > 1 ? 1 : 0;
-    // learned: in the BF16 form the bound after the nearest lists is safe (an upper bound of the
-    // k-th distance over the probed lists): keep it for the exact redo, then lower the bound to
-    // the seed. The FP32 form ran the nearest lists with the seed already (its redo has no bound).
-    float* bound_safe = nullptr;
-    auto apply_seed = [&]() {
-        if (!learned || !bf16) return;
-        HV_HIP(h->s_save.ensure(sizeof(float) * n));
-        bound_safe = h->s_save.as<float>();
-        hv::launch_apply_seed(bound, seed, bound_safe, bound_ref, n, s);
-    };
-    if (mfma) {
+    // learned: the ranks past the predictor's cap are never searched (qn <= nprobe_cap)
+    const int64_t rank_end = learned ? std::min<int64_t>(nprobe, nprobe_cap) : nprobe;
+    if (mfma && learned_seed) {
+        // learned, BF16: the seed filters from the nearest lists on; the ranks go in groups
+        // [0, 1), [1, 4), [4, 16), ..., each followed by the compaction, which lowers the bound
+        // to the k-th smallest upper bound so far (when below the seed) and drops the pairs
+        // whose lower bound exceeds it
+        for (int64_t r0 = 0; r0 < rank_end;) {
+            const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
+            mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, r0 == 0 ? stats : stats + 8);
+            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
+            r0 = r1;
+        }
+        HV_CHECK_LAUNCH("compact");
+    } else if (mfma) {
         // nearest list: vectors below cap go to their own slot unfiltered; the k-th smallest upper
         // bound (FP8: the kr-th smallest estimate) then lowers the bound and the pairs whose lower
         // bound exceeds it are dropped (vector ranges start on 16-vector blocks)
@@ -806,10 +810,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
         hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, s);
         HV_CHECK_LAUNCH("compact");
-        if (learned && bf16)
-            apply_seed();
-        else
-            HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
         if (!fp8 && nprobe <= kBf16SinglePhase) {
@@ -830,13 +831,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             // estimate so far; the probe ranks go in groups of growing width ([1, 4), [4, 16),
             // ...), each followed by the shortlist, which keeps the kr smallest and lowers the
             // filter. The last shortlist is what the exact re-rank sees.
-            for (int64_t r0 = 1; r0 < nprobe;) {
-                const int64_t r1 = std::min<int64_t>(nprobe, r0 * 4);
+            for (int64_t r0 = 1; r0 < rank_end;) {
+                const int64_t r1 = std::min<int64_t>(rank_end, r0 * 4);
                 mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, stats + 8);
                 hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
                 r0 = r1;
             }
-            if (nprobe == 1) hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
+            if (rank_end == 1) hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
             HV_CHECK_LAUNCH("shortlist");
         }
     } else {
@@ -845,7 +846,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             scan_phase(0, 1, INT32_MAX, prune, 0, stats);
             hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
             HV_CHECK_LAUNCH("compact");
-            apply_seed();
             r_next = 1;
         }
         scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
@@ -877,14 +877,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     // 7. the exact path redoes the queries whose survivor buffer overflowed (rare) and, in learned
     //    mode, those whose k-th result is missing or above the seeded bound (a neighbour may have
-    //    been pruned), over the same probe lists, with the safe bound in learned mode (none in the
-    //    FP32 form). The list and its count stay on the device: the kernels take the first batch
+    //    been pruned), over the same probe lists, with no bound in learned mode. The list and its count stay on the device: the kernels take the first batch
     //    by themselves, and the host adds batches only when the final count exceeds one.
     // (the check is against the bound the seed set, bound_ref: later compactions only tighten
     //  the working bound towards the k-th distance itself)
     hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
                          redo_cnt, stats + 12, s);
-    const float* redo_bound = fp8 ? nullptr : (learned ? bound_safe : bound);
+    const float* redo_bound = fp8 || learned ? nullptr : bound;
     const int nseg = hv::exact_segments(h->max_list);
     const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
     const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
@@ -983,7 +982,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
-                      &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_save, &h->s_qnorm,
+                      &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_qnorm,
                       &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale,
                       &h->d_rcodes8, &h->d_lscale8};
     for (DevBuf* b : bufs) b->release();

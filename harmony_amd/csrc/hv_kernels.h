@@ -177,9 +177,6 @@ void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf
                       const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
                       float* bound, float* pred, hipStream_t s);
 
-// learned seed after the nearest lists: safe = bound; bound = ref = min(bound, seed)
-void launch_apply_seed(float* bound, const float* seed, float* safe, float* ref, int64_t n, hipStream_t s);
-
 // --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
 void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);
 float coarse_kappa(int d);

@@ -261,23 +261,4 @@ void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf
                        nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred);
 }
 
-
-// The learned seed applied to the bound after the nearest lists: safe[i] = bound[i] (kept for the
-// exact redo), then bound[i] = ref[i] = min(bound[i], seed[i]).
-__global__ void k_apply_seed(float* __restrict__ bound, const float* __restrict__ seed, float* __restrict__ safe,
-                             float* __restrict__ ref, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float b = bound[i];
-    safe[i] = b;
-    const float m = fminf(b, seed[i]);
-    bound[i] = m;
-    ref[i] = m;
-}
-
-void launch_apply_seed(float* bound, const float* seed, float* safe, float* ref, int64_t n, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_apply_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bound, seed, safe, ref, n);
-}
-
 }  // namespace hv
