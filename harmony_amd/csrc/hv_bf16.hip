@@ -356,10 +356,12 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
     constexpr int QBM = 1;  // query blocks per wavefront
     constexpr bool kPrefetch = NKS <= 8;                     // larger K keeps occupancy instead
     __shared__ int s_q[64], s_slot[64];
+    // per query row of the item: |rq|^2, the bound (NaN past the item's queries: every
+    // comparison fails), and the row's factors of the estimate and its error bound
     __shared__ __align__(16) float s_nrq[64];
     __shared__ __align__(16) float s_srq[64];  // BF16: |rq| + |eq|
     __shared__ __align__(16) float s_bound[64];
-    __shared__ __align__(16) float s_qsc[64];  // FP8: the slot's inverse scale; BF16: |eq|
+    __shared__ __align__(16) float s_qsc[64];  // BF16: kappa1 |eq|; FP8: -2 x the list's and slot's scales
     typedef typename E::vec vec;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, col16 = lane & 15;
@@ -371,14 +373,15 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
     for (int slot_i = (int)blockIdx.x / kNumXcd; slot_i < qlen; slot_i += per_xcd) {
         const ScanItem item = items[qbase + slot_i];
         const int nQ = item.pair_end - item.pair_begin;
+        const int l = item.list;
         if (tid < 64) {
             int q = 0, slot = 0;
-            float nr = 0.f, b = -1.f, qa = 0.f;  // rows past nQ: bound -1 keeps nothing
+            float nr = 0.f, b = __builtin_nanf(""), qa = 0.f;
             if (tid < nQ) {
                 slot = pair_slot[item.pair_begin + tid];
                 q = slot / nprobe;
                 nr = pdist[slot];
-                b = mode == 1 ? 0.f : bound[q];
+                b = mode == 1 ? __builtin_inff() : bound[q];  // mode 1 keeps every pair
                 qa = qaux[slot];
             }
             s_q[tid] = q;
@@ -386,10 +389,8 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
             s_nrq[tid] = nr;
             s_srq[tid] = __builtin_amdgcn_sqrtf(nr) + (E::kScaled ? 0.f : qa);
             s_bound[tid] = b;
-            s_qsc[tid] = qa;
+            s_qsc[tid] = E::kScaled ? -2.f * lscale[l] * qa : kappa1 * qa;  // powers of two: exact
         }
-        const int l = item.list;
-        const float lsc = E::kScaled ? lscale[l] : 1.f;
         const int64_t lbeg = list_off[l];
         const int nl = (int)min(list_off[l + 1] - lbeg, (long long)vmax);
         const int vend = min(nl, item.vec_begin + vsplit);
@@ -463,9 +464,11 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                 }
                 const int v = v0 + col16;
                 const bool cvalid = v < vend;
-                // per-vector parts of the error bound (BF16)
-                const float sx = __builtin_amdgcn_sqrtf(nx);  // within 1 ulp (the 1.01 factor covers it)
-                const float kg_x = kappa_g * (sx + ex), k2_x = kappa2 * nx;
+                // per-vector factors of the error bound (BF16):
+                //   err = kappa1 |eq| sx + (|rq| + |eq|) vx + kappa2 (|rq|^2 + |rx|^2),
+                //   vx = kappa1 |ex| + kappa_g (sx + |ex|)
+                const float sx = E::kScaled ? 0.f : __builtin_amdgcn_sqrtf(nx);  // within 1 ulp (covered)
+                const float vx = E::kScaled ? 0.f : __builtin_fmaf(kappa1, ex, kappa_g * (sx + ex));
                 const uint32_t pos = (uint32_t)(lbeg + v);
                 pairs_done += 256 * nqb;
 #pragma unroll
@@ -481,31 +484,30 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                     const float nrq[4] = {nr4.x, nr4.y, nr4.z, nr4.w};
                     const float bq[4] = {b4.x, b4.y, b4.z, b4.w};
                     const float qa[4] = {q4.x, q4.y, q4.z, q4.w};
-                    float dot[4] = {acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
                     float sqp[4] = {0.f, 0.f, 0.f, 0.f};
-                    if (E::kScaled) {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) dot[i] *= lsc * qa[i];
-                    } else {
+                    if (!E::kScaled) {
                         const float4 sr4 = *reinterpret_cast<const float4*>(&s_srq[r0]);
                         sqp[0] = sr4.x;
                         sqp[1] = sr4.y;
                         sqp[2] = sr4.z;
                         sqp[3] = sr4.w;
                     }
-                    float lbv[4], ubv[4];
+                    // estimate dt and its error bound er (FP8: estimates only, no bound); the
+                    // pair is kept when dt - er <= bound (never for rows past the item's queries)
+                    float dtv[4], erv[4];
                     bool kp[4];
                     bool anyk = false;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const float dt = nrq[i] + nx - 2.f * dot[i];
-                        // FP8: estimates only (no bound)
-                        const float err = E::kScaled ? 0.f
-                                                     : kappa1 * __builtin_fmaf(qa[i], sx, sqp[i] * ex) +
-                                                           __builtin_fmaf(kg_x, sqp[i], __builtin_fmaf(kappa2, nrq[i], k2_x));
-                        lbv[i] = fmaxf(dt - err, 0.f);
-                        ubv[i] = fmaxf(dt + err, 0.f);
-                        kp[i] = cvalid && bq[i] >= 0.f && (mode == 1 || lbv[i] <= bq[i]);
+                        const float nsum = nrq[i] + nx;
+                        if (E::kScaled) {
+                            dtv[i] = __builtin_fmaf(acc[j][i], qa[i], nsum);
+                            erv[i] = 0.f;
+                        } else {
+                            dtv[i] = __builtin_fmaf(-2.f, acc[j][i], nsum);
+                            erv[i] = __builtin_fmaf(qa[i], sx, __builtin_fmaf(sqp[i], vx, kappa2 * nsum));
+                        }
+                        kp[i] = cvalid && dtv[i] - erv[i] <= bq[i];
                         anyk |= kp[i];
                     }
                     if (mode == 1) {
@@ -514,8 +516,8 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                         for (int i = 0; i < 4; ++i) {
                             if (kp[i] && v < cap) {
                                 const int64_t o = (int64_t)s_q[r0 + i] * cap + v;
-                                surv[o] = hv_key(lbv[i], pos);
-                                aux[o] = ubv[i];
+                                surv[o] = hv_key(fmaxf(dtv[i] - erv[i], 0.f), pos);
+                                aux[o] = fmaxf(dtv[i] + erv[i], 0.f);
                             }
                         }
                         continue;
@@ -541,8 +543,8 @@ __global__ __launch_bounds__(256) void k_mfma_scan(const typename E::elem* __res
                             const unsigned int slot = bs + (unsigned int)__builtin_popcount(gm[i] & ((1u << col16) - 1u));
                             if (slot < (unsigned int)cap) {
                                 const int64_t o = (int64_t)s_q[r0 + i] * cap + slot;
-                                surv[o] = hv_key(lbv[i], pos);
-                                aux[o] = ubv[i];
+                                surv[o] = hv_key(fmaxf(dtv[i] - erv[i], 0.f), pos);
+                                aux[o] = fmaxf(dtv[i] + erv[i], 0.f);
                             }
                         }
                     }
