@@ -95,10 +95,11 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
 }
 
 // The exact chain of one (query, centroid) pair, acc = fma(q-c, q-c, acc) over the dimensions in
-// order. The centroid row is loaded 32 dims at a time (16-byte loads when d % 4 == 0), the next
-// 32 in flight while the current ones are chained; q is read where it lies (LDS, or memory with a
-// wavefront-uniform address).
-constexpr int kChainDims = 32;
+// order. The centroid row is loaded 16 dims at a time (16-byte loads when d % 4 == 0), the next
+// 16 in flight while the current ones are chained (32 registers for both: the select kernels
+// keep their occupancy); q is read where it lies (LDS, or memory with a wavefront-uniform
+// address).
+constexpr int kChainDims = 16;
 
 __device__ __forceinline__ void coarse_load(const float* __restrict__ row, int lo, int d, bool v4,
                                             float (&r)[kChainDims]) {
@@ -277,10 +278,10 @@ __global__ __launch_bounds__(256) void k_coarse_select(const float* __restrict__
 }
 
 // The same selection with one wavefront per query (P <= kWaveP, nc <= 64 * E): the upper bounds
-// live in registers, tau comes from wave_kth_u32, the candidates' exact keys (at most
-// kWaveCand, else the query is listed for k_coarse_exact_row) are sorted in registers.
+// live in registers, tau comes from wave_kth_u32, the candidates' exact keys (at most CAND, else
+// the query is listed for k_coarse_exact_row) are sorted in registers. CAND = 256 for P <= 128
+// (less LDS, more resident wavefronts), 512 beyond.
 constexpr int kWaveP = 256;
-constexpr int kWaveCand = 512;
 
 template <int EC>
 __device__ __forceinline__ void coarse_finish_wave(const unsigned long long* __restrict__ keys, int m, int P,
@@ -303,7 +304,7 @@ __device__ __forceinline__ void coarse_finish_wave(const unsigned long long* __r
     }
 }
 
-template <int E>
+template <int E, int CAND>
 __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restrict__ approx,
                                                             const float* __restrict__ Q,
                                                             const float* __restrict__ C, int nc, int d,
@@ -312,8 +313,8 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
                                                             int P, int* __restrict__ probes,
                                                             float* __restrict__ pdist, int* __restrict__ redo_list,
                                                             int* __restrict__ redo_cnt) {
-    __shared__ unsigned int s_cand[4][kWaveCand];
-    __shared__ unsigned long long s_ckey[4][kWaveCand];
+    __shared__ unsigned int s_cand[4][CAND];
+    __shared__ unsigned long long s_ckey[4][CAND];
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -322,32 +323,28 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
     const float* __restrict__ row = approx + q * nc;
     const float qq = qn[q];
     unsigned int ub[E];
-    float lbv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int c = e * 64 + lane;
         ub[e] = 0xffffffffu;
-        lbv[e] = 3.402823466e38f;
         if (c < nc) {
-            const float dt = row[c];
-            const float err = kappa * (qq + cn[c]);
-            lbv[e] = dt - err;
-            const unsigned int u = __float_as_uint(dt + err);
+            const unsigned int u = __float_as_uint(row[c] + kappa * (qq + cn[c]));
             ub[e] = (int)u < 0 ? 0u : u;  // negative upper bounds (and -0) order as 0
         }
     }
     const float tau = __uint_as_float(wave_kth_u32<E>(ub, P, s_tmp[w]));
+    // lower bounds from the row again (cache hits) rather than held through the search
     int m = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int c = e * 64 + lane;
-        const bool in = c < nc && lbv[e] <= tau;
+        const bool in = c < nc && row[c] - kappa * (qq + cn[c]) <= tau;
         const unsigned long long b = __ballot(in);
         const int slot = m + (int)__builtin_popcountll(b & ((1ull << lane) - 1ull));
-        if (in && slot < kWaveCand) s_cand[w][slot] = (unsigned int)c;
+        if (in && slot < CAND) s_cand[w][slot] = (unsigned int)c;
         m += (int)__builtin_popcountll(b);
     }
-    if (m > kWaveCand) {  // too wide a band: the exact kernel redoes this query
+    if (m > CAND) {  // too wide a band: the exact kernel redoes this query
         if (lane == 0) redo_list[atomicAdd(redo_cnt, 1)] = (int)q;
         return;
     }
@@ -366,7 +363,7 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
         coarse_finish_wave<1>(s_ckey[w], m, P, pq, dq);
     else if (m <= 128)
         coarse_finish_wave<2>(s_ckey[w], m, P, pq, dq);
-    else if (m <= 256)
+    else if (CAND <= 256 || m <= 256)
         coarse_finish_wave<4>(s_ckey[w], m, P, pq, dq);
     else
         coarse_finish_wave<8>(s_ckey[w], m, P, pq, dq);
@@ -419,20 +416,22 @@ void launch_coarse_select(const float* approx, const float* Q, const float* C, i
                           int* redo_list, int* redo_cnt, hipStream_t s) {
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
-    if (P <= kWaveP && nc <= 512)
-        hipLaunchKernelGGL(k_coarse_select_wave<8>, wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, nq, P,
-                           probes, pdist, redo_list, redo_cnt);
-    else if (P <= kWaveP && nc <= 1024)
-        hipLaunchKernelGGL(k_coarse_select_wave<16>, wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, nq, P,
-                           probes, pdist, redo_list, redo_cnt);
-    else if (P <= kWaveP && nc <= 2048)
-        hipLaunchKernelGGL(k_coarse_select_wave<32>, wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, nq, P,
-                           probes, pdist, redo_list, redo_cnt);
-    else {
+#define HV_CSEL(E, CAND)                                                                                        \
+    hipLaunchKernelGGL((k_coarse_select_wave<E, CAND>), wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, \
+                       nq, P, probes, pdist, redo_list, redo_cnt)
+    const bool narrow = P <= 128;
+    if (P <= kWaveP && nc <= 512) {
+        if (narrow) HV_CSEL(8, 256); else HV_CSEL(8, 512);
+    } else if (P <= kWaveP && nc <= 1024) {
+        if (narrow) HV_CSEL(16, 256); else HV_CSEL(16, 512);
+    } else if (P <= kWaveP && nc <= 2048) {
+        if (narrow) HV_CSEL(32, 256); else HV_CSEL(32, 512);
+    } else {
         const size_t lds = sizeof(unsigned long long) * kCoarseCand + sizeof(float) * ((d + 3) & ~3);
         hipLaunchKernelGGL(k_coarse_select, dim3((unsigned)nq), dim3(256), lds, s, approx, Q, C, nc, d, qn, cn, kappa,
                            P, probes, pdist, redo_list, redo_cnt);
     }
+#undef HV_CSEL
     const size_t lds2 = sizeof(unsigned long long) * hv_next_pow2(nc > 2 ? nc : 2) + sizeof(float) * ((d + 3) & ~3);
     hipLaunchKernelGGL(k_coarse_exact_row, dim3((unsigned)std::min<int64_t>(nq, 256)), dim3(256), lds2, s, redo_list,
                        redo_cnt, Q, C, nc, d, P, probes, pdist);
