@@ -170,11 +170,15 @@ __device__ __forceinline__ void load8_diff(const float* __restrict__ qv, const f
 template <int G>
 __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp, const float* __restrict__ centroids,
                                                     const int* __restrict__ probes, const int* __restrict__ qn,
-                                                    int64_t nslots, int nprobe, int d, int dpad,
+                                                    int64_t nitems, int nprobe, int nranks, int d, int dpad,
                                                     bf16x8* __restrict__ out, float* __restrict__ qerr) {
-    const int64_t slot = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    // item = (query, rank < nranks); its slot in the [nq][nprobe] arrays
+    const int64_t it = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int64_t qi = it / nranks;
+    const int j = (int)(it - qi * nranks);
+    const int64_t slot = qi * nprobe + j;
     const int g = threadIdx.x % G;
-    const bool live = slot < nslots && !(qn && (int)(slot % nprobe) >= qn[slot / nprobe]);
+    const bool live = it < nitems && !(qn && j >= qn[qi]);
     const int k0 = g * 8;
     float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, se = 0.f, sr = 0.f;
     bf16x8 r;
@@ -196,13 +200,13 @@ __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp
 }
 
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                       int nprobe, int d, int dpad, __bf16* out, float* qerr, hipStream_t s) {
-    const int64_t nslots = nq * nprobe;
-    if (nslots <= 0) return;
+                       int nprobe, int nranks, int d, int dpad, __bf16* out, float* qerr, hipStream_t s) {
+    const int64_t nitems = nq * nranks;
+    if (nitems <= 0) return;
     const int words = dpad / 8;  // <= 64
 #define HV_RESID(G)                                                                                          \
-    hipLaunchKernelGGL(k_pair_resid<G>, dim3((unsigned)((nslots + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
-                       Qp, centroids, probes, qn, nslots, nprobe, d, dpad, reinterpret_cast<bf16x8*>(out), qerr)
+    hipLaunchKernelGGL(k_pair_resid<G>, dim3((unsigned)((nitems + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
+                       Qp, centroids, probes, qn, nitems, nprobe, nranks, d, dpad, reinterpret_cast<bf16x8*>(out), qerr)
     if (words <= 4)
         HV_RESID(4);
     else if (words <= 8)
@@ -273,11 +277,14 @@ template <int G>
 __global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict__ Qp,
                                                         const float* __restrict__ centroids,
                                                         const int* __restrict__ probes, const int* __restrict__ qn,
-                                                        int64_t nslots, int nprobe, int d, int dpad,
+                                                        int64_t nitems, int nprobe, int nranks, int d, int dpad,
                                                         long* __restrict__ out, float* __restrict__ inv_scale) {
-    const int64_t slot = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int64_t it = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;  // (query, rank < nranks)
+    const int64_t qi = it / nranks;
+    const int j = (int)(it - qi * nranks);
+    const int64_t slot = qi * nprobe + j;
     const int g = threadIdx.x % G;
-    const bool live = slot < nslots && !(qn && (int)(slot % nprobe) >= qn[slot / nprobe]);
+    const bool live = it < nitems && !(qn && j >= qn[qi]);
     float r[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, m = 0.f;
     const int k0 = g * 8;
     if (live) {
@@ -296,13 +303,14 @@ __global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict_
 }
 
 void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                           int nprobe, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s) {
-    const int64_t nslots = nq * nprobe;
-    if (nslots <= 0) return;
+                           int nprobe, int nranks, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s) {
+    const int64_t nitems = nq * nranks;
+    if (nitems <= 0) return;
     const int words = dpad / 8;  // <= 64
 #define HV_RESID8(G)                                                                                           \
-    hipLaunchKernelGGL(k_pair_resid_fp8<G>, dim3((unsigned)((nslots + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
-                       Qp, centroids, probes, qn, nslots, nprobe, d, dpad, reinterpret_cast<long*>(out), inv_scale)
+    hipLaunchKernelGGL(k_pair_resid_fp8<G>, dim3((unsigned)((nitems + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
+                       Qp, centroids, probes, qn, nitems, nprobe, nranks, d, dpad, reinterpret_cast<long*>(out),   \
+                       inv_scale)
     if (words <= 4)
         HV_RESID8(4);
     else if (words <= 8)
@@ -637,7 +645,8 @@ int bf16_grid_size(int device) {
 // 256-bin LDS histogram, starting at the first byte where the smallest and largest values differ
 // (all values share the bytes above it). An overflowed query (count > cap) keeps its count: its
 // bound still tightens (the k-th over the stored subset is >= the k-th over all), and the exact
-// path redoes it.
+// path redoes it. With `last` given, a query whose count is what the previous compaction left
+// (no pair appended since) is skipped: its buffer and bound would come out unchanged.
 constexpr int kCompactPer = kMaxSelectRow / 256;
 constexpr int kCompactWaveMax = 2048;  // one wavefront per query up to here,
 constexpr int kCompactWaveBig = 4096;  // then with 64 values per lane; larger: the workgroup kernel
@@ -645,7 +654,7 @@ constexpr int kCompactWaveBig = 4096;  // then with 64 values per lane; larger: 
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
                                                       float* __restrict__ aux,
                                                       unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
-                                                      float* __restrict__ bound) {
+                                                      float* __restrict__ bound, unsigned int* __restrict__ last) {
     __shared__ unsigned int s_hist[256];
     __shared__ unsigned int s_red[2][4];
     __shared__ unsigned int s_sel[2];
@@ -654,7 +663,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
-    if (n <= kCompactWaveBig) return;  // k_bf16_compact_wave(_big)
+    if (n <= kCompactWaveBig || (last && last[q] == c)) return;  // k_bf16_compact_wave(_big); unchanged
     float b = bound[q];
     unsigned long long* __restrict__ buf = surv + q * cap;
     const unsigned int* __restrict__ ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
@@ -743,7 +752,10 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
         b = fminf(b, __uint_as_float(prefix));
     }
     if (c > (unsigned int)cap) {
-        if (t == 0) bound[q] = b;
+        if (t == 0) {
+            bound[q] = b;
+            if (last) last[q] = c;
+        }
         return;
     }
     // every key is in registers: rewrite the buffer with the pairs whose lower bound is <= b
@@ -766,6 +778,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     if (t == 0) {
         cnt[q] = s_out;
         bound[q] = b;
+        if (last) last[q] = s_out;
     }
 }
 
@@ -776,7 +789,8 @@ template <int E>
 __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restrict__ buf,
                                                     unsigned int* __restrict__ ubits, int n, unsigned int c,
                                                     int cap, int k, float b, unsigned int* __restrict__ cnt_q,
-                                                    float* __restrict__ bound_q, unsigned int* s_tmp) {
+                                                    float* __restrict__ bound_q, unsigned int* __restrict__ last_q,
+                                                    unsigned int* s_tmp) {
     const int lane = threadIdx.x & 63;
     if (n >= k) {
         unsigned int u[E];
@@ -793,7 +807,10 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
         b = fminf(b, __uint_as_float(wave_kth_u32<E>(u, k, s_tmp)));
     }
     if (c > (unsigned int)cap) {
-        if (lane == 0) *bound_q = b;
+        if (lane == 0) {
+            *bound_q = b;
+            if (last_q) *last_q = c;
+        }
         return;
     }
     // the keys pass through registers H per lane at a time: every key of a part is loaded before
@@ -825,13 +842,14 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
     if (lane == 0) {
         *cnt_q = out;
         *bound_q = b;
+        if (last_q) *last_q = out;
     }
 }
 
 __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* __restrict__ surv,
                                                            float* __restrict__ aux,
                                                            unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
-                                                           float* __restrict__ bound) {
+                                                           float* __restrict__ bound, unsigned int* __restrict__ last) {
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -839,24 +857,27 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     if (n > kCompactWaveMax) return;  // k_bf16_compact_wave_big, k_bf16_compact
+    if (last && last[q] == c) return;  // unchanged since the previous compaction
     unsigned long long* buf = surv + q * cap;
     unsigned int* ubits = reinterpret_cast<unsigned int*>(aux + q * cap);
     const float b = bound[q];
+    unsigned int* lq = last ? last + q : nullptr;
     if (n <= 256)
-        bf16_compact_wave_e<4>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+        bf16_compact_wave_e<4>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
     else if (n <= 512)
-        bf16_compact_wave_e<8>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+        bf16_compact_wave_e<8>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
     else if (n <= 1024)
-        bf16_compact_wave_e<16>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+        bf16_compact_wave_e<16>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
     else
-        bf16_compact_wave_e<32>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, s_tmp[w]);
+        bf16_compact_wave_e<32>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
 }
 
 // buffers of kCompactWaveMax < n <= kCompactWaveBig pairs (64 upper bounds per lane)
 __global__ __launch_bounds__(256) void k_bf16_compact_wave_big(unsigned long long* __restrict__ surv,
                                                                float* __restrict__ aux,
                                                                unsigned int* __restrict__ cnt, int64_t nq, int cap,
-                                                               int k, float* __restrict__ bound) {
+                                                               int k, float* __restrict__ bound,
+                                                               unsigned int* __restrict__ last) {
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -864,19 +885,22 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave_big(unsigned long lon
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     if (n <= kCompactWaveMax || n > kCompactWaveBig) return;
+    if (last && last[q] == c) return;  // unchanged since the previous compaction
     bf16_compact_wave_e<kCompactWaveBig / 64>(surv + q * cap, reinterpret_cast<unsigned int*>(aux + q * cap), n,
-                                              c, cap, k, bound[q], cnt + q, bound + q, s_tmp[w]);
+                                              c, cap, k, bound[q], cnt + q, bound + q, last ? last + q : nullptr,
+                                              s_tmp[w]);
 }
 
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
-                         float* bound, hipStream_t s) {
+                         float* bound, unsigned int* last, hipStream_t s) {
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
-    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
+    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last);
     if (cap > kCompactWaveMax)
-        hipLaunchKernelGGL(k_bf16_compact_wave_big, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
+        hipLaunchKernelGGL(k_bf16_compact_wave_big, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last);
     if (cap > kCompactWaveBig)
-        hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound);
+        hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound,
+                           last);
 }
 
 // FP8 shortlist: each query keeps its kr smallest estimates (keys' distance bits; every key tied

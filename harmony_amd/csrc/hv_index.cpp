@@ -253,7 +253,7 @@ struct hv_index {
     // search scratch (grow-only)
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_done, s_stats, s_pdist, s_pair_slot,
-        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_qnorm, s_cnorm, s_redo, s_qscale;
+        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_qnorm, s_cnorm, s_redo, s_qscale, s_last;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
 };
@@ -507,6 +507,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     const bool mfma = bf16 || fp8;
     const int64_t nprobe_cap = nprobe;
     if (learned) nprobe = std::min<int64_t>(nlist, std::max<int64_t>(nprobe, hv::kNearest));
+    // learned: the probe lists are selected kNearest deep for the features, but the ranks past the
+    // predictor's cap are never searched (qn <= nprobe_cap)
+    const int64_t rank_end = learned ? std::min<int64_t>(nprobe, nprobe_cap) : nprobe;
     int64_t wl = learned ? 0 : std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
     int64_t wls = h->sp.warmup_list_size == 0 ? k : h->sp.warmup_list_size;  // query.cpp:328-333
     if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
@@ -599,6 +602,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bound.ensure(sizeof(float) * n));
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
+    if (mfma) HV_HIP(h->s_last.ensure(sizeof(unsigned int) * n));
     {
         hv::FillSegs f{};
         auto add = [&](void* p, int64_t words, int v) {
@@ -613,6 +617,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         add(h->s_surv_cnt.p, n, 0);
         add(h->s_overflow.p, 2, 0);  // redo count, re-rank item count
         if (coarse_on_mfma(h, nprobe)) add(coarse_counter(h, n), 1, 0);
+        if (mfma) add(h->s_last.p, n, -1);  // no compaction yet: counts never match
         if (bound_fmax) {
             int fm;
             const float fmax = FLT_MAX;
@@ -663,9 +668,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
-    hv::launch_count_pairs(probes, n, (int)nprobe, qn, h->s_bucket.as<int>(), s);
+    hv::launch_count_pairs(probes, n, (int)nprobe, (int)rank_end, qn, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
-    hv::launch_scatter_pairs(probes, n, (int)nprobe, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
+    hv::launch_scatter_pairs(probes, n, (int)nprobe, (int)rank_end, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
                              h->s_pairs.as<int>(), h->s_pair_slot.as<int>(), s);
     HV_CHECK_LAUNCH("grouping");
     rec(2);
@@ -710,6 +715,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // MFMA filter forms: BF16 (certified error bound, exact results) and FP8 (estimates only, a
     // shortlist of k * rerank_r per query re-ranked exactly)
     float* aux = nullptr;
+    unsigned int* last = mfma ? h->s_last.as<unsigned int>() : nullptr;  // compaction counts
     float kappa1 = 0.f, kappa_g = 0.f, kappa2 = 0.f;
     if (mfma) {
         HV_HIP(h->s_aux.ensure(sizeof(float) * n * cap));
@@ -728,8 +734,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
         HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
         HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
-        hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)d, h->dpad,
-                              h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), s);
+        hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)rank_end, (int)d,
+                              h->dpad, h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), s);
         HV_CHECK_LAUNCH("pair residuals");
     }
     if (fp8) {
@@ -744,7 +750,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         }
         HV_HIP(h->s_qres.ensure(n * nprobe * h->dpad));
         HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
-        hv::launch_pair_resid_fp8(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)d, h->dpad,
+        hv::launch_pair_resid_fp8(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)rank_end, (int)d,
+                                  h->dpad,
                                   h->s_qres.as<uint8_t>(), h->s_qscale.as<float>(), s);
         HV_CHECK_LAUNCH("pair residuals");
     }
@@ -787,8 +794,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // 5. list scan: each query's nearest list first, then (bound lowered to the running k-th)
     //    the remaining probed lists
     const int prune = B > 1 ? 1 : 0;
-    // learned: the ranks past the predictor's cap are never searched (qn <= nprobe_cap)
-    const int64_t rank_end = learned ? std::min<int64_t>(nprobe, nprobe_cap) : nprobe;
     if (mfma && learned_seed) {
         // learned, BF16: the seed filters from the nearest lists on; the ranks go in groups
         // [0, 1), [1, 4), [4, 16), ..., each followed by the compaction, which lowers the bound
@@ -797,7 +802,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         for (int64_t r0 = 0; r0 < rank_end;) {
             const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
             mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, r0 == 0 ? stats : stats + 8);
-            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
+            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, s);
             r0 = r1;
         }
         HV_CHECK_LAUNCH("compact");
@@ -808,7 +813,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         const int vdirect = (int)(cap / 16 * 16);
         if (vdirect > 0) mfma_phase(0, 1, 0, vdirect, 1, stats);
         hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
-        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, s);
+        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, last, s);
         HV_CHECK_LAUNCH("compact");
         HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
@@ -822,7 +827,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             for (int64_t r0 = 1; r0 < nprobe;) {
                 const int64_t r1 = std::min<int64_t>(nprobe, r0 * 4);
                 mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, stats + 8);
-                hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, s);
+                hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, s);
                 r0 = r1;
             }
             HV_CHECK_LAUNCH("compact");
@@ -877,8 +882,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     // 7. the exact path redoes the queries whose survivor buffer overflowed (rare) and, in learned
     //    mode, those whose k-th result is missing or above the seeded bound (a neighbour may have
-    //    been pruned), over the same probe lists, with no bound in learned mode. The list and its count stay on the device: the kernels take the first batch
-    //    by themselves, and the host adds batches only when the final count exceeds one.
+    //    been pruned), over the same probe lists, with no bound in learned mode. The list and its
+    //    count stay on the device: the kernels take the first batch by themselves, and the host
+    //    adds batches only when the final count exceeds one.
     // (the check is against the bound the seed set, bound_ref: later compactions only tighten
     //  the working bound towards the k-th distance itself)
     hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
@@ -983,7 +989,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
                       &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_qnorm,
-                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale,
+                      &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale, &h->s_last,
                       &h->d_rcodes8, &h->d_lscale8};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)

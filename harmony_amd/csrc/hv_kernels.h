@@ -47,7 +47,7 @@ void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 // up to kFillSegs fills of 32-bit words (pointer, word count, value) in one launch
-constexpr int kFillSegs = 8;
+constexpr int kFillSegs = 10;
 struct FillSegs {
     int* p[kFillSegs];
     int64_t n[kFillSegs];
@@ -55,13 +55,15 @@ struct FillSegs {
     int nseg;
 };
 void launch_fill_multi(const FillSegs& f, hipStream_t s);
-// qn (optional): per-query probe counts (ranks at or past qn[q] are skipped)
-void launch_count_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, int* bucket_cnt, hipStream_t s);
+// qn (optional): per-query probe counts (ranks at or past qn[q] are skipped); nranks (<= nprobe):
+// the ranks looked at (every qn[q] <= nranks)
+void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int nranks, const int* qn, int* bucket_cnt,
+                        hipStream_t s);
 // exclusive scan of n ints into out (n+1 entries, out[n] = total); tmp >= scan_tmp_ints(n)
 int64_t scan_tmp_ints(int64_t n);
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s);
-void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, const int* bucket_off, int* fill,
-                          int* pairs, int* pair_slot, hipStream_t s);
+void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, int nranks, const int* qn, const int* bucket_off,
+                          int* fill, int* pairs, int* pair_slot, hipStream_t s);
 // queue: 3 x 8 ints (per-XCD length, head, base) + nlist ints (first item of each list rank)
 // vector range [vmin, vmax) of each list (vmin a multiple of the scan's vector tile)
 void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
@@ -104,7 +106,7 @@ void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, f
 // BF16 (q - c) rows of dpad dims for every (query, probe rank) slot, and the norm of each
 // slot's rounding error (+ 2^-23 of its norm) in qerr
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                       int nprobe, int d, int dpad, __bf16* out, float* qerr, hipStream_t s);
+                       int nprobe, int nranks, int d, int dpad, __bf16* out, float* qerr, hipStream_t s);
 // FP8 (e4m3) codes of every list from the rows in list order, scaled per list by a power of two
 // (inverse in inv_scale[l]), in the BF16 codes' order, one byte per value
 void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int64_t* list_off,
@@ -112,7 +114,7 @@ void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int
                             hipStream_t s);
 // FP8 (q - c) rows of every slot, scaled per slot (inverse in inv_scale[slot])
 void launch_pair_resid_fp8(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                           int nprobe, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s);
+                           int nprobe, int nranks, int d, int dpad, uint8_t* out, float* inv_scale, hipStream_t s);
 // the MFMA filter scan: elem 0 = BF16 codes (qaux = per-slot rounding-error norms, rerr =
 // per-vector ones, lscale unused), 1 = FP8 codes (qaux = per-slot inverse scales, lscale =
 // per-list ones, rerr and the kappas unused)
@@ -129,8 +131,10 @@ void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, i
 // counts after the direct-slot scan of the nearest lists (vectors v < cap)
 void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
                          unsigned int* cnt, hipStream_t s);
+// last (optional, nq counts; start at 0xffffffff): the count each query's previous compaction
+// left; a query whose count still equals it is skipped
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
-                         float* bound, hipStream_t s);
+                         float* bound, unsigned int* last, hipStream_t s);
 // re-rank + final top-k of the MFMA forms, in three launches:
 //  launch_rerank_items: work items of 64 stored pairs (items: nq * ceil(cap/64) entries, n_items
 //    zeroed by the caller); overflowed queries listed when overflow_list is given; stats[2]

@@ -49,43 +49,48 @@ __device__ __forceinline__ float blocked_l2_col(const float* qs, const float* __
 // rank the list first come first, so a query tile holds queries at similar distance from the
 // list and prunes (or survives) together.
 // qn (optional): per-query probe count; ranks at or past it are not searched
-__global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int nprobe, const int* __restrict__ qn,
-                              int* __restrict__ cnt) {
+// one thread per (query, rank < nranks); probes and slots are [nq][nprobe]
+__global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int nprobe, int nranks,
+                              const int* __restrict__ qn, int* __restrict__ cnt) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const int j = (int)(e % nprobe);
-    if (qn && j >= qn[e / nprobe]) return;
-    atomicAdd(&cnt[(int64_t)probes[e] * nprobe + j], 1);
+    const int64_t q = e / nranks;
+    const int j = (int)(e - q * nranks);
+    if (qn && j >= qn[q]) return;
+    atomicAdd(&cnt[(int64_t)probes[q * nprobe + j] * nprobe + j], 1);
 }
 
-void launch_count_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, int* bucket_cnt, hipStream_t s) {
-    const int64_t tot = nq * nprobe;
+void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int nranks, const int* qn, int* bucket_cnt,
+                        hipStream_t s) {
+    const int64_t tot = nq * nranks;
     if (tot <= 0) return;
-    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe, qn,
-                       bucket_cnt);
+    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe, nranks,
+                       qn, bucket_cnt);
 }
 
 // pairs[pos] = query, pair_slot[pos] = query * nprobe + rank (the probe slot, for the coarse
 // distance of the pair)
-__global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, int nprobe, const int* __restrict__ qn,
-                                const int* __restrict__ off, int* __restrict__ fill, int* __restrict__ pairs,
-                                int* __restrict__ pair_slot) {
+__global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, int nprobe, int nranks,
+                                const int* __restrict__ qn, const int* __restrict__ off, int* __restrict__ fill,
+                                int* __restrict__ pairs, int* __restrict__ pair_slot) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const int j = (int)(e % nprobe);
-    if (qn && j >= qn[e / nprobe]) return;
-    const int64_t b = (int64_t)probes[e] * nprobe + j;
+    const int64_t q = e / nranks;
+    const int j = (int)(e - q * nranks);
+    if (qn && j >= qn[q]) return;
+    const int64_t slot = q * nprobe + j;
+    const int64_t b = (int64_t)probes[slot] * nprobe + j;
     const int pos = off[b] + atomicAdd(&fill[b], 1);
-    pairs[pos] = (int)(e / nprobe);
-    pair_slot[pos] = (int)e;
+    pairs[pos] = (int)q;
+    pair_slot[pos] = (int)slot;
 }
 
-void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, const int* qn, const int* bucket_off, int* fill,
-                          int* pairs, int* pair_slot, hipStream_t s) {
-    const int64_t tot = nq * nprobe;
+void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, int nranks, const int* qn, const int* bucket_off,
+                          int* fill, int* pairs, int* pair_slot, hipStream_t s) {
+    const int64_t tot = nq * nranks;
     if (tot <= 0) return;
-    hipLaunchKernelGGL(k_scatter_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe, qn,
-                       bucket_off, fill, pairs, pair_slot);
+    hipLaunchKernelGGL(k_scatter_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, probes, tot, nprobe,
+                       nranks, qn, bucket_off, fill, pairs, pair_slot);
 }
 
 // Exclusive scan of n ints (three passes: tile scan, scan of tile sums, add).
