@@ -634,7 +634,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     float* pdist = h->s_pdist.as<float>();
     coarse_select(h, n, Qp, nprobe, probes, pdist, true);
     const int* qn = nullptr;
-    float* seed = nullptr;
     if (learned) {
         HV_HIP(h->s_feat.ensure(sizeof(float) * n * hv::kMlpIn));
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
@@ -644,10 +643,11 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_mlp_serve(h->s_feat.as<float>(), n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
                              h->m_params.as<float>(), h->m_norm.as<float>(), h->sp.learned_nprobe_scale,
                              h->sp.learned_bound_scale, std::max(1, h->sp.learned_min_nprobe), (int)nprobe_cap,
-                             h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(), s);
+                             h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(),
+                             learned_seed ? h->s_bound.as<float>() : nullptr,
+                             learned_seed ? h->s_bound_ref.as<float>() : nullptr, s);
         HV_CHECK_LAUNCH("predictor");
         qn = h->s_qn.as<int>();
-        seed = h->s_seed.as<float>();
     } else if (uneven) {
         stage_qn();
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
@@ -751,14 +751,57 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_qres.ensure(n * nprobe * h->dpad));
         HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
         hv::launch_pair_resid_fp8(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)rank_end, (int)d,
-                                  h->dpad,
-                                  h->s_qres.as<uint8_t>(), h->s_qscale.as<float>(), s);
+                                  h->dpad, h->s_qres.as<uint8_t>(), h->s_qscale.as<float>(), s);
         HV_CHECK_LAUNCH("pair residuals");
     }
+    // MFMA forms: every scan phase (ranks, vector range), in the order the phases run below, is
+    // planned here in one pair of launches, each phase into its own queue and items
+    std::vector<hv::PlanSpec> mph;
+    size_t mph_next = 0;
+    if (mfma) {
+        auto addp = [&](int64_t r0, int64_t r1, int vmin, int vmax, unsigned long long* st) {
+            mph.push_back(hv::PlanSpec{(int)r0, (int)r1, vmin, vmax, nullptr, nullptr, st});
+        };
+        if (learned_seed) {
+            for (int64_t r0 = 0; r0 < rank_end;) {
+                const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
+                addp(r0, r1, 0, INT32_MAX, r0 == 0 ? stats : stats + 8);
+                r0 = r1;
+            }
+        } else {
+            const int vdirect = (int)(cap / 16 * 16);
+            if (vdirect > 0) addp(0, 1, 0, vdirect, stats);
+            if (h->max_list > vdirect) addp(0, 1, vdirect, INT32_MAX, stats);
+            if (!fp8 && nprobe <= kBf16SinglePhase) {
+                if (nprobe > 1) addp(1, nprobe, 0, INT32_MAX, stats + 8);
+            } else {
+                const int64_t end = fp8 ? rank_end : nprobe;
+                for (int64_t r0 = 1; r0 < end;) {
+                    const int64_t r1 = std::min<int64_t>(end, r0 * 4);
+                    addp(r0, r1, 0, INT32_MAX, stats + 8);
+                    r0 = r1;
+                }
+            }
+        }
+        require((int)mph.size() <= hv::kMaxPlanPhases, HV_ERR_CAPACITY, "too many scan phases");
+        const int64_t qlen = 3 * hv::kNumXcd + nlist;
+        HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap * std::max<size_t>(1, mph.size())));
+        HV_HIP(h->s_queue.ensure(sizeof(int) * qlen * std::max<size_t>(1, mph.size())));
+        for (size_t i = 0; i < mph.size(); ++i) {
+            mph[i].queue = h->s_queue.as<int>() + i * qlen;
+            mph[i].items = h->s_items.as<hv::ScanItem>() + i * qcap;
+        }
+        hv::launch_plan_phases(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, 64, vsplit_mfma,
+                               h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap, mph.data(),
+                               (int)mph.size(), s);
+        HV_CHECK_LAUNCH("plan");
+    }
     auto mfma_phase = [&](int r0, int r1, int vmin, int vmax, int mode, unsigned long long* st_slot) {
-        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, vmin, vmax, 64, vsplit_mfma,
-                              h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap,
-                              h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
+        // the next planned phase must be this one
+        require(mph_next < mph.size() && mph[mph_next].r0 == r0 && mph[mph_next].r1 == r1 &&
+                    mph[mph_next].vmin == vmin && mph[mph_next].vmax == vmax && mph[mph_next].stats == st_slot,
+                HV_ERR_STATE, "scan phase out of plan");
+        const hv::PlanSpec& ph = mph[mph_next++];
         const bool main_phase = st_slot == stats + 8;
         if (main_phase && !main_recorded) rec(6);  // events 6-7 span the main phases
         main_recorded |= main_phase;
@@ -768,8 +811,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
                              h->d_rnorm.as<float>(), fp8 ? nullptr : h->d_rerr.as<float>(),
                              h->d_list_off.as<int64_t>(), h->s_bound.as<float>(), h->s_pair_slot.as<int>(),
-                             h->s_items.as<hv::ScanItem>(), queue, vmax, vsplit_mfma, mode, kappa1, kappa_g, kappa2,
-                             surv, aux, surv_cnt, (int)cap, st_slot, s);
+                             ph.items, ph.queue, vmax, vsplit_mfma, mode, kappa1, kappa_g, kappa2, surv, aux,
+                             surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("mfma scan");
         if (main_phase) rec(7);
     };
@@ -780,8 +823,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // learned (FP32 and BF16): the seed from the start, no warm-up. The FP8 form starts without a
     // bound and filters on estimates only.
     if (learned_seed) {
-        HV_HIP(hipMemcpyAsync(bound_ref, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-        HV_HIP(hipMemcpyAsync(bound, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        // the predictor wrote the seed into both
     } else if (external) {
         HV_HIP(hipMemcpyAsync(bound_ref, h->ext_bounds.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));

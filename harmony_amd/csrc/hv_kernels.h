@@ -69,6 +69,18 @@ void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, int nranks,
 void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
                        int vsplit, const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
                        int* queue, unsigned long long* stats, hipStream_t s);
+// several phases planned at once (one workgroup each): ranks [r0, r1), vectors [vmin, vmax), and
+// the phase's own queue, items (qcap entries) and counters
+constexpr int kMaxPlanPhases = 12;
+struct PlanSpec {
+    int r0, r1, vmin, vmax;
+    int* queue;
+    ScanItem* items;
+    unsigned long long* stats;
+};
+void launch_plan_phases(const int* bucket_off, int nlist, int nprobe, int qtile, int vsplit,
+                        const int* list_rank_order, const int64_t* list_off, int64_t qcap, const PlanSpec* phases,
+                        int nph, hipStream_t s);
 int scan_query_tile(int shape);   // queries per work item of a tile shape
 int scan_vector_tile(int shape);  // vectors per tile of a tile shape
 int scan_grid_size(int device, int shape, int d);
@@ -177,9 +189,10 @@ void launch_adam(float* P, const float* G, float* M, float* V, float lr, float b
 void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s);
 void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
                         const float* norm, float* out, hipStream_t s);
+// bound2, bound3 (optional): more copies of the seeded bound
 void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
                       const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
-                      float* bound, float* pred, hipStream_t s);
+                      float* bound, float* pred, float* bound2, float* bound3, hipStream_t s);
 
 // --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
 void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);

@@ -231,13 +231,15 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 }
 
 // Serving: per query, the probe count qn = clamp(ceil(exp(y1) * nprobe_scale), min_nprobe,
-// nprobe) and the seeded bound expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0).
+// nprobe) and the seeded bound expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also
+// into bound2 and bound3 when given (the search's working and reference bounds start there).
 __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ feat, int64_t n,
                                                    const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
                                                    const float* __restrict__ P, const float* __restrict__ norm,
                                                    float nprobe_scale, float bound_scale, int min_nprobe, int nprobe,
                                                    int* __restrict__ qn, float* __restrict__ bound,
-                                                   float* __restrict__ pred) {
+                                                   float* __restrict__ pred, float* __restrict__ bound2,
+                                                   float* __restrict__ bound3) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= n) return;
     const int lane = threadIdx.x & 63;
@@ -247,7 +249,11 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ fea
         const float np = expf(y1) * nprobe_scale;
         const int c = np >= (float)nprobe ? nprobe : max(min_nprobe, (int)ceilf(np));
         qn[q] = min(max(c, 1), nprobe);
-        bound[q] = bound_scale > 0.f ? fminf(expm1f(fmaxf(y0, 0.f)) * bound_scale, 3.402823466e38f) : 3.402823466e38f;
+        const float b =
+            bound_scale > 0.f ? fminf(expm1f(fmaxf(y0, 0.f)) * bound_scale, 3.402823466e38f) : 3.402823466e38f;
+        bound[q] = b;
+        if (bound2) bound2[q] = b;
+        if (bound3) bound3[q] = b;
         pred[q * 2] = y0;
         pred[q * 2 + 1] = y1;
     }
@@ -255,10 +261,10 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ fea
 
 void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
                       const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
-                      float* bound, float* pred, hipStream_t s) {
+                      float* bound, float* pred, float* bound2, float* bound3, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, feat, n, w1, w2, P, norm,
-                       nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred);
+                       nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred, bound2, bound3);
 }
 
 }  // namespace hv

@@ -173,12 +173,25 @@ void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStre
 // while every XCD gets a similar mix of list sizes. The queues lie back to back in `items`.
 // queue[0..8) = lengths, [8..16) = heads, [16..24) = bases, [24 + r] = first item of the list
 // of rank r. k_plan_count (one workgroup of 1024 threads, thread t owns ranks 8t..8t+7, one
-// per queue) sizes the queues; k_plan_write (one workgroup per list) fills them.
+// per queue) sizes the queues; k_plan_write (one workgroup per list) fills them. Several phases
+// (rank and vector ranges, each with its own queues and items) are planned in one launch of
+// each: blockIdx.y is the phase.
 struct PlanArgs {
     const int* bucket_off;
     const int* rank_order;
     const int64_t* list_off;
     int nlist, nprobe, r0, r1, vmin, vmax, qtile, vsplit;
+};
+struct PlanBatch {
+    const int* bucket_off;
+    const int* rank_order;
+    const int64_t* list_off;
+    int nlist, nprobe, qtile, vsplit, nph;
+    PlanSpec ph[kMaxPlanPhases];
+    __device__ PlanArgs args(int i) const {
+        return PlanArgs{bucket_off, rank_order, list_off, nlist, nprobe, ph[i].r0, ph[i].r1, ph[i].vmin,
+                        ph[i].vmax, qtile, vsplit};
+    }
 };
 
 __device__ __forceinline__ void plan_list(const PlanArgs& a, int r, int& l, int& gb, int& ge, int& nlv, int& nqt,
@@ -191,8 +204,10 @@ __device__ __forceinline__ void plan_list(const PlanArgs& a, int r, int& l, int&
     nvr = (nlv + a.vsplit - 1) / a.vsplit;
 }
 
-__global__ __launch_bounds__(1024) void k_plan_count(PlanArgs a, int* __restrict__ queue,
-                                                     unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(1024) void k_plan_count(PlanBatch pb) {
+    const PlanArgs a = pb.args(blockIdx.y);
+    int* __restrict__ queue = pb.ph[blockIdx.y].queue;
+    unsigned long long* __restrict__ stats = pb.ph[blockIdx.y].stats;
     __shared__ int s_scan[kNumXcd][1024];
     __shared__ int s_base[kNumXcd];
     const int t = threadIdx.x;
@@ -241,8 +256,10 @@ __global__ __launch_bounds__(1024) void k_plan_count(PlanArgs a, int* __restrict
     if ((t & 63) == 0 && cand) atomicAdd(&stats[0], cand);
 }
 
-__global__ __launch_bounds__(64) void k_plan_write(PlanArgs a, const int* __restrict__ queue, int64_t qcap,
-                                                   ScanItem* __restrict__ items) {
+__global__ __launch_bounds__(64) void k_plan_write(PlanBatch pb, int64_t qcap) {
+    const PlanArgs a = pb.args(blockIdx.y);
+    const int* __restrict__ queue = pb.ph[blockIdx.y].queue;
+    ScanItem* __restrict__ items = pb.ph[blockIdx.y].items;
     const int r = blockIdx.x;
     int l, gb, ge, nlv, nqt, nvr;
     plan_list(a, r, l, gb, ge, nlv, nqt, nvr);
@@ -259,12 +276,21 @@ __global__ __launch_bounds__(64) void k_plan_write(PlanArgs a, const int* __rest
     }
 }
 
+void launch_plan_phases(const int* bucket_off, int nlist, int nprobe, int qtile, int vsplit,
+                        const int* list_rank_order, const int64_t* list_off, int64_t qcap, const PlanSpec* phases,
+                        int nph, hipStream_t s) {
+    if (nph <= 0) return;
+    PlanBatch pb{bucket_off, list_rank_order, list_off, nlist, nprobe, qtile, vsplit, nph, {}};
+    for (int i = 0; i < nph; ++i) pb.ph[i] = phases[i];
+    hipLaunchKernelGGL(k_plan_count, dim3(1, (unsigned)nph), dim3(1024), 0, s, pb);
+    hipLaunchKernelGGL(k_plan_write, dim3((unsigned)nlist, (unsigned)nph), dim3(64), 0, s, pb, qcap);
+}
+
 void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
                        int vsplit, const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
                        int* queue, unsigned long long* stats, hipStream_t s) {
-    PlanArgs a{bucket_off, list_rank_order, list_off, nlist, nprobe, r0, r1, vmin, vmax, qtile, vsplit};
-    hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1024), 0, s, a, queue, stats);
-    hipLaunchKernelGGL(k_plan_write, dim3((unsigned)nlist), dim3(64), 0, s, a, queue, qcap, items);
+    const PlanSpec p{r0, r1, vmin, vmax, queue, items, stats};
+    launch_plan_phases(bucket_off, nlist, nprobe, qtile, vsplit, list_rank_order, list_off, qcap, &p, 1, s);
 }
 
 // ---------------------------------------------------------------------------------------
