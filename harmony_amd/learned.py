@@ -92,14 +92,18 @@ def recall_per_query(I, exact):
 
 
 def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.95, margin=0.01,
-              bound_quantile=0.999, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0)):
-    """Serving scales from held-out queries: the bound scale lets bound_quantile of the queries
-    keep their seeded bound (the rest fall back), the nprobe scale is the smallest of `scales`
-    whose mean recall against the exact ids reaches target_recall + margin. Leaves the index in
-    learned mode with those scales; returns (nprobe_scale, bound_scale, recall, mean probes)."""
+              bound_quantile=1.0, bound_margin=1.25, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0)):
+    """Serving scales from held-out queries: the bound scale is bound_margin times the
+    bound_quantile of true / predicted k-th distance, so that nearly every query keeps its seeded
+    bound (a query whose seed is below its k-th distance falls back to the exact path, ~50 us for
+    a handful of queries, while a looser seed costs little: the compaction after each scan phase
+    lowers the bound to the k-th upper bound anyway, so only the first phase's appends grow). The
+    nprobe scale is the smallest of `scales` whose mean recall against the exact ids reaches
+    target_recall + margin. Leaves the index in learned mode with those scales; returns
+    (nprobe_scale, bound_scale, recall, mean probes)."""
     pred = index.mlp_predict(feats)
     ratio = np.expm1(targets[:, 0].astype(np.float64)) / np.maximum(np.expm1(pred[:, 0].astype(np.float64)), 1e-30)
-    bound_scale = float(max(1.0, np.quantile(ratio, bound_quantile)))
+    bound_scale = float(max(1.0, bound_margin * np.quantile(ratio, bound_quantile)))
     index.nprobe = nprobe
     best = None
     for s in scales:
