@@ -603,6 +603,12 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
     if (mfma) HV_HIP(h->s_last.ensure(sizeof(unsigned int) * n));
+    // the exact redo's run scratch: batches of queries, each with nprobe_cap * nseg runs of k keys,
+    // then one count per query of the batch
+    const int nseg = hv::exact_segments(h->max_list);
+    const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
+    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
+    HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
     {
         hv::FillSegs f{};
         auto add = [&](void* p, int64_t words, int v) {
@@ -618,6 +624,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         add(h->s_overflow.p, 2, 0);  // redo count, re-rank item count
         if (coarse_on_mfma(h, nprobe)) add(coarse_counter(h, n), 1, 0);
         if (mfma) add(h->s_last.p, n, -1);  // no compaction yet: counts never match
+        add(h->s_runs.as<unsigned long long>() + batch * run_keys, batch, 0);  // first redo batch's counts
         if (bound_fmax) {
             int fm;
             const float fmax = FLT_MAX;
@@ -932,10 +939,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
                          redo_cnt, stats + 12, s);
     const float* redo_bound = fp8 || learned ? nullptr : bound;
-    const int nseg = hv::exact_segments(h->max_list);
-    const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
-    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
-    HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
     post.batch = batch;
     post.main_recorded = main_recorded;
     post.mfma = mfma;
@@ -950,7 +953,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         hv::launch_query_exact(redo_list, redo_cnt, q0, batch, Qp, (int)d, h->d_codes.as<float>(),
                                h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(),
                                h->d_ids.as<uint32_t>(), probes, (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune,
-                               redo_bound, (int)k, runs, D, I, s);
+                               redo_bound, (int)k, runs, D, I, q0 == 0, s);  // the first batch's counts: cleared above
     };
     post.redo(0);
     HV_CHECK_LAUNCH("exact redo");

@@ -47,7 +47,7 @@ void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 // up to kFillSegs fills of 32-bit words (pointer, word count, value) in one launch
-constexpr int kFillSegs = 10;
+constexpr int kFillSegs = 12;
 struct FillSegs {
     int* p[kFillSegs];
     int64_t n[kFillSegs];
@@ -97,14 +97,14 @@ void launch_select_survivors(const unsigned long long* surv, const unsigned int*
 // device) over their probe ranks [0, nranks) (and below qn[q] when given; probe rows are nprobe
 // wide), lists cut into exact_segments(max list length) segments: per-(query, rank, segment)
 // top-k runs appended per query (runs: nql_max * nranks * nseg * k keys, then nql_max ints of
-// counts), merged into D/I. bound may be null (none).
+// counts, which the call clears unless counts_zeroed), merged into D/I. bound may be null (none).
 constexpr int kExactGrid = 2048;
 int exact_segments(int64_t max_list);
 void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_max, const float* Qp, int d,
                         const float* codes, const int64_t* code_off, const int64_t* list_off, const int* list_stride,
                         const uint32_t* ids, const int* probes, int nprobe, int nranks, const int* qn, int nseg,
                         const ChunkPlan& plan, int prune, const float* bound, int k, unsigned long long* runs, float* D,
-                        int64_t* I, hipStream_t s);
+                        int64_t* I, bool counts_zeroed, hipStream_t s);
 
 // --- BF16 precision form (hv_bf16.hip)
 int bf16_ksteps(int d);  // 32-dim MFMA steps per vector (dpad = 32 * ksteps)

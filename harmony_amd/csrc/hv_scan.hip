@@ -894,12 +894,12 @@ void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_ma
                         const float* codes, const int64_t* code_off, const int64_t* list_off, const int* list_stride,
                         const uint32_t* ids, const int* probes, int nprobe, int nranks, const int* qn, int nseg,
                         const ChunkPlan& plan, int prune, const float* bound, int k, unsigned long long* runs, float* D,
-                        int64_t* I, hipStream_t s) {
+                        int64_t* I, bool counts_zeroed, hipStream_t s) {
     if (nql_max <= 0) return;
     const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
     const int64_t pairs = (int64_t)nql_max * nranks * nseg;
     int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * nranks * nseg * k);
-    launch_fill_i32(run_cnt, nql_max, 0, s);
+    if (!counts_zeroed) launch_fill_i32(run_cnt, nql_max, 0, s);
     hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, kExactGrid)), dim3(256), lds, s, qlist,
                        nql_dev, q0, nql_max, Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nranks,
                        qn, nseg, plan, prune, bound, k, runs, run_cnt);

@@ -196,21 +196,24 @@ __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* _
         });
 }
 
+// workgroups stride over the queries (most are skipped: the wavefront select did them)
 __global__ __launch_bounds__(256) void k_select_block(const unsigned long long* __restrict__ surv,
-                                                      const unsigned int* __restrict__ surv_cnt, int cap, int k,
-                                                      float* __restrict__ D, int64_t* __restrict__ I,
+                                                      const unsigned int* __restrict__ surv_cnt, int64_t nq, int cap,
+                                                      int k, float* __restrict__ D, int64_t* __restrict__ I,
                                                       const unsigned int* __restrict__ done) {
     extern __shared__ unsigned long long s_keys[];
-    const int64_t q = blockIdx.x;
-    const unsigned int n = surv_cnt[q];
-    if (n > (unsigned int)cap) return;  // overflow: the exact path
-    if (done ? done[q] != 0u : max((int)n, k) <= kWaveSortMax) return;  // the wavefront select did it
-    const int P = hv_next_pow2(max((int)n, k));
-    for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < (int)n ? surv[q * cap + i] : HV_KEY_PAD;
-    block_bitonic_sort(s_keys, P);
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
-        D[q * k + i] = hv_key_dist(s_keys[i]);
-        I[q * k + i] = hv_key_id(s_keys[i]);
+    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const unsigned int n = surv_cnt[q];
+        if (n > (unsigned int)cap) continue;  // overflow: the exact path
+        if (done ? done[q] != 0u : max((int)n, k) <= kWaveSortMax) continue;  // the wavefront select did it
+        const int P = hv_next_pow2(max((int)n, k));
+        for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < (int)n ? surv[q * cap + i] : HV_KEY_PAD;
+        block_bitonic_sort(s_keys, P);
+        for (int i = threadIdx.x; i < k; i += blockDim.x) {
+            D[q * k + i] = hv_key_dist(s_keys[i]);
+            I[q * k + i] = hv_key_id(s_keys[i]);
+        }
+        __syncthreads();  // s_keys is refilled for the next query
     }
 }
 
@@ -249,7 +252,8 @@ void launch_select_block(const unsigned long long* surv, const unsigned int* sur
                          float* D, int64_t* I, const unsigned int* done, hipStream_t s) {
     if (nq <= 0) return;
     const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
-    hipLaunchKernelGGL(k_select_block, dim3((unsigned)nq), dim3(256), lds, s, surv, surv_cnt, cap, k, D, I, done);
+    const dim3 grid((unsigned)std::min<int64_t>(nq, 1024));
+    hipLaunchKernelGGL(k_select_block, grid, dim3(256), lds, s, surv, surv_cnt, nq, cap, k, D, I, done);
 }
 
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
@@ -260,8 +264,8 @@ void launch_select_survivors(const unsigned long long* surv, const unsigned int*
                        I, overflow_list, overflow_cnt, stats);
     if (cap > kWaveSortMax || k > kWaveSortMax) {
         const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
-        hipLaunchKernelGGL(k_select_block, dim3((unsigned)nq), dim3(256), lds, s, surv, surv_cnt, cap, k, D, I,
-                           nullptr);
+        const dim3 grid((unsigned)std::min<int64_t>(nq, 1024));
+        hipLaunchKernelGGL(k_select_block, grid, dim3(256), lds, s, surv, surv_cnt, nq, cap, k, D, I, nullptr);
     }
 }
 
