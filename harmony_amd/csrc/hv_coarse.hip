@@ -45,7 +45,9 @@ void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t 
 
 // 64 queries x 64 centroids per workgroup; wavefront w computes queries 16w..16w+15 against the
 // 64 centroids (four 16 x 16 MFMA tiles). Lane l holds A[l%16][l/16], B[l/16][l%16] and
-// D[4(l/16)+i][l%16].
+// D[4(l/16)+i][l%16]. The tiles pass through LDS 32 dims at a time; with d % 4 == 0 each thread
+// loads 2 + 2 float4 per chunk and the next chunk's loads are in flight while the current one is
+// multiplied.
 __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q, int64_t nq,
                                                      const float* __restrict__ C, int nc, int d,
                                                      const float* __restrict__ qn, const float* __restrict__ cn,
@@ -59,19 +61,7 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
     floatx4 acc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < d; k0 += KC) {
-        const int kn = min(KC, d - k0);
-        for (int e = t; e < 64 * KC; e += 256) {
-            const int r = e / KC, kk = e % KC;
-            float v = 0.f, u = 0.f;
-            if (kk < kn) {
-                if (q0 + r < nq) v = Q[(q0 + r) * d + k0 + kk];
-                if (c0 + r < nc) u = C[(int64_t)(c0 + r) * d + k0 + kk];
-            }
-            Qs[r * LS + kk] = v;
-            Cs[r * LS + kk] = u;
-        }
-        __syncthreads();
+    auto mult = [&]() {
 #pragma unroll
         for (int s = 0; s < KC / 4; ++s) {
             const float a = Qs[(16 * w + (l & 15)) * LS + 4 * s + (l >> 4)];
@@ -81,7 +71,60 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
                 acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
             }
         }
-        __syncthreads();
+    };
+    if ((d & 3) == 0) {
+        // float4 e = t + 256 i (i = 0, 1) of a chunk: row e / 8, dims k0 + 4 (e % 8) .. +3
+        float4 qv[2], cv[2];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int e = t + 256 * i, r = e >> 3, k = k0 + 4 * (e & 7);
+                qv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                cv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (k < d) {
+                    if (q0 + r < nq) qv[i] = *reinterpret_cast<const float4*>(Q + (q0 + r) * d + k);
+                    if (c0 + r < nc) cv[i] = *reinterpret_cast<const float4*>(C + (int64_t)(c0 + r) * d + k);
+                }
+            }
+        };
+        load(0);
+        for (int k0 = 0; k0 < d; k0 += KC) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int e = t + 256 * i, r = e >> 3, kk = 4 * (e & 7);
+                float* qs = Qs + r * LS + kk;
+                float* cs = Cs + r * LS + kk;
+                qs[0] = qv[i].x;
+                qs[1] = qv[i].y;
+                qs[2] = qv[i].z;
+                qs[3] = qv[i].w;
+                cs[0] = cv[i].x;
+                cs[1] = cv[i].y;
+                cs[2] = cv[i].z;
+                cs[3] = cv[i].w;
+            }
+            __syncthreads();
+            if (k0 + KC < d) load(k0 + KC);  // next chunk in flight during the MFMAs
+            mult();
+            __syncthreads();
+        }
+    } else {
+        for (int k0 = 0; k0 < d; k0 += KC) {
+            const int kn = min(KC, d - k0);
+            for (int e = t; e < 64 * KC; e += 256) {
+                const int r = e / KC, kk = e % KC;
+                float v = 0.f, u = 0.f;
+                if (kk < kn) {
+                    if (q0 + r < nq) v = Q[(q0 + r) * d + k0 + kk];
+                    if (c0 + r < nc) u = C[(int64_t)(c0 + r) * d + k0 + kk];
+                }
+                Qs[r * LS + kk] = v;
+                Cs[r * LS + kk] = u;
+            }
+            __syncthreads();
+            mult();
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
