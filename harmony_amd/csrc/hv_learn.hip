@@ -192,6 +192,9 @@ __device__ __forceinline__ void mlp_eval_bf16(const float* __restrict__ f, int l
     const float* tmean = norm + 2 * kIn;
     const float* tstd = tmean + kOut;
     float h0 = P[kOffB1 + lane], h1 = P[kOffB1 + lane + 64];
+    // unrolled so that the loads of 13 inputs are in flight together (the sums keep their order)
+    static_assert(kIn % 13 == 0, "unroll factor");
+#pragma unroll 13
     for (int i = 0; i < kIn; ++i) {
         const float x = (f[i] - fmean[i]) / fstd[i];
         h0 = __builtin_fmaf(x, (float)w1[i * kHid + lane], h0);

@@ -116,6 +116,28 @@ def w1_model(w1):
     return feats, targets, exact
 
 
+def test_serving_outputs_follow_predictions(w1, w1_model):
+    """The search's predictor (weights staged in LDS, several queries per wavefront) gives each
+    query qn = clamp(ceil(exp(y1) * s_np), min, nprobe) and seed = expm1(max(y0, 0)) * s_b from
+    the same network outputs as hv_mlp_predict on the query's features."""
+    xb, xq, g = w1
+    g.nprobe = 10
+    s_np, s_b = 1.3, 1.1
+    g.preSearch(learned=True, learned_nprobe_scale=s_np, learned_bound_scale=s_b, learned_min_nprobe=2,
+                precision=HV_PRECISION_BF16, dim_blocks=4)
+    g.search(xq, 10)
+    qn = g.last_nprobe(xq.shape[0])
+    seed = g.last_bounds(xq.shape[0])
+    g.preSearch(learned=False, precision=HV_PRECISION_FP32, learned_min_nprobe=1)
+    y = g.mlp_predict(g.query_features(xq)).astype(np.float64)
+    np_f = np.exp(y[:, 1]) * s_np
+    ref_qn = np.where(np_f >= 10, 10, np.maximum(2, np.ceil(np_f))).clip(1, 10).astype(np.int32)
+    away = np.abs(np_f - np.round(np_f)) > 1e-4  # not at an integer (last-bit exp differences)
+    np.testing.assert_array_equal(qn[away], ref_qn[away])
+    ref_seed = np.expm1(np.maximum(y[:, 0], 0.0)) * s_b
+    np.testing.assert_allclose(seed, ref_seed.astype(np.float32), rtol=2e-6)
+
+
 @pytest.mark.parametrize("precision", [HV_PRECISION_FP32, HV_PRECISION_BF16])
 def test_learned_without_trimming_equals_plain(w1, w1_model, precision):
     """Every query keeps all nprobe lists; the seeded bound and the fallback leave the results
