@@ -590,7 +590,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     // counters, bucket counts and (when no warm-up, seed or caller's bound applies) the bounds
     // start cleared: one fill launch for all of them
-    const int64_t nb = nlist * nprobe;
+    const int64_t nb = nlist * rank_end;  // (list, rank) buckets of the searched ranks
     const bool learned_seed = learned && !fp8;  // the seed is the starting bound (FP32 and BF16)
     const bool bound_fmax = !learned_seed && !external && !(wl > 0 && !mfma && !learned);
     HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
@@ -693,7 +693,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // Tile shape: the caller's, or by the expected number of queries per list in the phase
     // (few queries per list -> narrow query tiles, long vector tiles).
     auto pick_shape = [&](int r0, int r1) {
-        if (h->sp.tile_shape > 0 && r1 == (int)nprobe && r1 - r0 > 1) return h->sp.tile_shape - 1;
+        if (h->sp.tile_shape > 0 && r1 == (int)rank_end && r1 - r0 > 1) return h->sp.tile_shape - 1;
         if (h->sp.tile_shape_first > 0 && r0 == 0 && r1 == 1) return h->sp.tile_shape_first - 1;
         const double g = (double)n * (r1 - r0) / (double)nlist;
         return g <= 16 ? 5 : (g <= 40 ? 2 : 0);
@@ -704,7 +704,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (h->scan_grid[shape] == 0) h->scan_grid[shape] = hv::scan_grid_size(h->device, shape, (int)d);
         const int vt = hv::scan_vector_tile(shape);
         const int vsplit = (kVecSplit + vt - 1) / vt * vt;
-        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, r0, r1, 0, vmax,
+        hv::launch_plan_items(h->s_bucket_off.as<int>(), (int)nlist, (int)rank_end, r0, r1, 0, vmax,
                               hv::scan_query_tile(shape), vsplit, h->d_rank_order.as<int>(),
                               h->d_list_off.as<int64_t>(), qcap, h->s_items.as<hv::ScanItem>(), queue, st_slot, s);
         const bool main_phase = st_slot == stats + 8;
@@ -798,7 +798,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             mph[i].queue = h->s_queue.as<int>() + i * qlen;
             mph[i].items = h->s_items.as<hv::ScanItem>() + i * qcap;
         }
-        hv::launch_plan_phases(h->s_bucket_off.as<int>(), (int)nlist, (int)nprobe, 64, vsplit_mfma,
+        hv::launch_plan_phases(h->s_bucket_off.as<int>(), (int)nlist, (int)rank_end, 64, vsplit_mfma,
                                h->d_rank_order.as<int>(), h->d_list_off.as<int64_t>(), qcap, mph.data(),
                                (int)mph.size(), s);
         HV_CHECK_LAUNCH("plan");
@@ -896,13 +896,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         }
     } else {
         int r_next = 0;
-        if ((h->sp.refine_bound || learned) && nprobe > 1) {
+        if ((h->sp.refine_bound || learned) && rank_end > 1) {
             scan_phase(0, 1, INT32_MAX, prune, 0, stats);
             hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
             HV_CHECK_LAUNCH("compact");
             r_next = 1;
         }
-        scan_phase(r_next, (int)nprobe, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
+        scan_phase(r_next, (int)rank_end, INT32_MAX, prune, 0, r_next == 1 ? stats + 8 : stats);
     }
     rec(4);
     // 6. top-k selection. The BF16 form re-ranks its survivors (the plain path's FP32 arithmetic)
@@ -916,7 +916,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_done.ensure(sizeof(unsigned int) * n));
         // queries in nearest-list order (their rank-0 pairs), so the re-rank reads shared rows once
         hv::launch_rerank_items(surv_cnt, n, (int)cap, h->s_pairs.as<int>(), h->s_pair_slot.as<int>(),
-                                h->s_bucket_off.as<int>() + nb, n * nprobe, (int)nprobe, h->s_ritems.as<int2>(),
+                                h->s_bucket_off.as<int>() + nb, n * rank_end, (int)nprobe, h->s_ritems.as<int2>(),
                                 n_items, nullptr, nullptr, stats, s);
         rec(8);  // events 8-9 bracket the re-rank kernel alone
         hv::launch_rerank(Qp, (int)d, h->d_rows.as<float>(), h->d_ids.as<uint32_t>(), plan, surv, n, (int)cap,

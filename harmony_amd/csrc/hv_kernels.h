@@ -56,7 +56,7 @@ struct FillSegs {
 };
 void launch_fill_multi(const FillSegs& f, hipStream_t s);
 // qn (optional): per-query probe counts (ranks at or past qn[q] are skipped); nranks (<= nprobe):
-// the ranks looked at (every qn[q] <= nranks)
+// the ranks looked at (every qn[q] <= nranks) and the bucket stride: buckets [nlist][nranks]
 void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int nranks, const int* qn, int* bucket_cnt,
                         hipStream_t s);
 // exclusive scan of n ints into out (n+1 entries, out[n] = total); tmp >= scan_tmp_ints(n)
@@ -66,7 +66,8 @@ void launch_scatter_pairs(const int* probes, int64_t nq, int nprobe, int nranks,
                           int* fill, int* pairs, int* pair_slot, hipStream_t s);
 // queue: 3 x 8 ints (per-XCD length, head, base) + nlist ints (first item of each list rank)
 // vector range [vmin, vmax) of each list (vmin a multiple of the scan's vector tile)
-void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
+// nbr: ranks per list in the buckets (bucket of (list l, rank j) = l * nbr + j; r1 <= nbr)
+void launch_plan_items(const int* bucket_off, int nlist, int nbr, int r0, int r1, int vmin, int vmax, int qtile,
                        int vsplit, const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
                        int* queue, unsigned long long* stats, hipStream_t s);
 // several phases planned at once (one workgroup each): ranks [r0, r1), vectors [vmin, vmax), and
@@ -78,7 +79,7 @@ struct PlanSpec {
     ScanItem* items;
     unsigned long long* stats;
 };
-void launch_plan_phases(const int* bucket_off, int nlist, int nprobe, int qtile, int vsplit,
+void launch_plan_phases(const int* bucket_off, int nlist, int nbr, int qtile, int vsplit,
                         const int* list_rank_order, const int64_t* list_off, int64_t qcap, const PlanSpec* phases,
                         int nph, hipStream_t s);
 int scan_query_tile(int shape);   // queries per work item of a tile shape

@@ -49,7 +49,8 @@ __device__ __forceinline__ float blocked_l2_col(const float* qs, const float* __
 // rank the list first come first, so a query tile holds queries at similar distance from the
 // list and prunes (or survives) together.
 // qn (optional): per-query probe count; ranks at or past it are not searched
-// one thread per (query, rank < nranks); probes and slots are [nq][nprobe]
+// one thread per (query, rank < nranks); probes and slots are [nq][nprobe], buckets
+// [nlist][nranks] (bucket of (list l, rank j) = l * nranks + j)
 __global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int nprobe, int nranks,
                               const int* __restrict__ qn, int* __restrict__ cnt) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,7 +58,7 @@ __global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int
     const int64_t q = e / nranks;
     const int j = (int)(e - q * nranks);
     if (qn && j >= qn[q]) return;
-    atomicAdd(&cnt[(int64_t)probes[q * nprobe + j] * nprobe + j], 1);
+    atomicAdd(&cnt[(int64_t)probes[q * nprobe + j] * nranks + j], 1);
 }
 
 void launch_count_pairs(const int* probes, int64_t nq, int nprobe, int nranks, const int* qn, int* bucket_cnt,
@@ -79,7 +80,7 @@ __global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, i
     const int j = (int)(e - q * nranks);
     if (qn && j >= qn[q]) return;
     const int64_t slot = q * nprobe + j;
-    const int64_t b = (int64_t)probes[slot] * nprobe + j;
+    const int64_t b = (int64_t)probes[slot] * nranks + j;
     const int pos = off[b] + atomicAdd(&fill[b], 1);
     pairs[pos] = (int)q;
     pair_slot[pos] = (int)slot;
