@@ -17,7 +17,7 @@ autotuner picks on the same batch; none of these change results):
   learned-bf16  the bf16 form with each query's probe count trimmed and its bound seeded by the
                 learned predictor (65 -> 128 -> 2 MLP trained on the GPU on traces of 200,000
                 training queries; serving scales calibrated on 10,000 separate validation
-                queries; nprobe capped at the operating point); recall judged on the 10,000
+                queries; nprobe capped at the operating point); recall measured on the 10,000
                 evaluation queries against the FP64 ground truth
 `value` is the learned-bf16 QPS (the headline form); every form's QPS, recall@100 and distance
 error against the FP64 ground truth (over the returned ids the ground truth also holds) are

@@ -196,7 +196,7 @@ def test_learned_gain_on_1m(tmp_path):
     vf, vt, vexact = learned.collect_traces(g, xq, k=100)
     losses = learned.train(g, feats, targets, epochs=20, batch=1024, lr=2e-3, seed=0)
     say("w2 predictor loss %.4f -> %.4f" % (losses[0], losses[-1]))
-    # calibrate on the first half of the held-out set, judge on the second half
+    # calibrate on the first half of the held-out set, evaluate on the second half
     h = xq.shape[0] // 2
     s, bs, rec_cal, probes_cal = learned.calibrate(g, vf[:h], vt[:h], xq[:h], vexact[:h], nprobe=16, k=100)
     say("w2 calibration: nprobe scale %.2f, bound scale %.3f, recall %.4f, probes %.2f" % (s, bs, rec_cal,
