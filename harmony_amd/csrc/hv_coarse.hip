@@ -47,14 +47,18 @@ void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t 
 // 64 centroids (four 16 x 16 MFMA tiles). Lane l holds A[l%16][l/16], B[l/16][l%16] and
 // D[4(l/16)+i][l%16]. The tiles pass through LDS 32 dims at a time; with d % 4 == 0 each thread
 // loads 2 + 2 float4 per chunk and the next chunk's loads are in flight while the current one is
-// multiplied.
+// multiplied, and the squared norms of the 64 + 64 rows are summed from the same loads (8 threads
+// per row, any order: kappa covers it): the workgroups of the first column write the query norms,
+// those of the first row the centroid norms, for the select. With d % 4 != 0 the norms come from
+// k_row_norms, launched before.
 __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q, int64_t nq,
                                                      const float* __restrict__ C, int nc, int d,
-                                                     const float* __restrict__ qn, const float* __restrict__ cn,
+                                                     float* __restrict__ qn, float* __restrict__ cn,
                                                      float* __restrict__ out) {
     constexpr int KC = 32, LS = KC + 1;  // odd row stride: the 16 rows a K-step reads hit 16 banks
     __shared__ float Qs[64 * LS];
     __shared__ float Cs[64 * LS];
+    __shared__ float s_qn[64], s_cn[64];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int64_t q0 = (int64_t)blockIdx.y * 64;
     const int c0 = blockIdx.x * 64;
@@ -75,6 +79,7 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
     if ((d & 3) == 0) {
         // float4 e = t + 256 i (i = 0, 1) of a chunk: row e / 8, dims k0 + 4 (e % 8) .. +3
         float4 qv[2], cv[2];
+        float sq[2] = {0.f, 0.f}, sc[2] = {0.f, 0.f};
         auto load = [&](int k0) {
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
@@ -94,6 +99,8 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
                 const int e = t + 256 * i, r = e >> 3, kk = 4 * (e & 7);
                 float* qs = Qs + r * LS + kk;
                 float* cs = Cs + r * LS + kk;
+                sq[i] += qv[i].x * qv[i].x + qv[i].y * qv[i].y + qv[i].z * qv[i].z + qv[i].w * qv[i].w;
+                sc[i] += cv[i].x * cv[i].x + cv[i].y * cv[i].y + cv[i].z * cv[i].z + cv[i].w * cv[i].w;
                 qs[0] = qv[i].x;
                 qs[1] = qv[i].y;
                 qs[2] = qv[i].z;
@@ -107,6 +114,22 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
             if (k0 + KC < d) load(k0 + KC);  // next chunk in flight during the MFMAs
             mult();
             __syncthreads();
+        }
+        // row r = (t >> 3) + 32 i is held by the 8 consecutive threads 8 (t >> 3) .. +7
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1) {
+                sq[i] += __shfl_xor(sq[i], o, 64);
+                sc[i] += __shfl_xor(sc[i], o, 64);
+            }
+            const int r = (t >> 3) + 32 * i;
+            if ((t & 7) == 0) {
+                s_qn[r] = sq[i];
+                s_cn[r] = sc[i];
+                if (blockIdx.x == 0 && q0 + r < nq) qn[q0 + r] = sq[i];
+                if (blockIdx.y == 0 && c0 + r < nc) cn[c0 + r] = sc[i];
+            }
         }
     } else {
         for (int k0 = 0; k0 < d; k0 += KC) {
@@ -125,14 +148,19 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
             mult();
             __syncthreads();
         }
+        if (t < 64) {
+            s_qn[t] = q0 + t < nq ? qn[q0 + t] : 0.f;
+            s_cn[t] = c0 + t < nc ? cn[c0 + t] : 0.f;
+        }
     }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int c = c0 + 16 * j + (l & 15);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int64_t q = q0 + 16 * w + 4 * (l >> 4) + i;
-            if (q < nq && c < nc) out[q * nc + c] = qn[q] + cn[c] - 2.f * acc[j][i];
+            if (q < nq && c < nc) out[q * nc + c] = s_qn[q - q0] + s_cn[16 * j + (l & 15)] - 2.f * acc[j][i];
         }
     }
 }
@@ -447,9 +475,13 @@ float coarse_kappa(int d) {
     return (float)(1.02 * (12.0 * d + 8.0) * u);
 }
 
-void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, const float* qn,
-                        const float* cn, float* approx, hipStream_t s) {
+void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, float* qn, float* cn,
+                        float* approx, hipStream_t s) {
     if (nq <= 0) return;
+    if ((d & 3) != 0) {  // the kernel sums the norms itself only from its 16-byte loads
+        launch_row_norms(C, nc, d, cn, s);
+        launch_row_norms(Q, nq, d, qn, s);
+    }
     const dim3 grid((unsigned)((nc + 63) / 64), (unsigned)((nq + 63) / 64));
     hipLaunchKernelGGL(k_coarse_mfma, grid, dim3(256), 0, s, Q, nq, C, nc, d, qn, cn, approx);
 }

@@ -395,8 +395,6 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
         HV_HIP(h->s_qnorm.ensure(sizeof(float) * n));
         HV_HIP(h->s_cnorm.ensure(sizeof(float) * nlist));
         int* redo_cnt = coarse_counter(h, n);
-        hv::launch_row_norms(h->d_centroids.as<float>(), nlist, (int)d, h->s_cnorm.as<float>(), s);
-        hv::launch_row_norms(Qp, n, (int)d, h->s_qnorm.as<float>(), s);
         const float kappa = hv::coarse_kappa((int)d);
         for (int64_t r0 = 0; r0 < n; r0 += per) {
             const int64_t rows = std::min(per, n - r0);

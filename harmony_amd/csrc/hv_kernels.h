@@ -198,8 +198,9 @@ void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf
 // --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
 void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);
 float coarse_kappa(int d);
-void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, const float* qn,
-                        const float* cn, float* approx, hipStream_t s);
+// also writes the squared norms of the rows (qn [nq], cn [nc]) that the select uses
+void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, float* qn, float* cn,
+                        float* approx, hipStream_t s);
 // P nearest per query, exact (distance, id) order; redo_cnt must be 0 on entry
 void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
