@@ -640,12 +640,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     coarse_select(h, n, Qp, nprobe, probes, pdist, true);
     const int* qn = nullptr;
     if (learned) {
-        HV_HIP(h->s_feat.ensure(sizeof(float) * n * hv::kMlpIn));
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
         HV_HIP(h->s_seed.ensure(sizeof(float) * n));
         HV_HIP(h->s_pred.ensure(sizeof(float) * n * 2));
-        hv::launch_features(pdist, (int)nprobe, Qp, (int)d, n, h->s_feat.as<float>(), s);
-        hv::launch_mlp_serve(h->s_feat.as<float>(), n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
+        hv::launch_mlp_serve(pdist, (int)nprobe, Qp, (int)d, n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
                              h->m_params.as<float>(), h->m_norm.as<float>(), h->sp.learned_nprobe_scale,
                              h->sp.learned_bound_scale, std::max(1, h->sp.learned_min_nprobe), (int)nprobe_cap,
                              h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(),

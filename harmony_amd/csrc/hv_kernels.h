@@ -190,10 +190,12 @@ void launch_adam(float* P, const float* G, float* M, float* V, float lr, float b
 void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s);
 void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
                         const float* norm, float* out, hipStream_t s);
-// bound2, bound3 (optional): more copies of the seeded bound
-void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
-                      const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
-                      float* bound, float* pred, float* bound2, float* bound3, hipStream_t s);
+// features from pdist [n][pc] (ascending coarse distances) and the queries Qp [n][d], as
+// launch_features forms them; bound2, bound3 (optional): more copies of the seeded bound
+void launch_mlp_serve(const float* pdist, int pc, const float* Qp, int d, int64_t n, const __bf16* w1,
+                      const __bf16* w2, const float* P, const float* norm, float nprobe_scale, float bound_scale,
+                      int min_nprobe, int nprobe, int* qn, float* bound, float* pred, float* bound2, float* bound3,
+                      hipStream_t s);
 
 // --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
 void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);

@@ -236,18 +236,32 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 // Serving: per query, the probe count qn = clamp(ceil(exp(y1) * nprobe_scale), min_nprobe,
 // nprobe) and the seeded bound expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also
 // into bound2 and bound3 when given (the search's working and reference bounds start there).
-__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ feat, int64_t n,
+// The features are formed here, in LDS, from the coarse distances (pc per query, ascending) and
+// the query, with k_features' arithmetic (so the same bits).
+__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdist, int pc,
+                                                   const float* __restrict__ Qp, int d, int64_t n,
                                                    const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
                                                    const float* __restrict__ P, const float* __restrict__ norm,
                                                    float nprobe_scale, float bound_scale, int min_nprobe, int nprobe,
                                                    int* __restrict__ qn, float* __restrict__ bound,
                                                    float* __restrict__ pred, float* __restrict__ bound2,
                                                    float* __restrict__ bound3) {
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ float s_f[4][kIn];
+    const int w = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= n) return;
     const int lane = threadIdx.x & 63;
+    float s = 0.f;
+    for (int i = lane; i < d; i += 64) {
+        const float v = Qp[q * d + i];
+        s = __builtin_fmaf(v, v, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    s_f[w][lane] = log1pf(fmaxf(pdist[q * pc + min(lane, pc - 1)], 0.f));
+    if (lane == 0) s_f[w][kNearest] = log1pf(s);
+    __threadfence_block();  // every lane reads the whole row its wavefront wrote
     float y0, y1;
-    mlp_eval_bf16(feat + q * kIn, lane, w1, w2, P, norm, y0, y1);
+    mlp_eval_bf16(s_f[w], lane, w1, w2, P, norm, y0, y1);
     if (lane == 0) {
         const float np = expf(y1) * nprobe_scale;
         const int c = np >= (float)nprobe ? nprobe : max(min_nprobe, (int)ceilf(np));
@@ -262,12 +276,13 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ fea
     }
 }
 
-void launch_mlp_serve(const float* feat, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
-                      const float* norm, float nprobe_scale, float bound_scale, int min_nprobe, int nprobe, int* qn,
-                      float* bound, float* pred, float* bound2, float* bound3, hipStream_t s) {
+void launch_mlp_serve(const float* pdist, int pc, const float* Qp, int d, int64_t n, const __bf16* w1,
+                      const __bf16* w2, const float* P, const float* norm, float nprobe_scale, float bound_scale,
+                      int min_nprobe, int nprobe, int* qn, float* bound, float* pred, float* bound2, float* bound3,
+                      hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, feat, n, w1, w2, P, norm,
-                       nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred, bound2, bound3);
+    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, pdist, pc, Qp, d, n, w1, w2, P,
+                       norm, nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred, bound2, bound3);
 }
 
 }  // namespace hv
