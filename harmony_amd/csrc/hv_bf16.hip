@@ -648,8 +648,8 @@ int bf16_grid_size(int device) {
 // path redoes it. With `last` given, a query whose count is what the previous compaction left
 // (no pair appended since) is skipped: its buffer and bound would come out unchanged.
 constexpr int kCompactPer = kMaxSelectRow / 256;
-constexpr int kCompactWaveMax = 2048;  // one wavefront per query up to here,
-constexpr int kCompactWaveBig = 4096;  // then with 64 values per lane; larger: the workgroup kernel
+constexpr int kCompactWaveMax = 2048;  // one wavefront per query up to here with <= 32 values per lane,
+constexpr int kCompactWaveBig = 4096;  // then with 64 per lane (same kernel); larger: the workgroup kernel
 
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
                                                       float* __restrict__ aux,
@@ -663,7 +663,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
-    if (n <= kCompactWaveBig || (last && last[q] == c)) return;  // k_bf16_compact_wave(_big); unchanged
+    if (n <= kCompactWaveBig || (last && last[q] == c)) return;  // k_bf16_compact_wave; unchanged
     float b = bound[q];
     unsigned long long* __restrict__ buf = surv + q * cap;
     const unsigned int* __restrict__ ubits = reinterpret_cast<const unsigned int*>(aux + q * cap);
@@ -856,7 +856,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
     if (q >= nq) return;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
-    if (n > kCompactWaveMax) return;  // k_bf16_compact_wave_big, k_bf16_compact
+    if (n > kCompactWaveBig) return;  // k_bf16_compact
     if (last && last[q] == c) return;  // unchanged since the previous compaction
     unsigned long long* buf = surv + q * cap;
     unsigned int* ubits = reinterpret_cast<unsigned int*>(aux + q * cap);
@@ -868,27 +868,10 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
         bf16_compact_wave_e<8>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
     else if (n <= 1024)
         bf16_compact_wave_e<16>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
-    else
+    else if (n <= kCompactWaveMax)
         bf16_compact_wave_e<32>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
-}
-
-// buffers of kCompactWaveMax < n <= kCompactWaveBig pairs (64 upper bounds per lane)
-__global__ __launch_bounds__(256) void k_bf16_compact_wave_big(unsigned long long* __restrict__ surv,
-                                                               float* __restrict__ aux,
-                                                               unsigned int* __restrict__ cnt, int64_t nq, int cap,
-                                                               int k, float* __restrict__ bound,
-                                                               unsigned int* __restrict__ last) {
-    __shared__ unsigned int s_tmp[4][64];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t q = (int64_t)blockIdx.x * 4 + w;
-    if (q >= nq) return;
-    const unsigned int c = cnt[q];
-    const int n = (int)min(c, (unsigned int)cap);
-    if (n <= kCompactWaveMax || n > kCompactWaveBig) return;
-    if (last && last[q] == c) return;  // unchanged since the previous compaction
-    bf16_compact_wave_e<kCompactWaveBig / 64>(surv + q * cap, reinterpret_cast<unsigned int*>(aux + q * cap), n,
-                                              c, cap, k, bound[q], cnt + q, bound + q, last ? last + q : nullptr,
-                                              s_tmp[w]);
+    else
+        bf16_compact_wave_e<kCompactWaveBig / 64>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
 }
 
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
@@ -896,8 +879,6 @@ void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
     hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last);
-    if (cap > kCompactWaveMax)
-        hipLaunchKernelGGL(k_bf16_compact_wave_big, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last);
     if (cap > kCompactWaveBig)
         hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound,
                            last);
