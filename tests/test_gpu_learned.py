@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP32, datagen, learned
+from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32, datagen, learned
 from harmony_amd.index import Index
 from oracle import mlp_oracle as M
 from oracle import oracle as O
@@ -178,6 +178,25 @@ def test_learned_per_query_probes_match_oracle(w1, w1_model, precision):
         np.testing.assert_array_equal(I[sel], oI)
         np.testing.assert_array_equal(D[sel], oD)
     g.preSearch(learned=False, precision=HV_PRECISION_FP32)
+
+
+def test_learned_fp8_wide_shortlist_matches_oracle(w1, w1_model):
+    """Learned probe counts with the FP8 form: a shortlist wider than any query's candidates
+    (k * r = 3000) filters nothing, so each query's results are the oracle's at its probe count."""
+    xb, xq, g = w1
+    o = O.IndexData.from_gpu_index(g)
+    g.nprobe = 10
+    g.preSearch(learned=True, learned_nprobe_scale=1.0, learned_bound_scale=1.2, precision=HV_PRECISION_FP8,
+                rerank_r=300, dim_blocks=4)
+    D, I = g.search(xq, 10)
+    qn = g.last_nprobe(xq.shape[0])
+    g.preSearch(learned=False, precision=HV_PRECISION_FP32, rerank_r=4)
+    assert qn.min() >= 1 and qn.max() <= 10 and qn.mean() < 10
+    for p in np.unique(qn):
+        sel = np.nonzero(qn == p)[0]
+        oD, oI, _, _ = O.search(o, xq[sel], 10, nprobe=int(p), dim_blocks=4)
+        np.testing.assert_array_equal(I[sel], oI)
+        np.testing.assert_array_equal(D[sel], oD)
 
 
 def test_learned_gain_on_1m(tmp_path):
