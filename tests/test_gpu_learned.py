@@ -199,6 +199,33 @@ def test_learned_fp8_wide_shortlist_matches_oracle(w1, w1_model):
         np.testing.assert_array_equal(D[sel], oD)
 
 
+@pytest.mark.parametrize("precision", [HV_PRECISION_FP32, HV_PRECISION_BF16])
+@pytest.mark.parametrize("bscale", [1.2, 0.5])
+def test_learned_graph_replays_match_eager(w1, w1_model, precision, bscale):
+    """A learned search captured into one HIP graph (second call) and replayed (later calls, with
+    host work between) returns the eager results and probe counts; with the lower bound scale
+    some queries fall back, so the exact redo runs inside the graph too."""
+    xb, xq, g = w1
+    n = xq.shape[0]
+    g.nprobe = 10
+    kw = dict(learned=True, learned_nprobe_scale=1.0, learned_bound_scale=bscale, precision=precision,
+              dim_blocks=4)
+    g.preSearch(graph=False, **kw)
+    D0, I0, st0 = g.search(xq, 10, return_stats=True)
+    qn0 = g.last_nprobe(n)
+    g.preSearch(graph=True, **kw)
+    for rep in range(4):
+        D, I, st = g.search(xq, 10, return_stats=True)
+        np.testing.assert_array_equal(I, I0)
+        np.testing.assert_array_equal(D, D0)
+        np.testing.assert_array_equal(g.last_nprobe(n), qn0)
+        assert st["fallback_queries"] == st0["fallback_queries"]
+        _ = np.sort(xq[: 100 * (rep + 1)], axis=0)  # the program's own work between replays
+    say("learned graph, precision %d, bound scale %.1f: %d fallbacks" % (precision, bscale,
+                                                                           st0["fallback_queries"]))
+    g.preSearch(learned=False, precision=HV_PRECISION_FP32, graph=False)
+
+
 def test_learned_gain_on_1m(tmp_path):
     """What the trained network gains over the method without it: on held-out queries of the
     1M workload, recall@100 >= 0.95 against the exact neighbours with far fewer candidates than
