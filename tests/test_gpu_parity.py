@@ -144,10 +144,13 @@ def test_graph_replays_match_eager(w1, precision):
     g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False)
 
 
-def test_graph_replay_follows_per_query_nprobe(w1):
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_graph_replay_follows_per_query_nprobe(w1, precision):
+    """Per-query probe counts change between replays of one captured graph (they are copied from
+    pinned memory at replay). FP8: a shortlist wider than any query's candidates, so exact."""
     xb, xq, g, o = w1
     g.nprobe = 10
-    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=True)
+    g.preSearch(pruning=True, dim_blocks=4, precision=precision, graph=True, rerank_r=300)
     rng = np.random.default_rng(5)
     for it in range(4):
         npq = rng.integers(1, 11, size=xq.shape[0]).astype(np.int32)
@@ -159,7 +162,7 @@ def test_graph_replay_follows_per_query_nprobe(w1):
             np.testing.assert_array_equal(I[rows], oI)
             np.testing.assert_array_equal(D[rows], oD)
     g.set_query_nprobe(None)
-    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False)
+    g.preSearch(pruning=True, dim_blocks=4, precision=0, graph=False, rerank_r=4)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
