@@ -105,11 +105,14 @@ def test_overflow_redo_in_several_batches(w1, precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_uneven_nprobe_matches_oracle_per_query(w1, precision):
-    """Per-query probe counts: each query's results are the oracle's at its own nprobe."""
+@pytest.mark.parametrize("nmax", [10, 60])
+def test_uneven_nprobe_matches_oracle_per_query(w1, precision, nmax):
+    """Per-query probe counts: each query's results are the oracle's at its own nprobe. Up to 60
+    probes the BF16 form scans the ranks in groups ([1, 4), [4, 16), [16, 60)) with a
+    compaction after each, which skips the queries that gained nothing."""
     xb, xq, g, o = w1
-    npq = np.random.default_rng(3).integers(1, 11, size=xq.shape[0]).astype(np.int32)
-    g.nprobe = 10
+    npq = np.random.default_rng(3).integers(1, nmax + 1, size=xq.shape[0]).astype(np.int32)
+    g.nprobe = nmax
     g.preSearch(pruning=True, dim_blocks=4, precision=precision)
     g.set_query_nprobe(npq)
     D, I = g.search(xq, 10)
