@@ -1137,9 +1137,10 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
 // emits the first k. Buffers of up to kSelLds pairs are loaded at once; longer ones are read in
 // 64-key steps, and a query whose kept keys do not fit kSelLds is left to the workgroup select
 // (done[q] = 0).
-constexpr int kSelLds = 1024;
+constexpr int kSelLds = 1024;  // LDS keys per wavefront: 1024, or 512 when k <= 256 (more resident)
 
-// sort the n (<= kSelLds) keys of s_keys (padded), emit the first k
+// sort the n (<= SEL) keys of s_keys (padded), emit the first k
+template <int SEL>
 __device__ __forceinline__ void sort_emit_lds(const unsigned long long* s_keys, int n, int k, float* __restrict__ Dq,
                                               int64_t* __restrict__ Iq) {
     const int m = max(n, k);
@@ -1156,17 +1157,17 @@ __device__ __forceinline__ void sort_emit_lds(const unsigned long long* s_keys, 
         wave_sort_emit_e<2>(m, key_of, emit);
     else if (m <= 256)
         wave_sort_emit_e<4>(m, key_of, emit);
-    else if (m <= 512)
+    else if (m <= 512 || SEL <= 512)
         wave_sort_emit_e<8>(m, key_of, emit);
     else
-        wave_sort_emit_e<16>(m, key_of, emit);
+        wave_sort_emit_e<SEL / 64>(m, key_of, emit);
 }
 
-template <int E>
+template <int E, int SEL>
 __device__ __forceinline__ void select_kept_e(const unsigned long long* __restrict__ buf, int c, int k,
                                               unsigned long long* s_keys, float* __restrict__ Dq,
                                               int64_t* __restrict__ Iq) {
-    static_assert(64 * E <= kSelLds, "kept keys must fit the LDS row");
+    static_assert(64 * E <= SEL, "kept keys must fit the LDS row");
     const int lane = threadIdx.x & 63;
     unsigned long long v[E];
 #pragma unroll
@@ -1183,14 +1184,17 @@ __device__ __forceinline__ void select_kept_e(const unsigned long long* __restri
         m += (int)__builtin_popcountll(b);
     }
     __threadfence_block();  // the keys other lanes stored are read below
-    sort_emit_lds(s_keys, m, k, Dq, Iq);
+    sort_emit_lds<SEL>(s_keys, m, k, Dq, Iq);
 }
 
+// SEL (512 or 1024) kept keys per wavefront fit its LDS row; the launcher takes 512 when k <= 256
+// (a quarter of the LDS per workgroup, so more of them resident).
+template <int SEL>
 __global__ __launch_bounds__(256) void k_select_kept(const unsigned long long* __restrict__ kept,
                                                      const unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
                                                      float* __restrict__ D, int64_t* __restrict__ I,
                                                      unsigned int* __restrict__ done) {
-    __shared__ unsigned long long s_keys[4][kSelLds];
+    __shared__ unsigned long long s_keys[4][SEL];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -1201,17 +1205,24 @@ __global__ __launch_bounds__(256) void k_select_kept(const unsigned long long* _
     const unsigned long long* __restrict__ buf = kept + q * cap;
     float* Dq = D + q * k;
     int64_t* Iq = I + q * k;
+    bool direct = true;
     if (c <= 64)
-        select_kept_e<1>(buf, c, k, s_keys[w], Dq, Iq);
+        select_kept_e<1, SEL>(buf, c, k, s_keys[w], Dq, Iq);
     else if (c <= 128)
-        select_kept_e<2>(buf, c, k, s_keys[w], Dq, Iq);
+        select_kept_e<2, SEL>(buf, c, k, s_keys[w], Dq, Iq);
     else if (c <= 256)
-        select_kept_e<4>(buf, c, k, s_keys[w], Dq, Iq);
+        select_kept_e<4, SEL>(buf, c, k, s_keys[w], Dq, Iq);
     else if (c <= 512)
-        select_kept_e<8>(buf, c, k, s_keys[w], Dq, Iq);
-    else if (c <= 1024)
-        select_kept_e<16>(buf, c, k, s_keys[w], Dq, Iq);
-    else {
+        select_kept_e<8, SEL>(buf, c, k, s_keys[w], Dq, Iq);
+    else
+        direct = false;
+    if constexpr (SEL >= 1024) {
+        if (!direct && c <= 1024) {
+            select_kept_e<16, SEL>(buf, c, k, s_keys[w], Dq, Iq);
+            direct = true;
+        }
+    }
+    if (!direct) {
         int m = 0;
         for (int j0 = 0; j0 < c; j0 += 64) {
             const int j = j0 + lane;
@@ -1219,15 +1230,15 @@ __global__ __launch_bounds__(256) void k_select_kept(const unsigned long long* _
             const bool keep = key != HV_KEY_PAD;
             const unsigned long long b = __ballot(keep);
             const int slot = m + (int)__builtin_popcountll(b & ((1ull << lane) - 1ull));
-            if (keep && slot < kSelLds) s_keys[w][slot] = key;
+            if (keep && slot < SEL) s_keys[w][slot] = key;
             m += (int)__builtin_popcountll(b);
         }
-        if (m > kSelLds) {
+        if (m > SEL || k > SEL) {
             if (lane == 0) done[q] = 0u;  // the workgroup select sorts the whole buffer
             return;
         }
         __threadfence_block();
-        sort_emit_lds(s_keys[w], m, k, Dq, Iq);
+        sort_emit_lds<SEL>(s_keys[w], m, k, Dq, Iq);
     }
     if (lane == 0) done[q] = 1u;
 }
@@ -1261,9 +1272,13 @@ void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint3
 void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt, int64_t nq, int cap, int k, float* D,
                         int64_t* I, unsigned int* done, hipStream_t s) {
     if (nq <= 0) return;
-    hipLaunchKernelGGL(k_select_kept, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, kept, cnt, nq, cap, k, D, I,
-                       done);
-    if (cap > kSelLds) launch_select_block(kept, cnt, nq, cap, k, D, I, done, s);
+    const dim3 grid((unsigned)((nq + 3) / 4));
+    const int sel = k <= 256 ? 512 : kSelLds;
+    if (sel == 512)
+        hipLaunchKernelGGL(k_select_kept<512>, grid, dim3(256), 0, s, kept, cnt, nq, cap, k, D, I, done);
+    else
+        hipLaunchKernelGGL(k_select_kept<kSelLds>, grid, dim3(256), 0, s, kept, cnt, nq, cap, k, D, I, done);
+    if (cap > sel || k > sel) launch_select_block(kept, cnt, nq, cap, k, D, I, done, s);
 }
 
 }  // namespace hv
