@@ -18,7 +18,7 @@ import math
 
 import numpy as np
 
-from . import _lib
+from . import _lib, metrics
 from ._lib import HV_PRECISION_BF16
 
 
@@ -84,11 +84,8 @@ def train(index, feats, targets, epochs=30, batch=1024, lr=2e-3, seed=0):
 
 
 def recall_per_query(I, exact):
-    k = exact.shape[1]
-    hit = np.zeros(I.shape[0], np.float64)
-    for row in range(I.shape[0]):
-        hit[row] = np.intersect1d(I[row], exact[row][exact[row] >= 0]).size
-    return hit / k
+    """Each query's hits / k against its exact ids (calculate_recall's counting, utils.h:676-685)."""
+    return metrics.recall_per_query(I, exact)
 
 
 def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.95, margin=0.01,

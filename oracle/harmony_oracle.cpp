@@ -21,7 +21,8 @@
 //
 // Parity pins (tests/test_oracle_*.py): heap vs the reference heap.hpp compiled from source
 // (oracle/_ref), top-k vs full sort, scan vs FP64 brute force at nprobe = nlist, pruning on
-// == off, bound >= true k-th, k-means step vs closed-form mean, assignment vs argmin.
+// == off, bound >= true k-th, k-means step vs closed-form mean, assignment vs argmin, recall
+// and r2 vs hand-computed values (tests/test_metrics.py).
 
 #include <algorithm>
 #include <cfloat>
@@ -451,25 +452,63 @@ void or_kmeans(idx_t n, const float* x, idx_t d, idx_t nlist, idx_t niter, idx_t
 }
 
 // ---------------------------------------------------------------- metrics (utils.h:660-756)
-// recall = |I_q ∩ GT_q| summed over queries / (nq * k); a -1 in I ends its row.
-double or_recall(idx_t nq, idx_t k, const idx_t* I, const idx_t* GT, idx_t gt_k) {
+// calculate_recall (utils.h:660-732). Returns 0 when D[0] < 0 (662-665); gt_k = 0 means k
+// (666-668); k > gt_k is an error (671-673: std::invalid_argument, here *status = -1). For each
+// query the ground-truth set is the first k ids of its GT row (676); the query's ids are
+// counted in order until the first -1 (678-680), each one found in the set is a hit, so a
+// repeated id is counted every time it appears. The distance sanity count (687-729: a result
+// is "correct" when its distance is within the row's last valid GT distance, or within a
+// relative 1e-4 of it; >= for IP) goes to *correct; the reference asserts true/correct > 0.99
+// in Debug builds only (730). Result: (float)hits / (nq * k) in FP32, as 731 computes it.
+// One deviation: a GT row whose first k ids are all -1 makes the reference's index walk
+// (691-694) run below the row (undefined); here it stops at the row's first entry.
+float or_recall(idx_t nq, idx_t k, const idx_t* I, const float* D, const idx_t* GT, const float* GD, idx_t gt_k,
+                int metric_l2, idx_t* correct_out, int* status) {
+    if (status) *status = 0;
+    if (correct_out) *correct_out = 0;
+    if (D && D[0] < 0) return 0.0f;
     if (gt_k == 0) gt_k = k;
-    idx_t hit = 0;
+    if (k > gt_k) {
+        if (status) *status = -1;
+        return 0.0f;
+    }
+    idx_t true_correct = 0, correct = 0;
     for (idx_t i = 0; i < nq; ++i) {
         std::vector<idx_t> g(GT + i * gt_k, GT + i * gt_k + k);
         std::sort(g.begin(), g.end());
         for (idx_t j = 0; j < k; ++j) {
-            idx_t v = I[i * k + j];
+            const idx_t v = I[i * k + j];
             if (v == -1) break;
-            if (std::binary_search(g.begin(), g.end(), v)) hit++;
+            if (std::binary_search(g.begin(), g.end(), v)) true_correct++;
         }
     }
-    return (double)hit / (double)(nq * k);
+    if (D && GD) {
+        for (idx_t i = 0; i < nq; ++i) {
+            idx_t ii = k - 1;
+            while (ii > 0 && GT[i * gt_k + ii] == -1) ii--;
+            const float topK = GD[i * gt_k + ii];
+            for (idx_t j = 0; j < k; ++j) {
+                if (I[i * k + j] == -1) break;
+                const float x = D[i * k + j];
+                const float rel = x == 0.0f ? std::fabs(topK) : std::fabs((x - topK) / x);  // utils.h:622-628
+                const bool within = metric_l2 ? x <= topK : x >= topK;
+                if (within || (double)rel < 1e-4) correct++;
+            }
+        }
+    }
+    if (correct_out) *correct_out = correct;
+    return (float)true_correct / (float)(nq * k);
 }
 
-// r2 = sum(D) / sum(GD) - 1 over the non-padded entries (utils.h:734-756).
-double or_r2(idx_t nq, idx_t k, const idx_t* I, const float* D, const float* GD, idx_t gt_k) {
+// calculate_r2 (utils.h:734-756): FP32 sums of D and of GD over each row's entries before the
+// first -1 in I, r2 = sum / g_sum - 1 in FP32; k > gt_k is an error (*status = -1).
+float or_r2(idx_t nq, idx_t k, const idx_t* I, const float* D, const float* GD, idx_t gt_k, int* status) {
+    if (status) *status = 0;
     if (gt_k == 0) gt_k = k;
+    if (k > gt_k) {
+        if (status) *status = -1;
+        return 0.0f;
+    }
     float gs = 0, s = 0;
     for (idx_t i = 0; i < nq; ++i)
         for (idx_t j = 0; j < k; ++j) {
@@ -477,7 +516,7 @@ double or_r2(idx_t nq, idx_t k, const idx_t* I, const float* D, const float* GD,
             gs += GD[i * gt_k + j];
             s += D[i * k + j];
         }
-    return (double)(s / gs - 1);
+    return s / gs - 1;
 }
 
 }  // extern "C"

@@ -33,8 +33,8 @@ _SIG = {
     "or_kmeans_init": (None, [_I, _P, _I, _I, _I, ctypes.c_uint64, _P, _P, _P]),
     "or_kmeans_step": (None, [_I, _P, _I, _I, _P, _C]),
     "or_kmeans": (None, [_I, _P, _I, _I, _I, _I, ctypes.c_uint64, _C, _P]),
-    "or_recall": (ctypes.c_double, [_I, _I, _P, _P, _I]),
-    "or_r2": (ctypes.c_double, [_I, _I, _P, _P, _P, _I]),
+    "or_recall": (ctypes.c_float, [_I, _I, _P, _P, _P, _P, _I, _C, _P, _P]),
+    "or_r2": (ctypes.c_float, [_I, _I, _P, _P, _P, _I, _P]),
 }
 
 _cache = {}
@@ -204,8 +204,30 @@ def build_index(xb, nlist, niter=10, max_pts=256, seed=0, nthreads=0):
     return IndexData(c, off, order, _f32(xb)[order])
 
 
-def recall(I, GT, k=None):
+def recall(I, GT, D=None, GD=None, metric_l2=True, return_correct=False):
+    """calculate_recall (utils.h:660-732) restated in or_recall: k = I's row length, gt_k = GT's.
+    D / GD may be None (then the D[0] < 0 rule and the distance sanity count are skipped).
+    Raises ValueError when k > gt_k (the reference throws std::invalid_argument)."""
     I, GT = _i64(I), _i64(GT)
-    nq, kk = I.shape
-    k = kk if k is None else k
-    return lib().or_recall(nq, k, _p(I), _p(GT), GT.shape[1])
+    nq, k = I.shape
+    Dp = _f32(D) if D is not None else None
+    GDp = _f32(GD) if GD is not None else None
+    correct = ctypes.c_int64(0)
+    status = ctypes.c_int(0)
+    r = lib().or_recall(nq, k, _p(I), _p(Dp) if Dp is not None else None, _p(GT),
+                        _p(GDp) if GDp is not None else None, GT.shape[1], 1 if metric_l2 else 0,
+                        ctypes.byref(correct), ctypes.byref(status))
+    if status.value != 0:
+        raise ValueError("k should be less than or equal to gt_k.")
+    return (float(r), int(correct.value)) if return_correct else float(r)
+
+
+def r2(I, D, GD):
+    """calculate_r2 (utils.h:734-756) restated in or_r2: k = I's row length, gt_k = GD's."""
+    I, D, GD = _i64(I), _f32(D), _f32(GD)
+    nq, k = I.shape
+    status = ctypes.c_int(0)
+    r = lib().or_r2(nq, k, _p(I), _p(D), _p(GD), GD.shape[1], ctypes.byref(status))
+    if status.value != 0:
+        raise ValueError("k should be less than or equal to gt_k.")
+    return float(r)
