@@ -101,9 +101,10 @@ SIGNATURES = {
     "hv_get_list_offsets": (ctypes.c_int, [_P, _P]),
     "hv_get_list_ids": (ctypes.c_int, [_P, _P]),
     "hv_get_list_codes": (ctypes.c_int, [_P, _P]),
-    "hv_get_last_probes": (ctypes.c_int, [_P, _P]),
-    "hv_get_last_bounds": (ctypes.c_int, [_P, _P]),
-    "hv_get_last_nprobe": (ctypes.c_int, [_P, _P]),
+    "hv_last_n": (_I64, [_P]),
+    "hv_get_last_probes": (ctypes.c_int, [_P, _P, _I64]),
+    "hv_get_last_bounds": (ctypes.c_int, [_P, _P, _I64]),
+    "hv_get_last_nprobe": (ctypes.c_int, [_P, _P, _I64]),
     "hv_last_probe_stride": (_I64, [_P]),
     "hv_query_features": (ctypes.c_int, [_P, _I64, _P, _P]),
     "hv_mlp_set_norm": (ctypes.c_int, [_P, _P]),

@@ -512,7 +512,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     int64_t wls = h->sp.warmup_list_size == 0 ? k : h->sp.warmup_list_size;  // query.cpp:328-333
     if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
         require(wl * wls <= hv::kMaxWarmup, HV_ERR_CAPACITY, "warmup_list * warmup_list_size > 8192");
-    const bool external = !learned && !fp8 && h->sp.bound_mode == HV_BOUND_EXTERNAL;
+    require(h->sp.bound_mode != HV_BOUND_EXTERNAL || (!learned && !fp8), HV_ERR_INVALID_ARG,
+            "HV_BOUND_EXTERNAL cannot be combined with the FP8 form or the learned predictor");
+    const bool external = h->sp.bound_mode == HV_BOUND_EXTERNAL;
     if (external)
         require((int64_t)h->ext_bounds.size() == n, HV_ERR_STATE, "external bounds not set for this batch");
     // FP8 form: the shortlist of the kr smallest estimates per query is re-ranked exactly
@@ -1251,7 +1253,11 @@ int hv_save(const hv_index_t* hc, const char* path) {
     });
 }
 
-// Index::load_index (Index.cpp:1735-1770) + IVF::load_IVF (IVF.cpp:101-134).
+// Index::load_index (Index.cpp:1733-1770) + IVF::load_IVF (IVF.cpp:101-134). The whole file is
+// parsed and checked (sizes against the bytes left in the file, optional arrays included) before
+// the handle is touched, so a bad or truncated file leaves the handle as it was. Like the
+// reference, the search opt level restarts at OPT_NONE (Index.cpp:1743) and added_opt_level is the
+// file's.
 int hv_load(hv_index_t* h, const char* path) {
     if (!h) return HV_ERR_INVALID_ARG;
     return guarded(h, [&] {
@@ -1259,6 +1265,15 @@ int hv_load(hv_index_t* h, const char* path) {
         FILE* f = std::fopen(path, "rb");
         require(f != nullptr, HV_ERR_IO, std::string("Cannot open file ") + path);
         FileCloser fc{f};
+        require(std::fseek(f, 0, SEEK_END) == 0, HV_ERR_IO, "cannot size the index file");
+        const long fsize = std::ftell(f);
+        require(fsize >= 0 && std::fseek(f, 0, SEEK_SET) == 0, HV_ERR_IO, "cannot size the index file");
+        // bytes left after the current position must cover `need`
+        auto need = [&](uint64_t bytes, const char* what) {
+            const long pos = std::ftell(f);
+            require(pos >= 0 && (uint64_t)(fsize - pos) >= bytes, HV_ERR_IO,
+                    std::string("truncated index file while reading ") + what);
+        };
         uint64_t hd[2];
         int32_t en[2];
         uint64_t sub[3];
@@ -1267,53 +1282,45 @@ int hv_load(hv_index_t* h, const char* path) {
         read_exact(f, sub, 3, "header");
         const int64_t d = (int64_t)hd[0], nlist = (int64_t)hd[1];
         require(d > 0 && d <= 65536 && nlist > 0 && nlist <= (1 << 24), HV_ERR_IO, "implausible d / nlist in header");
-        require(en[0] == HV_METRIC_L2, HV_ERR_INVALID_ARG, "only METRIC_L2 indexes are supported");
+        require(en[0] == HV_METRIC_L2 || en[0] == HV_METRIC_IP, HV_ERR_IO, "unknown metric in header");
+        require(en[0] == h->metric, HV_ERR_INVALID_ARG, "index file metric differs from the handle's");
+        require(en[1] >= 0 && en[1] <= HV_OPT_ALL, HV_ERR_IO, "unknown opt level in header");
+        need(sizeof(float) * (uint64_t)nlist * d, "centroids");
         std::vector<float> cent(nlist * d);
         read_exact(f, cent.data(), cent.size(), "centroids");
         std::vector<std::vector<uint64_t>> lids(nlist);
         std::vector<std::vector<float>> lcodes(nlist);
+        std::vector<char> seen(nlist, 0);
         int64_t total = 0;
         for (;;) {
             uint64_t lid;
             if (std::fread(&lid, sizeof(lid), 1, f) != 1) {
-                require(std::feof(f), HV_ERR_IO, "read error");
+                require(std::feof(f) && std::ftell(f) == fsize, HV_ERR_IO, "truncated list record");
                 break;
             }
             uint64_t rec[3];
             int32_t opt;
             read_exact(f, rec, 3, "list header");
             read_exact(f, &opt, 1, "list header");
-            const int64_t n = (int64_t)rec[0], ld = (int64_t)rec[1], sk = (int64_t)rec[2];
-            require((int64_t)lid < nlist && ld == d && n >= 0 && n < (1ll << 32), HV_ERR_IO, "bad list record");
-            require(lids[lid].empty(), HV_ERR_IO, "list stored twice");
+            const uint64_t n = rec[0], ld = rec[1], sk = rec[2];
+            require(lid < (uint64_t)nlist && ld == (uint64_t)d && n < (1ull << 32), HV_ERR_IO, "bad list record");
+            require(opt >= 0 && opt <= HV_OPT_ALL && sk < (1ull << 20), HV_ERR_IO, "bad list record");
+            require(!seen[lid], HV_ERR_IO, "list stored twice");
+            seen[lid] = 1;
+            // ids, codes, norms, then the optional arrays of IVF.cpp:117-134
+            uint64_t extra = 0;
+            if ((opt & HV_OPT_TRIANGLE) || (opt & HV_OPT_SUBNN_IP)) extra += 2ull * 4 * n;
+            if (opt & HV_OPT_SUBNN_L2) extra += n * sk * (8 + 4);
+            if (opt & HV_OPT_SUBNN_IP) extra += 2ull * n * sk * (8 + 4);
+            need(n * 8 + n * (uint64_t)d * 4 + n * 4 + extra, "list arrays");
             lids[lid].resize(n);
             lcodes[lid].resize(n * d);
             read_exact(f, lids[lid].data(), n, "ids");
             read_exact(f, lcodes[lid].data(), n * d, "codes");
-            std::vector<float> skip(n);
-            read_exact(f, skip.data(), n, "norms");
-            // optional arrays (IVF.cpp:117-134): skipped, the L2 search does not use them
-            int64_t extra = 0;
-            if ((opt & HV_OPT_TRIANGLE) || (opt & HV_OPT_SUBNN_IP)) extra += 2 * 4 * n;
-            if (opt & HV_OPT_SUBNN_L2) extra += n * sk * (8 + 4);
-            if (opt & HV_OPT_SUBNN_IP) extra += 2 * n * sk * (8 + 4);
-            require(std::fseek(f, (long)extra, SEEK_CUR) == 0, HV_ERR_IO, "truncated optional arrays");
-            total += n;
+            require(std::fseek(f, (long)(n * 4 + extra), SEEK_CUR) == 0, HV_ERR_IO, "seek failed");
+            total += (int64_t)n;
         }
         require(total < 0xffffffffll, HV_ERR_CAPACITY, "ntotal must be < 2^32 - 1");
-        // re-initialise the handle to the file's shape
-        if (d != h->d) {
-            h->perm.clear();
-            h->d_perm.release();
-        }
-        h->d = d;
-        h->nlist = nlist;
-        h->trained = h->added = false;
-        h->added_opt_level = en[1];
-        h->opt_level = en[1];
-        h->last_n = 0;
-        int rc = hv_set_centroids(h, cent.data());
-        if (rc != HV_OK) throw HvError{rc, h->err};
         std::vector<float> rows(total * d);
         std::vector<int64_t> off(nlist + 1, 0), ids(total);
         for (int64_t l = 0; l < nlist; ++l) {
@@ -1324,12 +1331,33 @@ int hv_load(hv_index_t* h, const char* path) {
                 ids[off[l] + t] = (int64_t)lids[l][t];
             }
         }
-        DevBuf raw, perm_buf;
-        const float* xp = total ? upload_rows(h, rows.data(), total, raw, perm_buf) : nullptr;
-        std::vector<int> order(total);
-        std::iota(order.begin(), order.end(), 0);
-        install_lists(h, xp, order, off, ids);
-        h->added_opt_level = en[1];
+        lcodes.clear();
+        // the file is good: re-initialise the handle to its shape. A device failure from here on
+        // leaves the handle untrained (searches refuse with HV_ERR_STATE) rather than half loaded.
+        if (d != h->d) {
+            h->perm.clear();
+            h->d_perm.release();
+        }
+        h->d = d;
+        h->nlist = nlist;
+        h->trained = h->added = false;
+        h->opt_level = HV_OPT_NONE;
+        h->sp.opt_level = HV_OPT_NONE;
+        h->last_n = 0;
+        h->graph.reset();
+        try {
+            int rc = hv_set_centroids(h, cent.data());
+            if (rc != HV_OK) throw HvError{rc, h->err};
+            DevBuf raw, perm_buf;
+            const float* xp = total ? upload_rows(h, rows.data(), total, raw, perm_buf) : nullptr;
+            std::vector<int> order(total);
+            std::iota(order.begin(), order.end(), 0);
+            install_lists(h, xp, order, off, ids);
+            h->added_opt_level = en[1];
+        } catch (...) {
+            h->trained = h->added = false;
+            throw;
+        }
     });
 }
 
@@ -1546,9 +1574,10 @@ int hv_get_list_codes(const hv_index_t* h, float* out) {
     });
 }
 
-int hv_get_last_probes(const hv_index_t* h, int64_t* out) {
+int hv_get_last_probes(const hv_index_t* h, int64_t* out, int64_t capacity) {
     if (!h || !out) return HV_ERR_INVALID_ARG;
     if (h->last_n == 0) return HV_ERR_STATE;
+    if (capacity < h->last_n * h->last_stride) return HV_ERR_INVALID_ARG;
     std::vector<int> p(h->last_n * h->last_stride);
     if (hipMemcpy(p.data(), h->s_probes.p, sizeof(int) * p.size(), hipMemcpyDeviceToHost) != hipSuccess)
         return HV_ERR_DEVICE;
@@ -1556,17 +1585,19 @@ int hv_get_last_probes(const hv_index_t* h, int64_t* out) {
     return HV_OK;
 }
 
-int hv_get_last_bounds(const hv_index_t* h, float* out) {
+int hv_get_last_bounds(const hv_index_t* h, float* out, int64_t capacity) {
     if (!h || !out) return HV_ERR_INVALID_ARG;
     if (h->last_n == 0) return HV_ERR_STATE;
+    if (capacity < h->last_n) return HV_ERR_INVALID_ARG;
     if (hipMemcpy(out, h->s_bound_ref.p, sizeof(float) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
         return HV_ERR_DEVICE;
     return HV_OK;
 }
 
-int hv_get_last_nprobe(const hv_index_t* h, int32_t* out) {
+int hv_get_last_nprobe(const hv_index_t* h, int32_t* out, int64_t capacity) {
     if (!h || !out) return HV_ERR_INVALID_ARG;
     if (h->last_n == 0) return HV_ERR_STATE;
+    if (capacity < h->last_n) return HV_ERR_INVALID_ARG;
     if (h->last_learned) {
         if (hipMemcpy(out, h->s_qn.p, sizeof(int32_t) * h->last_n, hipMemcpyDeviceToHost) != hipSuccess)
             return HV_ERR_DEVICE;
@@ -1575,6 +1606,8 @@ int hv_get_last_nprobe(const hv_index_t* h, int32_t* out) {
     }
     return HV_OK;
 }
+
+int64_t hv_last_n(const hv_index_t* h) { return h ? h->last_n : 0; }
 
 int64_t hv_last_probe_stride(const hv_index_t* h) { return h ? h->last_stride : 0; }
 

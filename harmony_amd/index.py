@@ -217,6 +217,8 @@ class Index:
         L = lib()
         self.d = int(L.hv_dim(self._h))
         self.nlist = int(L.hv_nlist(self._h))
+        # the load resets the search opt level (Index.cpp:1743): keep the mirror in step
+        check(self._h, L.hv_get_search_params(self._h, ctypes.byref(self._params)))
 
     # -- introspection
     @property
@@ -243,17 +245,24 @@ class Index:
         check(self._h, lib().hv_get_list_codes(self._h, _ptr(out)))
         return out
 
-    def last_probes(self, n):
-        """Probe lists of the last search ([n][stride]: nprobe, or >= 64 deep with the predictor)."""
+    def last_n(self):
+        """Queries of the last search."""
+        return int(lib().hv_last_n(self._h))
+
+    def last_probes(self, n=None):
+        """Probe lists of the last search ([n][stride]: nprobe, or >= 64 deep with the predictor).
+        n defaults to the last search's query count; a smaller n is rejected by the library."""
+        n = self.last_n() if n is None else int(n)
         stride = int(lib().hv_last_probe_stride(self._h)) or min(self.nprobe, self.nlist)
         out = np.empty((n, stride), dtype=np.int64)
-        check(self._h, lib().hv_get_last_probes(self._h, _ptr(out)))
+        check(self._h, lib().hv_get_last_probes(self._h, _ptr(out), out.size))
         return out
 
-    def last_nprobe(self, n):
+    def last_nprobe(self, n=None):
         """Per-query probe counts of the last search."""
+        n = self.last_n() if n is None else int(n)
         out = np.empty(n, dtype=np.int32)
-        check(self._h, lib().hv_get_last_nprobe(self._h, _ptr(out)))
+        check(self._h, lib().hv_get_last_nprobe(self._h, _ptr(out), out.size))
         return out
 
     # -- learned probe/bound predictor
@@ -313,7 +322,8 @@ class Index:
         check(self._h, lib().hv_mlp_predict(self._h, x.shape[0], _ptr(x), _ptr(out)))
         return out
 
-    def last_bounds(self, n):
+    def last_bounds(self, n=None):
+        n = self.last_n() if n is None else int(n)
         out = np.empty(n, dtype=np.float32)
-        check(self._h, lib().hv_get_last_bounds(self._h, _ptr(out)))
+        check(self._h, lib().hv_get_last_bounds(self._h, _ptr(out), out.size))
         return out

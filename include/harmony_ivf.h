@@ -46,7 +46,9 @@ extern "C" {
 
 /* bound modes for the pruning bound */
 #define HV_BOUND_WARMUP 0   /* Index::warmUpSearch (Index.cpp:999-1051) */
-#define HV_BOUND_EXTERNAL 1 /* caller-provided per-query bound (--fullWarmUp, query.cpp:716-725) */
+#define HV_BOUND_EXTERNAL 1 /* caller-provided per-query bound (--fullWarmUp, query.cpp:716-725); FP32 and
+                               BF16 forms only: a search with it and HV_PRECISION_FP8 or learned = 1
+                               returns HV_ERR_INVALID_ARG */
 
 /* precision of the candidate distances */
 #define HV_PRECISION_FP32 0 /* FP32 blocked partial sums on every candidate (reference arithmetic) */
@@ -167,8 +169,11 @@ int hv_ground_truth_f64(hv_index_t* h, int64_t n, const float* q, int64_t k, dou
  *   list_size, size_t d, size_t sub_k, int32 opt_level, size_t ids[list_size],
  *   float codes[list_size][d], float norms[list_size] (+ optional arrays per opt_level bits),
  *   read until end of file. hv_save writes per-list opt_level 0 (no optional arrays);
- *   hv_load skips optional arrays, replaces the handle's d / nlist with the file's and keeps
- *   each list's vector order. Codes are stored in caller dimension order. */
+ *   hv_load skips optional arrays, replaces the handle's d / nlist with the file's, keeps each
+ *   list's vector order, sets the search opt level to HV_OPT_NONE (Index.cpp:1743) and the added
+ *   opt level to the file's. The file is parsed and size-checked completely before the handle
+ *   changes: a truncated or malformed file returns HV_ERR_IO and leaves the handle as it was.
+ *   Codes are stored in caller dimension order. */
 int hv_save(const hv_index_t* h, const char* path);
 int hv_load(hv_index_t* h, const char* path);
 
@@ -180,11 +185,17 @@ int hv_get_centroids(const hv_index_t* h, float* out);       /* nlist*d, caller 
 int hv_get_list_offsets(const hv_index_t* h, int64_t* out);  /* nlist+1 */
 int hv_get_list_ids(const hv_index_t* h, int64_t* out);      /* ntotal, list order */
 int hv_get_list_codes(const hv_index_t* h, float* out);      /* ntotal*d, list order, caller dims */
-int hv_get_last_probes(const hv_index_t* h, int64_t* out);   /* n*nprobe of the last search */
-int hv_get_last_bounds(const hv_index_t* h, float* out);     /* n bounds of the last search */
-/* per-query probe counts of the last search (nprobe each, unless the predictor trimmed them),
- * and the probe-list stride of hv_get_last_probes (nprobe, or at least 64 with the predictor) */
-int hv_get_last_nprobe(const hv_index_t* h, int32_t* out);
+/* Introspection of the last search (listidqueries / heapTops, src/Index.h:142-147). Each call
+ * takes the capacity of `out` in elements and returns HV_ERR_INVALID_ARG without writing when
+ * it is below what the last search holds: hv_last_n(h) * hv_last_probe_stride(h) probes,
+ * hv_last_n(h) bounds or counts. HV_ERR_STATE when no search has run since the last build. */
+int64_t hv_last_n(const hv_index_t* h);                                     /* queries of the last search */
+int hv_get_last_probes(const hv_index_t* h, int64_t* out, int64_t capacity); /* [n][stride] list ids */
+int hv_get_last_bounds(const hv_index_t* h, float* out, int64_t capacity);   /* n pruning bounds */
+/* per-query probe counts of the last search (nprobe each, unless the predictor or
+ * hv_set_query_nprobe trimmed them), and the probe-list stride of hv_get_last_probes (nprobe, or
+ * at least 64 with the predictor) */
+int hv_get_last_nprobe(const hv_index_t* h, int32_t* out, int64_t capacity);
 int64_t hv_last_probe_stride(const hv_index_t* h);
 
 /* ---- learned probe/bound predictor (no counterpart in the reference; the reference always

@@ -65,3 +65,12 @@ def test_null_handle_calls_return_invalid_arg():
     assert L.hv_add(None, 1, None) == _lib.HV_ERR_INVALID_ARG
     assert L.hv_search(None, 1, None, 1, None, None, None) == _lib.HV_ERR_INVALID_ARG
     assert L.hv_ntotal(None) == -1
+
+
+def test_last_getters_null_and_no_search():
+    from harmony_amd import _lib
+    L = _lib.lib()
+    assert L.hv_get_last_probes(None, None, 0) == _lib.HV_ERR_INVALID_ARG
+    assert L.hv_get_last_bounds(None, None, 0) == _lib.HV_ERR_INVALID_ARG
+    assert L.hv_get_last_nprobe(None, None, 0) == _lib.HV_ERR_INVALID_ARG
+    assert L.hv_last_n(None) == 0

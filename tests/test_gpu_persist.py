@@ -144,3 +144,62 @@ def test_load_rejects_truncated_file(built, tmp_path):
         h.load_index(t)
     with pytest.raises(HvError):
         h.load_index(tmp_path / "missing.index")
+
+
+def test_load_rejects_truncated_optional_arrays(tmp_path):
+    """A file cut inside the last list's optional arrays (IVF.cpp:117-122) is an I/O error, not a
+    silent seek past the end."""
+    rng = np.random.default_rng(5)
+    cent = rng.integers(0, 100, size=(4, 8)).astype(np.float32)
+    lists = {l: (np.arange(10 * l, 10 * l + 10), rng.integers(0, 100, size=(10, 8)).astype(np.float32))
+             for l in range(4)}
+    p = tmp_path / "opt.index"
+    write_reference_index(p, cent, lists, opt=1)
+    data = open(p, "rb").read()
+    for cut in (4, 40, 79):  # inside sqrt_candidate2centroid / candidate2centroid of list 3
+        t = tmp_path / ("cut%d.index" % cut)
+        open(t, "wb").write(data[: len(data) - cut])
+        h = Index(8, 4)
+        with pytest.raises(HvError, match="truncated"):
+            h.load_index(t)
+
+
+def test_failed_load_leaves_handle_intact(built, tmp_path):
+    xb, xq, g = built
+    p = tmp_path / "w1.index"
+    g.save_index(p)
+    h = Index(128, 100)
+    h.load_index(p)
+    h.nprobe = 10
+    h.preSearch(pruning=True, dim_blocks=4)
+    D0, I0 = h.search(xq, 10)
+    data = open(p, "rb").read()
+    t = tmp_path / "trunc.index"
+    open(t, "wb").write(data[: len(data) // 2])
+    with pytest.raises(HvError):
+        h.load_index(t)
+    assert (h.ntotal, h.nlist) == (10000, 100)
+    D1, I1 = h.search(xq, 10)
+    np.testing.assert_array_equal(I0, I1)
+    np.testing.assert_array_equal(D0, D1)
+
+
+def test_load_resets_search_opt_level(tmp_path):
+    """load_index sets opt_level = OPT_NONE and keeps the file's added_opt_level (Index.cpp:1742-1743):
+    a search at OPT_NONE passes the subset check, hv_save writes the file's level back."""
+    rng = np.random.default_rng(6)
+    cent = rng.integers(0, 100, size=(3, 8)).astype(np.float32)
+    lists = {l: (np.arange(5 * l, 5 * l + 5), rng.integers(0, 100, size=(5, 8)).astype(np.float32))
+             for l in range(3)}
+    p = tmp_path / "opt.index"
+    write_reference_index(p, cent, lists, opt=1)
+    h = Index(8, 3)
+    h.opt_level = 1
+    h.load_index(p)
+    assert h.opt_level == 0
+    h.nprobe = 3
+    h.preSearch(pruning=False)
+    h.search(cent, 2)
+    q = tmp_path / "again.index"
+    h.save_index(q)
+    assert read_reference_index(q)[3] == 1
