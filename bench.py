@@ -1,37 +1,42 @@
 """Benchmark of the IVF-flat search path on one MI355X.
 
-A step is one hv_search_device call over the whole query batch (queries already in HBM):
+A step is one hv_search_device call over the whole query batch (queries and results in HBM):
 coarse quantizer -> bound -> grouped list scan -> per-query top-k. The workload is the
 1,000,000 x 128 integer-mixture index (nlist 1024, k 100, 4 dimension blocks, pruning on) with
-10,000 queries; the operating point is the smallest nprobe of {16, 64, 256} whose recall@100
-against the FP64 ground truth is >= 0.95.
+10,000 queries. Each method runs at its own operating point: the smallest nprobe of the sweep
+{1, 2, 3, 4, 6, 8, 12, 16} (or, for the learned form, the calibrated policy) whose recall@100
+against the FP64 ground truth (calculate_recall's rule) reaches 0.95.
 
-Forms measured at that nprobe (each with the tile shapes, MFMA item length and graph mode its
-autotuner picks on the same batch; none of these change results):
+Forms (each with the tile shapes, MFMA item length and graph mode its autotuner picks on the
+same batch; none of these change results):
   plain-fp32    warm-up bound, FP32 blocked partial sums with pruning on every candidate
   bf16          BF16 MFMA filter with a certified error bound, exact FP32 re-rank of survivors
                 (same ids and distances as plain-fp32)
   fp8           FP8 e4m3 MFMA estimates (power-of-two scales per list and per query slot), the
-                k * r smallest per query re-ranked in FP32; r is the smallest of 1, 2, 4 whose
-                recall@100 reaches 0.95 (approximate: recall below the plain path's is possible)
-  learned-bf16  the bf16 form with each query's probe count trimmed and its bound seeded by the
+                k * r smallest per query re-ranked in FP32; (nprobe, r) the first of the sweep x
+                {1, 2, 4} reaching the target (approximate: recall below the plain path's is possible)
+  learned-bf16  the bf16 form with each query's probe count (at most 16) and bound from the
                 learned predictor (65 -> 128 -> 2 MLP trained on the GPU on traces of 200,000
                 training queries; serving scales calibrated on 10,000 separate validation
-                queries; nprobe capped at the operating point); recall measured on the 10,000
-                evaluation queries against the FP64 ground truth
-`value` is the learned-bf16 QPS (the headline form); every form's QPS, recall@100 and distance
-error against the FP64 ground truth (over the returned ids the ground truth also holds) are
-under config.forms.
+                queries to the target + 0.003); recall measured on the 10,000 evaluation queries
+`value` is the learned-bf16 QPS (the headline form). config.forms holds every form's QPS, recall,
+r2, distance error against FP64, stage and kernel times, executed block fraction and, for plain,
+bf16 and learned-bf16, the host-to-host QPS of hv_search (queries and results in host memory,
+PCIe included). config.learned_gain compares the learned form with the fixed-nprobe BF16 form at
+equal recall for higher targets (--gain-targets).
 
-Prints one JSON line. `roofline` is for the headline form's dominant kernel, the longer of
-  - the scan of the remaining probed lists: algorithmic FLOPs per (query, vector) pair (2d
-    multiply-adds for the BF16 dot product, 3d for the FP32 subtract + multiply-add) times the
-    kernel's unpruned pair count, against the MFMA (BF16) or FP32 peak;
-  - the exact re-rank of the BF16 forms: algorithmic bytes per re-ranked pair (the FP32 row,
-    4d, plus its id, 4, and the key read and written, 16) against the HBM peak;
-each divided by the kernel's duration measured with HIP events on the library's stream.
-`cpu_baseline` is the oracle's restatement of the reference CPU scan (built with the
-reference's Release flags), timed on this host on a bounded query sample.
+Prints one JSON line. `roofline` is for the headline form's dominant kernel, the one with the
+larger summed device time per step:
+  - the candidate scan: FP32 VALU for the plain form (3d FLOPs per unpruned pair against 157.3
+    TFLOP/s); for the MFMA forms whichever bound is larger of HBM (the codes, norms and error
+    norms of every touched list, once per step, plus the query residuals) and MFMA (2d FLOPs per
+    pair);
+  - the exact re-rank of the MFMA forms: bytes per re-ranked pair (the FP32 row, 4d, plus its
+    id, 4, and the key read and written, 16) against the HBM peak;
+each divided by the kernel's summed duration from HIP events on the library's stream.
+`cpu_baseline` is the oracle's restatement of the reference CPU scan (built with the reference's
+Release flags) at the plain operating point, timed on this host: this GPU's share of the cores
+(pruning on and off) and one thread.
 
 `--workload skewed-420d` measures the 1,000,000 x 420 float mixture with per-dimension scales
 exp(N(0,1)) (nlist 2048, nprobe 64, k 100, B = 8 blocks over the dimensions in descending
@@ -85,31 +90,6 @@ def profiled_traffic(kernel):
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
-
-
-def cpu_baseline(index, xq, nprobe, k, dim_blocks, budget_s):
-    """Oracle restatement of the reference per-node scan, reference Release flags, on a
-    bounded, growing sample of the query batch (about budget_s seconds of CPU work)."""
-    from oracle import oracle as O
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
-    data = O.IndexData.from_gpu_index(index)
-    n = 64
-    spent = 0.0
-    qps = None
-    while True:
-        sub = xq[:n]
-        t0 = time.perf_counter()
-        O.search(data, sub, k, nprobe=nprobe, dim_blocks=dim_blocks, pruning=True, nthreads=threads, fast=True)
-        dt = time.perf_counter() - t0
-        spent += dt
-        qps = n / dt
-        if dt > budget_s / 3 or n >= xq.shape[0] or spent > budget_s:
-            break
-        n = min(xq.shape[0], int(n * max(2.0, (budget_s / 3) / max(dt, 1e-3))))
-    return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": "%d of %d queries, nprobe %d, k %d, B=%d pruning on, single process, %d OpenMP threads"
-                      % (n, xq.shape[0], nprobe, k, dim_blocks, threads)}
 
 
 def bench_skewed(args):
@@ -252,22 +232,146 @@ def bench_uneven(args):
     print(json.dumps(out), flush=True)
 
 
+def host_info():
+    """CPU of this host: what lscpu reports, and the cores this process may use."""
+    info = {"nproc": os.cpu_count(),
+            "affinity_cores": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+            if k.strip() == "Flags":
+                info["avx512f"] = " avx512f " in " " + v + " "
+    except Exception as e:  # lscpu missing: the counts above still stand
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_leg(O, data, xq, nprobe, k, dim_blocks, pruning, threads, budget_s):
+    """QPS of the oracle's restatement (reference Release flags) on a growing sample of the batch,
+    then the mean of three timed runs of the final sample (the growing runs are the warm-up)."""
+    n = 64
+    while True:
+        t0 = time.perf_counter()
+        O.search(data, xq[:n], k, nprobe=nprobe, dim_blocks=dim_blocks, pruning=pruning, nthreads=threads, fast=True)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 6 or n >= xq.shape[0]:
+            break
+        n = min(xq.shape[0], int(n * max(2.0, (budget_s / 6) / max(dt, 1e-3))))
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.search(data, xq[:n], k, nprobe=nprobe, dim_blocks=dim_blocks, pruning=pruning, nthreads=threads, fast=True)
+        ts.append(time.perf_counter() - t0)
+    return n / float(np.mean(ts)), n
+
+
+def cpu_baseline(index, xq, nprobe, k, dim_blocks, budget_s):
+    """The oracle's restatement of the reference per-node scan (coarse + warm-up bound + blocked
+    scan + merge + sort, Release flags), single process, timed on this host: with the box's CPU
+    share of threads (pruning on and off) and with one thread."""
+    from oracle import oracle as O
+    info = host_info()
+    threads = max(1, min(info["affinity_cores"] or 1, 16))  # one GPU's share of the host
+    data = O.IndexData.from_gpu_index(index)
+    legs = {}
+    for name, thr, pr in (("threads_pruning_on", threads, True), ("threads_pruning_off", threads, False),
+                          ("1_thread_pruning_on", 1, True)):
+        qps, n = cpu_leg(O, data, xq, nprobe, k, dim_blocks, pr, thr, budget_s / 3)
+        legs[name] = {"qps": round(qps, 1), "threads": thr, "pruning": pr, "sample_queries": n}
+        log("cpu %s: %.1f QPS (%d queries, %d threads)" % (name, qps, n, thr))
+    main = legs["threads_pruning_on"]
+    return {"value": main["qps"], "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": "%d of %d queries at nprobe %d, k %d, B=%d pruning on, single process, %d OpenMP threads "
+                      "(this GPU's share of the host); mean of 3 timed runs after the sizing runs"
+                      % (main["sample_queries"], xq.shape[0], nprobe, k, dim_blocks, threads),
+            "host": info, "legs": legs}
+
+
+def touched(index, sizes):
+    """(touched vectors, searched (query, rank) slots) of the last search: vectors of the lists that
+    at least one query searched, and the probe slots searched (each query's first nprobe_q ranks)."""
+    pr = index.last_probes()
+    qn = index.last_nprobe()
+    mask = np.arange(pr.shape[1])[None, :] < qn[:, None]
+    lists = np.unique(pr[mask])
+    return int(sizes[lists].sum()), int(qn.sum())
+
+
+def roofline_of(name, st, d, dpad, touched_vecs, slots):
+    """Roofline of the kernel that dominates the search's device time: the candidate scan (FP32
+    VALU for the plain form; the MFMA filter, HBM-bound on the codes of the touched lists, for the
+    others) or the exact re-rank (HBM: the FP32 row, id and key of every re-ranked pair)."""
+    fp8 = name.startswith("fp8")
+    plain = name == "plain-fp32"
+    cands = {}
+    t_scan = st["ms_scan_kernels"] / 1000.0
+    if t_scan > 0:
+        if plain:
+            flops = 3.0 * d * st["scan_pairs"]
+            ach = flops / t_scan / 1e12
+            cands["scan"] = {"bound": "mfma", "pipe": "fp32 valu (direct-difference sub + fma; packed FP32 off)",
+                             "kernel": "k_list_scan", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 5),
+                             "algorithmic_flops_per_step": flops, "kernel_ms": round(st["ms_scan_kernels"], 4),
+                             "launches": int(st["scan_launches"]), "pairs": int(st["scan_pairs"])}
+        else:
+            s = 1 if fp8 else 2
+            nbytes = touched_vecs * (s * dpad + (4 if fp8 else 8)) + slots * s * dpad
+            flops = 2.0 * d * st["scan_pairs"]
+            peak_f = 5000.0 if fp8 else PEAK_BF16_TFLOPS
+            t_hbm = nbytes / (PEAK_HBM_GBS * 1e9)
+            t_mfma = flops / (peak_f * 1e12)
+            if t_hbm >= t_mfma:
+                ach = nbytes / t_scan / 1e9
+                cands["scan"] = {"bound": "hbm", "kernel": "k_mfma_scan", "achieved": round(ach, 1),
+                                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 5),
+                                 "algorithmic_bytes_per_step": nbytes, "touched_vectors": touched_vecs,
+                                 "mfma_frac": round(flops / t_scan / 1e12 / peak_f, 6)}
+            else:
+                ach = flops / t_scan / 1e12
+                cands["scan"] = {"bound": "mfma", "kernel": "k_mfma_scan", "achieved": round(ach, 3), "peak": peak_f,
+                                 "unit": "TFLOP/s", "frac": round(ach / peak_f, 5), "algorithmic_flops_per_step": flops}
+            cands["scan"].update({"kernel_ms": round(st["ms_scan_kernels"], 4), "launches": int(st["scan_launches"]),
+                                  "pairs": int(st["scan_pairs"])})
+    if st["ms_rerank"] > 0:
+        rbytes = (4.0 * d + 20.0) * st["rerank_pairs"]
+        gbs = rbytes / (st["ms_rerank"] / 1000.0) / 1e9
+        cands["rerank"] = {"bound": "hbm", "kernel": "k_rerank_flat", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                           "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 5),
+                           "algorithmic_bytes_per_step": rbytes, "kernel_ms": round(st["ms_rerank"], 4),
+                           "launches": 1, "pairs": int(st["rerank_pairs"])}
+    if not cands:
+        return None, {}
+    dom = max(cands, key=lambda c: cands[c]["kernel_ms"])
+    return dict(cands[dom]), cands
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nprobe", type=int, default=0, help="fixed nprobe (0: smallest of the sweep reaching 0.95)")
-    ap.add_argument("--sweep", default="16,64,256")
+    ap.add_argument("--sweep", default="1,2,3,4,6,8,12,16")
+    ap.add_argument("--target", type=float, default=0.95, help="recall@100 of the operating points")
+    ap.add_argument("--learned-cap", type=int, default=16, help="largest per-query nprobe of the learned form")
+    ap.add_argument("--gain-targets", default="0.99",
+                    help="extra recall targets where the learned form is compared with the fixed-nprobe BF16 form")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--nb", type=int, default=1000000)
-    ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--train-queries", type=int, default=200000)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-fp8", action="store_true")
+    ap.add_argument("--no-host-to-host", action="store_true",
+                    help="skip the host-buffer (hv_search) timings (profiler runs time device work only)")
     ap.add_argument("--workload", default="int-mixture-1m", choices=["int-mixture-1m", "skewed-420d", "zipf-96d-10m"])
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this size (a copy kernel of known traffic, "
@@ -285,8 +389,9 @@ def main():
         bench_skewed(args)
         return
 
-    from harmony_amd import datagen, learned, metrics
+    from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32, datagen, learned, metrics
     from harmony_amd._lib import lib
+    from harmony_amd.autotune import autotune
     from harmony_amd.index import DeviceArray, Index
 
     if args.copy_calibration_mb:
@@ -298,6 +403,7 @@ def main():
         del src, dst
 
     k, d, nlist, B = 100, 128, 1024, 4
+    dpad = (d + 31) // 32 * 32
     t0 = time.perf_counter()
     xb, xq = datagen.make(2, nb=args.nb, nq=args.nq)
     log("data %.1fs" % (time.perf_counter() - t0))
@@ -305,71 +411,46 @@ def main():
     index = Index(d, nlist, nprobe=16)
     index.train(xb, niter=10, seed=0, max_points_per_centroid=256)
     index.add(xb)
+    nb = int(xb.shape[0])
+    del xb
+    sizes = np.diff(index.list_offsets())
     log("index build (GPU k-means + add) %.1fs" % (time.perf_counter() - t0))
-    index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B)
     nq = xq.shape[0]
     q_dev = DeviceArray.from_host(xq)
     D_dev = DeviceArray(nq * k * 4)
     I_dev = DeviceArray(nq * k * 8)
 
-    sweep = [int(s) for s in args.sweep.split(",")]
-    recall_at = {}
-    if not args.no_recall:
-        t0 = time.perf_counter()
-        index.nprobe = max(sweep)
-        GD, GI = index.ground_truth_f64(xq, k)
-        log("fp64 ground truth %.1fs" % (time.perf_counter() - t0))
-        for npb in sweep:
-            index.nprobe = npb
-            index.search_device(q_dev, nq, k, D_dev, I_dev)
-            I = I_dev.to_host(np.int64, (nq, k))
-            recall_at[npb] = metrics.recall(I, GI)
-            log("nprobe %d recall@100 %.4f" % (npb, recall_at[npb]))
-    if args.nprobe:
-        nprobe = args.nprobe
-    elif recall_at:
-        ok = [p for p in sweep if recall_at[p] >= 0.95]
-        nprobe = min(ok) if ok else max(sweep)
-    else:
-        nprobe = sweep[0]
-    index.nprobe = nprobe
-
-    from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32
-
-    from harmony_amd.autotune import autotune
-
-    def measure(precision, use_learned=False, rerank_r=4):
+    def params(precision, use_learned=False, rerank_r=4, nprobe=None):
+        if nprobe is not None:
+            index.nprobe = nprobe
         index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B, precision=precision,
                         learned=use_learned, rerank_r=rerank_r, graph=False, tile_shape=0, tile_shape_first=0,
                         scan_vsplit=0)
-        tuned = None
-        if not args.no_autotune:
-            tuned, _ = autotune(index, xq, k, sample=nq, reps=3)
-            log("autotuned: %s" % tuned)
-        for _ in range(args.warmup):
-            index.search_device(q_dev, nq, k, D_dev, I_dev)
-        times, sts = [], []
-        for _ in range(args.steps):
-            t0 = time.perf_counter()
-            st = index.search_device(q_dev, nq, k, D_dev, I_dev)
-            times.append(time.perf_counter() - t0)
-            sts.append(st)
-        ms = 1000.0 * float(np.mean(times))
-        avg = {kk: float(np.mean([st[kk] for st in sts])) for kk in sts[0]}
-        if tuned and tuned.get("graph"):
-            # graph replays report the whole device work as one figure: take the kernel timings
-            # from an eager call with the same tuned shapes
-            index.preSearch(graph=False)
-            est = index.search_device(q_dev, nq, k, D_dev, I_dev)
-            for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan", "ms_select", "ms_main", "ms_rerank"):
-                avg[kk] = est[kk]
-            index.preSearch(graph=True)
-        return ms, avg, I_dev.to_host(np.int64, (nq, k)), D_dev.to_host(np.float32, (nq, k)), tuned
+
+    def run_once():
+        st = index.search_device(q_dev, nq, k, D_dev, I_dev)
+        return I_dev.to_host(np.int64, (nq, k)), D_dev.to_host(np.float32, (nq, k)), st
+
+    # 1. ground truth and the recall of every nprobe of the sweep (plain and BF16 return the same
+    #    ids, checked at the operating point); operating point = smallest nprobe reaching the target
+    sweep = sorted(int(x) for x in args.sweep.split(","))
+    t0 = time.perf_counter()
+    index.nprobe = max(sweep)
+    index.preSearch(pruning=True, dim_blocks=B)
+    GD, GI = index.ground_truth_f64(xq, k)
+    GD32 = GD.astype(np.float32)
+    log("fp64 ground truth %.1fs" % (time.perf_counter() - t0))
+    recall_at = {}
+    for npb in sweep:
+        params(HV_PRECISION_FP32, nprobe=npb)
+        I, D, _ = run_once()
+        recall_at[npb] = metrics.recall(I, GI, D, GD32)
+        log("plain nprobe %d recall@100 %.5f" % (npb, recall_at[npb]))
+    ok = [p for p in sweep if recall_at[p] >= args.target]
+    op_plain = min(ok) if ok else max(sweep)
 
     def dist_error(D, I):
         """max relative |D - D_fp64| over the returned ids that the FP64 ground truth holds"""
-        if args.no_recall:
-            return None
         worst = 0.0
         for i in range(0, nq, 7):
             m = dict(zip(GI[i].tolist(), GD[i].tolist()))
@@ -379,27 +460,97 @@ def main():
                     worst = max(worst, abs(float(dist) - ref) / ref)
         return worst
 
+    def measure(name, precision, use_learned=False, rerank_r=4, nprobe=None, h2h=False):
+        if use_learned:
+            index.preSearch(precision=precision, learned=True, graph=False, tile_shape=0, tile_shape_first=0,
+                            scan_vsplit=0)
+            if nprobe is not None:
+                index.nprobe = nprobe
+        else:
+            params(precision, False, rerank_r, nprobe)
+        tuned = None
+        if not args.no_autotune:
+            tuned, _ = autotune(index, xq, k, sample=nq, reps=3)
+            log("%s autotuned: %s" % (name, tuned))
+        for _ in range(args.warmup):
+            index.search_device(q_dev, nq, k, D_dev, I_dev)
+        times, sts = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            st = index.search_device(q_dev, nq, k, D_dev, I_dev)
+            times.append(time.perf_counter() - t0)
+            sts.append(st)
+        ms = 1000.0 * float(np.mean(times))
+        I, D = I_dev.to_host(np.int64, (nq, k)), D_dev.to_host(np.float32, (nq, k))
+        tv, slots = touched(index, sizes)
+        avg = {kk: float(np.mean([st[kk] for st in sts])) for kk in sts[0]}
+        if tuned and tuned.get("graph"):
+            # graph replays report the device work as one figure: kernel timings from eager calls
+            # with the same tuned shapes
+            index._params.graph = 0  # the tuned tile shapes stay
+            index._push()
+            ests = [index.search_device(q_dev, nq, k, D_dev, I_dev) for _ in range(3)]
+            for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan", "ms_select", "ms_main", "ms_rerank",
+                       "ms_scan_kernels", "scan_launches"):
+                avg[kk] = float(np.mean([e[kk] for e in ests]))
+            index._params.graph = 1
+            index._push()
+        roof, cands = roofline_of(name, avg, d, dpad, tv, slots)
+        rec, correct = metrics.recall(I, GI, D, GD32, return_correct=True)
+        f = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 4), "nprobe": int(index.nprobe),
+             "recall_at_100": rec, "r2": metrics.r2(I, D, GD32), "distance_sanity_correct": correct,
+             "max_rel_dist_error_vs_fp64": dist_error(D, I), "autotuned": tuned,
+             "stage_ms": {kk: round(avg[kk], 4) for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan",
+                                                           "ms_select", "ms_scan_kernels", "ms_rerank")},
+             "candidates": int(avg["candidates"]), "scan_pairs": int(avg["scan_pairs"]),
+             "survivors": int(avg["survivors"]), "overflow_queries": int(avg["overflow_queries"]),
+             "fallback_queries": int(avg["fallback_queries"]), "touched_vectors": tv,
+             "executed_block_fraction": round(avg["block_evals"] / max(1.0, avg["block_evals_max"]), 5),
+             "roofline": roof, "kernels": cands}
+        if use_learned:
+            f["mean_probes"] = round(float(index.last_nprobe().mean()), 4)
+        if precision == HV_PRECISION_FP8:
+            f["rerank_r"] = rerank_r
+        if h2h and not args.no_host_to_host:
+            index.search(xq, k)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                index.search(xq, k)
+            f["host_to_host_qps"] = round(nq / ((time.perf_counter() - t0) / 3), 1)
+        log("%s: nprobe %d, %.3f ms/step, %.0f QPS%s, recall %.5f, dominant %s %.3f ms (%.1f%% of its peak)"
+            % (name, f["nprobe"], ms, f["qps"],
+               (", host-to-host %.0f QPS" % f["host_to_host_qps"]) if "host_to_host_qps" in f else "", rec,
+               roof["kernel"] if roof else "-", roof["kernel_ms"] if roof else 0, 100 * roof["frac"] if roof else 0))
+        return f, I
+
     forms = {}
-    ids = {}
-    runs = [("plain-fp32", HV_PRECISION_FP32, False, 3.0, PEAK_FP32_TFLOPS, "fp32-valu (direct-difference sub+fma)",
-             "k_list_scan"),
-            ("bf16", HV_PRECISION_BF16, False, 2.0, PEAK_BF16_TFLOPS, "bf16 mfma (v_mfma_f32_16x16x32_bf16)",
-             "k_mfma_scan")]
-    fp8_r = None
+    f, I_plain = measure("plain-fp32", HV_PRECISION_FP32, nprobe=op_plain, h2h=True)
+    forms["plain-fp32"] = f
+    f, I_bf16 = measure("bf16", HV_PRECISION_BF16, nprobe=op_plain, h2h=True)
+    forms["bf16"] = f
+    same = bool(np.array_equal(I_plain, I_bf16))
+    op = {"plain-fp32": {"nprobe": op_plain, "recall_at_100": recall_at[op_plain]},
+          "bf16": {"nprobe": op_plain, "recall_at_100": forms["bf16"]["recall_at_100"], "ids_equal_plain": same}}
     if not args.no_fp8:
-        fp8_r = 4
-        if not args.no_recall:
+        found = None
+        for npb in [p for p in sweep if p >= op_plain]:
             for r in (1, 2, 4):
-                index.preSearch(precision=HV_PRECISION_FP8, rerank_r=r, learned=False)
-                index.search_device(q_dev, nq, k, D_dev, I_dev)
-                rr = metrics.recall(I_dev.to_host(np.int64, (nq, k)), GI)
-                log("fp8 r %d recall@100 %.4f" % (r, rr))
-                if rr >= 0.95:
-                    fp8_r = r
+                params(HV_PRECISION_FP8, rerank_r=r, nprobe=npb)
+                I, D, _ = run_once()
+                rr = metrics.recall(I, GI)
+                log("fp8 nprobe %d r %d recall@100 %.5f" % (npb, r, rr))
+                if rr >= args.target:
+                    found = (npb, r)
                     break
-        runs.append(("fp8", HV_PRECISION_FP8, False, 2.0, 5000.0, "fp8 mfma (v_mfma_f32_16x16x32_fp8_fp8)",
-                     "k_mfma_scan"))
-    learned_info = None
+            if found:
+                break
+        if found:
+            f, _ = measure("fp8", HV_PRECISION_FP8, rerank_r=found[1], nprobe=found[0])
+            forms["fp8"] = f
+            op["fp8"] = {"nprobe": found[0], "rerank_r": found[1], "recall_at_100": f["recall_at_100"]}
+
+    learned_info, gain = None, []
+    best = "bf16"
     if not args.no_learned:
         t0 = time.perf_counter()
         _, xt = datagen.make(2, nb=1, nq=args.train_queries, query_set=1)
@@ -410,67 +561,44 @@ def main():
         t1 = time.perf_counter()
         losses = learned.train(index, feats, targets, epochs=20, batch=1024, lr=2e-3, seed=0)
         t2 = time.perf_counter()
-        s_np, s_b, rec_v, probes_v = learned.calibrate(index, vf, vt, xv, vexact, nprobe=nprobe, k=k)
+        cap = args.learned_cap
+        index.preSearch(precision=HV_PRECISION_BF16)
+        s_np, s_b, rec_v, probes_v = learned.calibrate(index, vf, vt, xv, vexact, nprobe=cap, k=k,
+                                                       target_recall=args.target, margin=0.003)
         learned_info = {"train_queries": int(feats.shape[0]), "trace_s": round(t1 - t0, 2),
                         "train_s": round(t2 - t1, 2), "epochs": 20, "loss_first": float(losses[0]),
-                        "loss_last": float(losses[-1]), "nprobe_scale": s_np, "bound_scale": round(s_b, 4),
-                        "validation_recall": round(rec_v, 5), "validation_mean_probes": round(probes_v, 3)}
+                        "loss_last": float(losses[-1]), "nprobe_cap": cap, "nprobe_scale": s_np,
+                        "bound_scale": round(s_b, 4), "validation_recall": round(rec_v, 5),
+                        "validation_mean_probes": round(probes_v, 3)}
         log("predictor: traces %.1fs, training %.1fs (loss %.4f -> %.4f), calibration: nprobe scale %.2f, "
             "bound scale %.3f, validation recall %.4f, mean probes %.2f"
             % (t1 - t0, t2 - t1, losses[0], losses[-1], s_np, s_b, rec_v, probes_v))
-        runs.append(("learned-bf16", HV_PRECISION_BF16, True, 2.0, PEAK_BF16_TFLOPS,
-                     "bf16 mfma (v_mfma_f32_16x16x32_bf16)", "k_mfma_scan"))
-    for name, prec, use_learned, flops_per_dim, peak, pipe, kern in runs:
-        ms, st, I, Dh, tuned = measure(prec, use_learned, fp8_r if prec == HV_PRECISION_FP8 else 4)
-        ids[name] = I
-        flops = flops_per_dim * d * st["main_pairs"]
-        ach = flops / (st["ms_main"] / 1000.0) / 1e12 if st["ms_main"] > 0 else 0.0
-        roof = {"bound": "mfma", "pipe": pipe, "kernel": kern, "achieved": round(ach, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(ach / peak, 5), "algorithmic_flops_per_launch": flops,
-                "kernel_ms": round(st["ms_main"], 4)}
-        if st["ms_rerank"] > st["ms_main"]:
-            rbytes = (4.0 * d + 20.0) * st["rerank_pairs"]
-            gbs = rbytes / (st["ms_rerank"] / 1000.0) / 1e9
-            roof = {"bound": "hbm", "kernel": "k_rerank_flat", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 5), "algorithmic_bytes_per_launch": rbytes,
-                    "kernel_ms": round(st["ms_rerank"], 4), "pairs": int(st["rerank_pairs"])}
-        rec = metrics.recall(I, GI) if not args.no_recall else None
-        forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 4),
-                       "recall_at_100": rec, "max_rel_dist_error_vs_fp64": dist_error(Dh, I),
-                       "autotuned": tuned,
-                       "stage_ms": {kk: round(st[kk], 4) for kk in ("ms_coarse", "ms_warmup", "ms_group", "ms_scan",
-                                                                    "ms_select", "ms_main")},
-                       "candidates": int(st["candidates"]), "main_pairs": int(st["main_pairs"]),
-                       "survivors": int(st["survivors"]), "overflow_queries": int(st["overflow_queries"]),
-                       "fallback_queries": int(st["fallback_queries"]),
-                       "rerank_ms": round(st["ms_rerank"], 4),
-                       "roofline": roof}
-        if use_learned:
-            forms[name]["mean_probes"] = round(float(index.last_nprobe(nq).mean()), 3)
-        if prec == HV_PRECISION_FP8:
-            forms[name]["rerank_r"] = fp8_r
-        log("%s: %.3f ms/step, %.0f QPS, recall %s, main scan %.3f ms (%.1f TFLOP/s), re-rank %.3f ms; "
-            "dominant %s %.1f%% of peak" % (name, ms, nq / (ms / 1000.0), rec, st["ms_main"], ach, st["ms_rerank"],
-                                           roof["kernel"], 100 * roof["frac"]))
-    same = bool(np.array_equal(ids["plain-fp32"], ids["bf16"]))
-    best = "learned-bf16" if "learned-bf16" in forms else "bf16"
-    # secondary figure: the host-buffer C-ABI call (queries and results crossing PCIe), headline form
-    index.preSearch(learned=best == "learned-bf16", precision=HV_PRECISION_BF16, graph=False)
-    index.search(xq, k)
-    t0 = time.perf_counter()
-    for _ in range(3):
-        index.search(xq, k)
-    h2h_ms = 1000.0 * (time.perf_counter() - t0) / 3
-    forms[best]["host_to_host_qps"] = round(nq / (h2h_ms / 1000.0), 1)
-    log("%s host-to-host (hv_search, PCIe included): %.3f ms, %.0f QPS" % (best, h2h_ms, nq / (h2h_ms / 1000.0)))
-    index.preSearch(learned=False)
-    ms = forms[best]["ms_per_step"]
-    qps = forms[best]["qps"]
+        f, _ = measure("learned-bf16", HV_PRECISION_BF16, use_learned=True, nprobe=cap, h2h=True)
+        forms["learned-bf16"] = f
+        op["learned-bf16"] = {"nprobe_cap": cap, "nprobe_scale": s_np, "mean_probes": f["mean_probes"],
+                              "recall_at_100": f["recall_at_100"]}
+        best = "learned-bf16"
+        # what the network gains over the method without it at equal recall: the BF16 form at the
+        # smallest fixed nprobe reaching each target, against the learned form calibrated to it
+        for tgt in [float(x) for x in args.gain_targets.split(",") if x]:
+            okp = [p for p in sweep if recall_at[p] >= tgt]
+            if not okp:
+                continue
+            fixed, _ = measure("bf16@%g" % tgt, HV_PRECISION_BF16, nprobe=min(okp))
+            s2 = learned.calibrate(index, vf, vt, xv, vexact, nprobe=cap, k=k, target_recall=tgt, margin=0.003)
+            lf, _ = measure("learned-bf16@%g" % tgt, HV_PRECISION_BF16, use_learned=True, nprobe=cap)
+            gain.append({"target": tgt, "fixed_nprobe": min(okp), "fixed_qps": fixed["qps"],
+                         "fixed_recall": fixed["recall_at_100"], "fixed_candidates": fixed["candidates"],
+                         "learned_nprobe_scale": s2[0], "learned_qps": lf["qps"], "learned_recall": lf["recall_at_100"],
+                         "learned_candidates": lf["candidates"], "learned_mean_probes": lf["mean_probes"],
+                         "speedup": round(lf["qps"] / fixed["qps"], 3)})
+        index.preSearch(learned=False)
 
     base = None
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        base = cpu_baseline(index, xq, nprobe, k, B, args.cpu_budget)
+        index.nprobe = op_plain
+        base = cpu_baseline(index, xq, op_plain, k, B, args.cpu_budget)
         log("cpu baseline %.1f qps (%s) in %.1fs" % (base["value"], base["sample"], time.perf_counter() - t0))
 
     roof = dict(forms[best]["roofline"])
@@ -478,6 +606,8 @@ def main():
     roof["traffic"] = tr["bytes_per_launch"] if tr else None
     if tr:
         roof["traffic_source"] = tr["source"]
+    qps = forms[best]["qps"]
+    h2h = {n: forms[n]["host_to_host_qps"] for n in forms if "host_to_host_qps" in forms[n]}
     out = {
         "metric": METRIC,
         "value": qps,
@@ -485,21 +615,21 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": forms[best]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16+f32",
         "data": "synthetic",
-        "config": {"workload": "int-mixture-1m: 1,000,000 x 128 fp32 base, 10,000 queries, nlist 1024, k 100, "
-                               "B=4 dimension blocks, pruning on",
-                   "form": best,
-                   "nprobe": nprobe, "nb": int(xb.shape[0]), "nq": nq,
-                   "recall_at_100": forms[best]["recall_at_100"], "recall_sweep_plain": recall_at,
-                   "forms": forms,
-                   "bf16_identical_to_plain": same,
-                   "predictor": learned_info,
-                   "speedup_vs_plain_gpu": round(qps / forms["plain-fp32"]["qps"], 3)},
+        "config": {"workload": "int-mixture-1m: %d x 128 fp32 base, %d queries, nlist 1024, k 100, B=4 dimension "
+                               "blocks, pruning on" % (nb, nq),
+                   "form": best, "target_recall": args.target, "operating_points": op,
+                   "recall_sweep_plain": recall_at, "forms": forms,
+                   "host_to_host_qps": h2h,
+                   "speedup_vs_plain_gpu": round(qps / forms["plain-fp32"]["qps"], 3),
+                   "speedup_vs_plain_gpu_host_to_host":
+                       round(h2h[best] / h2h["plain-fp32"], 3) if best in h2h and "plain-fp32" in h2h else None,
+                   "bf16_identical_to_plain": same, "predictor": learned_info, "learned_gain": gain},
         "roofline": roof,
         "cpu_baseline": base,
     }

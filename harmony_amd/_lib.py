@@ -64,7 +64,9 @@ class Stats(ctypes.Structure):
                [(n, ctypes.c_int64) for n in ("candidates", "block_evals", "block_evals_max", "survivors",
                                               "overflow_queries", "work_items")] + \
                [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64), ("fallback_queries", ctypes.c_int64),
-                ("ms_rerank", ctypes.c_double), ("rerank_pairs", ctypes.c_int64)]
+                ("ms_rerank", ctypes.c_double), ("rerank_pairs", ctypes.c_int64),
+                ("ms_scan_kernels", ctypes.c_double), ("scan_launches", ctypes.c_int64),
+                ("scan_pairs", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -10,13 +10,14 @@ any setting; the FP8 form's shortlist is the same too):
 The dimension-block count B is not a knob here: it fixes the order of the FP32 partial sums
 (node.cpp:639-646), hence the last bit of every distance. A coordinate search: each knob in
 turn over its candidates with the others fixed at their best so far, each candidate timed as
-the median of `reps` searches after two warm-up calls (in graph mode the first sizes the
-buffers and the second captures). No counterpart in the reference (its per-node scan has no
+the median of `reps` device-resident searches (hv_search_device) after two warm-up calls (in
+graph mode the first sizes the buffers and the second captures). No counterpart in the reference (its per-node scan has no
 tiles).
 """
 import numpy as np
 
 from ._lib import HV_PRECISION_FP32
+from .index import DeviceArray
 
 SHAPES = {
     0: "automatic", 1: "64 queries x 128 vectors", 2: "64 x 64", 3: "32 x 128", 4: "32 x 64",
@@ -24,12 +25,13 @@ SHAPES = {
 }
 
 
-def _time(index, q, k, reps):
-    index.search(q, k)
-    index.search(q, k)
+def _time(index, bufs, k, reps):
+    q, n, D, I = bufs
+    index.search_device(q, n, k, D, I)
+    index.search_device(q, n, k, D, I)
     ts = []
     for _ in range(reps):
-        _, _, st = index.search(q, k, return_stats=True)
+        st = index.search_device(q, n, k, D, I)
         ts.append(st["ms_total"])
     return float(np.median(ts))
 
@@ -39,6 +41,9 @@ def autotune(index, queries, k, sample=4096, reps=3, vsplits=(128, 256, 512, 102
     up to `sample` of `queries`; leaves the best setting applied and returns
     (best parameters dict, {setting: ms} of every trial)."""
     q = np.ascontiguousarray(queries[:sample], dtype=np.float32)
+    n = q.shape[0]
+    # timed on device-resident queries and results (the path a serving caller keeps hot)
+    bufs = (DeviceArray.from_host(q), n, DeviceArray(n * k * 4), DeviceArray(n * k * 8))
     p = index._params
     fp32 = p.precision == HV_PRECISION_FP32
     best = {"tile_shape_first": int(p.tile_shape_first), "tile_shape": int(p.tile_shape),
@@ -56,7 +61,7 @@ def autotune(index, queries, k, sample=4096, reps=3, vsplits=(128, 256, 512, 102
         key = tuple(sorted(cfg.items()))
         if key not in trials:
             apply(cfg)
-            trials[key] = _time(index, q, k, reps)
+            trials[key] = _time(index, bufs, k, reps)
             if log:
                 log("autotune %s: %.3f ms" % (dict(key), trials[key]))
         return trials[key]

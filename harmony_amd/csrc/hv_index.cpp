@@ -216,6 +216,10 @@ struct hv_index {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
+    // begin/end pairs around each candidate-scan kernel launch of an eager search
+    static constexpr int kScanEv = 16;
+    hipEvent_t ev_scan[2 * kScanEv] = {};
+    int nscan_ev = 0;
     int scan_grid[hv::kNumShapes] = {};
 
     std::vector<int> perm;  // empty = identity
@@ -456,6 +460,12 @@ void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, 
                 HV_HIP(hipEventElapsedTime(&mr, h->ev[8], h->ev[9]));
                 st->ms_rerank += mr;
             }
+            for (int i = 0; i < h->nscan_ev; ++i) {
+                float ms1 = 0.f;
+                HV_HIP(hipEventElapsedTime(&ms1, h->ev_scan[2 * i], h->ev_scan[2 * i + 1]));
+                st->ms_scan_kernels += ms1;
+            }
+            st->scan_launches += h->nscan_ev;
         } else {
             float mg = 0.f;  // the whole device work of the graph
             HV_HIP(hipEventElapsedTime(&mg, h->ev[0], h->ev[5]));
@@ -469,6 +479,7 @@ void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, 
         st->overflow_queries += (int64_t)hs[12];
         st->fallback_queries += (int64_t)hs[13];
         if (p.main_recorded) st->main_pairs += (int64_t)hs[8];
+        st->scan_pairs += (int64_t)(hs[0] + hs[4] + hs[8]);
         if (p.mfma) st->rerank_pairs += (int64_t)hs[2];  // the select sees exactly the re-ranked pairs
         st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -562,6 +573,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     const bool capture = graph_ok && h->graph.pending_sig == sig && h->graph.pending_gen == g_alloc_gen.load();
     auto rec = [&](int i) {
         if (!capture) HV_HIP(hipEventRecord(h->ev[i], s));
+    };
+    h->nscan_ev = 0;
+    // events around one candidate-scan launch (eager searches; launches past kScanEv go untimed)
+    auto scan_ev = [&](bool end) {
+        if (capture || h->nscan_ev >= hv_index::kScanEv) return;
+        HV_HIP(hipEventRecord(h->ev_scan[2 * h->nscan_ev + (end ? 1 : 0)], s));
+        if (end) ++h->nscan_ev;
     };
     HV_HIP(hipEventRecord(h->ev[0], s));
     if (capture) {
@@ -708,12 +726,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         const bool main_phase = st_slot == stats + 8;
         main_recorded |= main_phase;
         if (main_phase) rec(6);
+        scan_ev(false);
         hv::launch_list_scan(shape, Qp, (int)d, h->d_codes.as<float>(), h->d_code_off.as<int64_t>(),
                              h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), plan,
                              prune, h->s_bound.as<float>(), h->s_pairs.as<int>(), h->s_items.as<hv::ScanItem>(),
                              queue, vmax, vsplit, accept_all, surv, surv_cnt, (int)cap, st_slot, h->scan_grid[shape],
                              s);
         HV_CHECK_LAUNCH("list scan");
+        scan_ev(true);
         if (main_phase) rec(7);
     };
     // BF16 precision form: one phase of the MFMA filter scan over probe ranks [r0, r1)
@@ -811,6 +831,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (main_phase && !main_recorded) rec(6);  // events 6-7 span the main phases
         main_recorded |= main_phase;
         // per slot: FP8 inverse scales or BF16 rounding-error norms (s_qscale holds either)
+        scan_ev(false);
         hv::launch_mfma_scan(h->bf16_grid, fp8 ? 1 : 0, h->s_qres.p, h->s_qscale.as<float>(), (int)d, pdist,
                              (int)nprobe, fp8 ? h->d_rcodes8.p : h->d_rcodes.p,
                              fp8 ? h->d_lscale8.as<float>() : nullptr, h->d_roff.as<int64_t>(),
@@ -819,6 +840,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              ph.items, ph.queue, vmax, vsplit_mfma, mode, kappa1, kappa_g, kappa2, surv, aux,
                              surv_cnt, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("mfma scan");
+        scan_ev(true);
         if (main_phase) rec(7);
     };
     // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
@@ -1008,13 +1030,20 @@ hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_leve
     hv_default_search_params(&h->sp);
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
-        delete h;
+        hv_destroy(h);
         return nullptr;
+    }
+    for (auto& e : h->ev_scan) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            g_create_error = "hipEventCreate failed";
+            hv_destroy(h);
+            return nullptr;
+        }
     }
     for (auto& e : h->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             g_create_error = "hipEventCreate failed";
-            delete h;
+            hv_destroy(h);
             return nullptr;
         }
     }
@@ -1036,6 +1065,8 @@ void hv_destroy(hv_index_t* h) {
                       &h->d_rcodes8, &h->d_lscale8};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : h->ev_scan)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

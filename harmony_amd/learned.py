@@ -20,6 +20,7 @@ import numpy as np
 
 from . import _lib, metrics
 from ._lib import HV_PRECISION_BF16
+from .index import DeviceArray
 
 
 def id_to_list(index):
@@ -30,6 +31,15 @@ def id_to_list(index):
     for l in range(len(off) - 1):
         out[ids[off[l]:off[l + 1]]] = l
     return out
+
+
+def search_dev(index, q, k):
+    """(D, I) of a search with the queries and results in device memory (hv_search_device)."""
+    q = np.ascontiguousarray(q, dtype=np.float32)
+    n = q.shape[0]
+    qd, Dd, Id = DeviceArray.from_host(q), DeviceArray(n * k * 4), DeviceArray(n * k * 8)
+    index.search_device(qd, n, k, Dd, Id)
+    return Dd.to_host(np.float32, (n, k)), Id.to_host(np.int64, (n, k))
 
 
 def collect_traces(index, queries, k, target_recall=0.95, chunk=10000):
@@ -51,7 +61,7 @@ def collect_traces(index, queries, k, target_recall=0.95, chunk=10000):
             qc = q[c0:c0 + chunk]
             m = qc.shape[0]
             feats[c0:c0 + m] = index.query_features(qc)
-            D, I = index.search(qc, k)
+            D, I = search_dev(index, qc, k)
             probes = index.last_probes(m)[:, :nlist]
             rank = np.empty((m, nlist), np.int32)
             np.put_along_axis(rank, probes, np.arange(nlist, dtype=np.int32)[None, :].repeat(m, 0), axis=1)
@@ -105,7 +115,7 @@ def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.
     best = None
     for s in scales:
         index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale)
-        _, I = index.search(queries, k)
+        _, I = search_dev(index, queries, k)
         rec = float(recall_per_query(I, exact).mean())
         probes = float(index.last_nprobe(queries.shape[0]).mean())
         best = (s, bound_scale, rec, probes)

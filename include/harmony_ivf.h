@@ -122,6 +122,10 @@ typedef struct {
     int64_t fallback_queries;/* learned search: queries whose seeded bound was too low, redone exactly */
     double ms_rerank;        /* BF16 form: the exact FP32 re-rank kernel */
     int64_t rerank_pairs;    /* pairs it re-ranked */
+    double ms_scan_kernels;  /* summed duration of every candidate-scan kernel launch (the FP32 list
+                                scan, warm-up included, or the MFMA filter scan), eager searches only */
+    int64_t scan_launches;   /* those launches */
+    int64_t scan_pairs;      /* (query, vector) pairs they covered, unpruned (warm-up included) */
 } hv_stats;
 
 /* ---- lifetime (Index::Index, Index.h:15-25) ---- */

@@ -225,9 +225,9 @@ def test_padding_and_clamp():
     assert np.all(I[:, 40:] == -1) and np.all(D[:, 40:] == FMAX)
 
 
-@pytest.mark.parametrize("B", [8, 1])
-def test_float_data_bit_exact(B):
-    say("float workload 3 sample: 4000 x 420, nlist 40, B=%d" % B)
+@pytest.fixture(scope="module")
+def w3s():
+    say("float workload 3 sample: 4000 x 420, nlist 40")
     xb, xq = datagen.make(3, nb=4000, nq=40)
     perm = datagen.variance_order(xb)
     g = gpu_index(xb, 40, 12, niter=4, dim_order=perm)
@@ -235,11 +235,28 @@ def test_float_data_bit_exact(B):
     np.testing.assert_array_equal(g.centroids()[:, perm], o_c)
     o = O.IndexData.from_gpu_index(g)
     o = O.IndexData(o.centroids[:, perm], o.list_off, o.ids, o.codes[:, perm])
-    g.preSearch(pruning=True, dim_blocks=B)
+    return xq, perm, g, o
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("B,pruning", [(8, True), (1, True), (8, False), (4, False)])
+def test_float_data_bit_exact(w3s, B, pruning, precision):
+    """Float data (d = 420, wide per-dimension scales): pruning on sums B per-block chains
+    (node.cpp:639-651), pruning off one full chain (node.cpp:658-667, the ORIGINAL scan of
+    Index.cpp:1420-1554); the two differ in the last bits here, and each matches the oracle's."""
+    xq, perm, g, o = w3s
+    say("w3 sample B=%d pruning=%s precision=%d" % (B, pruning, precision))
+    g.nprobe = 12
+    g.preSearch(pruning=pruning, dim_blocks=B, precision=precision)
     D, I = g.search(xq, 20)
-    oD, oI, _, _ = O.search(o, xq[:, perm], 20, nprobe=12, dim_blocks=B)
+    g.preSearch(pruning=True, dim_blocks=4, precision=0)
+    oD, oI, _, _ = O.search(o, xq[:, perm], 20, nprobe=12, dim_blocks=B, pruning=pruning)
     np.testing.assert_array_equal(I, oI)
     np.testing.assert_array_equal(D, oD)
+    if not pruning:
+        # the full chain is not the blocked sum on this data, so the test tells them apart
+        bD, _, _, _ = O.search(o, xq[:, perm], 20, nprobe=12, dim_blocks=8, pruning=True)
+        assert np.any(bD != oD)
 
 
 @pytest.fixture(scope="module")
