@@ -373,6 +373,9 @@ def main():
     ap.add_argument("--no-host-to-host", action="store_true",
                     help="skip the host-buffer (hv_search) timings (profiler runs time device work only)")
     ap.add_argument("--workload", default="int-mixture-1m", choices=["int-mixture-1m", "skewed-420d", "zipf-96d-10m"])
+    ap.add_argument("--trace-markers", action="store_true",
+                    help="before each form's timed steps run a device copy of (form number) MiB, so the steps "
+                         "can be found in a profiler's kernel trace")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this size (a copy kernel of known traffic, "
                          "for scaling HBM counters in a profiler run)")
@@ -460,6 +463,13 @@ def main():
                     worst = max(worst, abs(float(dist) - ref) / ref)
         return worst
 
+    forms_seen = []
+    mark_src, mark_dst = (DeviceArray(16 << 20), DeviceArray(16 << 20)) if args.trace_markers else (None, None)
+
+    def trace_mark(i):
+        assert lib().hv_memcpy_dtod(mark_dst.ptr, mark_src.ptr, i << 20) == 0
+        log("trace marker: %d MiB copy before the timed steps of form %d" % (i, i))
+
     def measure(name, precision, use_learned=False, rerank_r=4, nprobe=None, h2h=False):
         if use_learned:
             index.preSearch(precision=precision, learned=True, graph=False, tile_shape=0, tile_shape_first=0,
@@ -474,6 +484,11 @@ def main():
             log("%s autotuned: %s" % (name, tuned))
         for _ in range(args.warmup):
             index.search_device(q_dev, nq, k, D_dev, I_dev)
+        if args.trace_markers:
+            # a device copy of a size unique to this form marks where its timed steps start in a
+            # profiler's kernel trace (1 MiB times the form's number)
+            trace_mark(len(forms_seen) + 1)
+        forms_seen.append(name)
         times, sts = [], []
         for _ in range(args.steps):
             t0 = time.perf_counter()
