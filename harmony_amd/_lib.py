@@ -93,6 +93,7 @@ SIGNATURES = {
     "hv_set_query_nprobe": (ctypes.c_int, [_P, _I64, _P]),
     "hv_search": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_search_device": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
+    "hv_search_device_noalloc": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_ground_truth_f64": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P]),
     "hv_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "hv_load": (ctypes.c_int, [_P, ctypes.c_char_p]),

@@ -21,6 +21,14 @@
 
 #include <atomic>
 #include <functional>
+#include <mutex>
+
+// Scratch of hv_search_device_noalloc: a fixed block of device memory on every GPU the library
+// runs on, reserved when its code object is loaded and held until the process ends. Host code
+// never uses this name as an address: hv_create looks up each device's copy with
+// hipGetSymbolAddress (with that device current) and the search passes that address on.
+constexpr size_t kNoAllocScratchBytes = 128ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
@@ -30,11 +38,34 @@ thread_local std::string g_create_error;
 // holds their addresses and is replayed only while this has not moved.
 std::atomic<uint64_t> g_alloc_gen{0};
 
+// Bump allocator over a fixed block of device memory (the no-allocation search's scratch): the
+// search buffers are carved from it in order, 256-byte aligned; a buffer that was the last one
+// carved grows in place. Carving past the end fails instead of allocating.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* last = nullptr;
+    size_t high = 0;  // largest `used` of the current call
+};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    Arena* arena = nullptr;  // set: ensure() carves from the arena and never allocates
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
+        if (arena) {
+            const size_t b = (std::max<size_t>(bytes, 256) + 255) / 256 * 256;
+            char* at = (p && p == arena->last) ? static_cast<char*>(p) : arena->base + arena->used;
+            const size_t end = (size_t)(at - arena->base) + b;
+            if (end > arena->cap) return hipErrorOutOfMemory;
+            p = at;
+            cap = b;
+            arena->last = at;
+            arena->used = end;
+            arena->high = std::max(arena->high, end);
+            return hipSuccess;
+        }
         g_alloc_gen.fetch_add(1);
         if (p) {
             hipError_t e = hipFree(p);
@@ -48,6 +79,11 @@ struct DevBuf {
         return e;
     }
     void release() {
+        if (arena) {  // arena memory is never freed
+            p = nullptr;
+            cap = 0;
+            return;
+        }
         if (p) (void)hipFree(p);
         g_alloc_gen.fetch_add(1);
         p = nullptr;
@@ -132,6 +168,33 @@ void require(bool ok, int code, const std::string& msg) {
     if (!ok) throw HvError{code, msg};
 }
 
+// Per-device address of `scratch` (looked up once per device) and the lock that serializes the
+// no-allocation searches on that device, which share it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_setup_mu;
+std::mutex g_scratch_mu[kMaxDevices];
+void* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// With `dev` current: the device address of its scratch block, or null (error in *e).
+void* scratch_setup(int dev, hipError_t* e) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        *e = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_setup_mu);
+    if (!g_scratch_done[dev]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_err[dev] = r;
+        g_scratch_ptr[dev] = r == hipSuccess ? p : nullptr;
+        g_scratch_done[dev] = true;
+    }
+    *e = g_scratch_err[dev];
+    return g_scratch_ptr[dev];
+}
+
 // SplitMix64 + Fisher-Yates: the shuffle used for the k-means subsample and init
 // (replacing std::shuffle(default_random_engine(seed)) of Clustering.cpp:58,81 with a
 // portable stream).
@@ -182,8 +245,13 @@ constexpr int kVecSplitBf16 = 256;
 // beyond it in groups of growing width with the bound tightened in between
 constexpr int64_t kBf16SinglePhase = 16;
 
-// Scratch of one batch of exact redos (per-(query, rank, segment) runs of k keys)
+// Scratch of one batch of exact redos (per-(query, rank, segment) runs of k keys); the
+// no-allocation search uses smaller batches
 constexpr int64_t kRedoRunBytes = 64ll << 20;
+constexpr int64_t kRedoRunBytesNoAlloc = 8ll << 20;
+// Coarse distance rows are formed in slices of at most this many bytes (no-allocation: smaller)
+constexpr int64_t kCoarseSliceBytes = 256ll << 20;
+constexpr int64_t kCoarseSliceBytesNoAlloc = 16ll << 20;
 
 // Dimension blocks of node.cpp:639-646 (B blocks of floor(d/B) dims, the last takes the
 // rest), each cut into staging chunks of at most kChunkDims dims.
@@ -247,6 +315,10 @@ struct hv_index {
     std::vector<float> ext_bounds;
     std::vector<int32_t> query_nprobe;  // per-query probe counts (empty: nprobe for all)
     PinnedBuf qn_host;                  // the clamped counts of the current search
+    std::vector<int32_t> qn_stage;      // the same for a no-allocation search (sized when set)
+    // no-allocation search in progress: scratch comes from the static arena, graphs are off;
+    // size_only: stop after sizing the scratch (the no-allocation entry point's first pass)
+    bool noalloc = false, size_only = false;
     SearchGraph graph;
 
     // learned probe/bound predictor: FP32 parameters and Adam moments, BF16 serving weights,
@@ -380,7 +452,8 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
 // with an exact re-check of the candidates near the P-th (hv_coarse.hip); otherwise full exact
 // distance rows and a select. Rows in slices of <= 256 MB.
 int64_t coarse_rows_per_slice(const hv_index_t* h, int64_t n) {
-    return std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(256ll << 20) / (h->nlist * 4)));
+    const int64_t slice = h->noalloc ? kCoarseSliceBytesNoAlloc : kCoarseSliceBytes;
+    return std::max<int64_t>(1, std::min<int64_t>(n, slice / (h->nlist * 4)));
 }
 bool coarse_on_mfma(const hv_index_t* h, int64_t P) { return P <= 512 && h->nlist <= 4096; }
 // the MFMA path's re-check counter and list (slice rows + 1 ints)
@@ -541,15 +614,23 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
 
     hipStream_t s = h->stream;
     auto t0 = std::chrono::steady_clock::now();
-    // per-query probe counts, clamped, in pinned memory (a captured search copies them from there)
-    auto stage_qn = [&]() {
-        HV_HIP(h->qn_host.ensure(sizeof(int) * n));
-        int* qh = static_cast<int*>(h->qn_host.p);
+    // per-query probe counts, clamped, in pinned memory (a captured search copies them from there;
+    // a no-allocation search uses the host vector sized when the counts were set)
+    auto stage_qn = [&]() -> const int* {
+        int* qh;
+        if (h->noalloc) {
+            require((int64_t)h->qn_stage.size() >= n, HV_ERR_STATE, "per-query nprobe staging not sized");
+            qh = h->qn_stage.data();
+        } else {
+            HV_HIP(h->qn_host.ensure(sizeof(int) * n));
+            qh = static_cast<int*>(h->qn_host.p);
+        }
         for (int64_t i = 0; i < n; ++i) qh[i] = (int)std::min<int64_t>(nprobe, std::max<int64_t>(1, h->query_nprobe[i]));
+        return qh;
     };
     // graph mode: the device work of a search is captured on the second call with the same
     // signature (the first one sizes every buffer) and replayed from then on
-    const bool graph_ok = h->sp.graph != 0 && !external;
+    const bool graph_ok = h->sp.graph != 0 && !external && !h->noalloc;
     std::vector<unsigned char> sig;
     if (graph_ok) {
         auto put = [&](const void* v, size_t b) {
@@ -570,6 +651,104 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             return;
         }
     }
+    // Every scratch buffer of the search is sized here, before the first launch: a no-allocation
+    // search carves them all from its arena (or fails before doing anything); later ensure() calls
+    // of the same buffers are no-ops.
+    const int64_t nb = nlist * rank_end;  // (list, rank) buckets of the searched ranks
+    const bool learned_seed = learned && !fp8;  // the seed is the starting bound (FP32 and BF16)
+    const bool bound_fmax = !learned_seed && !external && !(wl > 0 && !mfma && !learned);
+    // the exact redo's run scratch: batches of queries, each with nprobe_cap * nseg runs of k keys,
+    // then one count per query of the batch
+    const int nseg = hv::exact_segments(h->max_list);
+    const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
+    const int64_t redo_bytes = h->noalloc ? kRedoRunBytesNoAlloc : kRedoRunBytes;
+    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, redo_bytes / (8 * run_keys)));
+    // work items of one phase: at most (query tiles) x (vector ranges); query tiles hold >= 16
+    // queries, vector ranges >= kVecSplit vectors
+    const int vsplit_mfma = h->sp.scan_vsplit > 0 ? h->sp.scan_vsplit : kVecSplitBf16;
+    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / std::min(kVecSplit, vsplit_mfma) + 1);
+    require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
+    // survivor buffer: room for the warm-up set too
+    if (!learned && h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
+        cap = std::max<int64_t>(cap, hv_next_pow2((int)(wl * wls)));
+    // MFMA forms: the scan phases (probe ranks [r0, r1), vectors [vmin, vmax)) in the order they run
+    struct PhaseRange {
+        int64_t r0, r1;
+        int vmin, vmax;
+        bool main;
+    };
+    std::vector<PhaseRange> ranges;
+    if (mfma) {
+        if (learned_seed) {
+            for (int64_t r0 = 0; r0 < rank_end;) {
+                const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
+                ranges.push_back({r0, r1, 0, INT32_MAX, r0 != 0});
+                r0 = r1;
+            }
+        } else {
+            const int vdirect = (int)(cap / 16 * 16);
+            if (vdirect > 0) ranges.push_back({0, 1, 0, vdirect, false});
+            if (h->max_list > vdirect) ranges.push_back({0, 1, vdirect, INT32_MAX, false});
+            if (!fp8 && nprobe <= kBf16SinglePhase) {
+                if (nprobe > 1) ranges.push_back({1, nprobe, 0, INT32_MAX, true});
+            } else {
+                const int64_t end = fp8 ? rank_end : nprobe;
+                for (int64_t r0 = 1; r0 < end;) {
+                    const int64_t r1 = std::min<int64_t>(end, r0 * 4);
+                    ranges.push_back({r0, r1, 0, INT32_MAX, true});
+                    r0 = r1;
+                }
+            }
+        }
+        require((int)ranges.size() <= hv::kMaxPlanPhases, HV_ERR_CAPACITY, "too many scan phases");
+    }
+    const int64_t qlen = 3 * hv::kNumXcd + nlist;
+    const int64_t nph = std::max<int64_t>(1, (int64_t)ranges.size());
+    {
+        auto need = [&](DevBuf& b, int64_t bytes) { HV_HIP(b.ensure((size_t)std::max<int64_t>(bytes, 1))); };
+        if (!h->perm.empty()) need(h->s_qp, 4 * n * d);
+        need(h->s_bucket, 4 * nb);
+        need(h->s_fill, 4 * nb);
+        need(h->s_stats, 8 * 14);
+        need(h->s_surv_cnt, 4 * n);
+        need(h->s_bound, 4 * n);
+        need(h->s_bound_ref, 4 * n);
+        need(h->s_overflow, 4 * (n + 2));
+        if (mfma) need(h->s_last, 4 * n);
+        need(h->s_runs, (8 * run_keys + 4) * batch);
+        need(h->s_probes, 4 * n * nprobe);
+        need(h->s_pdist, 4 * n * nprobe);
+        const int64_t per = coarse_rows_per_slice(h, n);
+        need(h->s_dist, 4 * per * nlist);
+        if (coarse_on_mfma(h, nprobe)) {
+            need(h->s_qnorm, 4 * n);
+            need(h->s_cnorm, 4 * nlist);
+            need(h->s_redo, 4 * (per + 1));
+        }
+        if (learned) {
+            need(h->s_qn, 4 * n);
+            need(h->s_seed, 4 * n);
+            need(h->s_pred, 8 * n);
+        } else if (uneven) {
+            need(h->s_qn, 4 * n);
+        }
+        need(h->s_bucket_off, 4 * (nb + 1));
+        need(h->s_scan_tmp, 4 * hv::scan_tmp_ints(nb));
+        need(h->s_pairs, 4 * n * nprobe);
+        need(h->s_pair_slot, 4 * n * nprobe);
+        need(h->s_items, (int64_t)sizeof(hv::ScanItem) * qcap * nph);
+        need(h->s_queue, 4 * qlen * nph);
+        need(h->s_surv, 8 * n * cap);
+        if (mfma) {
+            need(h->s_aux, 4 * n * cap);
+            need(h->s_qres, (fp8 ? 1 : 2) * n * nprobe * h->dpad);
+            need(h->s_qscale, 4 * n * nprobe);
+            need(h->s_ritems, 8 * n * ((cap + 63) / 64));
+            need(h->s_kept, 8 * n * cap);
+            need(h->s_done, 4 * n);
+        }
+    }
+    if (h->size_only) return;
     const bool capture = graph_ok && h->graph.pending_sig == sig && h->graph.pending_gen == g_alloc_gen.load();
     auto rec = [&](int i) {
         if (!capture) HV_HIP(hipEventRecord(h->ev[i], s));
@@ -608,9 +787,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     // counters, bucket counts and (when no warm-up, seed or caller's bound applies) the bounds
     // start cleared: one fill launch for all of them
-    const int64_t nb = nlist * rank_end;  // (list, rank) buckets of the searched ranks
-    const bool learned_seed = learned && !fp8;  // the seed is the starting bound (FP32 and BF16)
-    const bool bound_fmax = !learned_seed && !external && !(wl > 0 && !mfma && !learned);
     HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
     HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
     // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
@@ -621,11 +797,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
     if (mfma) HV_HIP(h->s_last.ensure(sizeof(unsigned int) * n));
-    // the exact redo's run scratch: batches of queries, each with nprobe_cap * nseg runs of k keys,
-    // then one count per query of the batch
-    const int nseg = hv::exact_segments(h->max_list);
-    const int64_t run_keys = nprobe_cap * nseg * k;  // run keys of one redone query
-    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
     HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
     {
         hv::FillSegs f{};
@@ -672,9 +843,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_CHECK_LAUNCH("predictor");
         qn = h->s_qn.as<int>();
     } else if (uneven) {
-        stage_qn();
+        const int* qh = stage_qn();
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
-        HV_HIP(hipMemcpyAsync(h->s_qn.p, h->qn_host.p, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        HV_HIP(hipMemcpyAsync(h->s_qn.p, qh, sizeof(int) * n, hipMemcpyHostToDevice, s));
         qn = h->s_qn.as<int>();
     }
     rec(1);
@@ -683,11 +854,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_scan_tmp.ensure(sizeof(int) * hv::scan_tmp_ints(nb)));
     HV_HIP(h->s_pairs.ensure(sizeof(int) * n * nprobe));
     HV_HIP(h->s_pair_slot.ensure(sizeof(int) * n * nprobe));
-    // work items of one phase: at most (query tiles) x (vector ranges); query tiles hold >= 16
-    // queries, vector ranges >= kVecSplit vectors
-    const int vsplit_mfma = h->sp.scan_vsplit > 0 ? h->sp.scan_vsplit : kVecSplitBf16;
-    const int64_t qcap = (n * nprobe / 16 + nlist + 1) * (h->max_list / std::min(kVecSplit, vsplit_mfma) + 1);
-    require(qcap < (1ll << 31), HV_ERR_CAPACITY, "too many work items");
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
@@ -699,9 +865,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     rec(2);
 
     int* queue = h->s_queue.as<int>();
-    // survivor buffer: room for the warm-up set too
-    if (!learned && h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
-        cap = std::max<int64_t>(cap, hv_next_pow2((int)(wl * wls)));
     HV_HIP(h->s_surv.ensure(sizeof(unsigned long long) * n * cap));
     unsigned long long* surv = h->s_surv.as<unsigned long long>();
     unsigned int* surv_cnt = h->s_surv_cnt.as<unsigned int>();
@@ -784,32 +947,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     std::vector<hv::PlanSpec> mph;
     size_t mph_next = 0;
     if (mfma) {
-        auto addp = [&](int64_t r0, int64_t r1, int vmin, int vmax, unsigned long long* st) {
-            mph.push_back(hv::PlanSpec{(int)r0, (int)r1, vmin, vmax, nullptr, nullptr, st});
-        };
-        if (learned_seed) {
-            for (int64_t r0 = 0; r0 < rank_end;) {
-                const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
-                addp(r0, r1, 0, INT32_MAX, r0 == 0 ? stats : stats + 8);
-                r0 = r1;
-            }
-        } else {
-            const int vdirect = (int)(cap / 16 * 16);
-            if (vdirect > 0) addp(0, 1, 0, vdirect, stats);
-            if (h->max_list > vdirect) addp(0, 1, vdirect, INT32_MAX, stats);
-            if (!fp8 && nprobe <= kBf16SinglePhase) {
-                if (nprobe > 1) addp(1, nprobe, 0, INT32_MAX, stats + 8);
-            } else {
-                const int64_t end = fp8 ? rank_end : nprobe;
-                for (int64_t r0 = 1; r0 < end;) {
-                    const int64_t r1 = std::min<int64_t>(end, r0 * 4);
-                    addp(r0, r1, 0, INT32_MAX, stats + 8);
-                    r0 = r1;
-                }
-            }
-        }
-        require((int)mph.size() <= hv::kMaxPlanPhases, HV_ERR_CAPACITY, "too many scan phases");
-        const int64_t qlen = 3 * hv::kNumXcd + nlist;
+        for (const PhaseRange& r : ranges)
+            mph.push_back(hv::PlanSpec{(int)r.r0, (int)r.r1, r.vmin, r.vmax, nullptr, nullptr, r.main ? stats + 8 : stats});
         HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap * std::max<size_t>(1, mph.size())));
         HV_HIP(h->s_queue.ensure(sizeof(int) * qlen * std::max<size_t>(1, mph.size())));
         for (size_t i = 0; i < mph.size(); ++i) {
@@ -1032,6 +1171,13 @@ hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_leve
         g_create_error = "hipStreamCreate failed";
         hv_destroy(h);
         return nullptr;
+    }
+    {
+        // the no-allocation search's scratch on this device (current here); a failure only
+        // disables that entry point, which reports it
+        hipError_t e = hipSuccess;
+        (void)scratch_setup(h->device, &e);
+        (void)hipGetLastError();
     }
     for (auto& e : h->ev_scan) {
         if (hipEventCreate(&e) != hipSuccess) {
@@ -1458,6 +1604,7 @@ int hv_set_query_nprobe(hv_index_t* h, int64_t n, const int32_t* nprobe) {
     return guarded(h, [&] {
         require(n >= 0 && (nprobe || n == 0), HV_ERR_INVALID_ARG, "bad per-query nprobe");
         h->query_nprobe.assign(nprobe, nprobe + n);
+        h->qn_stage.assign(n, 0);
     });
 }
 
@@ -1466,6 +1613,75 @@ int hv_search_device(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, fl
     if (!h) return HV_ERR_INVALID_ARG;
     return guarded(h, [&] {
         if (st) std::memset(st, 0, sizeof(*st));
+        do_search(h, n, q_dev, k, D_dev, I_dev, st);
+    });
+}
+
+int hv_search_device_noalloc(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D_dev, int64_t* I_dev,
+                             hv_stats* st) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        if (st) std::memset(st, 0, sizeof(*st));
+        hipError_t e = hipSuccess;
+        void* base = scratch_setup(h->device, &e);
+        require(base != nullptr, HV_ERR_DEVICE,
+                std::string("no-allocation scratch unavailable on this device: ") + hipGetErrorString(e));
+        require(h->sp.precision != HV_PRECISION_FP8 || h->fp8_built, HV_ERR_STATE,
+                "the FP8 codes are built by the first FP8 search (hv_search / hv_search_device); a no-allocation "
+                "search cannot build them");
+        // the search's scratch buffers, swapped for arena-backed ones for the length of the call
+        DevBuf* bufs[] = {&h->s_qp, &h->s_dist, &h->s_redo, &h->s_qnorm, &h->s_cnorm, &h->s_bucket, &h->s_fill,
+                          &h->s_stats, &h->s_surv_cnt, &h->s_bound, &h->s_bound_ref, &h->s_overflow, &h->s_last,
+                          &h->s_runs, &h->s_probes, &h->s_pdist, &h->s_qn, &h->s_seed, &h->s_pred, &h->s_bucket_off,
+                          &h->s_scan_tmp, &h->s_pairs, &h->s_pair_slot, &h->s_items, &h->s_queue, &h->s_surv,
+                          &h->s_aux, &h->s_qres, &h->s_qscale, &h->s_ritems, &h->s_kept, &h->s_done};
+        constexpr int nbufs = (int)(sizeof(bufs) / sizeof(bufs[0]));
+        std::lock_guard<std::mutex> lk(g_scratch_mu[h->device]);
+        struct Swap {
+            hv_index_t* h;
+            DevBuf** b;
+            void* p[nbufs];
+            size_t cap[nbufs];
+            Swap(hv_index_t* h_, DevBuf** b_) : h(h_), b(b_) {
+                for (int i = 0; i < nbufs; ++i) {
+                    p[i] = b[i]->p;
+                    cap[i] = b[i]->cap;
+                }
+                h->noalloc = true;
+            }
+            void attach(Arena* a) {
+                for (int i = 0; i < nbufs; ++i) {
+                    b[i]->p = nullptr;
+                    b[i]->cap = 0;
+                    b[i]->arena = a;
+                }
+            }
+            ~Swap() {
+                // the scratch is free for the next call only once this call's work is done
+                (void)hipStreamSynchronize(h->stream);
+                for (int i = 0; i < nbufs; ++i) {
+                    b[i]->p = p[i];
+                    b[i]->cap = cap[i];
+                    b[i]->arena = nullptr;
+                }
+                h->noalloc = false;
+                h->size_only = false;
+                h->last_n = 0;  // the last search's buffers are not kept
+            }
+        } swap(h, bufs);
+        // pass 1 sizes every buffer (no launches) against an unbounded counter; pass 2 runs the
+        // search in the real block
+        Arena count{static_cast<char*>(base), (size_t)1 << 62, 0, nullptr, 0};
+        swap.attach(&count);
+        h->size_only = true;
+        do_search(h, n, q_dev, k, D_dev, I_dev, nullptr);
+        h->size_only = false;
+        require(count.high <= kNoAllocScratchBytes, HV_ERR_CAPACITY,
+                "no-allocation search: this batch needs " + std::to_string((count.high + (1 << 20) - 1) >> 20) +
+                    " MiB of scratch, the limit is " + std::to_string(kNoAllocScratchBytes >> 20) +
+                    " MiB (search fewer queries per call, or lower survivor_cap)");
+        Arena arena{static_cast<char*>(base), kNoAllocScratchBytes, 0, nullptr, 0};
+        swap.attach(&arena);
         do_search(h, n, q_dev, k, D_dev, I_dev, st);
     });
 }

@@ -200,6 +200,13 @@ class Index:
                                               ctypes.byref(st)))
         return st.as_dict()
 
+    def search_device_noalloc(self, q_dev, n, k, D_dev, I_dev):
+        """search_device with the library's fixed per-GPU scratch block: allocates nothing."""
+        st = Stats()
+        check(self._h, lib().hv_search_device_noalloc(self._h, int(n), q_dev.ptr, int(k), D_dev.ptr, I_dev.ptr,
+                                                      ctypes.byref(st)))
+        return st.as_dict()
+
     def ground_truth_f64(self, queries, k):
         q = _f32(queries, self.d)
         n = q.shape[0]

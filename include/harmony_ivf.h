@@ -159,6 +159,20 @@ int hv_search(hv_index_t* h, int64_t n, const float* q, int64_t k, float* D, int
 /* device buffers (already in HBM): same layouts */
 int hv_search_device(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D_dev, int64_t* I_dev,
                      hv_stats* st);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): hv_search_device with every
+ * scratch buffer of the search carved from a fixed block of 128 MiB of device memory that the
+ * library holds on each GPU it runs on (reserved when the library is loaded, kept until the process
+ * ends), so the call allocates no device or pinned host memory. Same results as hv_search_device.
+ * Limits: the batch's scratch must fit the block (about 90 KB per query at k = 100 with the default
+ * 4096-entry survivor buffer, BF16 form; otherwise HV_ERR_CAPACITY before any work is queued:
+ * search fewer queries per call or lower survivor_cap); graph mode is ignored; the FP8 form needs
+ * its codes built by an earlier FP8 hv_search / hv_search_device (else HV_ERR_STATE);
+ * hv_get_last_* report nothing afterwards. Calls on one GPU share its block: the library
+ * serializes them (a call waits for the one in progress), and the block keeps the last call's
+ * contents (a search clears what it needs cleared with its own kernels). */
+int hv_search_device_noalloc(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D_dev,
+                             int64_t* I_dev, hv_stats* st);
 
 /* Exact kNN in FP64 over every vector of the index, ordered by (distance, id): the ground
  * truth recall is measured against (the reference computes it with faiss IndexIVFFlat at
