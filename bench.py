@@ -45,9 +45,9 @@ the FP64 ground truth.
 
 `--workload zipf-96d-10m` instead measures the uneven workload: 10,000,000 x 96 unit vectors from
 a Zipf-weighted mixture (nlist 4096), 20,000 queries with per-query nprobe round(exp U[ln 8,
-ln 512]) and k in {10, 100} (two calls per step, one per k), each form in graph mode (one
-captured HIP graph per call) with its autotuned item length; recall of each form against the
-plain path's results at the same per-query nprobe.
+ln 512]) and k in {10, 100}, all in one search call per step (per-query nprobe and k), each form
+autotuned on the batch (tile shapes / MFMA item length) and run as one captured HIP graph; recall
+of each form against the plain path's results at the same per-query nprobe and k.
 """
 import argparse
 import json
@@ -161,8 +161,9 @@ def bench_skewed(args):
 
 
 def bench_uneven(args):
-    """The zipf-96d-10m workload: uneven per-query nprobe and k, graph-captured searches."""
+    """The zipf-96d-10m workload: per-query nprobe and k, the whole batch in one graph-captured search."""
     from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP8, HV_PRECISION_FP32, datagen, metrics
+    from harmony_amd.autotune import autotune
     from harmony_amd.index import DeviceArray, Index
     w = datagen.WORKLOADS[4]
     d, nlist, B = w["d"], w["nlist"], w["dim_blocks"]
@@ -180,43 +181,43 @@ def bench_uneven(args):
     log("index build %.1fs; list sizes min %d median %d max %d" % (time.perf_counter() - t0, sizes.min(),
                                                                    int(np.median(sizes)), sizes.max()))
     del xb
-    groups = []
-    for kk in (10, 100):
-        rows = np.where(k_q == kk)[0]
-        q = np.ascontiguousarray(xq[rows])
-        groups.append({"k": kk, "rows": rows, "nprobe": np.ascontiguousarray(nprobe_q[rows]),
-                       "q": DeviceArray.from_host(q), "D": DeviceArray(rows.size * kk * 4),
-                       "I": DeviceArray(rows.size * kk * 8), "host_q": q})
+    kmax = int(k_q.max())
+    q_dev, D_dev, I_dev = DeviceArray.from_host(xq), DeviceArray(nq * kmax * 4), DeviceArray(nq * kmax * 8)
+    index.set_query_nprobe(nprobe_q)
+    index.set_query_k(k_q)
 
-    def run_step():
-        for g in groups:
-            index.set_query_nprobe(g["nprobe"])
-            index.search_device(g["q"], g["rows"].size, g["k"], g["D"], g["I"])
-
-    forms, ref = {}, {}
+    forms, ref = {}, None
     for name, prec, r in (("plain-fp32", HV_PRECISION_FP32, 4), ("bf16", HV_PRECISION_BF16, 4),
                           ("fp8", HV_PRECISION_FP8, 2)):
         index.preSearch(warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=B, precision=prec,
                         rerank_r=r, graph=True, scan_vsplit=0, tile_shape=0, tile_shape_first=0)
+        tuned = None
+        if not args.no_autotune:
+            tuned, _ = autotune(index, xq, kmax, sample=nq, reps=3)
+            index._params.graph = 1  # one captured graph per search, whatever the tuner timed
+            index._push()
+            log("%s autotuned: %s" % (name, tuned))
         for _ in range(max(2, args.warmup)):  # graph mode: sizing call, capture
-            run_step()
+            index.search_device(q_dev, nq, kmax, D_dev, I_dev)
         times = []
         for _ in range(args.steps):
             t0 = time.perf_counter()
-            run_step()
+            index.search_device(q_dev, nq, kmax, D_dev, I_dev)
             times.append(time.perf_counter() - t0)
         ms = 1000.0 * float(np.mean(times))
-        res = [g["I"].to_host(np.int64, (g["rows"].size, g["k"])) for g in groups]
+        res = I_dev.to_host(np.int64, (nq, kmax))
         if name == "plain-fp32":
             ref = res
-        rec = float(np.mean([metrics.recall(a, b) for a, b in zip(res, ref)]))
+        rec = float(np.mean([metrics.recall(res[k_q == kk][:, :kk], ref[k_q == kk][:, :kk])
+                             for kk in np.unique(k_q)]))
         forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 3),
-                       "recall_vs_plain": round(rec, 6)}
+                       "recall_vs_plain": round(rec, 6), "autotuned": tuned, "calls_per_step": 1}
         if prec == HV_PRECISION_FP8:
             forms[name]["rerank_r"] = r
-        log("%s: %.3f ms per step (%d queries, two calls), %.0f QPS, recall vs plain %.5f"
+        log("%s: %.3f ms per step (%d queries, one graph launch), %.0f QPS, recall vs plain %.5f"
             % (name, ms, nq, nq / (ms / 1000.0), rec))
     index.set_query_nprobe(None)
+    index.set_query_k(None)
     best = max(forms, key=lambda f: forms[f]["qps"] if forms[f]["recall_vs_plain"] >= 0.95 else 0)
     out = {
         "metric": "QPS, 10M x 96 Zipf mixture, per-query nprobe in [8, 512], k in {10, 100}",
@@ -225,8 +226,8 @@ def bench_uneven(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "fp8+f32" if best == "fp8" else "bf16+f32",
         "data": "synthetic",
         "config": {"workload": "zipf-96d-10m: %d x %d unit vectors, nlist %d, %d queries, per-query nprobe "
-                               "round(exp U[ln 8, ln 512]) (mean %.1f), k in {10, 100}, B=%d, graph mode"
-                               % (nb, d, nlist, nq, float(nprobe_q.mean()), B),
+                               "round(exp U[ln 8, ln 512]) (mean %.1f) and k in {10, 100} in one search call, "
+                               "B=%d, graph mode" % (nb, d, nlist, nq, float(nprobe_q.mean()), B),
                    "form": best, "forms": forms},
     }
     print(json.dumps(out), flush=True)
@@ -524,6 +525,8 @@ def main():
              "roofline": roof, "kernels": cands}
         if use_learned:
             f["mean_probes"] = round(float(index.last_nprobe().mean()), 4)
+            f["widened_queries"] = int(avg["widened_queries"])
+            f["widen_margin"] = float(index._params.learned_margin)
         if precision == HV_PRECISION_FP8:
             f["rerank_r"] = rerank_r
         if h2h and not args.no_host_to_host:
@@ -583,15 +586,15 @@ def main():
         learned_info = {"train_queries": int(feats.shape[0]), "trace_s": round(t1 - t0, 2),
                         "train_s": round(t2 - t1, 2), "epochs": 20, "loss_first": float(losses[0]),
                         "loss_last": float(losses[-1]), "nprobe_cap": cap, "nprobe_scale": s_np,
-                        "bound_scale": round(s_b, 4), "validation_recall": round(rec_v, 5),
-                        "validation_mean_probes": round(probes_v, 3)}
+                        "bound_scale": round(s_b, 4), "widen_margin": float(index._params.learned_margin),
+                        "validation_recall": round(rec_v, 5), "validation_mean_probes": round(probes_v, 3)}
         log("predictor: traces %.1fs, training %.1fs (loss %.4f -> %.4f), calibration: nprobe scale %.2f, "
             "bound scale %.3f, validation recall %.4f, mean probes %.2f"
             % (t1 - t0, t2 - t1, losses[0], losses[-1], s_np, s_b, rec_v, probes_v))
         f, _ = measure("learned-bf16", HV_PRECISION_BF16, use_learned=True, nprobe=cap, h2h=True)
         forms["learned-bf16"] = f
-        op["learned-bf16"] = {"nprobe_cap": cap, "nprobe_scale": s_np, "mean_probes": f["mean_probes"],
-                              "recall_at_100": f["recall_at_100"]}
+        op["learned-bf16"] = {"nprobe_cap": cap, "nprobe_scale": s_np, "widen_margin": learned_info["widen_margin"],
+                              "mean_probes": f["mean_probes"], "recall_at_100": f["recall_at_100"]}
         best = "learned-bf16"
         # what the network gains over the method without it at equal recall: the BF16 form at the
         # smallest fixed nprobe reaching each target, against the learned form calibrated to it
@@ -602,7 +605,8 @@ def main():
             fixed, _ = measure("bf16@%g" % tgt, HV_PRECISION_BF16, nprobe=min(okp))
             s2 = learned.calibrate(index, vf, vt, xv, vexact, nprobe=cap, k=k, target_recall=tgt, margin=0.003)
             lf, _ = measure("learned-bf16@%g" % tgt, HV_PRECISION_BF16, use_learned=True, nprobe=cap)
-            gain.append({"target": tgt, "fixed_nprobe": min(okp), "fixed_qps": fixed["qps"],
+            gain.append({"target": tgt, "widen_margin": float(index._params.learned_margin),
+                         "fixed_nprobe": min(okp), "fixed_qps": fixed["qps"],
                          "fixed_recall": fixed["recall_at_100"], "fixed_candidates": fixed["candidates"],
                          "learned_nprobe_scale": s2[0], "learned_qps": lf["qps"], "learned_recall": lf["recall_at_100"],
                          "learned_candidates": lf["candidates"], "learned_mean_probes": lf["mean_probes"],

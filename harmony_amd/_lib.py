@@ -55,7 +55,7 @@ class SearchParams(ctypes.Structure):
                 ("tile_shape_first", ctypes.c_int32), ("learned", ctypes.c_int32),
                 ("learned_nprobe_scale", ctypes.c_float), ("learned_bound_scale", ctypes.c_float),
                 ("learned_min_nprobe", ctypes.c_int32), ("rerank_r", ctypes.c_int32),
-                ("graph", ctypes.c_int32), ("scan_vsplit", ctypes.c_int32)]
+                ("graph", ctypes.c_int32), ("scan_vsplit", ctypes.c_int32), ("learned_margin", ctypes.c_float)]
 
 
 class Stats(ctypes.Structure):
@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
                [("ms_main", ctypes.c_double), ("main_pairs", ctypes.c_int64), ("fallback_queries", ctypes.c_int64),
                 ("ms_rerank", ctypes.c_double), ("rerank_pairs", ctypes.c_int64),
                 ("ms_scan_kernels", ctypes.c_double), ("scan_launches", ctypes.c_int64),
-                ("scan_pairs", ctypes.c_int64)]
+                ("scan_pairs", ctypes.c_int64), ("widened_queries", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -91,6 +91,7 @@ SIGNATURES = {
     "hv_get_search_params": (ctypes.c_int, [_P, ctypes.POINTER(SearchParams)]),
     "hv_set_external_bounds": (ctypes.c_int, [_P, _I64, _P]),
     "hv_set_query_nprobe": (ctypes.c_int, [_P, _I64, _P]),
+    "hv_set_query_k": (ctypes.c_int, [_P, _I64, _P]),
     "hv_search": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_search_device": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),
     "hv_search_device_noalloc": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Stats)]),

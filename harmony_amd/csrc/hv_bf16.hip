@@ -653,13 +653,15 @@ constexpr int kCompactWaveBig = 4096;  // then with 64 per lane (same kernel); l
 
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
                                                       float* __restrict__ aux,
-                                                      unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
-                                                      float* __restrict__ bound, unsigned int* __restrict__ last) {
+                                                      unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
+                                                      float* __restrict__ bound, unsigned int* __restrict__ last,
+                                                      const int* __restrict__ kq, int kmul) {
     __shared__ unsigned int s_hist[256];
     __shared__ unsigned int s_red[2][4];
     __shared__ unsigned int s_sel[2];
     __shared__ unsigned int s_out;
     const int64_t q = blockIdx.x;
+    const int k = kq ? min(k_all, kmul * hv_query_k(kq, q, k_all)) : k_all;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
@@ -848,12 +850,14 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
 
 __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* __restrict__ surv,
                                                            float* __restrict__ aux,
-                                                           unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
-                                                           float* __restrict__ bound, unsigned int* __restrict__ last) {
+                                                           unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
+                                                           float* __restrict__ bound, unsigned int* __restrict__ last,
+                                                           const int* __restrict__ kq, int kmul) {
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= nq) return;
+    const int k = kq ? min(k_all, kmul * hv_query_k(kq, q, k_all)) : k_all;
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     if (n > kCompactWaveBig) return;  // k_bf16_compact
@@ -875,13 +879,13 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
 }
 
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
-                         float* bound, unsigned int* last, hipStream_t s) {
+                         float* bound, unsigned int* last, const int* kq, int kmul, hipStream_t s) {
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
-    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last);
+    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last, kq, kmul);
     if (cap > kCompactWaveBig)
         hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound,
-                           last);
+                           last, kq, kmul);
 }
 
 // FP8 shortlist: each query keeps its kr smallest estimates (keys' distance bits; every key tied
@@ -929,12 +933,13 @@ __device__ __forceinline__ void shortlist_wave_e(unsigned long long* __restrict_
 
 template <bool BIG>
 __global__ __launch_bounds__(256) void k_shortlist(unsigned long long* __restrict__ surv,
-                                                   unsigned int* __restrict__ cnt, int64_t nq, int cap, int kr,
-                                                   float* __restrict__ bound) {
+                                                   unsigned int* __restrict__ cnt, int64_t nq, int cap, int kr_all,
+                                                   float* __restrict__ bound, const int* __restrict__ kq, int kmul) {
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= nq) return;
+    const int kr = kq ? min(kr_all, kmul * hv_query_k(kq, q, kr_all)) : kr_all;
     const unsigned int c = cnt[q];
     if (c > (unsigned int)cap || c <= (unsigned int)kr) return;
     const int n = (int)c;
@@ -954,12 +959,12 @@ __global__ __launch_bounds__(256) void k_shortlist(unsigned long long* __restric
 }
 
 void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, float* bound,
-                      hipStream_t s) {
+                      const int* kq, int kmul, hipStream_t s) {
     if (nq <= 0) return;
     const dim3 grid((unsigned)((nq + 3) / 4));
-    hipLaunchKernelGGL(k_shortlist<false>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr, bound);
+    hipLaunchKernelGGL(k_shortlist<false>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr, bound, kq, kmul);
     if (cap > kCompactWaveMax)
-        hipLaunchKernelGGL(k_shortlist<true>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr, bound);
+        hipLaunchKernelGGL(k_shortlist<true>, grid, dim3(256), 0, s, surv, cnt, nq, cap, kr, bound, kq, kmul);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -43,6 +43,11 @@ __device__ void block_bitonic_sort(T* s, int n) {
     __syncthreads();
 }
 
+// A query's own k when per-query counts are given (clamped to [1, k]), else k.
+__device__ __forceinline__ int hv_query_k(const int* kq, int64_t q, int k) {
+    return kq ? max(1, min(kq[q], k)) : k;
+}
+
 __host__ __device__ inline int hv_next_pow2(int v) {
     int p = 1;
     while (p < v) p <<= 1;

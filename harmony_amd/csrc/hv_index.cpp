@@ -128,6 +128,7 @@ struct SearchPost {
     unsigned long long* stats = nullptr;
     int* redo_cnt = nullptr;
     std::function<void(int)> redo;
+    std::function<void()> truncate;
 };
 
 // A search captured as a HIP graph, replayed while the signature and the buffers match
@@ -304,6 +305,7 @@ struct hv_index {
     DevBuf d_codes, d_list_off, d_code_off, d_list_stride, d_ids, d_rank_order;
     // BF16 residual codes per list (MFMA operand order), their FP32 norms, and the FP32 rows in list order
     DevBuf d_rcodes, d_roff, d_rnorm, d_rerr, d_rows;
+    DevBuf d_lrad;  // each list's radius around its centroid (the learned search's margin test)
     int dpad = 0, bf16_grid = 0;
     // FP8 (e4m3) codes per list in the same order and their per-list inverse scales, built on the
     // first FP8-form search after an add
@@ -316,6 +318,9 @@ struct hv_index {
     std::vector<int32_t> query_nprobe;  // per-query probe counts (empty: nprobe for all)
     PinnedBuf qn_host;                  // the clamped counts of the current search
     std::vector<int32_t> qn_stage;      // the same for a no-allocation search (sized when set)
+    std::vector<int32_t> query_k;       // per-query k (empty: the search's k for all)
+    PinnedBuf qk_host;                  // their clamped values for the current search
+    std::vector<int32_t> qk_stage;      // the same for a no-allocation search
     // no-allocation search in progress: scratch comes from the static arena, graphs are off;
     // size_only: stop after sizing the scratch (the no-allocation entry point's first pass)
     bool noalloc = false, size_only = false;
@@ -327,6 +332,7 @@ struct hv_index {
     DevBuf m_params, m_grad, m_adam_m, m_adam_v, m_w1b, m_w2b, m_norm;
 
     // search scratch (grow-only)
+    DevBuf s_qk;  // per-query k of the current search
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_done, s_stats, s_pdist, s_pair_slot,
         s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_qnorm, s_cnorm, s_redo, s_qscale, s_last;
@@ -437,6 +443,9 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
                                h->d_rcodes.as<__bf16>(), h->d_rnorm.as<float>(), h->d_rerr.as<float>(), h->stream);
         hv::launch_gather_rows(xp, n, (int)d, order_buf.as<int>(), h->d_rows.as<float>(), h->stream);
     }
+    HV_HIP(h->d_lrad.ensure(sizeof(float) * nlist));
+    hv::launch_list_radius(h->d_rnorm.as<float>(), h->d_list_off.as<int64_t>(), (int)nlist, h->d_lrad.as<float>(),
+                           h->stream);
     HV_CHECK_LAUNCH("bf16 layout");
     HV_HIP(hipStreamSynchronize(h->stream));
     order_buf.release();
@@ -500,13 +509,14 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
 void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, bool timed,
                    std::chrono::steady_clock::time_point t0) {
     hipStream_t s = h->stream;
-    unsigned long long hs[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long hs[15] = {};
     int nredo = 0;
     HV_HIP(hipMemcpyAsync(hs, p.stats, sizeof(hs), hipMemcpyDeviceToHost, s));
     HV_HIP(hipMemcpyAsync(&nredo, p.redo_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
     if (nredo > p.batch) {
         for (int q0 = p.batch; q0 < nredo; q0 += p.batch) p.redo(q0);
+        if (p.truncate) p.truncate();
         HV_CHECK_LAUNCH("exact redo");
         HV_HIP(hipStreamSynchronize(s));
     }
@@ -551,6 +561,7 @@ void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, 
         st->survivors += (int64_t)hs[2];
         st->overflow_queries += (int64_t)hs[12];
         st->fallback_queries += (int64_t)hs[13];
+        st->widened_queries += (int64_t)hs[14];
         if (p.main_recorded) st->main_pairs += (int64_t)hs[8];
         st->scan_pairs += (int64_t)(hs[0] + hs[4] + hs[8]);
         if (p.mfma) st->rerank_pairs += (int64_t)hs[2];  // the select sees exactly the re-ranked pairs
@@ -584,6 +595,11 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     if (learned)
         require(h->mlp_params && h->mlp_norm, HV_ERR_STATE,
                 "learned search needs a predictor (hv_mlp_init/train or set_params, and hv_mlp_set_norm)");
+    const bool kq_on = !h->query_k.empty();
+    if (kq_on) {
+        require((int64_t)h->query_k.size() == n, HV_ERR_STATE, "per-query k not set for this batch");
+        require(!learned, HV_ERR_INVALID_ARG, "per-query k and the learned predictor are exclusive");
+    }
     const bool fp8 = h->sp.precision == HV_PRECISION_FP8;
     const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
     const bool mfma = bf16 || fp8;
@@ -628,6 +644,19 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         for (int64_t i = 0; i < n; ++i) qh[i] = (int)std::min<int64_t>(nprobe, std::max<int64_t>(1, h->query_nprobe[i]));
         return qh;
     };
+    // per-query k, clamped to [1, k], staged the same way
+    auto stage_qk = [&]() -> const int* {
+        int* kh;
+        if (h->noalloc) {
+            require((int64_t)h->qk_stage.size() >= n, HV_ERR_STATE, "per-query k staging not sized");
+            kh = h->qk_stage.data();
+        } else {
+            HV_HIP(h->qk_host.ensure(sizeof(int) * n));
+            kh = static_cast<int*>(h->qk_host.p);
+        }
+        for (int64_t i = 0; i < n; ++i) kh[i] = (int)std::min<int64_t>(k, std::max<int64_t>(1, h->query_k[i]));
+        return kh;
+    };
     // graph mode: the device work of a search is captured on the second call with the same
     // signature (the first one sizes every buffer) and replayed from then on
     const bool graph_ok = h->sp.graph != 0 && !external && !h->noalloc;
@@ -637,13 +666,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             const unsigned char* c = static_cast<const unsigned char*>(v);
             sig.insert(sig.end(), c, c + b);
         };
-        const int64_t shape[6] = {n, k, h->ntotal, h->max_list, (int64_t)uneven, (int64_t)h->fp8_built};
+        const int64_t shape[7] = {n, k, h->ntotal, h->max_list, (int64_t)uneven, (int64_t)h->fp8_built, (int64_t)kq_on};
         const void* ptrs[3] = {q_dev, D, I};
         put(shape, sizeof(shape));
         put(ptrs, sizeof(ptrs));
         put(&h->sp, sizeof(h->sp));
         if (h->graph.exec && h->graph.sig == sig && h->graph.gen == g_alloc_gen.load()) {
             if (uneven) stage_qn();
+            if (kq_on) stage_qk();
             HV_HIP(hipEventRecord(h->ev[0], s));
             HV_HIP(hipGraphLaunch(h->graph.exec, s));
             HV_HIP(hipEventRecord(h->ev[5], s));
@@ -709,7 +739,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (!h->perm.empty()) need(h->s_qp, 4 * n * d);
         need(h->s_bucket, 4 * nb);
         need(h->s_fill, 4 * nb);
-        need(h->s_stats, 8 * 14);
+        need(h->s_stats, 8 * 15);
         need(h->s_surv_cnt, 4 * n);
         need(h->s_bound, 4 * n);
         need(h->s_bound_ref, 4 * n);
@@ -732,6 +762,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         } else if (uneven) {
             need(h->s_qn, 4 * n);
         }
+        if (kq_on) need(h->s_qk, 4 * n);
         need(h->s_bucket_off, 4 * (nb + 1));
         need(h->s_scan_tmp, 4 * hv::scan_tmp_ints(nb));
         need(h->s_pairs, 4 * n * nprobe);
@@ -790,8 +821,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_bucket.ensure(sizeof(int) * nb));
     HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
     // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
-    // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 14));
+    // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos, [14] learned
+    // queries whose probe count the margin test widened
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 15));
     HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
     HV_HIP(h->s_bound.ensure(sizeof(float) * n));
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
@@ -808,7 +840,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         };
         add(h->s_bucket.p, nb, 0);
         add(h->s_fill.p, nb, 0);
-        add(h->s_stats.p, 28, 0);
+        add(h->s_stats.p, 30, 0);
         add(h->s_surv_cnt.p, n, 0);
         add(h->s_overflow.p, 2, 0);  // redo count, re-rank item count
         if (coarse_on_mfma(h, nprobe)) add(coarse_counter(h, n), 1, 0);
@@ -834,12 +866,18 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
         HV_HIP(h->s_seed.ensure(sizeof(float) * n));
         HV_HIP(h->s_pred.ensure(sizeof(float) * n * 2));
-        hv::launch_mlp_serve(pdist, (int)nprobe, Qp, (int)d, n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
+        // margin widening only where the search verifies the seed afterwards (FP32 and BF16); the
+        // slack covers the FP32 roundings of the coarse distances, residual norms and square roots
+        const float slack = (float)(4.0 * d * std::ldexp(1.0, -24) + std::ldexp(1.0, -20));
+        hv::launch_mlp_serve(pdist, probes, (int)nprobe, Qp, (int)d, n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
                              h->m_params.as<float>(), h->m_norm.as<float>(), h->sp.learned_nprobe_scale,
                              h->sp.learned_bound_scale, std::max(1, h->sp.learned_min_nprobe), (int)nprobe_cap,
+                             h->d_lrad.as<float>(), h->d_centroids.as<float>(),
+                             learned_seed ? h->sp.learned_margin : 0.f, slack,
                              h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(),
                              learned_seed ? h->s_bound.as<float>() : nullptr,
-                             learned_seed ? h->s_bound_ref.as<float>() : nullptr, s);
+                             learned_seed ? h->s_bound_ref.as<float>() : nullptr, h->s_stats.as<unsigned long long>() + 14,
+                             s);
         HV_CHECK_LAUNCH("predictor");
         qn = h->s_qn.as<int>();
     } else if (uneven) {
@@ -847,6 +885,12 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
         HV_HIP(hipMemcpyAsync(h->s_qn.p, qh, sizeof(int) * n, hipMemcpyHostToDevice, s));
         qn = h->s_qn.as<int>();
+    }
+    const int* kq = nullptr;
+    if (kq_on) {
+        const int* kh = stage_qk();
+        HV_HIP(hipMemcpyAsync(h->s_qk.p, kh, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        kq = h->s_qk.as<int>();
     }
     rec(1);
     // 3. grouping by (list, rank)
@@ -995,7 +1039,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         HV_HIP(hipMemcpyAsync(bound, bound_ref, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     } else if (wl > 0 && !mfma && !learned) {
         scan_phase(0, (int)wl, (int)wls, 0, 1, stats + 4);
-        hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, s);
+        hv::launch_kth_survivors(surv, surv_cnt, n, (int)cap, (int)k, bound_ref, bound, kq, s);
     }  // otherwise both start at FLT_MAX (the fill above)
     HV_CHECK_LAUNCH("bound");
     rec(3);
@@ -1010,7 +1054,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         for (int64_t r0 = 0; r0 < rank_end;) {
             const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
             mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, r0 == 0 ? stats : stats + 8);
-            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, s);
+            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s);
             r0 = r1;
         }
         HV_CHECK_LAUNCH("compact");
@@ -1021,7 +1065,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         const int vdirect = (int)(cap / 16 * 16);
         if (vdirect > 0) mfma_phase(0, 1, 0, vdirect, 1, stats);
         hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
-        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, last, s);
+        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, last, kq, (int)(kr / k), s);
         HV_CHECK_LAUNCH("compact");
         HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
@@ -1035,7 +1079,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             for (int64_t r0 = 1; r0 < nprobe;) {
                 const int64_t r1 = std::min<int64_t>(nprobe, r0 * 4);
                 mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, stats + 8);
-                hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, s);
+                hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s);
                 r0 = r1;
             }
             HV_CHECK_LAUNCH("compact");
@@ -1047,17 +1091,17 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             for (int64_t r0 = 1; r0 < rank_end;) {
                 const int64_t r1 = std::min<int64_t>(rank_end, r0 * 4);
                 mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, stats + 8);
-                hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
+                hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, kq, (int)(kr / k), s);
                 r0 = r1;
             }
-            if (rank_end == 1) hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, s);
+            if (rank_end == 1) hv::launch_shortlist(surv, surv_cnt, n, (int)cap, (int)kr, bound, kq, (int)(kr / k), s);
             HV_CHECK_LAUNCH("shortlist");
         }
     } else {
         int r_next = 0;
         if ((h->sp.refine_bound || learned) && rank_end > 1) {
             scan_phase(0, 1, INT32_MAX, prune, 0, stats);
-            hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, s);
+            hv::launch_compact_topk(surv, surv_cnt, n, (int)cap, (int)k, bound, kq, s);
             HV_CHECK_LAUNCH("compact");
             r_next = 1;
         }
@@ -1095,9 +1139,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     //    adds batches only when the final count exceeds one.
     // (the check is against the bound the seed set, bound_ref: later compactions only tighten
     //  the working bound towards the k-th distance itself)
+    // (a learned query that is redone is redone over all nprobe_cap lists: its result is then the
+    //  plain search's at nprobe_cap, and its reported probe count says so)
     hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
-                         redo_cnt, stats + 12, s);
-    const float* redo_bound = fp8 || learned ? nullptr : bound;
+                         redo_cnt, stats + 12, learned ? h->s_qn.as<int>() : nullptr, (int)nprobe_cap,
+                         learned && !fp8 ? bound : nullptr, s);
+    const int* redo_qn = learned ? nullptr : qn;
+    // learned (FP32, BF16): each redone query's own k-th result (written above); FP8: no bound
+    const float* redo_bound = fp8 ? nullptr : bound;
     post.batch = batch;
     post.main_recorded = main_recorded;
     post.mfma = mfma;
@@ -1111,11 +1160,15 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     post.redo = [=](int q0) {
         hv::launch_query_exact(redo_list, redo_cnt, q0, batch, Qp, (int)d, h->d_codes.as<float>(),
                                h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(), h->d_list_stride.as<int>(),
-                               h->d_ids.as<uint32_t>(), probes, (int)nprobe, (int)nprobe_cap, qn, nseg, plan, prune,
-                               redo_bound, (int)k, runs, D, I, q0 == 0, s);  // the first batch's counts: cleared above
+                               h->d_ids.as<uint32_t>(), probes, (int)nprobe, (int)nprobe_cap, redo_qn, nseg, plan,
+                               prune, redo_bound, (int)k, runs, D, I, q0 == 0, s);  // the first batch's counts: cleared above
     };
     post.redo(0);
     HV_CHECK_LAUNCH("exact redo");
+    // per-query k: each row cut to its query's k (again after any redo batch the host adds)
+    post.truncate = [=]() { hv::launch_truncate_rows(D, I, n, (int)k, kq, s); };
+    post.truncate();
+    HV_CHECK_LAUNCH("truncate");
     if (capture) {
         hipGraph_t g = nullptr;
         guard.active = false;
@@ -1201,6 +1254,7 @@ void hv_destroy(hv_index_t* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->graph.reset();
     h->qn_host.release();
+    h->qk_host.release();
     DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
@@ -1208,7 +1262,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
                       &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_qnorm,
                       &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale, &h->s_last,
-                      &h->d_rcodes8, &h->d_lscale8};
+                      &h->d_rcodes8, &h->d_lscale8, &h->d_lrad, &h->s_qk};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1556,6 +1610,7 @@ void hv_default_search_params(hv_search_params* p) {
     p->learned_nprobe_scale = 1.0f;
     p->learned_bound_scale = 1.0f;
     p->learned_min_nprobe = 1;
+    p->learned_margin = 1.0f;
 }
 
 int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
@@ -1581,6 +1636,7 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
         require(p->learned == 0 || p->learned == 1, HV_ERR_INVALID_ARG, "learned must be 0 or 1");
         require(p->learned_nprobe_scale > 0.f && p->learned_min_nprobe >= 1, HV_ERR_INVALID_ARG,
                 "learned_nprobe_scale must be > 0 and learned_min_nprobe >= 1");
+        require(p->learned_margin >= 0.f, HV_ERR_INVALID_ARG, "learned_margin must be >= 0");
         h->sp = *p;
     });
 }
@@ -1605,6 +1661,16 @@ int hv_set_query_nprobe(hv_index_t* h, int64_t n, const int32_t* nprobe) {
         require(n >= 0 && (nprobe || n == 0), HV_ERR_INVALID_ARG, "bad per-query nprobe");
         h->query_nprobe.assign(nprobe, nprobe + n);
         h->qn_stage.assign(n, 0);
+    });
+}
+
+int hv_set_query_k(hv_index_t* h, int64_t n, const int32_t* k) {
+    if (!h) return HV_ERR_INVALID_ARG;
+    return guarded(h, [&] {
+        require(n >= 0 && (k || n == 0), HV_ERR_INVALID_ARG, "bad per-query k");
+        for (int64_t i = 0; i < n; ++i) require(k[i] >= 1, HV_ERR_INVALID_ARG, "per-query k must be >= 1");
+        h->query_k.assign(k, k + n);
+        h->qk_stage.assign(n, 0);
     });
 }
 
@@ -1634,7 +1700,7 @@ int hv_search_device_noalloc(hv_index_t* h, int64_t n, const float* q_dev, int64
                           &h->s_stats, &h->s_surv_cnt, &h->s_bound, &h->s_bound_ref, &h->s_overflow, &h->s_last,
                           &h->s_runs, &h->s_probes, &h->s_pdist, &h->s_qn, &h->s_seed, &h->s_pred, &h->s_bucket_off,
                           &h->s_scan_tmp, &h->s_pairs, &h->s_pair_slot, &h->s_items, &h->s_queue, &h->s_surv,
-                          &h->s_aux, &h->s_qres, &h->s_qscale, &h->s_ritems, &h->s_kept, &h->s_done};
+                          &h->s_aux, &h->s_qres, &h->s_qscale, &h->s_ritems, &h->s_kept, &h->s_done, &h->s_qk};
         constexpr int nbufs = (int)(sizeof(bufs) / sizeof(bufs[0]));
         std::lock_guard<std::mutex> lk(g_scratch_mu[h->device]);
         struct Swap {

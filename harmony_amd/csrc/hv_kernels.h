@@ -40,10 +40,13 @@ void launch_argmin_rows(const float* dist, int64_t n, int m, int* labels, float*
 
 // --- search path
 void launch_permute_rows(const float* src, int64_t n, int d, const int* perm, float* dst, hipStream_t s);
+// kq (optional, here and below): each query's own k (<= k); the bound is taken at the query's k
 void launch_kth_survivors(const unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k,
-                          float* bound_ref, float* bound, hipStream_t s);
+                          float* bound_ref, float* bound, const int* kq, hipStream_t s);
 void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k, float* bound,
-                         hipStream_t s);
+                         const int* kq, hipStream_t s);
+// entries at or past each query's own k -> (FLT_MAX, -1); nothing when kq is null
+void launch_truncate_rows(float* D, int64_t* I, int64_t n, int k, const int* kq, hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 // up to kFillSegs fills of 32-bit words (pointer, word count, value) in one launch
@@ -139,15 +142,17 @@ void launch_mfma_scan(int grid, int elem, const void* qres, const float* qaux, i
                       int cap, unsigned long long* stats, hipStream_t s);
 // FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place;
 // bound[q] drops to the kr-th estimate
+// with kq: each query keeps min(kr, kmul * k_q)
 void launch_shortlist(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int kr, float* bound,
-                      hipStream_t s);
+                      const int* kq, int kmul, hipStream_t s);
 // counts after the direct-slot scan of the nearest lists (vectors v < cap)
 void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_t* list_off, int cap,
                          unsigned int* cnt, hipStream_t s);
 // last (optional, nq counts; start at 0xffffffff): the count each query's previous compaction
 // left; a query whose count still equals it is skipped
+// with kq: the bound at each query's min(k, kmul * k_q)-th upper bound
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
-                         float* bound, unsigned int* last, hipStream_t s);
+                         float* bound, unsigned int* last, const int* kq, int kmul, hipStream_t s);
 // re-rank + final top-k of the MFMA forms, in three launches:
 //  launch_rerank_items: work items of 64 stored pairs (items: nq * ceil(cap/64) entries, n_items
 //    zeroed by the caller); overflowed queries listed when overflow_list is given; stats[2]
@@ -169,8 +174,11 @@ void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt,
 // the queries the exact path redoes -> list / list_cnt (both zeroed by the caller): overflowed
 // survivor buffers (cnt[q] > cap; counts[0]) and, with check_bound, queries whose k-th result is
 // missing or above bound[q] (counts[1])
+// qn_out (optional): each listed query's probe count is set to qn_full (its redo covers them all);
+// bound_out (optional): its bound to the k-th result it has (FLT_MAX when overflowed or short)
 void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
-                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, hipStream_t s);
+                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, int* qn_out,
+                      int qn_full, float* bound_out, hipStream_t s);
 // the workgroup select alone: queries whose done flag is 0 (with done), else those with more than
 // kWaveSortMax survivors
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
@@ -191,11 +199,17 @@ void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s);
 void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf16* w2, const float* P,
                         const float* norm, float* out, hipStream_t s);
 // features from pdist [n][pc] (ascending coarse distances) and the queries Qp [n][d], as
-// launch_features forms them; bound2, bound3 (optional): more copies of the seeded bound
-void launch_mlp_serve(const float* pdist, int pc, const float* Qp, int d, int64_t n, const __bf16* w1,
-                      const __bf16* w2, const float* P, const float* norm, float nprobe_scale, float bound_scale,
-                      int min_nprobe, int nprobe, int* qn, float* bound, float* pred, float* bound2, float* bound3,
-                      hipStream_t s);
+// launch_features forms them; bound2, bound3 (optional): more copies of the seeded bound. With
+// margin > 0, lrad (list radii) and cent (centroids), each query's count widens to every list of
+// its first nprobe probes whose triangle or Voronoi lower bound is within the seed (widened:
+// count of such queries).
+void launch_mlp_serve(const float* pdist, const int* probes, int pc, const float* Qp, int d, int64_t n,
+                      const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
+                      float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
+                      float margin, float slack, int* qn, float* bound, float* pred, float* bound2, float* bound3,
+                      unsigned long long* widened, hipStream_t s);
+// lrad[l] = max over list l of sqrt(rnorm) (the list's radius around its centroid)
+void launch_list_radius(const float* rnorm, const int64_t* list_off, int nlist, float* lrad, hipStream_t s);
 
 // --- coarse quantizer on the matrix cores with an exact boundary re-check (hv_coarse.hip)
 void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t s);

@@ -234,18 +234,33 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 }
 
 // Serving: per query, the probe count qn = clamp(ceil(exp(y1) * nprobe_scale), min_nprobe,
-// nprobe) and the seeded bound expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also
+// nprobe) and the seeded bound R = expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also
 // into bound2 and bound3 when given (the search's working and reference bounds start there).
 // The features are formed here, in LDS, from the coarse distances (pc per query, ascending) and
 // the query, with k_features' arithmetic (so the same bits).
-__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdist, int pc,
-                                                   const float* __restrict__ Qp, int d, int64_t n,
+//
+// Margin widening (margin > 0, with a finite R): a list l past the predicted count can hold a
+// vector within sqrt(R) of q only if both lower bounds of |q - x| over its vectors x are <= sqrt(R):
+//  - the triangle bound |q - c_l| - rad_l (rad_l = the largest |x - c_l| of the list);
+//  - the Voronoi bound (|q - c_l|^2 - |q - c_0|^2) / (2 |c_l - c_0|), c_0 = the query's nearest
+//    centroid: every x of list l is at least as close to c_l as to c_0 (vectors are filed under
+//    their nearest centroid), so it lies beyond the bisector of c_0 and c_l, whose distance from
+//    q is that quotient.
+// qn grows to cover every list of the first nprobe whose larger bound g satisfies g <= 0 or
+// g^2 <= R * margin, with relative slack for the FP32 roundings of the distances, norms and
+// square roots involved. With margin >= 1, and once the search has checked that the query's k-th
+// result is <= R, no list left out holds a vector of the top k over all nprobe lists: the query's
+// result is the plain search's at nprobe. Queries whose count grew are counted in *widened.
+__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdist, const int* __restrict__ probes,
+                                                   int pc, const float* __restrict__ Qp, int d, int64_t n,
                                                    const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
                                                    const float* __restrict__ P, const float* __restrict__ norm,
                                                    float nprobe_scale, float bound_scale, int min_nprobe, int nprobe,
+                                                   const float* __restrict__ lrad, const float* __restrict__ cent,
+                                                   float margin, float slack,
                                                    int* __restrict__ qn, float* __restrict__ bound,
                                                    float* __restrict__ pred, float* __restrict__ bound2,
-                                                   float* __restrict__ bound3) {
+                                                   float* __restrict__ bound3, unsigned long long* __restrict__ widened) {
     __shared__ float s_f[4][kIn];
     const int w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -262,12 +277,37 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdi
     __threadfence_block();  // every lane reads the whole row its wavefront wrote
     float y0, y1;
     mlp_eval_bf16(s_f[w], lane, w1, w2, P, norm, y0, y1);
+    // every lane holds y0, y1 (reduced across the wavefront), so every lane forms the same values
+    const float np = expf(y1) * nprobe_scale;
+    const int c0 = np >= (float)nprobe ? nprobe : max(min_nprobe, (int)ceilf(np));
+    const int qpred = min(max(c0, 1), nprobe);
+    const float b =
+        bound_scale > 0.f ? fminf(expm1f(fmaxf(y0, 0.f)) * bound_scale, 3.402823466e38f) : 3.402823466e38f;
+    int c = qpred;
+    if (margin > 0.f && b < 3.402823466e38f && lrad != nullptr) {
+        const float thr = b * margin * (1.f + slack);
+        const float d0 = fmaxf(pdist[q * pc], 0.f);
+        const float* c0 = cent + (int64_t)probes[q * pc] * d;
+        for (int j = qpred; j < nprobe; ++j) {  // wave-uniform loop over the ranks past the prediction
+            const int l = probes[q * pc + j];
+            const float dl = fmaxf(pdist[q * pc + j], 0.f);
+            const float* cl = cent + (int64_t)l * d;
+            float e = 0.f;
+            for (int i = lane; i < d; i += 64) {
+                const float t = cl[i] - c0[i];
+                e = __builtin_fmaf(t, t, e);
+            }
+            for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+            const float rb = sqrtf(dl) * (1.f - slack) - lrad[l] * (1.f + slack);
+            const float num = (dl - d0) - 2.f * slack * (dl + d0);
+            const float vb = e > 0.f ? num / (2.f * sqrtf(e) * (1.f + slack)) : 0.f;
+            const float g = fmaxf(rb, vb);
+            if (g <= 0.f || g * g <= thr) c = j + 1;
+        }
+    }
     if (lane == 0) {
-        const float np = expf(y1) * nprobe_scale;
-        const int c = np >= (float)nprobe ? nprobe : max(min_nprobe, (int)ceilf(np));
-        qn[q] = min(max(c, 1), nprobe);
-        const float b =
-            bound_scale > 0.f ? fminf(expm1f(fmaxf(y0, 0.f)) * bound_scale, 3.402823466e38f) : 3.402823466e38f;
+        qn[q] = c;
+        if (c > qpred && widened) atomicAdd(widened, 1ull);
         bound[q] = b;
         if (bound2) bound2[q] = b;
         if (bound3) bound3[q] = b;
@@ -276,13 +316,33 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdi
     }
 }
 
-void launch_mlp_serve(const float* pdist, int pc, const float* Qp, int d, int64_t n, const __bf16* w1,
-                      const __bf16* w2, const float* P, const float* norm, float nprobe_scale, float bound_scale,
-                      int min_nprobe, int nprobe, int* qn, float* bound, float* pred, float* bound2, float* bound3,
-                      hipStream_t s) {
+void launch_mlp_serve(const float* pdist, const int* probes, int pc, const float* Qp, int d, int64_t n,
+                      const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
+                      float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
+                      float margin, float slack, int* qn, float* bound, float* pred, float* bound2, float* bound3,
+                      unsigned long long* widened, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, pdist, pc, Qp, d, n, w1, w2, P,
-                       norm, nprobe_scale, bound_scale, min_nprobe, nprobe, qn, bound, pred, bound2, bound3);
+    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, pdist, probes, pc, Qp, d, n, w1,
+                       w2, P, norm, nprobe_scale, bound_scale, min_nprobe, nprobe, lrad, cent, margin, slack, qn, bound,
+                       pred, bound2, bound3, widened);
+}
+
+// Radius of every list: the largest |x - c| over its vectors, from their squared residual norms
+// (one wavefront per list; 0 for an empty list).
+__global__ __launch_bounds__(256) void k_list_radius(const float* __restrict__ rnorm, const int64_t* __restrict__ list_off,
+                                                     int nlist, float* __restrict__ lrad) {
+    const int l = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (l >= nlist) return;
+    const int lane = threadIdx.x & 63;
+    float m = 0.f;
+    for (int64_t i = list_off[l] + lane; i < list_off[l + 1]; i += 64) m = fmaxf(m, rnorm[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0) lrad[l] = sqrtf(m);
+}
+
+void launch_list_radius(const float* rnorm, const int64_t* list_off, int nlist, float* lrad, hipStream_t s) {
+    if (nlist <= 0) return;
+    hipLaunchKernelGGL(k_list_radius, dim3((unsigned)((nlist + 3) / 4)), dim3(256), 0, s, rnorm, list_off, nlist, lrad);
 }
 
 }  // namespace hv

@@ -61,10 +61,12 @@ void launch_select_rows(const float* dist, int64_t n, int m, int p, int* out_ids
 // the k-th smallest of the query's warm-up distances, FLT_MAX when fewer than k were seen.
 // Resets the query's candidate count for the scan that follows.
 __global__ __launch_bounds__(256) void k_kth_wave(const unsigned long long* __restrict__ surv,
-                                                  unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
-                                                  float* __restrict__ bound_ref, float* __restrict__ bound) {
+                                                  unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
+                                                  float* __restrict__ bound_ref, float* __restrict__ bound,
+                                                  const int* __restrict__ kq) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
+    const int k = hv_query_k(kq, q, k_all);
     const int n = (int)min(cnt[q], (unsigned int)cap);
     if (n > kWaveSortMax) return;  // k_kth_block
     const unsigned long long* __restrict__ src = surv + q * cap;
@@ -81,10 +83,12 @@ __global__ __launch_bounds__(256) void k_kth_wave(const unsigned long long* __re
 }
 
 __global__ __launch_bounds__(256) void k_kth_block(const unsigned long long* __restrict__ surv,
-                                                   unsigned int* __restrict__ cnt, int cap, int k,
-                                                   float* __restrict__ bound_ref, float* __restrict__ bound) {
+                                                   unsigned int* __restrict__ cnt, int cap, int k_all,
+                                                   float* __restrict__ bound_ref, float* __restrict__ bound,
+                                                   const int* __restrict__ kq) {
     extern __shared__ unsigned long long s_keys[];
     const int64_t q = blockIdx.x;
+    const int k = hv_query_k(kq, q, k_all);
     const int n = (int)min(cnt[q], (unsigned int)cap);
     if (n <= kWaveSortMax) return;  // k_kth_wave
     const int P = hv_next_pow2(n);
@@ -97,14 +101,15 @@ __global__ __launch_bounds__(256) void k_kth_block(const unsigned long long* __r
 }
 
 void launch_kth_survivors(const unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k,
-                          float* bound_ref, float* bound, hipStream_t s) {
+                          float* bound_ref, float* bound, const int* kq, hipStream_t s) {
     if (nq <= 0) return;
     if (cap > kWaveSortMax) {  // before the wave kernel resets the counts
         const size_t lds = sizeof(unsigned long long) * hv_next_pow2(cap);
-        hipLaunchKernelGGL(k_kth_block, dim3((unsigned)nq), dim3(256), lds, s, surv, cnt, cap, k, bound_ref, bound);
+        hipLaunchKernelGGL(k_kth_block, dim3((unsigned)nq), dim3(256), lds, s, surv, cnt, cap, k, bound_ref, bound,
+                           kq);
     }
     hipLaunchKernelGGL(k_kth_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, cnt, nq, cap, k, bound_ref,
-                       bound);
+                       bound, kq);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -115,10 +120,11 @@ void launch_kth_survivors(const unsigned long long* surv, unsigned int* cnt, int
 // overflowed keeps its keys (it goes to the exact path) but still gets the tighter bound of
 // the keys that were stored.
 __global__ __launch_bounds__(256) void k_compact_wave(unsigned long long* __restrict__ surv,
-                                                      unsigned int* __restrict__ cnt, int64_t nq, int cap, int k,
-                                                      float* __restrict__ bound) {
+                                                      unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
+                                                      float* __restrict__ bound, const int* __restrict__ kq) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
+    const int k = hv_query_k(kq, q, k_all);
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     if (n == 0 || n > kWaveSortMax) return;
@@ -135,10 +141,11 @@ __global__ __launch_bounds__(256) void k_compact_wave(unsigned long long* __rest
 }
 
 __global__ __launch_bounds__(256) void k_compact_block(unsigned long long* __restrict__ surv,
-                                                       unsigned int* __restrict__ cnt, int cap, int k,
-                                                       float* __restrict__ bound) {
+                                                       unsigned int* __restrict__ cnt, int cap, int k_all,
+                                                       float* __restrict__ bound, const int* __restrict__ kq) {
     extern __shared__ unsigned long long s_keys[];
     const int64_t q = blockIdx.x;
+    const int k = hv_query_k(kq, q, k_all);
     const unsigned int c = cnt[q];
     const int n = (int)min(c, (unsigned int)cap);
     if (n <= kWaveSortMax) return;  // k_compact_wave
@@ -155,13 +162,32 @@ __global__ __launch_bounds__(256) void k_compact_block(unsigned long long* __res
 }
 
 void launch_compact_topk(unsigned long long* surv, unsigned int* cnt, int64_t nq, int cap, int k, float* bound,
-                         hipStream_t s) {
+                         const int* kq, hipStream_t s) {
     if (nq <= 0) return;
     if (cap > kWaveSortMax) {  // before the wave kernel shrinks the counts
         const size_t lds = sizeof(unsigned long long) * hv_next_pow2(cap);
-        hipLaunchKernelGGL(k_compact_block, dim3((unsigned)nq), dim3(256), lds, s, surv, cnt, cap, k, bound);
+        hipLaunchKernelGGL(k_compact_block, dim3((unsigned)nq), dim3(256), lds, s, surv, cnt, cap, k, bound, kq);
     }
-    hipLaunchKernelGGL(k_compact_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, cnt, nq, cap, k, bound);
+    hipLaunchKernelGGL(k_compact_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, cnt, nq, cap, k, bound,
+                       kq);
+}
+
+// Per-query k: entries at or past each query's own k become (FLT_MAX, -1). The top k_q by
+// (distance, id) is the first k_q of the top k, so a search at k cut this way is the search at k_q.
+__global__ __launch_bounds__(256) void k_truncate_rows(float* __restrict__ D, int64_t* __restrict__ I, int64_t n, int k,
+                                                       const int* __restrict__ kq) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t q = e / k;
+    if ((int)(e - q * k) >= hv_query_k(kq, q, k)) {
+        D[e] = kFltMax;
+        I[e] = -1;
+    }
+}
+
+void launch_truncate_rows(float* D, int64_t* I, int64_t n, int k, const int* kq, hipStream_t s) {
+    if (n <= 0 || !kq) return;
+    hipLaunchKernelGGL(k_truncate_rows, dim3((unsigned)((n * k + 255) / 256)), dim3(256), 0, s, D, I, n, k, kq);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -221,7 +247,8 @@ __global__ __launch_bounds__(256) void k_redo_list(const unsigned int* __restric
                                                    const float* __restrict__ D, const int64_t* __restrict__ I, int k,
                                                    const float* __restrict__ bound, int check_bound, int64_t n,
                                                    int* __restrict__ list, int* __restrict__ list_cnt,
-                                                   unsigned long long* __restrict__ counts) {
+                                                   unsigned long long* __restrict__ counts, int* __restrict__ qn_out,
+                                                   int qn_full, float* __restrict__ bound_out) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     bool ovf = false, redo = false;
@@ -238,14 +265,20 @@ __global__ __launch_bounds__(256) void k_redo_list(const unsigned int* __restric
         if (m != mo) atomicAdd(&counts[1], (unsigned long long)__builtin_popcountll(m & ~mo));
     }
     base = __shfl(base, 0, 64);
-    if (redo) list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
+    if (redo) {
+        list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
+        if (qn_out) qn_out[q] = qn_full;
+        // the query's own k-th result bounds the k-th over a superset of its lists
+        if (bound_out) bound_out[q] = (!ovf && I[q * k + k - 1] >= 0) ? D[q * k + k - 1] : 3.402823466e38f;
+    }
 }
 
 void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
-                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, hipStream_t s) {
+                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, int* qn_out,
+                      int qn_full, float* bound_out, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_redo_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cnt, cap, D, I, k, bound,
-                       check_bound, n, list, list_cnt, counts);
+                       check_bound, n, list, list_cnt, counts, qn_out, qn_full, bound_out);
 }
 
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,

@@ -119,7 +119,7 @@ class Index:
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
                   survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None,
                   learned=None, learned_nprobe_scale=None, learned_bound_scale=None, learned_min_nprobe=None,
-                  rerank_r=None, graph=None, scan_vsplit=None):
+                  rerank_r=None, graph=None, scan_vsplit=None, learned_margin=None):
         """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59).
 
         precision: HV_PRECISION_FP32 (default), HV_PRECISION_BF16 (MFMA filter + FP32 distances
@@ -129,8 +129,11 @@ class Index:
         learned (None keeps the current settings): per-query probe count and pruning bound from
         the trained predictor (mlp_*): probes = ceil(predicted nprobe * learned_nprobe_scale),
         capped at nprobe; bound = predicted k-th distance * learned_bound_scale (<= 0: none).
-        Queries whose k-th result lands above the bound the search used are redone exactly over
-        the same lists with the bound of their nearest list (no seed)."""
+        learned_margin (FP32 and BF16 forms): a query's probe count also grows to every list of
+        its first nprobe whose triangle lower bound (coarse distance minus the list's radius) is
+        within sqrt(margin * seeded bound); a query whose k-th result lands above its seeded bound
+        is redone exactly over all nprobe lists. With margin >= 1 (the default) every learned
+        result is the plain search's at nprobe; 0 keeps the predicted counts alone."""
         p = self._params
         if precision is not None:
             p.precision = int(precision)
@@ -142,6 +145,8 @@ class Index:
             p.learned_bound_scale = float(learned_bound_scale)
         if learned_min_nprobe is not None:
             p.learned_min_nprobe = int(learned_min_nprobe)
+        if learned_margin is not None:
+            p.learned_margin = float(learned_margin)
         if rerank_r is not None:
             p.rerank_r = int(rerank_r)
         if graph is not None:
@@ -168,6 +173,12 @@ class Index:
         nprobe_per_query[i] lists (clamped to [1, nprobe])."""
         a = np.zeros(0, np.int32) if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.int32)
         check(self._h, lib().hv_set_query_nprobe(self._h, a.size, _ptr(a)))
+
+    def set_query_k(self, k_per_query):
+        """Per-query k for the next searches (None clears): row i holds query i's k_i results,
+        then (FLT_MAX, -1) padding (k_i clamped to [1, k] of the search)."""
+        a = np.zeros(0, np.int32) if k_per_query is None else np.ascontiguousarray(k_per_query, np.int32)
+        check(self._h, lib().hv_set_query_k(self._h, a.size, _ptr(a)))
 
     # -- build
     def train(self, x, niter=10, seed=0, max_points_per_centroid=256):

@@ -3,9 +3,10 @@
 The predictor (a 65 -> 128 -> 2 MLP run by the library's own HIP kernels) reads a query's
 distances to its 64 nearest centroids and its norm, and predicts the k-th neighbour distance and
 the number of lists needed to reach a target recall. Serving uses the prediction to trim each
-query's probe list and to seed its pruning bound; a query whose k-th result lands above the
-seeded bound is redone exactly over the same lists without the seed (the plain path's
-arithmetic), and a predicted nprobe at or above the cap falls back to the plain nprobe. No
+query's probe list and to seed its pruning bound. The margin test widens a query's probe count to
+every list (of its first nprobe) that the triangle inequality cannot rule out within the seeded
+bound, and a query whose k-th result lands above its seed is redone exactly over all nprobe lists
+(the plain path), so with the default margin every result is the plain search's at nprobe. No
 counterpart in the reference, which probes a fixed nprobe for every query and takes its bound
 from a warm-up scan.
 
@@ -99,27 +100,41 @@ def recall_per_query(I, exact):
 
 
 def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.95, margin=0.01,
-              bound_quantile=1.0, bound_margin=1.25, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0)):
-    """Serving scales from held-out queries: the bound scale is bound_margin times the
+              bound_quantile=1.0, bound_margin=1.25,
+              scales=(0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0),
+              widen_margins=(0.0, 0.5, 1.0)):
+    """Serving settings from held-out queries. The bound scale is bound_margin times the
     bound_quantile of true / predicted k-th distance, so that nearly every query keeps its seeded
-    bound (a query whose seed is below its k-th distance falls back to the exact path, ~50 us for
-    a handful of queries, while a looser seed costs little: the compaction after each scan phase
-    lowers the bound to the k-th upper bound anyway, so only the first phase's appends grow). The
-    nprobe scale is the smallest of `scales` whose mean recall against the exact ids reaches
-    target_recall + margin. Leaves the index in learned mode with those scales; returns
-    (nprobe_scale, bound_scale, recall, mean probes)."""
+    bound (a query whose seed is below its k-th distance is redone exactly over all nprobe lists,
+    ~50 us for a handful of queries, while a looser seed costs little: the compaction after each
+    scan phase lowers the bound to the k-th upper bound anyway). Then, for each widening margin
+    (0: the predicted counts alone; 1: counts widened to every list the triangle and Voronoi bounds
+    cannot exclude within the seed, so results equal the plain search's at nprobe), the smallest of
+    `scales` whose mean recall against the exact ids reaches target_recall + margin; of those the
+    setting with the fewest candidates. Leaves the index in learned mode with it; returns
+    (nprobe_scale, bound_scale, recall, mean probes); the chosen widening margin is in
+    index._params.learned_margin."""
     pred = index.mlp_predict(feats)
     ratio = np.expm1(targets[:, 0].astype(np.float64)) / np.maximum(np.expm1(pred[:, 0].astype(np.float64)), 1e-30)
     bound_scale = float(max(1.0, bound_margin * np.quantile(ratio, bound_quantile)))
     index.nprobe = nprobe
-    best = None
-    for s in scales:
-        index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale)
-        _, I = search_dev(index, queries, k)
-        rec = float(recall_per_query(I, exact).mean())
-        probes = float(index.last_nprobe(queries.shape[0]).mean())
-        best = (s, bound_scale, rec, probes)
-        if rec >= target_recall + margin:
-            break
-    index.preSearch(learned=True, learned_nprobe_scale=best[0], learned_bound_scale=best[1])
-    return best
+    q = np.ascontiguousarray(queries, dtype=np.float32)
+    n = q.shape[0]
+    qd, Dd, Id = DeviceArray.from_host(q), DeviceArray(n * k * 4), DeviceArray(n * k * 8)
+    best, fallback = None, None
+    for wm in widen_margins:
+        for s in scales:
+            index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale, learned_margin=wm)
+            st = index.search_device(qd, n, k, Dd, Id)
+            I = Id.to_host(np.int64, (n, k))
+            rec = float(recall_per_query(I, exact).mean())
+            cand = (st["candidates"], s, wm, rec, float(index.last_nprobe().mean()))
+            fallback = cand if fallback is None or rec > fallback[3] else fallback
+            if rec >= target_recall + margin:
+                if best is None or cand[0] < best[0]:
+                    best = cand
+                break
+    pick = best if best is not None else fallback
+    _, s, wm, rec, probes = pick
+    index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale, learned_margin=wm)
+    return s, bound_scale, rec, probes

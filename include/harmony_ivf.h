@@ -99,6 +99,12 @@ typedef struct {
     int32_t graph;
     /* vectors per work item of the MFMA filter scans (a multiple of 16; 0 = 256) */
     int32_t scan_vsplit;
+    /* learned search, FP32 and BF16 forms: a query's probe count grows past the prediction to every
+     * list of its first nprobe whose lower bound sqrt(|q - c|^2) - radius (radius = the list's
+     * largest |x - c|) is within sqrt(margin * seeded bound). 0: the prediction alone; >= 1 (default
+     * 1): with the search's check that the k-th result is <= the seed (else the query is redone
+     * over all nprobe lists), every learned result equals the plain search's at nprobe. */
+    float learned_margin;
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */
@@ -126,6 +132,7 @@ typedef struct {
                                 scan, warm-up included, or the MFMA filter scan), eager searches only */
     int64_t scan_launches;   /* those launches */
     int64_t scan_pairs;      /* (query, vector) pairs they covered, unpruned (warm-up included) */
+    int64_t widened_queries; /* learned search: queries whose probe count the margin test widened */
 } hv_stats;
 
 /* ---- lifetime (Index::Index, Index.h:15-25) ---- */
@@ -154,6 +161,12 @@ int hv_set_external_bounds(hv_index_t* h, int64_t n, const float* bounds);
  * clears. No counterpart in the reference (one nprobe for every query, Index.h:117); not
  * combined with the learned predictor, which sets its own counts. */
 int hv_set_query_nprobe(hv_index_t* h, int64_t n, const int32_t* nprobe);
+/* per-query k for the next searches of n queries (each clamped to [1, k] of the search); n = 0
+ * clears. Row q of D / I then holds the query's k_q results, padded with (FLT_MAX, -1) after them:
+ * the same as a search of that query alone at k = k_q. Bounds are taken at each query's own k.
+ * No counterpart in the reference (one k per call, Index.h:82); not combined with the learned
+ * predictor (trained for one k). */
+int hv_set_query_k(hv_index_t* h, int64_t n, const int32_t* k);
 /* host buffers: q [n][d] f32 row-major, D [n][k] f32, I [n][k] int64 */
 int hv_search(hv_index_t* h, int64_t n, const float* q, int64_t k, float* D, int64_t* I, hv_stats* st);
 /* device buffers (already in HBM): same layouts */
