@@ -20,6 +20,9 @@ HV_ERR_NO_DEVICE = -6
 HV_METRIC_IP = 0
 HV_METRIC_L2 = 1
 HV_OPT_NONE = 0
+HV_OPT_TRIANGLE = 1
+HV_OPT_SUBNN_L2 = 2
+HV_OPT_SUBNN_IP = 4
 HV_OPT_ALL = 7
 HV_BOUND_WARMUP = 0
 HV_BOUND_EXTERNAL = 1
@@ -104,6 +107,7 @@ SIGNATURES = {
     "hv_get_centroids": (ctypes.c_int, [_P, _P]),
     "hv_get_list_offsets": (ctypes.c_int, [_P, _P]),
     "hv_get_list_ids": (ctypes.c_int, [_P, _P]),
+    "hv_get_list_c2c": (ctypes.c_int, [_P, _P]),
     "hv_get_list_codes": (ctypes.c_int, [_P, _P]),
     "hv_last_n": (_I64, [_P]),
     "hv_get_last_probes": (ctypes.c_int, [_P, _P, _I64]),

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libharmony_amd.so")
 ARCH = os.environ.get("HV_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["hv_build.hip", "hv_scan.hip", "hv_select.hip", "hv_gt.hip", "hv_bf16.hip", "hv_learn.hip",
-           "hv_coarse.hip", "hv_index.cpp"]
+           "hv_coarse.hip", "hv_ip.hip", "hv_index.cpp"]
 HEADERS = ["hv_device.h", "hv_kernels.h", os.path.join("..", "..", "include", "harmony_ivf.h")]
 
 # -packed-fp32-ops off: hipcc -O3 would otherwise pair FP32 ops into v_pk_* forms.

@@ -20,6 +20,20 @@ __device__ __forceinline__ int64_t hv_key_id(uint64_t k) {
     return lo == HV_ID_NONE ? (int64_t)-1 : (int64_t)lo;
 }
 
+// Inner-product keys: the high word is a decreasing map of the FP32 inner product onto uint32
+// (larger products first), the low word the id, so ascending keys are (inner product descending,
+// id ascending); -0.0 counts as +0.0. HV_IP_KEY_PAD sorts after every real key.
+#define HV_IP_KEY_PAD 0xffffffffffffffffull
+__device__ __forceinline__ uint64_t hv_ip_key(float ip, uint32_t id) {
+    const uint32_t b = __float_as_uint(ip == 0.f ? 0.f : ip);
+    const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);  // increasing with ip
+    return ((uint64_t)(~o) << 32) | (uint64_t)id;
+}
+__device__ __forceinline__ float hv_ip_key_value(uint64_t k) {
+    const uint32_t o = ~(uint32_t)(k >> 32);
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
 // Workgroup-wide ascending bitonic sort of n keys in LDS (n a power of two, n >= 2).
 // Every thread of the workgroup must call it. Ends with a barrier.
 template <typename T>

@@ -306,6 +306,11 @@ struct hv_index {
     // BF16 residual codes per list (MFMA operand order), their FP32 norms, and the FP32 rows in list order
     DevBuf d_rcodes, d_roff, d_rnorm, d_rerr, d_rows;
     DevBuf d_lrad;  // each list's radius around its centroid (the learned search's margin test)
+    // inner-product metric: each vector's inner product with its centroid, list order (descending
+    // within a list when c2c_sorted), the candidate2centroid array of the reference's IVF
+    DevBuf d_c2c;
+    std::vector<float> c2c;
+    bool c2c_sorted = false;
     int dpad = 0, bf16_grid = 0;
     // FP8 (e4m3) codes per list in the same order and their per-list inverse scales, built on the
     // first FP8-form search after an add
@@ -375,10 +380,17 @@ void assign_rows(hv_index_t* h, const float* xp, int64_t n, int* labels, float* 
     HV_HIP(h->s_dist.ensure(sizeof(float) * per * h->nlist));
     for (int64_t r0 = 0; r0 < n; r0 += per) {
         const int64_t rows = std::min(per, n - r0);
-        hv::launch_pair_l2_full(xp + r0 * h->d, rows, h->d_centroids.as<float>(), h->nlist, (int)h->d,
-                                h->s_dist.as<float>(), h->nlist, h->stream);
-        hv::launch_argmin_rows(h->s_dist.as<float>(), rows, (int)h->nlist, labels + r0, best ? best + r0 : nullptr,
-                               h->stream);
+        if (h->metric == HV_METRIC_IP) {  // the largest inner product, ties to the lower id
+            hv::launch_ip_rows(xp + r0 * h->d, rows, h->d_centroids.as<float>(), h->nlist, (int)h->d,
+                               h->s_dist.as<float>(), h->nlist, h->stream);
+            hv::launch_argmax_rows(h->s_dist.as<float>(), rows, (int)h->nlist, labels + r0, best ? best + r0 : nullptr,
+                                   h->stream);
+        } else {
+            hv::launch_pair_l2_full(xp + r0 * h->d, rows, h->d_centroids.as<float>(), h->nlist, (int)h->d,
+                                    h->s_dist.as<float>(), h->nlist, h->stream);
+            hv::launch_argmin_rows(h->s_dist.as<float>(), rows, (int)h->nlist, labels + r0, best ? best + r0 : nullptr,
+                                   h->stream);
+        }
         HV_CHECK_LAUNCH("assignment");
     }
 }
@@ -397,7 +409,7 @@ const float* upload_rows(hv_index_t* h, const float* x, int64_t n, DevBuf& raw, 
 // gathered through order[] into the list-major, dimension-major slabs; off[nlist+1] delimits
 // the lists and ids[] (list order) are the vector ids.
 void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order, const std::vector<int64_t>& off,
-                   const std::vector<int64_t>& ids) {
+                   const std::vector<int64_t>& ids, const std::vector<float>* c2c = nullptr) {
     const int64_t d = h->d, nlist = h->nlist, n = (int64_t)order.size();
     h->list_off = off;
     h->list_stride.assign(nlist, 0);
@@ -442,6 +454,18 @@ void install_lists(hv_index_t* h, const float* xp, const std::vector<int>& order
                                h->d_centroids.as<float>(), h->d_roff.as<int64_t>(), (int)nlist,
                                h->d_rcodes.as<__bf16>(), h->d_rnorm.as<float>(), h->d_rerr.as<float>(), h->stream);
         hv::launch_gather_rows(xp, n, (int)d, order_buf.as<int>(), h->d_rows.as<float>(), h->stream);
+    }
+    if (h->metric == HV_METRIC_IP) {
+        // candidate2centroid in list order; the triangle band needs it descending within each list
+        h->c2c = c2c ? *c2c : std::vector<float>(n, 0.f);
+        h->c2c_sorted = c2c != nullptr;
+        for (int64_t l = 0; l < nlist && h->c2c_sorted; ++l)
+            for (int64_t t = off[l] + 1; t < off[l + 1]; ++t)
+                if (!(h->c2c[t] <= h->c2c[t - 1])) {
+                    h->c2c_sorted = false;
+                    break;
+                }
+        upload(h->d_c2c, h->c2c.data(), sizeof(float) * std::max<int64_t>(1, n), h->stream);
     }
     HV_HIP(h->d_lrad.ensure(sizeof(float) * nlist));
     hv::launch_list_radius(h->d_rnorm.as<float>(), h->d_list_off.as<int64_t>(), (int)nlist, h->d_lrad.as<float>(),
@@ -569,7 +593,121 @@ void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, 
     }
 }
 
+// Inner-product search: Index::single_thread_search's IP branch (Index.cpp:1440-1449, 1509-1551)
+// on the GPU. The nprobe centroids with the largest inner product (exact FP32 chains, ties to the
+// lower id), then the exact top-k of the probed lists by (inner product descending, id ascending).
+// With opt level OPT_TRIANGLE a query's lists are scanned only over the angular band a result
+// better than its current k-th needs: the k-th after its nearest list (a first pass), so every
+// list is cut against a value at most the final k-th (the reference cuts each list against the
+// running k-th of the lists before it). For unit vectors the cut drops only vectors that cannot be
+// results, so the results equal the unpruned search's.
+void do_search_ip(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D, int64_t* I, hv_stats* st) {
+    require(h->trained && h->added, HV_ERR_STATE, "search before train/add");
+    require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
+    require(k >= 1 && k <= hv::kMaxK, HV_ERR_CAPACITY, "k must be in [1, 1024]");
+    require((h->sp.opt_level & h->added_opt_level) == h->sp.opt_level, HV_ERR_INVALID_ARG,
+            "opt_level is not subset of added_opt_level");  // Index.cpp:1642-1645
+    if (n == 0) return;
+    require(q_dev && D && I, HV_ERR_INVALID_ARG, "null buffer");
+    require(h->sp.precision == HV_PRECISION_FP32 && !h->sp.learned && h->query_nprobe.empty() && h->query_k.empty() &&
+                h->sp.bound_mode == HV_BOUND_WARMUP,
+            HV_ERR_INVALID_ARG,
+            "the inner-product metric runs the FP32 form with one nprobe and one k per call (no learned "
+            "predictor, no external bound)");
+    require(!h->noalloc, HV_ERR_INVALID_ARG, "the no-allocation search is L2 only");
+    const int64_t d = h->d, nlist = h->nlist;
+    const int64_t nprobe = std::min<int64_t>(h->sp.nprobe, nlist);
+    require(nprobe >= 1, HV_ERR_INVALID_ARG, "nprobe must be >= 1");
+    require(nlist <= hv::kMaxSelectRow, HV_ERR_CAPACITY, "nlist > 8192");
+    require((int64_t)n * nprobe < (1ll << 31), HV_ERR_CAPACITY, "n * nprobe >= 2^31");
+    const bool tri = (h->sp.opt_level & HV_OPT_TRIANGLE) && h->c2c_sorted;
+    hipStream_t s = h->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    HV_HIP(hipEventRecord(h->ev[0], s));
+    const float* Qp = q_dev;
+    if (!h->perm.empty()) {
+        HV_HIP(h->s_qp.ensure(sizeof(float) * n * d));
+        hv::launch_permute_rows(q_dev, n, (int)d, h->d_perm.as<int>(), h->s_qp.as<float>(), s);
+        Qp = h->s_qp.as<float>();
+    }
+    // coarse: inner-product rows in slices, the nprobe largest of each
+    HV_HIP(h->s_probes.ensure(sizeof(int) * n * nprobe));
+    HV_HIP(h->s_pdist.ensure(sizeof(float) * n * nprobe));
+    const int64_t per = coarse_rows_per_slice(h, n);
+    HV_HIP(h->s_dist.ensure(sizeof(float) * per * nlist));
+    int* probes = h->s_probes.as<int>();
+    float* pval = h->s_pdist.as<float>();
+    for (int64_t r0 = 0; r0 < n; r0 += per) {
+        const int64_t rows = std::min(per, n - r0);
+        hv::launch_ip_rows(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d, h->s_dist.as<float>(), nlist,
+                           s);
+        hv::launch_ip_select_rows(h->s_dist.as<float>(), rows, (int)nlist, (int)nprobe, probes + r0 * nprobe,
+                                  pval + r0 * nprobe, s);
+    }
+    HV_CHECK_LAUNCH("ip coarse");
+    HV_HIP(hipEventRecord(h->ev[1], s));
+    // exact top-k over the probed lists, query batches sized to the run scratch
+    HV_HIP(h->s_redo.ensure(sizeof(int) * (n + 1)));
+    int* qcount = h->s_redo.as<int>();
+    hv::launch_iota_list(qcount, n, s);
+    const int nseg = hv::exact_segments(h->max_list);
+    const int64_t run_keys = nprobe * nseg * k;
+    const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(n, kRedoRunBytes / (8 * run_keys)));
+    HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
+    HV_HIP(h->s_bound.ensure(sizeof(float) * n));
+    // slack of the band for FP32 roundings: the products (d u each for unit vectors) and the square
+    // roots of 1 - c^2 and 1 - s^2 (an error e there moves them by at most sqrt(e))
+    const double du = (double)d * std::ldexp(1.0, -24);
+    const float slack = (float)(4.0 * std::sqrt(2.0 * du) + 8.0 * du);
+    unsigned long long* runs = h->s_runs.as<unsigned long long>();
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 2));
+    unsigned long long* scanned = h->s_stats.as<unsigned long long>();  // [0] first pass, [1] second
+    hv::launch_fill_u64(scanned, 2, 0, s);
+    for (int64_t q0 = 0; q0 < n; q0 += batch) {
+        auto pass = [&](int r0, int r1, const float* simi, bool append, unsigned long long* cnt) {
+            hv::launch_query_exact_ip(qcount + 1, qcount, (int)q0, batch, Qp, (int)d, h->d_codes.as<float>(),
+                                      h->d_code_off.as<int64_t>(), h->d_list_off.as<int64_t>(),
+                                      h->d_list_stride.as<int>(), h->d_ids.as<uint32_t>(), probes, (int)nprobe, r0, r1,
+                                      nseg, h->d_c2c.as<float>(), pval, simi, slack, (int)k, runs, append, D, I, cnt,
+                                      s);
+        };
+        if (tri && nprobe > 1) {
+            // each query's nearest list first: its k-th bounds the final k-th from below; the
+            // other lists then append their runs to it, each cut to the band that bound allows
+            pass(0, 1, nullptr, false, scanned);
+            hv::launch_ip_simi(qcount + 1, qcount, (int)q0, batch, D, I, (int)k, h->s_bound.as<float>(), s);
+            pass(1, (int)nprobe, h->s_bound.as<float>(), true, scanned + 1);
+        } else {
+            pass(0, (int)nprobe, nullptr, false, scanned + 1);
+        }
+        HV_CHECK_LAUNCH("ip scan");
+    }
+    HV_HIP(hipEventRecord(h->ev[5], s));
+    HV_HIP(hipStreamSynchronize(s));
+    h->last_n = n;
+    h->last_nprobe = nprobe;
+    h->last_stride = nprobe;
+    h->last_learned = false;
+    if (st) {
+        float mc = 0.f, ms = 0.f;
+        HV_HIP(hipEventElapsedTime(&mc, h->ev[0], h->ev[1]));
+        HV_HIP(hipEventElapsedTime(&ms, h->ev[1], h->ev[5]));
+        st->ms_coarse += mc;
+        st->ms_scan += ms;
+        st->ms_scan_kernels += ms;
+        unsigned long long cnt[2] = {0, 0};
+        HV_HIP(hipMemcpy(cnt, scanned, sizeof(cnt), hipMemcpyDeviceToHost));
+        st->scan_launches += tri && nprobe > 1 ? 2 : 1;
+        st->scan_pairs += (int64_t)(cnt[0] + cnt[1]);
+        st->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
 void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D, int64_t* I, hv_stats* st) {
+    if (h->metric == HV_METRIC_IP) {
+        do_search_ip(h, n, q_dev, k, D, I, st);
+        return;
+    }
     require(h->trained && h->added, HV_ERR_STATE, "search before train/add");
     require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
     require(k >= 1 && k <= hv::kMaxK, HV_ERR_CAPACITY, "k must be in [1, 1024]");
@@ -1201,8 +1339,8 @@ hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_leve
         g_create_error = "d and nlist must be positive";
         return nullptr;
     }
-    if (metric != HV_METRIC_L2) {
-        g_create_error = "only METRIC_L2 is implemented";
+    if (metric != HV_METRIC_L2 && metric != HV_METRIC_IP) {
+        g_create_error = "metric must be HV_METRIC_L2 or HV_METRIC_IP";
         return nullptr;
     }
     if (opt_level < 0 || opt_level > HV_OPT_ALL) {
@@ -1262,7 +1400,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
                       &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_qnorm,
                       &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale, &h->s_last,
-                      &h->d_rcodes8, &h->d_lscale8, &h->d_lrad, &h->s_qk};
+                      &h->d_rcodes8, &h->d_lscale8, &h->d_lrad, &h->s_qk, &h->d_c2c};
     for (DevBuf* b : bufs) b->release();
     for (auto& e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1331,8 +1469,15 @@ int hv_train(hv_index_t* h, int64_t n, const float* x, const hv_kmeans_params* k
         require(ns < (1ll << 31), HV_ERR_CAPACITY, "training sample too large");
         std::vector<float> sample(ns * d);
         for (int64_t i = 0; i < ns; ++i) std::memcpy(&sample[i * d], x + rows[i] * d, sizeof(float) * d);
-        DevBuf raw, perm_buf, labels, off_buf, order_buf;
+        DevBuf raw, perm_buf, labels, off_buf, order_buf, xn_buf;
         const float* xp = upload_rows(h, sample.data(), ns, raw, perm_buf);
+        // IP: rows are assigned by <normalized row, centroid> (Clustering.cpp:122-130), averaged raw
+        const float* xa = xp;
+        if (h->metric == HV_METRIC_IP) {
+            HV_HIP(xn_buf.ensure(sizeof(float) * ns * d));
+            hv::launch_normalize_rows(xp, ns, (int)d, xn_buf.as<float>(), h->stream);
+            xa = xn_buf.as<float>();
+        }
         h->centroids.assign(nlist * d, 0.f);
         for (int64_t c = 0; c < nlist; ++c)
             for (int64_t t = 0; t < d; ++t)
@@ -1344,7 +1489,7 @@ int hv_train(hv_index_t* h, int64_t n, const float* x, const hv_kmeans_params* k
         std::vector<int> lab(ns), order(ns);
         std::vector<int64_t> off(nlist + 1);
         for (int64_t it = 0; it < p.niter; ++it) {
-            assign_rows(h, xp, ns, labels.as<int>(), nullptr);
+            assign_rows(h, xa, ns, labels.as<int>(), nullptr);
             HV_HIP(hipMemcpyAsync(lab.data(), labels.p, sizeof(int) * ns, hipMemcpyDeviceToHost, h->stream));
             HV_HIP(hipStreamSynchronize(h->stream));
             // stable counting sort of the sample by cluster (ascending point order inside)
@@ -1360,6 +1505,10 @@ int hv_train(hv_index_t* h, int64_t n, const float* x, const hv_kmeans_params* k
                                            h->d_centroids.as<float>(), h->stream);
             HV_CHECK_LAUNCH("centroid update");
         }
+        if (h->metric == HV_METRIC_IP) {  // unit centroids (Index.cpp:408-418)
+            hv::launch_normalize_centroids(h->d_centroids.as<float>(), (int)nlist, (int)d, h->stream);
+            HV_CHECK_LAUNCH("centroid normalization");
+        }
         HV_HIP(hipMemcpyAsync(h->centroids.data(), h->d_centroids.p, sizeof(float) * nlist * d,
                               hipMemcpyDeviceToHost, h->stream));
         HV_HIP(hipStreamSynchronize(h->stream));
@@ -1368,6 +1517,7 @@ int hv_train(hv_index_t* h, int64_t n, const float* x, const hv_kmeans_params* k
         labels.release();
         off_buf.release();
         order_buf.release();
+        xn_buf.release();
         h->trained = true;
         h->added = false;
     });
@@ -1402,11 +1552,18 @@ int hv_add(hv_index_t* h, int64_t n, const float* x) {
         std::vector<int64_t> cur(off.begin(), off.end() - 1);
         std::vector<int> order(n);
         for (int64_t i = 0; i < n; ++i) order[cur[lab[i]]++] = (int)i;
+        const bool ip = h->metric == HV_METRIC_IP;  // IP: descending inner product (Index.cpp:643-651)
         for (int64_t l = 0; l < nlist; ++l)
             std::sort(order.begin() + off[l], order.begin() + off[l + 1], [&](int a, int b) {
-                return dist[a] != dist[b] ? dist[a] < dist[b] : a < b;
+                if (dist[a] != dist[b]) return ip ? dist[a] > dist[b] : dist[a] < dist[b];
+                return a < b;
             });
-        install_lists(h, xp, order, off, std::vector<int64_t>(order.begin(), order.end()));
+        std::vector<float> c2c;
+        if (ip) {
+            c2c.resize(n);
+            for (int64_t t = 0; t < n; ++t) c2c[t] = dist[order[t]];
+        }
+        install_lists(h, xp, order, off, std::vector<int64_t>(order.begin(), order.end()), ip ? &c2c : nullptr);
         raw.release();
         perm_buf.release();
         h->added_opt_level = h->opt_level;
@@ -1463,23 +1620,30 @@ int hv_save(const hv_index_t* hc, const char* path) {
         write_exact(f, cent.data(), cent.size());
         std::vector<uint64_t> ids;
         std::vector<float> norms;
+        const bool ip = h->metric == HV_METRIC_IP;
         for (int64_t l = 0; l < nlist; ++l) {
             const int64_t b = h->list_off[l], n = h->list_off[l + 1] - b;
             if (n == 0) continue;
             const uint64_t rec[4] = {(uint64_t)l, (uint64_t)n, (uint64_t)d, 0};
-            const int32_t opt = 0;
+            // IP lists carry candidate2centroid (the TRIANGLE arrays, IVF.cpp:87-90)
+            const int32_t opt = ip ? HV_OPT_TRIANGLE : 0;
             write_exact(f, rec, 4);
             write_exact(f, &opt, 1);
             ids.assign(h->ids.begin() + b, h->ids.begin() + b + n);
             write_exact(f, ids.data(), ids.size());
             write_exact(f, codes.data() + b * d, (size_t)(n * d));
-            norms.resize(n);
-            for (int64_t t = 0; t < n; ++t) {  // candidate_norms = <x, x> (Index.cpp:689-696)
+            norms.assign(n, 0.f);  // the reference fills candidate_norms for L2 only (Index.cpp:689-696)
+            for (int64_t t = 0; t < n && !ip; ++t) {
                 float s = 0.f;
                 for (int64_t i = 0; i < d; ++i) s = std::fmaf(codes[(b + t) * d + i], codes[(b + t) * d + i], s);
                 norms[t] = s;
             }
             write_exact(f, norms.data(), norms.size());
+            if (ip) {
+                write_exact(f, h->c2c.data() + b, (size_t)n);
+                const std::vector<float> zeros(n, 0.f);  // sqrt_candidate2centroid: L2 only (Index.cpp:678-686)
+                write_exact(f, zeros.data(), zeros.size());
+            }
         }
     });
 }
@@ -1514,13 +1678,14 @@ int hv_load(hv_index_t* h, const char* path) {
         const int64_t d = (int64_t)hd[0], nlist = (int64_t)hd[1];
         require(d > 0 && d <= 65536 && nlist > 0 && nlist <= (1 << 24), HV_ERR_IO, "implausible d / nlist in header");
         require(en[0] == HV_METRIC_L2 || en[0] == HV_METRIC_IP, HV_ERR_IO, "unknown metric in header");
-        require(en[0] == h->metric, HV_ERR_INVALID_ARG, "index file metric differs from the handle's");
+        const bool ip = en[0] == HV_METRIC_IP;
         require(en[1] >= 0 && en[1] <= HV_OPT_ALL, HV_ERR_IO, "unknown opt level in header");
         need(sizeof(float) * (uint64_t)nlist * d, "centroids");
         std::vector<float> cent(nlist * d);
         read_exact(f, cent.data(), cent.size(), "centroids");
         std::vector<std::vector<uint64_t>> lids(nlist);
-        std::vector<std::vector<float>> lcodes(nlist);
+        std::vector<std::vector<float>> lcodes(nlist), lc2c(nlist);
+        bool all_c2c = true;  // every stored list carries candidate2centroid
         std::vector<char> seen(nlist, 0);
         int64_t total = 0;
         for (;;) {
@@ -1548,15 +1713,33 @@ int hv_load(hv_index_t* h, const char* path) {
             lcodes[lid].resize(n * d);
             read_exact(f, lids[lid].data(), n, "ids");
             read_exact(f, lcodes[lid].data(), n * d, "codes");
-            require(std::fseek(f, (long)(n * 4 + extra), SEEK_CUR) == 0, HV_ERR_IO, "seek failed");
+            require(std::fseek(f, (long)(n * 4), SEEK_CUR) == 0, HV_ERR_IO, "seek failed");  // norms
+            if ((opt & HV_OPT_TRIANGLE) || (opt & HV_OPT_SUBNN_IP)) {
+                lc2c[lid].resize(n);
+                read_exact(f, lc2c[lid].data(), n, "candidate2centroid");
+                extra -= 4 * n;
+            } else {
+                all_c2c = false;
+            }
+            require(std::fseek(f, (long)extra, SEEK_CUR) == 0, HV_ERR_IO, "seek failed");
             total += (int64_t)n;
         }
         require(total < 0xffffffffll, HV_ERR_CAPACITY, "ntotal must be < 2^32 - 1");
-        std::vector<float> rows(total * d);
+        std::vector<float> rows(total * d), c2c;
         std::vector<int64_t> off(nlist + 1, 0), ids(total);
+        if (ip) c2c.resize(total);
         for (int64_t l = 0; l < nlist; ++l) {
             off[l + 1] = off[l] + (int64_t)lids[l].size();
             std::copy(lcodes[l].begin(), lcodes[l].end(), rows.begin() + off[l] * d);
+            for (size_t t = 0; ip && t < lids[l].size(); ++t) {
+                if (all_c2c) {
+                    c2c[off[l] + t] = lc2c[l][t];
+                } else {  // <x, c> as the add computes it (FP32 chain in the file's dimension order)
+                    float acc = 0.f;
+                    for (int64_t i = 0; i < d; ++i) acc = std::fmaf(lcodes[l][t * d + i], cent[l * d + i], acc);
+                    c2c[off[l] + t] = acc;
+                }
+            }
             for (size_t t = 0; t < lids[l].size(); ++t) {
                 require(lids[l][t] < 0xffffffffull, HV_ERR_CAPACITY, "id >= 2^32 - 1");
                 ids[off[l] + t] = (int64_t)lids[l][t];
@@ -1571,6 +1754,7 @@ int hv_load(hv_index_t* h, const char* path) {
         }
         h->d = d;
         h->nlist = nlist;
+        h->metric = en[0];
         h->trained = h->added = false;
         h->opt_level = HV_OPT_NONE;
         h->sp.opt_level = HV_OPT_NONE;
@@ -1583,7 +1767,7 @@ int hv_load(hv_index_t* h, const char* path) {
             const float* xp = total ? upload_rows(h, rows.data(), total, raw, perm_buf) : nullptr;
             std::vector<int> order(total);
             std::iota(order.begin(), order.end(), 0);
-            install_lists(h, xp, order, off, ids);
+            install_lists(h, xp, order, off, ids, ip ? &c2c : nullptr);
             h->added_opt_level = en[1];
         } catch (...) {
             h->trained = h->added = false;
@@ -1783,6 +1967,7 @@ int hv_search(hv_index_t* h, int64_t n, const float* q, int64_t k, float* D, int
 int hv_ground_truth_f64(hv_index_t* h, int64_t n, const float* q, int64_t k, double* D, int64_t* I) {
     if (!h) return HV_ERR_INVALID_ARG;
     return guarded(h, [&] {
+        require(h->metric == HV_METRIC_L2, HV_ERR_INVALID_ARG, "the FP64 ground truth is L2 only");
         require(n >= 0, HV_ERR_INVALID_ARG, "n < 0");
         if (n == 0) return;
         require(q && D && I, HV_ERR_INVALID_ARG, "null buffer");
@@ -1866,6 +2051,14 @@ int hv_get_list_ids(const hv_index_t* h, int64_t* out) {
     if (!h || !out) return HV_ERR_INVALID_ARG;
     if (!h->added) return HV_ERR_STATE;
     std::memcpy(out, h->ids.data(), sizeof(int64_t) * h->ntotal);
+    return HV_OK;
+}
+
+int hv_get_list_c2c(const hv_index_t* h, float* out) {
+    if (!h || !out) return HV_ERR_INVALID_ARG;
+    if (!h->added) return HV_ERR_STATE;
+    if (h->metric != HV_METRIC_IP) return HV_ERR_INVALID_ARG;
+    std::memcpy(out, h->c2c.data(), sizeof(float) * h->ntotal);
     return HV_OK;
 }
 

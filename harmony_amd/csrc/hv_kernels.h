@@ -222,6 +222,30 @@ void launch_coarse_select(const float* approx, const float* Q, const float* C, i
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
                           int* redo_list, int* redo_cnt, hipStream_t s);
 
+// --- inner-product metric (hv_ip.hip; the IP list scan is launch_query_exact_ip)
+void launch_ip_rows(const float* X, int64_t n, const float* C, int64_t nc, int d, float* out, int64_t ldo,
+                    hipStream_t s);
+void launch_argmax_rows(const float* v, int64_t n, int m, int* labels, float* best, hipStream_t s);
+void launch_normalize_rows(const float* x, int64_t n, int d, float* out, hipStream_t s);
+void launch_normalize_centroids(float* c, int nlist, int d, hipStream_t s);
+// the P largest of each row [n][m] (m <= kMaxSelectRow), descending, ties to the lower column
+void launch_ip_select_rows(const float* v, int64_t n, int m, int P, int* ids, float* vals, hipStream_t s);
+void launch_ip_simi(const int* qlist, const int* nql_dev, int q0, int nql_max, const float* D, const int64_t* I, int k,
+                    float* simi, hipStream_t s);
+void launch_iota_list(int* list, int64_t n, hipStream_t s);  // list[0] = n, list[1 + i] = i
+// exact inner-product top-k of the listed queries over probe ranks [r0, r1): keys by (inner product
+// descending, id ascending), D = inner products, padding (-FLT_MAX, -1). The kept keys of every
+// list are runs (nprobe * nseg * k keys per query, then nql_max counts); append keeps the runs of
+// an earlier call (ranks before r0) and D, I are the top-k of all of them. With simi, each list is
+// scanned only over the angular band of candidate2centroid (c2c) a result better than simi[q]
+// needs; scanned (may be null) accumulates the inner products computed.
+void launch_query_exact_ip(const int* qlist, const int* nql_dev, int q0, int nql_max, const float* Qp, int d,
+                           const float* codes, const int64_t* code_off, const int64_t* list_off,
+                           const int* list_stride, const uint32_t* ids, const int* probes, int nprobe, int r0,
+                           int r1, int nseg, const float* c2c, const float* pval, const float* simi, float slack,
+                           int k, unsigned long long* runs, bool append, float* D, int64_t* I,
+                           unsigned long long* scanned, hipStream_t s);
+
 // --- FP64 ground truth
 void launch_gt_f64(const float* Qp, int64_t nq, int d, const float* codes, const int64_t* code_off,
                    const int64_t* list_off, const int* list_stride, const uint32_t* ids, int nlist,

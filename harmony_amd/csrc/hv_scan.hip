@@ -769,6 +769,13 @@ __device__ void topk_reduce(unsigned long long* s, const unsigned long long* s_c
     __syncthreads();
 }
 
+// IP (inner-product metric): keys are hv_ip_key of the FP32 product chain (no bound filter,
+// no dimension blocks); with simi given, a query's list j is scanned only over the vectors whose
+// candidate2centroid (c2c, descending within the list) lies in the angular band that a result
+// better than simi[q] needs (Index.cpp:1515-1546), widened by `slack`:
+//   c = <q, centroid> (pval), lo = simi c - sqrt(1-simi^2) sqrt(1-c^2), hi = simi c + ... (no hi
+//   bound when simi < c); NaN (|q| > 1, or no simi yet) leaves the list whole.
+template <bool IP>
 __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qlist,
                                                      const int* __restrict__ nql_dev, int q0, int nql_max,
                                                      const float* __restrict__ Qp, int d,
@@ -777,10 +784,14 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
                                                      const int64_t* __restrict__ list_off,
                                                      const int* __restrict__ list_stride,
                                                      const uint32_t* __restrict__ ids, const int* __restrict__ probes,
-                                                     int nprobe, int nranks, const int* __restrict__ qn, int nseg,
-                                                     ChunkPlan plan, int prune, const float* __restrict__ bound, int k,
-                                                     unsigned long long* __restrict__ runs,
-                                                     int* __restrict__ run_cnt) {
+                                                     int nprobe, int r0, int nranks, const int* __restrict__ qn,
+                                                     int nseg, ChunkPlan plan, int prune,
+                                                     const float* __restrict__ bound, int k,
+                                                     unsigned long long* __restrict__ runs, int64_t run_stride,
+                                                     int* __restrict__ run_cnt, const float* __restrict__ c2c,
+                                                     const float* __restrict__ pval, const float* __restrict__ simi,
+                                                     float slack, unsigned long long* __restrict__ scanned) {
+    constexpr unsigned long long kPad = IP ? HV_IP_KEY_PAD : HV_KEY_PAD;
     extern __shared__ __align__(16) unsigned char s_raw[];
     // kept keys [kExactTop] | keys collected this segment [kExactSeg] | reduction output [kExactTop]
     unsigned long long* s = reinterpret_cast<unsigned long long*>(s_raw);
@@ -791,11 +802,11 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
     __shared__ unsigned long long s_sel;
     __shared__ int s_cnt, s_top, s_base;
     __shared__ unsigned long long s_kth;
-    const int64_t per_q = (int64_t)nranks * nseg;  // ranks [0, nranks) of rows nprobe wide
+    const int64_t per_q = (int64_t)(nranks - r0) * nseg;  // ranks [r0, nranks) of rows nprobe wide
     const int nql = min(*nql_dev - q0, nql_max);
     for (int64_t w = blockIdx.x; w < (int64_t)nql * per_q; w += gridDim.x) {
         const int64_t q = qlist[q0 + w / per_q];
-        const int j = (int)(w % per_q) / nseg, seg = (int)(w % nseg);
+        const int j = r0 + (int)(w % per_q) / nseg, seg = (int)(w % nseg);
         const float b = bound ? bound[q] : 3.402823466e38f;
         {
             // an empty segment (rank past the query's probe count, or past the list's end)
@@ -809,22 +820,58 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
         if (threadIdx.x == 0) {
             s_cnt = 0;
             s_top = 0;
-            s_kth = HV_KEY_PAD;
+            s_kth = kPad;
         }
         __syncthreads();
         const int l = probes[q * nprobe + j];
         const int64_t lb = list_off[l];
-        const int n = (int)min(list_off[l + 1] - lb, (long long)(seg + 1) * kExactSpan);
+        const int len = (int)(list_off[l + 1] - lb);
+        int n = min(len, (seg + 1) * kExactSpan), start = seg * kExactSpan;
+        if (IP && simi) {
+            const float sv = simi[q];
+            if (sv > -3.402823466e38f) {
+                const float c = pval[q * nprobe + j];
+                const float sc = sqrtf(1.f - c * c), ss = sqrtf(1.f - sv * sv);
+                const float lo = sv * c - ss * sc - slack;
+                const float hi = sv < c ? __builtin_inff() : sv * c + ss * sc + slack;
+                if (lo == lo && hi == hi) {  // not NaN
+                    // c2c descends within the list: [begin, end) = { t : lo <= c2c[t] <= hi }
+                    int a = 0, z = len;
+                    while (a < z) {
+                        const int m = (a + z) >> 1;
+                        if (c2c[lb + m] > hi) a = m + 1;
+                        else z = m;
+                    }
+                    const int begin = a;
+                    z = len;
+                    while (a < z) {
+                        const int m = (a + z) >> 1;
+                        if (c2c[lb + m] >= lo) a = m + 1;
+                        else z = m;
+                    }
+                    start = max(start, begin);
+                    n = min(n, a);
+                }
+            }
+        }
+        if (IP && scanned && threadIdx.x == 0 && n > start) atomicAdd(scanned, (unsigned long long)(n - start));
         const float* X = codes + code_off[l];
         const int ps = list_stride[l];
-        for (int t0 = seg * kExactSpan; t0 < n; t0 += blockDim.x) {
+        for (int t0 = start; t0 < n; t0 += blockDim.x) {
             const int t = t0 + threadIdx.x;
             const unsigned long long kth = s_kth;
             if (t < n) {
-                const float dis = blocked_l2_col(qs, X, ps, t, plan, prune != 0, b);
-                if (dis <= b) {
-                    const unsigned long long key = hv_key(dis, ids[lb + t]);
+                if (IP) {
+                    float acc = 0.f;
+                    for (int i = 0; i < d; ++i) acc = __builtin_fmaf(qs[i], X[(int64_t)i * ps + t], acc);
+                    const unsigned long long key = hv_ip_key(acc, ids[lb + t]);
                     if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
+                } else {
+                    const float dis = blocked_l2_col(qs, X, ps, t, plan, prune != 0, b);
+                    if (dis <= b) {
+                        const unsigned long long key = hv_key(dis, ids[lb + t]);
+                        if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
+                    }
                 }
             }
             __syncthreads();
@@ -832,21 +879,23 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
             if (s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= n)
                 topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
         }
-        // append the kept keys (at most k) to the query's runs
+        // append the kept keys (at most k) to the query's runs (run_stride keys per query)
         const int top = s_top;
         const int64_t wq = w / per_q;
         if (threadIdx.x == 0) s_base = top ? atomicAdd(&run_cnt[wq], top) : 0;
         __syncthreads();
-        unsigned long long* dst = runs + wq * per_q * k + s_base;
+        unsigned long long* dst = runs + wq * run_stride + s_base;
         for (int i = threadIdx.x; i < top; i += blockDim.x) dst[i] = s[i];
     }
 }
 
+template <bool IP>
 __global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qlist, const int* __restrict__ nql_dev,
-                                                     int q0, int nql_max, int nruns, int k,
+                                                     int q0, int nql_max, int64_t run_stride, int k,
                                                      const unsigned long long* __restrict__ runs,
                                                      const int* __restrict__ run_cnt, float* __restrict__ D,
                                                      int64_t* __restrict__ I) {
+    constexpr unsigned long long kPad = IP ? HV_IP_KEY_PAD : HV_KEY_PAD;
     __shared__ unsigned long long s[2 * kExactTop + kExactSeg];
     unsigned long long* s_col = s + kExactTop;
     unsigned long long* s_out = s_col + kExactSeg;
@@ -857,13 +906,13 @@ __global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qli
     const int nql = min(*nql_dev - q0, nql_max);
     for (int w = blockIdx.x; w < nql; w += gridDim.x) {
         const int64_t q = qlist[q0 + w];
-        const unsigned long long* __restrict__ src = runs + (int64_t)w * nruns * k;
+        const unsigned long long* __restrict__ src = runs + (int64_t)w * run_stride;
         const int total = run_cnt[w];
         __syncthreads();
         if (threadIdx.x == 0) {
             s_cnt = 0;
             s_top = 0;
-            s_kth = HV_KEY_PAD;
+            s_kth = kPad;
         }
         __syncthreads();
         for (int t0 = 0; t0 < total; t0 += blockDim.x) {
@@ -879,12 +928,14 @@ __global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qli
         }
         const int top = s_top;
         const int p2 = hv_next_pow2(max(k, 2));
-        for (int i = top + threadIdx.x; i < p2; i += blockDim.x) s[i] = HV_KEY_PAD;
+        for (int i = top + threadIdx.x; i < p2; i += blockDim.x) s[i] = kPad;
         __syncthreads();
         block_bitonic_sort(s, p2);
         for (int i = threadIdx.x; i < k; i += blockDim.x) {
-            D[q * k + i] = hv_key_dist(s[i]);
-            I[q * k + i] = hv_key_id(s[i]);
+            const int64_t id = hv_key_id(s[i]);
+            if (IP) D[q * k + i] = id < 0 ? -3.402823466e38f : hv_ip_key_value(s[i]);  // heap_init's lowest
+            else D[q * k + i] = hv_key_dist(s[i]);
+            I[q * k + i] = id;
         }
     }
 }
@@ -899,13 +950,36 @@ void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_ma
     if (nql_max <= 0) return;
     const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
     const int64_t pairs = (int64_t)nql_max * nranks * nseg;
-    int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * nranks * nseg * k);
+    const int64_t stride = (int64_t)nranks * nseg * k;
+    int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * stride);
     if (!counts_zeroed) launch_fill_i32(run_cnt, nql_max, 0, s);
-    hipLaunchKernelGGL(k_exact_lists, dim3((unsigned)std::min<int64_t>(pairs, kExactGrid)), dim3(256), lds, s, qlist,
-                       nql_dev, q0, nql_max, Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe, nranks,
-                       qn, nseg, plan, prune, bound, k, runs, run_cnt);
-    hipLaunchKernelGGL(k_exact_merge, dim3((unsigned)std::min(nql_max, kExactGrid)), dim3(256), 0, s, qlist, nql_dev,
-                       q0, nql_max, nranks * nseg, k, runs, run_cnt, D, I);
+    hipLaunchKernelGGL(k_exact_lists<false>, dim3((unsigned)std::min<int64_t>(pairs, kExactGrid)), dim3(256), lds, s,
+                       qlist, nql_dev, q0, nql_max, Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe,
+                       0, nranks, qn, nseg, plan, prune, bound, k, runs, stride, run_cnt, nullptr, nullptr, nullptr,
+                       0.f, nullptr);
+    hipLaunchKernelGGL(k_exact_merge<false>, dim3((unsigned)std::min(nql_max, kExactGrid)), dim3(256), 0, s, qlist,
+                       nql_dev, q0, nql_max, stride, k, runs, run_cnt, D, I);
+}
+
+void launch_query_exact_ip(const int* qlist, const int* nql_dev, int q0, int nql_max, const float* Qp, int d,
+                           const float* codes, const int64_t* code_off, const int64_t* list_off,
+                           const int* list_stride, const uint32_t* ids, const int* probes, int nprobe, int r0,
+                           int r1, int nseg, const float* c2c, const float* pval, const float* simi, float slack,
+                           int k, unsigned long long* runs, bool append, float* D, int64_t* I,
+                           unsigned long long* scanned, hipStream_t s) {
+    if (nql_max <= 0 || r1 <= r0) return;
+    const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
+    const int64_t pairs = (int64_t)nql_max * (r1 - r0) * nseg;
+    const int64_t stride = (int64_t)nprobe * nseg * k;  // room for every rank's runs
+    int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * stride);
+    if (!append) launch_fill_i32(run_cnt, nql_max, 0, s);
+    ChunkPlan none{};
+    hipLaunchKernelGGL(k_exact_lists<true>, dim3((unsigned)std::min<int64_t>(pairs, kExactGrid)), dim3(256), lds, s,
+                       qlist, nql_dev, q0, nql_max, Qp, d, codes, code_off, list_off, list_stride, ids, probes, nprobe,
+                       r0, r1, nullptr, nseg, none, 0, nullptr, k, runs, stride, run_cnt, c2c, pval, simi, slack,
+                       scanned);
+    hipLaunchKernelGGL(k_exact_merge<true>, dim3((unsigned)std::min(nql_max, kExactGrid)), dim3(256), 0, s, qlist,
+                       nql_dev, q0, nql_max, stride, k, runs, run_cnt, D, I);
 }
 
 }  // namespace hv

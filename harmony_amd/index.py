@@ -258,6 +258,12 @@ class Index:
         check(self._h, lib().hv_get_list_ids(self._h, _ptr(out)))
         return out
 
+    def list_c2c(self):
+        """METRIC_IP: each vector's inner product with its centroid, list order."""
+        out = np.empty(self.ntotal, dtype=np.float32)
+        check(self._h, lib().hv_get_list_c2c(self._h, _ptr(out)))
+        return out
+
     def list_codes(self):
         out = np.empty((self.ntotal, self.d), dtype=np.float32)
         check(self._h, lib().hv_get_list_codes(self._h, _ptr(out)))

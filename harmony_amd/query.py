@@ -13,6 +13,9 @@ subset rule (368-382) and loaded with --cache, else built and saved; search time
 host-to-host searches after one untimed search when loop > 1 (630-639); recall and r2 by
 calculate_recall / calculate_r2 against the ground truth; --fullWarmUp uses the ground truth's k-th
 distance as every query's bound (716-725); --dataset_info writes "nb, nq, d" to --csv.
+--metric ip runs the inner-product index (Index.cpp:1440-1551) in every mode as the FP32 exact scan,
+with triangle pruning when the opt level has OPT_TRIANGLE; its ground truth when absent is that
+scan over every list.
 Modes on one GPU: original = the unpruned scan (Index.cpp:1420-1554); base = the per-node scan with
 4 dimension blocks and pruning against the warm-up bound (node.cpp:606-678); block = the same with
 --block dimension blocks; brute = every list (the exact FP32 kNN). group / team / worker / MPI
@@ -159,13 +162,18 @@ def run(argv=None):
     if args.train_only:
         return 0
 
-    # ground truth: the file, or the library's FP64 brute force written in the reference's format
+    # ground truth: the file, or the library's FP64 brute force (L2) / the exact inner-product
+    # search over every list (IP) written in the reference's format
     if os.path.exists(paths["groundtruth"]):
         GI, GD = H.load_results(paths["groundtruth"], nq, k)
     else:
         index.nprobe = nlist
-        gD, GI = index.ground_truth_f64(query, k)
-        GD = gD.astype(np.float32)
+        if metric == HV_METRIC_L2:
+            gD, GI = index.ground_truth_f64(query, k)
+            GD = gD.astype(np.float32)
+        else:
+            index.opt_level = 0
+            GD, GI = index.search(query, k)
         H.write_results(paths["groundtruth"], GI, GD)
         log(args, "Groundtruth file %s created" % paths["groundtruth"])
 
@@ -181,12 +189,14 @@ def run(argv=None):
                 index.nprobe = nlist if mode == "brute" else nprobe
                 index.opt_level = opt
                 bound_mode = HV_BOUND_WARMUP
-                if args.fullWarmUp:
+                if args.fullWarmUp and metric == HV_METRIC_L2:
                     index.set_external_bounds(GD[:, k - 1].copy())
                     bound_mode = HV_BOUND_EXTERNAL
                 t0 = time.perf_counter()
-                index.preSearch(warmup_list=args.warmup_list, warmup_list_size=wls, pruning=pruning,
-                                dim_blocks=blocks, bound_mode=bound_mode, precision=precision, rerank_r=args.rerank_r)
+                if metric == HV_METRIC_L2:
+                    index.preSearch(warmup_list=args.warmup_list, warmup_list_size=wls, pruning=pruning,
+                                    dim_blocks=blocks, bound_mode=bound_mode, precision=precision,
+                                    rerank_r=args.rerank_r)
                 presearch_s = time.perf_counter() - t0
                 if args.loop > 1:
                     index.search(query, k)

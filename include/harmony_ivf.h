@@ -31,7 +31,15 @@ extern "C" {
 #define HV_ERR_CAPACITY (-5)    /* size beyond a documented limit */
 #define HV_ERR_NO_DEVICE (-6)   /* no GPU visible */
 
-/* MetricType (src/common.h:20-21). Only L2 is implemented on the GPU path. */
+/* MetricType (src/common.h:20-21). METRIC_IP (Index.cpp:408-418, 567-671, 1440-1551): k-means
+ * with normalized assignment and normalized centroids, lists ordered by descending inner product
+ * with their centroid, and the FP32 search: the nprobe largest <q, c>, then the exact top-k by
+ * (inner product descending, id ascending), padded with (-FLT_MAX, -1). With OPT_TRIANGLE both at
+ * add and at search, each list is scanned only over the angular band a result can lie in (the
+ * reference's triangle pruning, Index.cpp:1509-1546); on unit vectors that changes no result.
+ * The IP search takes the FP32 form with one nprobe and one k per call: no BF16 / FP8 forms,
+ * learned predictor, external bounds, per-query nprobe or k, ground truth or no-allocation
+ * variant (those return HV_ERR_INVALID_ARG). */
 #define HV_METRIC_IP 0
 #define HV_METRIC_L2 1
 
@@ -131,7 +139,8 @@ typedef struct {
     double ms_scan_kernels;  /* summed duration of every candidate-scan kernel launch (the FP32 list
                                 scan, warm-up included, or the MFMA filter scan), eager searches only */
     int64_t scan_launches;   /* those launches */
-    int64_t scan_pairs;      /* (query, vector) pairs they covered, unpruned (warm-up included) */
+    int64_t scan_pairs;      /* (query, vector) pairs they covered, unpruned (warm-up included);
+                                METRIC_IP: the inner products computed (after the triangle cut) */
     int64_t widened_queries; /* learned search: queries whose probe count the margin test widened */
 } hv_stats;
 
@@ -216,6 +225,9 @@ int hv_get_centroids(const hv_index_t* h, float* out);       /* nlist*d, caller 
 int hv_get_list_offsets(const hv_index_t* h, int64_t* out);  /* nlist+1 */
 int hv_get_list_ids(const hv_index_t* h, int64_t* out);      /* ntotal, list order */
 int hv_get_list_codes(const hv_index_t* h, float* out);      /* ntotal*d, list order, caller dims */
+/* METRIC_IP: each vector's inner product with its list's centroid, list order (the per-list
+ * candidate2centroid arrays of src/IVF.h:21, filled at Index.cpp:662-664) */
+int hv_get_list_c2c(const hv_index_t* h, float* out);        /* ntotal */
 /* Introspection of the last search (listidqueries / heapTops, src/Index.h:142-147). Each call
  * takes the capacity of `out` in elements and returns HV_ERR_INVALID_ARG without writing when
  * it is below what the last search holds: hv_last_n(h) * hv_last_probe_stride(h) probes,

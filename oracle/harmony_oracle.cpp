@@ -76,6 +76,22 @@ inline float l2_blocked(const float* q, const float* x, idx_t d, idx_t B, float 
     return dis;
 }
 
+// Inner product (utils.h:502-520): sum += a[i] * b[i] in index order, the FMA chain a contracting
+// compiler emits (the reference-flags build writes the loop as the reference does).
+#ifdef ORACLE_REFERENCE_FLAGS
+inline float ip_range(const float* a, const float* b, idx_t n) {
+    float sum = 0.0f;
+    for (idx_t i = 0; i < n; ++i) sum += a[i] * b[i];
+    return sum;
+}
+#else
+inline float ip_range(const float* a, const float* b, idx_t n) {
+    float acc = 0.0f;
+    for (idx_t i = 0; i < n; ++i) acc = std::fmaf(a[i], b[i], acc);
+    return acc;
+}
+#endif
+
 // ---------------------------------------------------------------- heap (heap.hpp:13-90)
 // Verbatim restatement of heap_replace_top / heap_sort for METRIC_L2 (max-heap in place,
 // 1-based child arithmetic on pointers shifted by one).
@@ -449,6 +465,158 @@ void or_kmeans(idx_t n, const float* x, idx_t d, idx_t nlist, idx_t niter, idx_t
     std::vector<float> sample(cap * d);
     or_kmeans_init(n, x, d, nlist, max_pts, seed, sample.data(), &ns, centroids);
     for (idx_t it = 0; it < niter; ++it) or_kmeans_step(ns, sample.data(), d, nlist, centroids, nthreads);
+}
+
+// ---------------------------------------------------------------- inner-product metric
+// Row normalization (Eigen's colwise().normalized() in Clustering.cpp:122, with the norm from the
+// FP32 chain): x / sqrt(<x, x>), a zero row unchanged.
+void or_ip_normalize_rows(idx_t n, const float* x, idx_t d, float* out) {
+    for (idx_t i = 0; i < n; ++i) {
+        const float nn = std::sqrt(ip_range(x + i * d, x + i * d, d));
+        for (idx_t t = 0; t < d; ++t) out[i * d + t] = nn > 0.0f ? x[i * d + t] / nn : x[i * d + t];
+    }
+}
+
+// Centroid normalization after training (Index.cpp:408-418): norm = <c, c>; if non-zero every
+// coordinate is divided by sqrt(norm).
+void or_ip_normalize_centroids(idx_t nlist, idx_t d, float* c) {
+    for (idx_t l = 0; l < nlist; ++l) {
+        const float norm = ip_range(c + l * d, c + l * d, d);
+        if (norm != 0.0f) {
+            const float r = std::sqrt(norm);
+            for (idx_t t = 0; t < d; ++t) c[l * d + t] /= r;
+        }
+    }
+}
+
+// Assignment by the largest inner product (the IP scanner's lite_scan_codes with a 1-heap: strict
+// '>' in centroid order, so ties go to the lower id; Index.cpp:567-579 / IVFScan.hpp:116-123).
+void or_ip_assign(idx_t n, const float* x, idx_t d, idx_t nlist, const float* centroids, int nthreads, idx_t* labels,
+                  float* best) {
+    for_batches(n, nthreads, [&](idx_t i) {
+        float bv = -FLT_MAX;
+        idx_t bl = -1;
+        for (idx_t c = 0; c < nlist; ++c) {
+            const float v = ip_range(x + i * d, centroids + c * d, d);
+            if (v > bv || bl < 0) {
+                bv = v;
+                bl = c;
+            }
+        }
+        labels[i] = bl;
+        best[i] = bv;
+    });
+}
+
+// IP k-means (Clustering.cpp:121-151 + Index.cpp:408-418): the same subsample and init as L2; per
+// iteration each sample row goes to the centroid with the largest <normalized row, centroid>, the
+// centroid becomes the FP64 mean of its raw rows (empty clusters keep theirs); after the last
+// iteration the centroids are normalized.
+void or_ip_kmeans(idx_t n, const float* x, idx_t d, idx_t nlist, idx_t niter, idx_t max_pts, uint64_t seed,
+                  int nthreads, float* centroids) {
+    idx_t ns = 0;
+    idx_t cap = (max_pts > 0 && n > nlist * max_pts) ? nlist * max_pts : n;
+    std::vector<float> sample(cap * d), xn(cap * d);
+    or_kmeans_init(n, x, d, nlist, max_pts, seed, sample.data(), &ns, centroids);
+    or_ip_normalize_rows(ns, sample.data(), d, xn.data());
+    std::vector<idx_t> lab(ns);
+    std::vector<float> best(ns);
+    for (idx_t it = 0; it < niter; ++it) {
+        or_ip_assign(ns, xn.data(), d, nlist, centroids, nthreads, lab.data(), best.data());
+        std::vector<double> sum(nlist * d, 0.0);
+        std::vector<idx_t> cnt(nlist, 0);
+        for (idx_t i = 0; i < ns; ++i) {
+            cnt[lab[i]]++;
+            for (idx_t t = 0; t < d; ++t) sum[lab[i] * d + t] += (double)sample[i * d + t];
+        }
+        for (idx_t c = 0; c < nlist; ++c)
+            if (cnt[c] > 0)
+                for (idx_t t = 0; t < d; ++t) centroids[c * d + t] = (float)(sum[c * d + t] / (double)cnt[c]);
+    }
+    or_ip_normalize_centroids(nlist, d, centroids);
+}
+
+// IP lists (Index.cpp:581-671): vectors by descending inner product with their centroid (the
+// candidate2centroid array), ties by id; off[nlist+1], order[n].
+void or_ip_build_lists(idx_t n, const idx_t* labels, const float* best, idx_t nlist, idx_t* off, idx_t* order) {
+    std::fill(off, off + nlist + 1, 0);
+    for (idx_t i = 0; i < n; ++i) off[labels[i] + 1]++;
+    for (idx_t l = 0; l < nlist; ++l) off[l + 1] += off[l];
+    std::vector<idx_t> cur(off, off + nlist);
+    for (idx_t i = 0; i < n; ++i) order[cur[labels[i]]++] = i;
+    for (idx_t l = 0; l < nlist; ++l)
+        std::sort(order + off[l], order + off[l + 1], [&](idx_t a, idx_t b) {
+            return best[a] != best[b] ? best[a] > best[b] : a < b;
+        });
+}
+
+// IP search, ORIGINAL path (Index.cpp:1420-1449, 1509-1551): the nprobe centroids with the largest
+// inner product (ties to the lower id), then each probed list in that order. With triangle pruning
+// (opt level bit OPT_TRIANGLE) the scanned range of a list is cut from both ends by the angular
+// band the current k-th result allows (Index.cpp:1515-1546): s = the heap's worst inner product
+// (-FLT_MAX until k results are held, which leaves the band undefined and the list whole),
+// c = <q, centroid>; below s < c only the tail with candidate2centroid < s c - sqrt(1-s^2) sqrt(1-c^2)
+// is cut, otherwise also the head with candidate2centroid > s c + sqrt(1-s^2) sqrt(1-c^2). Results
+// are kept as the top k by (inner product descending, id ascending), padded with (-FLT_MAX, -1).
+// skipped (may be null): vectors cut by the triangle ranges, summed over the queries.
+void or_ip_search(idx_t nq, const float* q, idx_t d, idx_t nlist, const float* centroids, const idx_t* list_off,
+                  const idx_t* ids, const float* codes, const float* c2c, idx_t k, idx_t nprobe, int triangle,
+                  int nthreads, float* D, idx_t* I, idx_t* probes, idx_t* skipped) {
+    nprobe = std::min(nprobe, nlist);
+    std::vector<idx_t> skip(nq, 0);
+    for_batches(nq, nthreads, [&](idx_t i) {
+        const float* qi = q + i * d;
+        std::vector<std::pair<float, idx_t>> cs(nlist);
+        for (idx_t c = 0; c < nlist; ++c) cs[c] = {ip_range(qi, centroids + c * d, d), c};
+        std::partial_sort(cs.begin(), cs.begin() + nprobe, cs.end(), [](const auto& a, const auto& b) {
+            return a.first != b.first ? a.first > b.first : a.second < b.second;
+        });
+        // canonical top-k: a list kept sorted by (ip desc, id asc), at most k long
+        std::vector<std::pair<float, idx_t>> top;
+        auto better = [](const std::pair<float, idx_t>& a, const std::pair<float, idx_t>& b) {
+            return a.first != b.first ? a.first > b.first : a.second < b.second;
+        };
+        for (idx_t j = 0; j < nprobe; ++j) {
+            const idx_t l = cs[j].second;
+            if (probes) probes[i * nprobe + j] = l;
+            const float cq = cs[j].first;
+            const float simi = (idx_t)top.size() < k ? -FLT_MAX : top.back().first;
+            const idx_t b = list_off[l], len = list_off[l + 1] - b;
+            idx_t begin = 0, end = len;
+            if (triangle) {
+                const float s_c = std::sqrt(1 - cq * cq);
+                const float s_s = std::sqrt(1 - simi * simi);
+                if (simi < cq) {
+                    const float min_cut = simi * cq - s_s * s_c;
+                    while (begin < end && c2c[b + end - 1] < min_cut) end--;
+                } else {
+                    const float max_cut = simi * cq + s_s * s_c;
+                    const float min_cut = simi * cq - s_s * s_c;
+                    while (begin < end && c2c[b + begin] > max_cut) begin++;
+                    while (begin < end && c2c[b + end - 1] < min_cut) end--;
+                }
+                skip[i] += begin + (len - end);
+            }
+            for (idx_t t = begin; t < end; ++t) {
+                const std::pair<float, idx_t> cand{ip_range(qi, codes + (b + t) * d, d), ids[b + t]};
+                if ((idx_t)top.size() < k) {
+                    top.insert(std::upper_bound(top.begin(), top.end(), cand, better), cand);
+                } else if (better(cand, top.back())) {
+                    top.pop_back();
+                    top.insert(std::upper_bound(top.begin(), top.end(), cand, better), cand);
+                }
+            }
+        }
+        for (idx_t j = 0; j < k; ++j) {
+            D[i * k + j] = j < (idx_t)top.size() ? top[j].first : -FLT_MAX;
+            I[i * k + j] = j < (idx_t)top.size() ? top[j].second : -1;
+        }
+    });
+    if (skipped) {
+        idx_t t = 0;
+        for (idx_t i = 0; i < nq; ++i) t += skip[i];
+        *skipped = t;
+    }
 }
 
 // ---------------------------------------------------------------- metrics (utils.h:660-756)

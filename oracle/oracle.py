@@ -33,6 +33,12 @@ _SIG = {
     "or_kmeans_init": (None, [_I, _P, _I, _I, _I, ctypes.c_uint64, _P, _P, _P]),
     "or_kmeans_step": (None, [_I, _P, _I, _I, _P, _C]),
     "or_kmeans": (None, [_I, _P, _I, _I, _I, _I, ctypes.c_uint64, _C, _P]),
+    "or_ip_normalize_rows": (None, [_I, _P, _I, _P]),
+    "or_ip_normalize_centroids": (None, [_I, _I, _P]),
+    "or_ip_assign": (None, [_I, _P, _I, _I, _P, _C, _P, _P]),
+    "or_ip_kmeans": (None, [_I, _P, _I, _I, _I, _I, ctypes.c_uint64, _C, _P]),
+    "or_ip_build_lists": (None, [_I, _P, _P, _I, _P, _P]),
+    "or_ip_search": (None, [_I, _P, _I, _I, _P, _P, _P, _P, _P, _I, _I, _C, _C, _P, _P, _P, _P]),
     "or_recall": (ctypes.c_float, [_I, _I, _P, _P, _P, _P, _I, _C, _P, _P]),
     "or_r2": (ctypes.c_float, [_I, _I, _P, _P, _P, _I, _P]),
 }
@@ -231,3 +237,73 @@ def r2(I, D, GD):
     if status.value != 0:
         raise ValueError("k should be less than or equal to gt_k.")
     return float(r)
+
+
+# ---------------------------------------------------------------- inner-product metric
+def ip_normalize_rows(x):
+    x = _f32(x)
+    out = np.empty_like(x)
+    lib().or_ip_normalize_rows(x.shape[0], _p(x), x.shape[1], _p(out))
+    return out
+
+
+def ip_normalize_centroids(c):
+    c = np.array(c, dtype=np.float32, copy=True)
+    lib().or_ip_normalize_centroids(c.shape[0], c.shape[1], _p(c))
+    return c
+
+
+def ip_assign(x, centroids, nthreads=0):
+    x, c = _f32(x), _f32(centroids)
+    n, d = x.shape
+    lab = np.empty(n, np.int64)
+    best = np.empty(n, np.float32)
+    lib().or_ip_assign(n, _p(x), d, c.shape[0], _p(c), nthreads, _p(lab), _p(best))
+    return lab, best
+
+
+def ip_kmeans(x, nlist, niter=10, max_pts=256, seed=0, nthreads=0):
+    x = _f32(x)
+    n, d = x.shape
+    c = np.empty((nlist, d), np.float32)
+    lib().or_ip_kmeans(n, _p(x), d, nlist, niter, max_pts, seed, nthreads, _p(c))
+    return c
+
+
+def ip_build_lists(labels, best, nlist):
+    labels, best = _i64(labels), _f32(best)
+    off = np.empty(nlist + 1, np.int64)
+    order = np.empty(labels.size, np.int64)
+    lib().or_ip_build_lists(labels.size, _p(labels), _p(best), nlist, _p(off), _p(order))
+    return off, order
+
+
+class IpIndexData(IndexData):
+    """An inner-product index: IndexData plus each vector's candidate2centroid (list order)."""
+
+    def __init__(self, centroids, list_off, ids, codes, c2c):
+        super().__init__(centroids, list_off, ids, codes)
+        self.c2c = _f32(c2c)
+
+
+def ip_build_index(xb, nlist, niter=10, max_pts=256, seed=0, nthreads=0):
+    """The oracle's own IP train + add."""
+    c = ip_kmeans(xb, nlist, niter, max_pts, seed, nthreads)
+    lab, best = ip_assign(xb, c, nthreads)
+    off, order = ip_build_lists(lab, best, nlist)
+    return IpIndexData(c, off, order, _f32(xb)[order], best[order])
+
+
+def ip_search(idx, q, k, nprobe, triangle=False, nthreads=0, return_skipped=False):
+    """(D, I, probes) of the IP search (descending inner products, (-FLT_MAX, -1) padding)."""
+    q = _f32(q)
+    nq, d = q.shape
+    npr = min(nprobe, idx.nlist)
+    D = np.empty((nq, k), np.float32)
+    I = np.empty((nq, k), np.int64)
+    probes = np.empty((nq, npr), np.int64)
+    skipped = ctypes.c_int64(0)
+    lib().or_ip_search(nq, _p(q), d, idx.nlist, _p(idx.centroids), _p(idx.list_off), _p(idx.ids), _p(idx.codes),
+                       _p(idx.c2c), k, nprobe, 1 if triangle else 0, nthreads, _p(D), _p(I), _p(probes),
+                       ctypes.byref(skipped))
+    return (D, I, probes, skipped.value) if return_skipped else (D, I, probes)

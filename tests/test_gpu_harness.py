@@ -72,3 +72,37 @@ def test_cli_end_to_end(tmp_path):
     # the unpruned mode over every list: the exact kNN
     rows3 = query.run(argv + ["--cache", "--mode", "original", "--nprobes", "44"])
     assert rows3[0]["recall"] == 1.0 and rows3[0]["variance"] == 0.0
+
+
+def test_cli_inner_product(tmp_path):
+    """--metric ip with OPT_TRIANGLE: index built and saved with candidate2centroid, ground truth
+    from the full-probe scan, recall 1 at nprobe = nlist, and the same recall with the cut off."""
+    bench = tmp_path / "bench"
+    origin = bench / "unit" / "origin"
+    origin.mkdir(parents=True)
+    rng = np.random.default_rng(4)
+    xb = rng.normal(size=(3000, 32)).astype(np.float32)
+    xb /= np.linalg.norm(xb, axis=1, keepdims=True)
+    xq = xb[rng.integers(0, 3000, size=50)] + 0.05 * rng.normal(size=(50, 32)).astype(np.float32)
+    xq /= np.linalg.norm(xq, axis=1, keepdims=True)
+    H.write_fvecs(origin / "unit_base.fvecs", xb)
+    H.write_fvecs(origin / "unit_query.fvecs", xq)
+    argv = ["--benchmarks_path", str(bench), "--dataset", "unit", "--k", "10", "--metric", "ip", "--nprobes", "3",
+            "0", "--opt_levels", "OPT_TRIANGLE", "--kmeans_niter", "6", "--kmeans_seed", "1"]
+    rows = query.run(argv)
+    assert [r["nprobe"] for r in rows] == [3, 54] and rows[-1]["recall"] == 1.0
+    ipath = bench / "unit" / "index" / "index_nlist_54_opt_1_subNprobeRatio_1.index"
+    g = Index(32, 54)
+    g.load_index(str(ipath))
+    c2c = g.list_c2c()
+    ids = g.list_ids()
+    off = g.list_offsets()
+    cent = g.centroids()
+    for l in range(54):
+        a, b = off[l], off[l + 1]
+        np.testing.assert_allclose(c2c[a:b], xb[ids[a:b]] @ cent[l], rtol=1e-5, atol=1e-6)
+    GI, _ = H.load_results(bench / "unit" / "result" / "groundtruth_10.bin", 50, 10)
+    s = xq.astype(np.float64) @ xb.astype(np.float64).T
+    assert np.mean(GI[:, 0] == np.argmax(s, axis=1)) == 1.0
+    rows2 = query.run(argv + ["--cache", "--opt_levels", "0"])  # the opt-1 file, searched without the cut
+    assert [r["recall"] for r in rows2] == [r["recall"] for r in rows]
