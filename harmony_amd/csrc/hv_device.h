@@ -107,6 +107,19 @@ __device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
     }
 }
 
+// out[(*cnt)++] = key for every lane with pred, one LDS atomic per wavefront (the lanes' slots
+// follow their order in the wavefront). Every lane of the wavefront that is active must call it.
+__device__ __forceinline__ void wave_append(bool pred, unsigned long long key, unsigned long long* out, int* cnt) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __builtin_popcountll(m));
+    base = __shfl(base, leader, 64);
+    if (pred) out[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
+}
+
 // Sum over the wavefront of a per-lane count below 128, as a wave-uniform value: one ballot per
 // bit of the count, no cross-lane data movement.
 __device__ __forceinline__ int wave_sum_small(unsigned int x) {

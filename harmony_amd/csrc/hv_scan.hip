@@ -670,31 +670,64 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 // Exact path for the queries the search redoes (overflowed candidate buffers, learned-mode
 // fallbacks). k_exact_lists: one workgroup per (query, probed list, segment of kExactSpan
 // vectors), one candidate per thread; candidates that pass the bound and beat the current k-th
-// key are collected in LDS (up to kExactSeg per segment of the list), the k-th smallest key of
-// the collected set is found by a block-wide radix select (8 passes of 8 bits over the 64-bit
-// keys, which are unique), and only the keys up to it are kept. It appends its (at most k) best
+// key are collected in LDS (up to kExactSeg per segment of the list; one LDS atomic per
+// wavefront), the k-th smallest key of the collected set is found by a block-wide radix select
+// over the bytes of the 64-bit keys, which are unique, and only the keys up to it are kept. It appends its (at most k) best
 // keys to the query's run buffer (an atomic count per query, order free). k_exact_merge: one
 // workgroup per query reduces the appended keys to the final top-k the same way and sorts them.
 // Same arithmetic, pruning and (distance <= bound) filter as the scan, so the result is the same.
 constexpr int kExactTop = 1024, kExactSeg = 4096, kExactSpan = 256;
 
 // k-th smallest (1-based) of the na keys of a and the nb keys of b (block-wide; every thread
-// gets it)
+// gets it; na + nb < 2^24). Radix select over the bytes of the 64-bit keys, from the highest byte
+// in which the smallest and the largest key differ (the bytes above it are common to every key,
+// so their passes would put every key in one bin); it stops as soon as the rank-k bin holds
+// exactly the keys of ranks <= k that remain (the k-th is then the largest key with that prefix,
+// one max reduction), so distinct distances take two or three passes instead of eight.
 __device__ unsigned long long block_radix_kth(const unsigned long long* a, int na, const unsigned long long* b, int nb,
                                               int k, unsigned int* hist, unsigned long long* shared_out) {
-    unsigned long long prefix = 0ull, mask = 0ull;
+    const int n = na + nb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = (blockDim.x + 63) >> 6;
+    unsigned long long* red = reinterpret_cast<unsigned long long*>(hist);  // 2 words per wavefront
+    unsigned long long mn = ~0ull, mx = 0ull;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long key = i < na ? a[i] : b[i - na];
+        mn = key < mn ? key : mn;
+        mx = key > mx ? key : mx;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(mn, o, 64), y = __shfl_xor(mx, o, 64);
+        mn = x < mn ? x : mn;
+        mx = y > mx ? y : mx;
+    }
+    if (lane == 0) {
+        red[2 * wave] = mn;
+        red[2 * wave + 1] = mx;
+    }
+    __syncthreads();
+    mn = red[0];
+    mx = red[1];
+    for (int w = 1; w < nwave; ++w) {
+        mn = red[2 * w] < mn ? red[2 * w] : mn;
+        mx = red[2 * w + 1] > mx ? red[2 * w + 1] : mx;
+    }
+    __syncthreads();  // red shares hist's words
+    if (mn == mx) return mn;
+    int shift = ((63 - __builtin_clzll(mn ^ mx)) >> 3) << 3;
+    unsigned long long mask = shift + 8 >= 64 ? 0ull : ~0ull << (shift + 8);
+    unsigned long long prefix = mn & mask;
     int kk = k;
-    for (int shift = 56; shift >= 0; shift -= 8) {
+    for (; shift >= 0; shift -= 8) {
         for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0u;
         __syncthreads();
-        for (int i = threadIdx.x; i < na + nb; i += blockDim.x) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const unsigned long long key = i < na ? a[i] : b[i - na];
             if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
         }
         __syncthreads();
         if (threadIdx.x < 64) {
             // wavefront 0: 4 bins per lane, inclusive scan, find the bin holding rank kk
-            const int lane = threadIdx.x;
             unsigned int h[4], sum = 0u;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -711,25 +744,45 @@ __device__ unsigned long long block_radix_kth(const unsigned long long* a, int n
             if (excl < (unsigned int)kk && (unsigned int)kk <= incl) {
                 unsigned int run = excl;
                 int digit = lane * 4 + 3;
-                unsigned int below = run + h[0] + h[1] + h[2];
+                unsigned int below = run + h[0] + h[1] + h[2], cnt = h[3];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     if (run + h[t] >= (unsigned int)kk) {
                         digit = lane * 4 + t;
                         below = run;
+                        cnt = h[t];
                         break;
                     }
                     run += h[t];
                 }
-                shared_out[0] = ((unsigned long long)digit << 32) | below;
+                shared_out[0] = ((unsigned long long)digit << 48) | ((unsigned long long)cnt << 24) | below;
             }
         }
         __syncthreads();
         const unsigned long long r = shared_out[0];
-        kk -= (int)(r & 0xffffffffull);
-        prefix |= (r >> 32) << shift;
+        kk -= (int)(r & 0xffffffull);
+        prefix |= ((r >> 48) & 255ull) << shift;
         mask |= 255ull << shift;
         __syncthreads();
+        if (kk == (int)((r >> 24) & 0xffffffull)) {
+            // every key with this prefix has rank <= k: the k-th is the largest of them
+            unsigned long long m = 0ull;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const unsigned long long key = i < na ? a[i] : b[i - na];
+                if ((key & mask) == prefix && key > m) m = key;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long x = __shfl_xor(m, o, 64);
+                m = x > m ? x : m;
+            }
+            if (lane == 0) red[wave] = m;
+            __syncthreads();
+            m = red[0];
+            for (int w = 1; w < nwave; ++w) m = red[w] > m ? red[w] : m;
+            __syncthreads();
+            return m;
+        }
     }
     return prefix;
 }
@@ -746,9 +799,10 @@ __device__ void topk_reduce(unsigned long long* s, const unsigned long long* s_c
         const unsigned long long kv = block_radix_kth(s, top, s_col, m, k, s_hist, s_sel);
         if (threadIdx.x == 0) s_cnt = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < top + m; i += blockDim.x) {
-            const unsigned long long key = i < top ? s[i] : s_col[i - top];
-            if (key <= kv) s_out[atomicAdd(&s_cnt, 1)] = key;
+        for (int i0 = 0; i0 < top + m; i0 += blockDim.x) {
+            const int i = i0 + threadIdx.x;
+            const unsigned long long key = i < top ? s[i] : (i < top + m ? s_col[i - top] : ~0ull);
+            wave_append(i < top + m && key <= kv, key, s_out, &s_cnt);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < k; i += blockDim.x) s[i] = s_out[i];
@@ -860,24 +914,27 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
         for (int t0 = start; t0 < n; t0 += blockDim.x) {
             const int t = t0 + threadIdx.x;
             const unsigned long long kth = s_kth;
+            unsigned long long key = kPad;
+            bool keep = false;
             if (t < n) {
                 if (IP) {
                     float acc = 0.f;
                     for (int i = 0; i < d; ++i) acc = __builtin_fmaf(qs[i], X[(int64_t)i * ps + t], acc);
-                    const unsigned long long key = hv_ip_key(acc, ids[lb + t]);
-                    if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
+                    key = hv_ip_key(acc, ids[lb + t]);
+                    keep = key < kth;
                 } else {
                     const float dis = blocked_l2_col(qs, X, ps, t, plan, prune != 0, b);
-                    if (dis <= b) {
-                        const unsigned long long key = hv_key(dis, ids[lb + t]);
-                        if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
-                    }
+                    key = hv_key(dis, ids[lb + t]);
+                    keep = dis <= b && key < kth;
                 }
             }
+            wave_append(keep, key, s_col, &s_cnt);
             __syncthreads();
-            // reduce when the next round might not fit, and at the end
-            if (s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= n)
-                topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
+            // reduce when the next round might not fit, and at the end; every thread reads the
+            // count before any wavefront can append the next round's keys to it
+            const bool reduce = s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= n;
+            __syncthreads();
+            if (reduce) topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
         }
         // append the kept keys (at most k) to the query's runs (run_stride keys per query)
         const int top = s_top;
@@ -915,16 +972,23 @@ __global__ __launch_bounds__(256) void k_exact_merge(const int* __restrict__ qli
             s_kth = kPad;
         }
         __syncthreads();
-        for (int t0 = 0; t0 < total; t0 += blockDim.x) {
-            const int t = t0 + threadIdx.x;
+        // kMergeKeys keys per thread per round: their loads are in flight together
+        constexpr int kMergeKeys = 8;
+        const int round = (int)blockDim.x * kMergeKeys;
+        for (int t0 = 0; t0 < total; t0 += round) {
             const unsigned long long kth = s_kth;
-            if (t < total) {
-                const unsigned long long key = src[t];
-                if (key < kth) s_col[atomicAdd(&s_cnt, 1)] = key;
+            unsigned long long v[kMergeKeys];
+#pragma unroll
+            for (int u = 0; u < kMergeKeys; ++u) {
+                const int t = t0 + u * (int)blockDim.x + (int)threadIdx.x;
+                v[u] = t < total ? src[t] : kPad;
             }
+#pragma unroll
+            for (int u = 0; u < kMergeKeys; ++u) wave_append(v[u] < kth, v[u], s_col, &s_cnt);
             __syncthreads();
-            if (s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= total)
-                topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
+            const bool reduce = s_cnt > kExactSeg - round || t0 + round >= total;
+            __syncthreads();  // the count is read by every thread before the next round appends
+            if (reduce) topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
         }
         const int top = s_top;
         const int p2 = hv_next_pow2(max(k, 2));
