@@ -24,6 +24,47 @@ namespace hv {
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 constexpr int kCoarseCand = 1024;  // exact re-checks per query in k_coarse_select
 
+// Bits of the upper bound d~ + kappa (|q|^2 + |c|^2) of a chain distance (one FMA, so every
+// kernel forms the same bits); negative bounds (and -0) order as 0.
+__device__ __forceinline__ unsigned int coarse_ub_bits(float approx, float qq, float cc, float kappa) {
+    const unsigned int u = __float_as_uint(__builtin_fmaf(kappa, qq + cc, approx));
+    return (int)u < 0 ? 0u : u;
+}
+
+// The F (<= 64) smallest of the 64 E values a wavefront holds E per lane (pad 0xffffffff),
+// ascending, as floats into out[0, kNearest) (lanes past F repeat the F-th): the F-th smallest by
+// wave_kth_u32, the values below it and then enough equal to it gathered in LDS (s_tmp, 64 words
+// of this wavefront), one register sort. Returns the lane's value of that order. All 64 lanes
+// must call it.
+template <int E>
+__device__ __forceinline__ unsigned int sorted_smallest_u32(const unsigned int (&u)[E], int F, unsigned int* s_tmp,
+                                                            float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const unsigned int thr = wave_kth_u32<E>(u, F, s_tmp);
+    int base = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool in = u[e] < thr;
+        const unsigned long long m = __ballot(in);
+        if (in) s_tmp[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = u[e];
+        base += __builtin_popcountll(m);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool eq = u[e] == thr;
+        const unsigned long long m = __ballot(eq);
+        const int pos = base + __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (eq && pos < F) s_tmp[pos] = u[e];
+        base += __builtin_popcountll(m);
+    }
+    __threadfence_block();  // the words other lanes stored are read below
+    unsigned long long x[1] = {lane < F ? (unsigned long long)s_tmp[lane] : ~0ull};
+    wave_bitonic_sort<1>(x);
+    const unsigned int val = (unsigned int)__shfl((int)(unsigned int)x[0], min(lane, F - 1), 64);
+    out[lane] = __uint_as_float(val);
+    return val;
+}
+
 // squared norms of rows (any summation order: kappa covers it)
 __global__ void k_row_norms(const float* __restrict__ X, int64_t n, int d, float* __restrict__ out) {
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -383,7 +424,7 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
                                                             const float* __restrict__ cn, float kappa, int64_t nq,
                                                             int P, int* __restrict__ probes,
                                                             float* __restrict__ pdist, int* __restrict__ redo_list,
-                                                            int* __restrict__ redo_cnt) {
+                                                            int* __restrict__ redo_cnt, float* __restrict__ near) {
     __shared__ unsigned int s_cand[4][CAND];
     __shared__ unsigned long long s_ckey[4][CAND];
     __shared__ unsigned int s_tmp[4][64];
@@ -397,13 +438,17 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int c = e * 64 + lane;
-        ub[e] = 0xffffffffu;
-        if (c < nc) {
-            const unsigned int u = __float_as_uint(row[c] + kappa * (qq + cn[c]));
-            ub[e] = (int)u < 0 ? 0u : u;  // negative upper bounds (and -0) order as 0
-        }
+        ub[e] = c < nc ? coarse_ub_bits(row[c], qq, cn[c], kappa) : 0xffffffffu;
     }
-    const float tau = __uint_as_float(wave_kth_u32<E>(ub, P, s_tmp[w]));
+    float tau;
+    if (near) {
+        // the kNearest smallest upper bounds, sorted (the learned features; P <= kNearest): their
+        // P-th is tau
+        const unsigned int v = sorted_smallest_u32<E>(ub, min(nc, kNearest), s_tmp[w], near + q * kNearest);
+        tau = __uint_as_float((unsigned int)__shfl((int)v, P - 1, 64));
+    } else {
+        tau = __uint_as_float(wave_kth_u32<E>(ub, P, s_tmp[w]));
+    }
     // lower bounds from the row again (cache hits) rather than held through the search
     int m = 0;
 #pragma unroll
@@ -467,6 +512,80 @@ __global__ __launch_bounds__(256) void k_coarse_exact_row(const int* __restrict_
     }
 }
 
+// The kNearest smallest distance values of each row of v [n][nc], ascending (a row of fewer
+// columns repeats its largest): the learned predictor's distance features. With qn (coarse
+// estimate rows from k_coarse_mfma), the values are the upper bounds d~ + kappa (|q|^2 + |c|^2)
+// that k_coarse_select_wave also forms (the same bits, so features do not depend on which kernel
+// produced them); without, the rows are exact distances, negatives as 0. One wavefront per row
+// (nc <= 64 E).
+template <int E>
+__global__ __launch_bounds__(256) void k_near_rows_wave(const float* __restrict__ v, int64_t n, int nc,
+                                                        const float* __restrict__ qn, const float* __restrict__ cn,
+                                                        float kappa, float* __restrict__ out) {
+    __shared__ unsigned int s_tmp[4][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + w;
+    if (r >= n) return;
+    const float* __restrict__ row = v + r * nc;
+    const float qq = qn ? qn[r] : 0.f;
+    unsigned int u[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int c = e * 64 + lane;
+        if (c >= nc) {
+            u[e] = 0xffffffffu;
+        } else if (qn) {
+            u[e] = coarse_ub_bits(row[c], qq, cn[c], kappa);
+        } else {
+            const float x = row[c];
+            u[e] = x > 0.f ? __float_as_uint(x) : 0u;  // non-negative floats order as uints
+        }
+    }
+    sorted_smallest_u32<E>(u, min(nc, kNearest), s_tmp[w], out + r * kNearest);
+}
+
+// The same for rows of up to kMaxSelectRow columns: one workgroup per row sorts (value, column)
+// keys in LDS.
+__global__ __launch_bounds__(256) void k_near_rows_block(const float* __restrict__ v, int64_t n, int nc,
+                                                         const float* __restrict__ qn, const float* __restrict__ cn,
+                                                         float kappa, float* __restrict__ out) {
+    extern __shared__ unsigned long long s_keys[];
+    const int64_t r = blockIdx.x;
+    const int M = hv_next_pow2(max(nc, 2));
+    for (int i = threadIdx.x; i < M; i += blockDim.x) {
+        unsigned int u = 0u;
+        if (i < nc) {
+            const float x = v[r * nc + i];
+            u = qn ? coarse_ub_bits(x, qn[r], cn[i], kappa) : (x > 0.f ? __float_as_uint(x) : 0u);
+        }
+        s_keys[i] = i < nc ? (((unsigned long long)u << 32) | (unsigned int)i) : ~0ull;
+    }
+    block_bitonic_sort(s_keys, M);
+    const int F = min(nc, kNearest);
+    for (int i = threadIdx.x; i < kNearest; i += blockDim.x)
+        out[r * kNearest + i] = __uint_as_float((unsigned int)(s_keys[min(i, F - 1)] >> 32));
+}
+
+void launch_near_rows(const float* v, int64_t n, int nc, const float* qn, const float* cn, float kappa, float* out,
+                      hipStream_t s) {
+    if (n <= 0 || nc <= 0) return;
+    const dim3 wgrid((unsigned)((n + 3) / 4));
+#define HV_NEAR(E) hipLaunchKernelGGL(k_near_rows_wave<E>, wgrid, dim3(256), 0, s, v, n, nc, qn, cn, kappa, out)
+    if (nc <= 512)
+        HV_NEAR(8);
+    else if (nc <= 1024)
+        HV_NEAR(16);
+    else if (nc <= 2048)
+        HV_NEAR(32);
+    else if (nc <= 4096)
+        HV_NEAR(64);
+    else
+        hipLaunchKernelGGL(k_near_rows_block, dim3((unsigned)n), dim3(256),
+                           sizeof(unsigned long long) * hv_next_pow2(nc), s, v, n, nc, qn, cn, kappa, out);
+#undef HV_NEAR
+}
+
 float coarse_kappa(int d) {
     // |d~ - chain| <= kappa (|q|^2 + |c|^2): the MFMA dot product (gamma_2d |q||c|), the FP32
     // norms (gamma_d each), the chain's own error against the real distance (2 gamma_2d of
@@ -488,12 +607,14 @@ void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int 
 
 void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
-                          int* redo_list, int* redo_cnt, hipStream_t s) {
+                          int* redo_list, int* redo_cnt, float* near, hipStream_t s) {
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
+    // the wavefront kernel forms the features itself when P <= kNearest
+    float* fused = P <= kNearest ? near : nullptr;
 #define HV_CSEL(E, CAND)                                                                                        \
     hipLaunchKernelGGL((k_coarse_select_wave<E, CAND>), wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, \
-                       nq, P, probes, pdist, redo_list, redo_cnt)
+                       nq, P, probes, pdist, redo_list, redo_cnt, fused)
     const bool narrow = P <= 128;
     if (P <= kWaveP && nc <= 512) {
         if (narrow) HV_CSEL(8, 256); else HV_CSEL(8, 512);
@@ -502,11 +623,13 @@ void launch_coarse_select(const float* approx, const float* Q, const float* C, i
     } else if (P <= kWaveP && nc <= 2048) {
         if (narrow) HV_CSEL(32, 256); else HV_CSEL(32, 512);
     } else {
+        fused = nullptr;
         const size_t lds = sizeof(unsigned long long) * kCoarseCand + sizeof(float) * ((d + 3) & ~3);
         hipLaunchKernelGGL(k_coarse_select, dim3((unsigned)nq), dim3(256), lds, s, approx, Q, C, nc, d, qn, cn, kappa,
                            P, probes, pdist, redo_list, redo_cnt);
     }
 #undef HV_CSEL
+    if (near && !fused) launch_near_rows(approx, nq, nc, qn, cn, kappa, near, s);
     const size_t lds2 = sizeof(unsigned long long) * hv_next_pow2(nc > 2 ? nc : 2) + sizeof(float) * ((d + 3) & ~3);
     hipLaunchKernelGGL(k_coarse_exact_row, dim3((unsigned)std::min<int64_t>(nq, 256)), dim3(256), lds2, s, redo_list,
                        redo_cnt, Q, C, nc, d, P, probes, pdist);

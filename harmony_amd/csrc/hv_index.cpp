@@ -340,7 +340,8 @@ struct hv_index {
     DevBuf s_qk;  // per-query k of the current search
     DevBuf s_q, s_qp, s_D, s_I, s_dist, s_probes, s_bound, s_bound_ref, s_bucket, s_fill, s_bucket_off, s_scan_tmp, s_pairs,
         s_items, s_queue, s_surv, s_surv_cnt, s_overflow, s_ritems, s_kept, s_done, s_stats, s_pdist, s_pair_slot,
-        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_qnorm, s_cnorm, s_redo, s_qscale, s_last;
+        s_aux, s_qres, s_runs, s_qn, s_feat, s_pred, s_seed, s_fb, s_qnorm, s_cnorm, s_redo, s_qscale, s_last,
+        s_near;
     int64_t last_n = 0, last_nprobe = 0, last_stride = 0;
     bool last_learned = false;
 };
@@ -494,9 +495,11 @@ int* coarse_counter(hv_index_t* h, int64_t n) {
     HV_HIP(h->s_redo.ensure(sizeof(int) * (coarse_rows_per_slice(h, n) + 1)));
     return h->s_redo.as<int>();
 }
-// counter_zeroed: the caller has cleared the re-check counter (the first slice skips its fill)
+// counter_zeroed: the caller has cleared the re-check counter (the first slice skips its fill);
+// near (optional, [n][kNearest]): the kNearest smallest distance estimates of each query, sorted
+// (the learned predictor's features), taken from the same rows
 void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist,
-                   bool counter_zeroed = false) {
+                   bool counter_zeroed = false, float* near = nullptr) {
     const int64_t d = h->d, nlist = h->nlist;
     hipStream_t s = h->stream;
     const int64_t per = coarse_rows_per_slice(h, n);
@@ -513,7 +516,8 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
             if (r0 > 0 || !counter_zeroed) hv::launch_fill_i32(redo_cnt, 1, 0, s);
             hv::launch_coarse_select(h->s_dist.as<float>(), Qp + r0 * d, h->d_centroids.as<float>(), (int)nlist,
                                      (int)d, h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), kappa, rows, (int)P,
-                                     probes + r0 * P, pdist + r0 * P, redo_cnt + 1, redo_cnt, s);
+                                     probes + r0 * P, pdist + r0 * P, redo_cnt + 1, redo_cnt,
+                                     near ? near + r0 * hv::kNearest : nullptr, s);
         }
         HV_CHECK_LAUNCH("coarse");
         return;
@@ -523,6 +527,9 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
         hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d, h->s_dist.as<float>(),
                                 nlist, s);
         hv::launch_select_rows(h->s_dist.as<float>(), rows, (int)nlist, (int)P, probes + r0 * P, pdist + r0 * P, s);
+        if (near)
+            hv::launch_near_rows(h->s_dist.as<float>(), rows, (int)nlist, nullptr, nullptr, 0.f,
+                                 near + r0 * hv::kNearest, s);
     }
     HV_CHECK_LAUNCH("coarse");
 }
@@ -722,17 +729,22 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     require(h->sp.dim_blocks >= 1 && h->sp.dim_blocks <= d, HV_ERR_INVALID_ARG, "dim_blocks must be in [1, d]");
     require(nlist <= hv::kMaxSelectRow, HV_ERR_CAPACITY, "nlist > 8192");
     const hv::ChunkPlan plan = make_plan((int)d, B);
-    // learned predictor: per-query probe counts (at most nprobe) and seeded bounds; the probe
-    // lists are then selected at least 64 deep (its features are the 64 nearest distances)
+    // learned predictor: per-query probe counts (at most nprobe) and seeded bounds; its features
+    // (the 64 smallest coarse distance estimates) come from the coarse rows beside the selection
     const bool learned = h->sp.learned != 0;
     const bool uneven = !h->query_nprobe.empty();
     if (uneven) {
         require((int64_t)h->query_nprobe.size() == n, HV_ERR_STATE, "per-query nprobe not set for this batch");
         require(!learned, HV_ERR_INVALID_ARG, "per-query nprobe and the learned predictor are exclusive");
     }
-    if (learned)
+    if (learned) {
         require(h->mlp_params && h->mlp_norm, HV_ERR_STATE,
                 "learned search needs a predictor (hv_mlp_init/train or set_params, and hv_mlp_set_norm)");
+        // the features come from the coarse rows: the matrix-core estimates whenever
+        // hv_query_features would take them too
+        require(coarse_on_mfma(h, nprobe) == coarse_on_mfma(h, 1), HV_ERR_INVALID_ARG,
+                "the learned search takes nprobe <= 512 (its features come from the coarse estimates)");
+    }
     const bool kq_on = !h->query_k.empty();
     if (kq_on) {
         require((int64_t)h->query_k.size() == n, HV_ERR_STATE, "per-query k not set for this batch");
@@ -742,10 +754,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     const bool bf16 = h->sp.precision == HV_PRECISION_BF16;
     const bool mfma = bf16 || fp8;
     const int64_t nprobe_cap = nprobe;
-    if (learned) nprobe = std::min<int64_t>(nlist, std::max<int64_t>(nprobe, hv::kNearest));
-    // learned: the probe lists are selected kNearest deep for the features, but the ranks past the
-    // predictor's cap are never searched (qn <= nprobe_cap)
-    const int64_t rank_end = learned ? std::min<int64_t>(nprobe, nprobe_cap) : nprobe;
+    const int64_t rank_end = nprobe;
     int64_t wl = learned ? 0 : std::min<int64_t>(h->sp.warmup_list, nprobe);  // Index.cpp:1007-1010
     int64_t wls = h->sp.warmup_list_size == 0 ? k : h->sp.warmup_list_size;  // query.cpp:328-333
     if (h->sp.bound_mode == HV_BOUND_WARMUP && wl > 0)
@@ -897,6 +906,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             need(h->s_qn, 4 * n);
             need(h->s_seed, 4 * n);
             need(h->s_pred, 8 * n);
+            need(h->s_near, 4 * n * hv::kNearest);
         } else if (uneven) {
             need(h->s_qn, 4 * n);
         }
@@ -998,7 +1008,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_pdist.ensure(sizeof(float) * n * nprobe));
     int* probes = h->s_probes.as<int>();
     float* pdist = h->s_pdist.as<float>();
-    coarse_select(h, n, Qp, nprobe, probes, pdist, true);
+    coarse_select(h, n, Qp, nprobe, probes, pdist, true, learned ? h->s_near.as<float>() : nullptr);
     const int* qn = nullptr;
     if (learned) {
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
@@ -1007,7 +1017,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         // margin widening only where the search verifies the seed afterwards (FP32 and BF16); the
         // slack covers the FP32 roundings of the coarse distances, residual norms and square roots
         const float slack = (float)(4.0 * d * std::ldexp(1.0, -24) + std::ldexp(1.0, -20));
-        hv::launch_mlp_serve(pdist, probes, (int)nprobe, Qp, (int)d, n, h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(),
+        hv::launch_mlp_serve(h->s_near.as<float>(), pdist, probes, (int)nprobe, Qp, (int)d, n, h->m_w1b.as<__bf16>(),
+                             h->m_w2b.as<__bf16>(),
                              h->m_params.as<float>(), h->m_norm.as<float>(), h->sp.learned_nprobe_scale,
                              h->sp.learned_bound_scale, std::max(1, h->sp.learned_min_nprobe), (int)nprobe_cap,
                              h->d_lrad.as<float>(), h->d_centroids.as<float>(),
@@ -1398,7 +1409,7 @@ void hv_destroy(hv_index_t* h) {
                       &h->s_bound, &h->s_bound_ref, &h->s_bucket, &h->s_fill, &h->s_bucket_off, &h->s_scan_tmp, &h->s_pairs,
                       &h->s_items, &h->s_queue, &h->s_surv, &h->s_surv_cnt, &h->s_overflow, &h->s_ritems,
                       &h->s_kept, &h->s_done, &h->s_stats, &h->s_pdist, &h->s_pair_slot, &h->s_aux, &h->s_qres,
-                      &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_qnorm,
+                      &h->s_runs, &h->s_qn, &h->s_feat, &h->s_pred, &h->s_seed, &h->s_fb, &h->s_qnorm, &h->s_near,
                       &h->s_cnorm, &h->s_redo, &h->d_rcodes, &h->d_roff, &h->d_rnorm, &h->d_rerr, &h->d_rows, &h->s_qscale, &h->s_last,
                       &h->d_rcodes8, &h->d_lscale8, &h->d_lrad, &h->s_qk, &h->d_c2c};
     for (DevBuf* b : bufs) b->release();
@@ -1884,7 +1895,8 @@ int hv_search_device_noalloc(hv_index_t* h, int64_t n, const float* q_dev, int64
                           &h->s_stats, &h->s_surv_cnt, &h->s_bound, &h->s_bound_ref, &h->s_overflow, &h->s_last,
                           &h->s_runs, &h->s_probes, &h->s_pdist, &h->s_qn, &h->s_seed, &h->s_pred, &h->s_bucket_off,
                           &h->s_scan_tmp, &h->s_pairs, &h->s_pair_slot, &h->s_items, &h->s_queue, &h->s_surv,
-                          &h->s_aux, &h->s_qres, &h->s_qscale, &h->s_ritems, &h->s_kept, &h->s_done, &h->s_qk};
+                          &h->s_aux, &h->s_qres, &h->s_qscale, &h->s_ritems, &h->s_kept, &h->s_done, &h->s_qk,
+                          &h->s_near};
         constexpr int nbufs = (int)(sizeof(bufs) / sizeof(bufs[0]));
         std::lock_guard<std::mutex> lk(g_scratch_mu[h->device]);
         struct Swap {
@@ -2125,7 +2137,6 @@ int hv_query_features(hv_index_t* h, int64_t n, const float* q, float* feat) {
         require(n >= 0 && (n == 0 || (q && feat)), HV_ERR_INVALID_ARG, "bad buffers");
         if (n == 0) return;
         const int64_t d = h->d;
-        const int64_t P = std::min<int64_t>(h->nlist, hv::kNearest);
         hipStream_t s = h->stream;
         upload(h->s_q, q, sizeof(float) * n * d, s);
         const float* Qp = h->s_q.as<float>();
@@ -2134,11 +2145,13 @@ int hv_query_features(hv_index_t* h, int64_t n, const float* q, float* feat) {
             hv::launch_permute_rows(Qp, n, (int)d, h->d_perm.as<int>(), h->s_qp.as<float>(), s);
             Qp = h->s_qp.as<float>();
         }
-        HV_HIP(h->s_probes.ensure(sizeof(int) * n * P));
-        HV_HIP(h->s_pdist.ensure(sizeof(float) * n * P));
-        coarse_select(h, n, Qp, P, h->s_probes.as<int>(), h->s_pdist.as<float>());
+        // the search's coarse step (one probe) with the same feature rows
+        HV_HIP(h->s_probes.ensure(sizeof(int) * n));
+        HV_HIP(h->s_pdist.ensure(sizeof(float) * n));
+        HV_HIP(h->s_near.ensure(sizeof(float) * n * hv::kNearest));
+        coarse_select(h, n, Qp, 1, h->s_probes.as<int>(), h->s_pdist.as<float>(), false, h->s_near.as<float>());
         HV_HIP(h->s_feat.ensure(sizeof(float) * n * hv::kMlpIn));
-        hv::launch_features(h->s_pdist.as<float>(), (int)P, Qp, (int)d, n, h->s_feat.as<float>(), s);
+        hv::launch_features(h->s_near.as<float>(), hv::kNearest, Qp, (int)d, n, h->s_feat.as<float>(), s);
         HV_CHECK_LAUNCH("features");
         HV_HIP(hipMemcpyAsync(feat, h->s_feat.p, sizeof(float) * n * hv::kMlpIn, hipMemcpyDeviceToHost, s));
         HV_HIP(hipStreamSynchronize(s));

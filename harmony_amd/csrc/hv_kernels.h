@@ -203,7 +203,8 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 // margin > 0, lrad (list radii) and cent (centroids), each query's count widens to every list of
 // its first nprobe probes whose triangle or Voronoi lower bound is within the seed (widened:
 // count of such queries).
-void launch_mlp_serve(const float* pdist, const int* probes, int pc, const float* Qp, int d, int64_t n,
+void launch_mlp_serve(const float* near, const float* pdist, const int* probes, int pc, const float* Qp, int d,
+                      int64_t n,
                       const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
                       float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
                       float margin, float slack, int* qn, float* bound, float* pred, float* bound2, float* bound3,
@@ -217,10 +218,17 @@ float coarse_kappa(int d);
 // also writes the squared norms of the rows (qn [nq], cn [nc]) that the select uses
 void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, float* qn, float* cn,
                         float* approx, hipStream_t s);
-// P nearest per query, exact (distance, id) order; redo_cnt must be 0 on entry
+// the kNearest smallest distance values of each row of v [n][nc] ascending, a short row's largest
+// repeated, into out [n][kNearest] (nc <= kMaxSelectRow): with qn / cn, v holds coarse estimates
+// and the values are their upper bounds d~ + kappa (|q|^2 + |c|^2); without, v holds exact
+// distances (negatives as 0)
+void launch_near_rows(const float* v, int64_t n, int nc, const float* qn, const float* cn, float kappa, float* out,
+                      hipStream_t s);
+// P nearest per query, exact (distance, id) order; redo_cnt must be 0 on entry. near (optional):
+// the kNearest smallest upper bounds of each query, sorted, as launch_near_rows forms them
 void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
-                          int* redo_list, int* redo_cnt, hipStream_t s);
+                          int* redo_list, int* redo_cnt, float* near, hipStream_t s);
 
 // --- inner-product metric (hv_ip.hip; the IP list scan is launch_query_exact_ip)
 void launch_ip_rows(const float* X, int64_t n, const float* C, int64_t nc, int d, float* out, int64_t ldo,

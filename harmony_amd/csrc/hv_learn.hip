@@ -2,8 +2,10 @@
 // nearest centroids and its squared norm, predicting log1p(k-th neighbour distance) and
 // log(nprobe needed for the target recall).
 //
-// Features (raw): f_i = log1p(i-th smallest coarse distance), i < 64 (the last one repeated when
-// there are fewer lists), f_64 = log1p(|q|^2). The network works on standardized features
+// Features (raw): f_i = log1p(i-th smallest coarse distance bound), i < 64 (the last one repeated
+// when there are fewer lists), f_64 = log1p(|q|^2). The bounds are the coarse step's upper bounds
+// d~ + kappa (|q|^2 + |c|^2) of the FP32 chain distances (d~ the MFMA estimate), sorted: the
+// search's probe selection forms them on the way (hv_coarse.hip). The network works on standardized features
 // xn = (f - fmean) / fstd and standardized targets; outputs are mapped back with the target
 // mean/std. Training keeps FP32 master weights (forward, backward and Adam below, all FP32);
 // serving uses BF16 copies of the weight matrices with FP32 accumulation.
@@ -22,7 +24,7 @@ constexpr int kOffB1 = kIn * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHi
 static_assert(kOffB2 + kOut == kMlpParams, "parameter layout");
 
 // ---------------------------------------------------------------------------------------
-// Features from the sorted coarse distances (pdist [n][P], ascending) and the queries.
+// Features from the sorted coarse distance estimates (pdist [n][P], ascending) and the queries.
 __global__ void k_features(const float* __restrict__ pdist, int P, const float* __restrict__ Qp, int d, int64_t n,
                            float* __restrict__ feat) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -236,8 +238,9 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 // Serving: per query, the probe count qn = clamp(ceil(exp(y1) * nprobe_scale), min_nprobe,
 // nprobe) and the seeded bound R = expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also
 // into bound2 and bound3 when given (the search's working and reference bounds start there).
-// The features are formed here, in LDS, from the coarse distances (pc per query, ascending) and
-// the query, with k_features' arithmetic (so the same bits).
+// The features are formed here, in LDS, from the query's kNearest smallest coarse distance
+// estimates (near, ascending) and the query, with k_features' arithmetic (so the same bits);
+// pdist / probes (pc per query) are the exact probe lists the margin test walks.
 //
 // Margin widening (margin > 0, with a finite R): a list l past the predicted count can hold a
 // vector within sqrt(R) of q only if both lower bounds of |q - x| over its vectors x are <= sqrt(R):
@@ -251,7 +254,8 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 // square roots involved. With margin >= 1, and once the search has checked that the query's k-th
 // result is <= R, no list left out holds a vector of the top k over all nprobe lists: the query's
 // result is the plain search's at nprobe. Queries whose count grew are counted in *widened.
-__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdist, const int* __restrict__ probes,
+__global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ near, const float* __restrict__ pdist,
+                                                   const int* __restrict__ probes,
                                                    int pc, const float* __restrict__ Qp, int d, int64_t n,
                                                    const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
                                                    const float* __restrict__ P, const float* __restrict__ norm,
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdi
         s = __builtin_fmaf(v, v, s);
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    s_f[w][lane] = log1pf(fmaxf(pdist[q * pc + min(lane, pc - 1)], 0.f));
+    s_f[w][lane] = log1pf(fmaxf(near[q * kNearest + lane], 0.f));
     if (lane == 0) s_f[w][kNearest] = log1pf(s);
     __threadfence_block();  // every lane reads the whole row its wavefront wrote
     float y0, y1;
@@ -316,13 +320,14 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ pdi
     }
 }
 
-void launch_mlp_serve(const float* pdist, const int* probes, int pc, const float* Qp, int d, int64_t n,
+void launch_mlp_serve(const float* near, const float* pdist, const int* probes, int pc, const float* Qp, int d,
+                      int64_t n,
                       const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
                       float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
                       float margin, float slack, int* qn, float* bound, float* pred, float* bound2, float* bound3,
                       unsigned long long* widened, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, pdist, probes, pc, Qp, d, n, w1,
+    hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, near, pdist, probes, pc, Qp, d, n, w1,
                        w2, P, norm, nprobe_scale, bound_scale, min_nprobe, nprobe, lrad, cent, margin, slack, qn, bound,
                        pred, bound2, bound3, widened);
 }

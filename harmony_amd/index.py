@@ -274,7 +274,7 @@ class Index:
         return int(lib().hv_last_n(self._h))
 
     def last_probes(self, n=None):
-        """Probe lists of the last search ([n][stride]: nprobe, or >= 64 deep with the predictor).
+        """Probe lists of the last search ([n][stride], stride = nprobe).
         n defaults to the last search's query count; a smaller n is rejected by the library."""
         n = self.last_n() if n is None else int(n)
         stride = int(lib().hv_last_probe_stride(self._h)) or min(self.nprobe, self.nlist)
@@ -291,7 +291,8 @@ class Index:
 
     # -- learned probe/bound predictor
     def query_features(self, queries):
-        """[n][65]: log1p of the 64 nearest squared centroid distances, ascending; log1p(|q|^2)."""
+        """[n][65]: log1p of the 64 smallest upper bounds of the squared centroid distances that the
+        coarse step forms (ascending), log1p(|q|^2): the learned search's features."""
         q = _f32(queries, self.d)
         out = np.empty((q.shape[0], _lib.HV_MLP_IN), dtype=np.float32)
         check(self._h, lib().hv_query_features(self._h, q.shape[0], _ptr(q), _ptr(out)))

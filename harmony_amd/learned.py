@@ -1,7 +1,7 @@
 """Learned probe/bound predictor: traces from the plain search, GPU training, calibration.
 
 The predictor (a 65 -> 128 -> 2 MLP run by the library's own HIP kernels) reads a query's
-distances to its 64 nearest centroids and its norm, and predicts the k-th neighbour distance and
+64 smallest centroid distances (the coarse step's upper bounds of them) and its norm, and predicts the k-th neighbour distance and
 the number of lists needed to reach a target recall. Serving uses the prediction to trim each
 query's probe list and to seed its pruning bound. The margin test widens a query's probe count to
 every list (of its first nprobe) that the triangle inequality cannot rule out within the seeded

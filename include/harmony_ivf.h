@@ -243,8 +243,12 @@ int64_t hv_last_probe_stride(const hv_index_t* h);
 
 /* ---- learned probe/bound predictor (no counterpart in the reference; the reference always
  * probes nprobe lists and takes its bound from a warm-up scan).
- * A 65 -> 128 (ReLU) -> 2 MLP. Features of a query (HV_MLP_IN): log1p of its squared
- * distances to its 64 nearest centroids, ascending, then log1p(|q|^2). Targets (raw, HV_MLP_OUT):
+ * A 65 -> 128 (ReLU) -> 2 MLP. Features of a query (HV_MLP_IN): log1p of the 64 smallest upper
+ * bounds of its squared centroid distances that the coarse step forms on the matrix cores
+ * (d~ + kappa (|q|^2 + |c|^2), d~ = |q|^2 + |c|^2 - 2 q.c, kappa = 1.02 (12 d + 8) 2^-24: between
+ * the FP32 chain distance and that plus 2 kappa (|q|^2 + |c|^2); exact distances when nlist >
+ * 4096), ascending, then log1p(|q|^2). The learned search needs nprobe <= 512 when nlist <= 4096
+ * (its features then come from the same rows). Targets (raw, HV_MLP_OUT):
  * log1p(k-th neighbour distance), log(nprobe needed for the target recall). The network maps
  * standardized features to standardized targets (hv_mlp_set_norm). Parameters, flat FP32
  * (HV_MLP_PARAMS): W1 [65][128], b1 [128], W2 [128][2], b2 [2]. Training (FP32, Adam) and

@@ -97,11 +97,28 @@ def test_adam_step_and_training_match_oracle(w1):
 
 
 def test_query_features_match_oracle(w1):
+    """The distance features are the coarse step's upper bounds d~ + kappa (|q|^2 + |c|^2) of the
+    chain distances (d~ the MFMA estimate, |d~ - chain| <= kappa (|q|^2 + |c|^2)), sorted: each
+    lies between the exact distance of its rank and that plus 2 kappa (|q|^2 + max |c|^2)
+    (sorting moves no value by more than the largest shift), so its log1p is at least the
+    oracle's and above it by at most that over 1 + distance; the norm feature is the oracle's to
+    rounding."""
     _, xq, g = w1
-    f = g.query_features(xq)
+    f = g.query_features(xq).astype(np.float64)
     _, pd = O.coarse(xq, g.centroids(), 64)
-    ref = M.features_from_coarse(pd, xq)
-    np.testing.assert_allclose(f, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    ref = M.features_from_coarse(pd, xq).astype(np.float64)
+    d = xq.shape[1]
+    kappa = 1.02 * (12 * d + 8) / 2.0 ** 24
+    c = g.centroids().astype(np.float64)
+    err = kappa * ((xq.astype(np.float64) ** 2).sum(1) + (c ** 2).sum(1).max())
+    pd64 = pd.astype(np.float64)
+    eps = 4e-7 * np.abs(ref[:, :64]) + 1e-7  # FP32 log1p and chain roundings
+    diff = f[:, :64] - ref[:, :64]
+    assert np.all(diff >= -eps)
+    assert np.all(diff <= 2 * err[:, None] / (1 + pd64) + eps)
+    np.testing.assert_allclose(f[:, 64], ref[:, 64], rtol=0, atol=2e-6 * np.abs(ref).max())
+    assert np.all(np.diff(f[:, :64], axis=1) >= 0)
+    say("features: largest shift %.3g (bound %.3g)" % (diff.max(), (2 * err).max()))
 
 
 @pytest.fixture(scope="module")
