@@ -439,7 +439,7 @@ def main():
     #    ids, checked at the operating point); operating point = smallest nprobe reaching the target
     sweep = sorted(int(x) for x in args.sweep.split(","))
     t0 = time.perf_counter()
-    index.nprobe = max(sweep)
+    index.nprobe = max(sweep + [16])  # the ground truth's candidate bound comes from this search
     index.preSearch(pruning=True, dim_blocks=B)
     GD, GI = index.ground_truth_f64(xq, k)
     GD32 = GD.astype(np.float32)

@@ -120,6 +120,116 @@ __device__ __forceinline__ void wave_append(bool pred, unsigned long long key, u
     if (pred) out[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
 }
 
+// k-th smallest (1-based) of the na keys of a and the nb keys of b (block-wide; every thread
+// gets it; na + nb < 2^24). Radix select over the bytes of the 64-bit keys, from the highest byte
+// in which the smallest and the largest key differ (the bytes above it are common to every key,
+// so their passes would put every key in one bin); it stops as soon as the rank-k bin holds
+// exactly the keys of ranks <= k that remain (the k-th is then the largest key with that prefix,
+// one max reduction), so distinct distances take two or three passes instead of eight.
+__device__ inline unsigned long long block_radix_kth(const unsigned long long* a, int na, const unsigned long long* b,
+                                                     int nb, int k, unsigned int* hist,
+                                                     unsigned long long* shared_out) {
+    const int n = na + nb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = (blockDim.x + 63) >> 6;
+    unsigned long long* red = reinterpret_cast<unsigned long long*>(hist);  // 2 words per wavefront
+    unsigned long long mn = ~0ull, mx = 0ull;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long key = i < na ? a[i] : b[i - na];
+        mn = key < mn ? key : mn;
+        mx = key > mx ? key : mx;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_xor(mn, o, 64), y = __shfl_xor(mx, o, 64);
+        mn = x < mn ? x : mn;
+        mx = y > mx ? y : mx;
+    }
+    if (lane == 0) {
+        red[2 * wave] = mn;
+        red[2 * wave + 1] = mx;
+    }
+    __syncthreads();
+    mn = red[0];
+    mx = red[1];
+    for (int w = 1; w < nwave; ++w) {
+        mn = red[2 * w] < mn ? red[2 * w] : mn;
+        mx = red[2 * w + 1] > mx ? red[2 * w + 1] : mx;
+    }
+    __syncthreads();  // red shares hist's words
+    if (mn == mx) return mn;
+    int shift = ((63 - __builtin_clzll(mn ^ mx)) >> 3) << 3;
+    unsigned long long mask = shift + 8 >= 64 ? 0ull : ~0ull << (shift + 8);
+    unsigned long long prefix = mn & mask;
+    int kk = k;
+    for (; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0u;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const unsigned long long key = i < na ? a[i] : b[i - na];
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // wavefront 0: 4 bins per lane, inclusive scan, find the bin holding rank kk
+            unsigned int h[4], sum = 0u;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                h[t] = hist[lane * 4 + t];
+                sum += h[t];
+            }
+            unsigned int incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const unsigned int excl = incl - sum;
+            if (excl < (unsigned int)kk && (unsigned int)kk <= incl) {
+                unsigned int run = excl;
+                int digit = lane * 4 + 3;
+                unsigned int below = run + h[0] + h[1] + h[2], cnt = h[3];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (run + h[t] >= (unsigned int)kk) {
+                        digit = lane * 4 + t;
+                        below = run;
+                        cnt = h[t];
+                        break;
+                    }
+                    run += h[t];
+                }
+                shared_out[0] = ((unsigned long long)digit << 48) | ((unsigned long long)cnt << 24) | below;
+            }
+        }
+        __syncthreads();
+        const unsigned long long r = shared_out[0];
+        kk -= (int)(r & 0xffffffull);
+        prefix |= ((r >> 48) & 255ull) << shift;
+        mask |= 255ull << shift;
+        __syncthreads();
+        if (kk == (int)((r >> 24) & 0xffffffull)) {
+            // every key with this prefix has rank <= k: the k-th is the largest of them
+            unsigned long long m = 0ull;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const unsigned long long key = i < na ? a[i] : b[i - na];
+                if ((key & mask) == prefix && key > m) m = key;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long x = __shfl_xor(m, o, 64);
+                m = x > m ? x : m;
+            }
+            if (lane == 0) red[wave] = m;
+            __syncthreads();
+            m = red[0];
+            for (int w = 1; w < nwave; ++w) m = red[w] > m ? red[w] : m;
+            __syncthreads();
+            return m;
+        }
+    }
+    return prefix;
+}
+
 // Sum over the wavefront of a per-lane count below 128, as a wave-uniform value: one ballot per
 // bit of the count, no cross-lane data movement.
 __device__ __forceinline__ int wave_sum_small(unsigned int x) {

@@ -222,24 +222,63 @@ __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* _
         });
 }
 
-// workgroups stride over the queries (most are skipped: the wavefront select did them)
+// The queries the wavefront select left (more keys than it sorts, or not marked done): the
+// workgroups read the counts of 256 queries at a time in parallel and list the ones to do in LDS.
+// A query's k smallest keys are found by a block radix select (block_radix_kth) and only those
+// k are sorted (LDS bitonic), instead of sorting all of its up to cap keys.
 __global__ __launch_bounds__(256) void k_select_block(const unsigned long long* __restrict__ surv,
                                                       const unsigned int* __restrict__ surv_cnt, int64_t nq, int cap,
                                                       int k, float* __restrict__ D, int64_t* __restrict__ I,
                                                       const unsigned int* __restrict__ done) {
-    extern __shared__ unsigned long long s_keys[];
-    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
-        const unsigned int n = surv_cnt[q];
-        if (n > (unsigned int)cap) continue;  // overflow: the exact path
-        if (done ? done[q] != 0u : max((int)n, k) <= kWaveSortMax) continue;  // the wavefront select did it
-        const int P = hv_next_pow2(max((int)n, k));
-        for (int i = threadIdx.x; i < P; i += blockDim.x) s_keys[i] = i < (int)n ? surv[q * cap + i] : HV_KEY_PAD;
-        block_bitonic_sort(s_keys, P);
-        for (int i = threadIdx.x; i < k; i += blockDim.x) {
-            D[q * k + i] = hv_key_dist(s_keys[i]);
-            I[q * k + i] = hv_key_id(s_keys[i]);
+    extern __shared__ unsigned long long s_keys[];  // [P = pow2(max(cap, k))] | [pow2(k)]
+    __shared__ int s_list[256];
+    __shared__ int s_n, s_cnt;
+    __shared__ unsigned int s_hist[256];
+    __shared__ unsigned long long s_sel;
+    const int pk = hv_next_pow2(max(k, 2));
+    unsigned long long* s_top = s_keys + hv_next_pow2(max(max(cap, k), 2));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x; base < nq; base += stride) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        const int64_t q0 = base + (int64_t)threadIdx.x * gridDim.x;
+        if (q0 < nq) {
+            const unsigned int n = surv_cnt[q0];
+            // overflow: the exact path; otherwise unless the wavefront select did it
+            const bool need = n <= (unsigned int)cap && (done ? done[q0] == 0u : max((int)n, k) > kWaveSortMax);
+            if (need) s_list[atomicAdd(&s_n, 1)] = (int)q0;
         }
-        __syncthreads();  // s_keys is refilled for the next query
+        __syncthreads();
+        const int todo = s_n;
+        for (int j = 0; j < todo; ++j) {
+            const int64_t q = s_list[j];
+            const int n = (int)surv_cnt[q];
+            for (int i = threadIdx.x; i < n; i += blockDim.x) s_keys[i] = surv[q * cap + i];
+            if (threadIdx.x == 0) s_cnt = 0;
+            __syncthreads();
+            if (n > k) {
+                // the k keys <= the k-th (keys are unique) into s_top
+                const unsigned long long kv = block_radix_kth(s_keys, n, nullptr, 0, k, s_hist, &s_sel);
+                for (int i0 = 0; i0 < n; i0 += blockDim.x) {
+                    const int i = i0 + threadIdx.x;
+                    const unsigned long long key = i < n ? s_keys[i] : HV_KEY_PAD;
+                    wave_append(i < n && key <= kv, key, s_top, &s_cnt);
+                }
+            } else {
+                for (int i = threadIdx.x; i < n; i += blockDim.x) s_top[i] = s_keys[i];
+                if (threadIdx.x == 0) s_cnt = n;
+            }
+            __syncthreads();
+            const int m = s_cnt;
+            for (int i = m + threadIdx.x; i < pk; i += blockDim.x) s_top[i] = HV_KEY_PAD;
+            block_bitonic_sort(s_top, pk);
+            for (int i = threadIdx.x; i < k; i += blockDim.x) {
+                D[q * k + i] = hv_key_dist(s_top[i]);
+                I[q * k + i] = hv_key_id(s_top[i]);
+            }
+            __syncthreads();  // s_keys and s_top are refilled for the next query
+        }
     }
 }
 
@@ -281,12 +320,16 @@ void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const in
                        check_bound, n, list, list_cnt, counts, qn_out, qn_full, bound_out);
 }
 
+static size_t select_block_lds(int cap, int k) {
+    return sizeof(unsigned long long) * (hv_next_pow2(std::max(std::max(cap, k), 2)) + hv_next_pow2(std::max(k, 2)));
+}
+
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
                          float* D, int64_t* I, const unsigned int* done, hipStream_t s) {
     if (nq <= 0) return;
-    const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
     const dim3 grid((unsigned)std::min<int64_t>(nq, 1024));
-    hipLaunchKernelGGL(k_select_block, grid, dim3(256), lds, s, surv, surv_cnt, nq, cap, k, D, I, done);
+    hipLaunchKernelGGL(k_select_block, grid, dim3(256), select_block_lds(cap, k), s, surv, surv_cnt, nq, cap, k, D, I,
+                       done);
 }
 
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
@@ -296,9 +339,9 @@ void launch_select_survivors(const unsigned long long* surv, const unsigned int*
     hipLaunchKernelGGL(k_select_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, surv_cnt, nq, cap, k, D,
                        I, overflow_list, overflow_cnt, stats);
     if (cap > kWaveSortMax || k > kWaveSortMax) {
-        const size_t lds = sizeof(unsigned long long) * hv_next_pow2(max(max(cap, k), 2));
         const dim3 grid((unsigned)std::min<int64_t>(nq, 1024));
-        hipLaunchKernelGGL(k_select_block, grid, dim3(256), lds, s, surv, surv_cnt, nq, cap, k, D, I, nullptr);
+        hipLaunchKernelGGL(k_select_block, grid, dim3(256), select_block_lds(cap, k), s, surv, surv_cnt, nq, cap, k, D,
+                           I, nullptr);
     }
 }
 
