@@ -1022,7 +1022,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              h->m_params.as<float>(), h->m_norm.as<float>(), h->sp.learned_nprobe_scale,
                              h->sp.learned_bound_scale, std::max(1, h->sp.learned_min_nprobe), (int)nprobe_cap,
                              h->d_lrad.as<float>(), h->d_centroids.as<float>(),
-                             learned_seed ? h->sp.learned_margin : 0.f, slack,
+                             learned_seed ? h->sp.learned_margin : 0.f, slack, h->d_list_off.as<int64_t>(), (int)k,
                              h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(),
                              learned_seed ? h->s_bound.as<float>() : nullptr,
                              learned_seed ? h->s_bound_ref.as<float>() : nullptr, h->s_stats.as<unsigned long long>() + 14,

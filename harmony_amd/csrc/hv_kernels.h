@@ -207,8 +207,8 @@ void launch_mlp_serve(const float* near, const float* pdist, const int* probes, 
                       int64_t n,
                       const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
                       float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
-                      float margin, float slack, int* qn, float* bound, float* pred, float* bound2, float* bound3,
-                      unsigned long long* widened, hipStream_t s);
+                      float margin, float slack, const int64_t* list_off, int kmin, int* qn, float* bound,
+                      float* pred, float* bound2, float* bound3, unsigned long long* widened, hipStream_t s);
 // lrad[l] = max over list l of sqrt(rnorm) (the list's radius around its centroid)
 void launch_list_radius(const float* rnorm, const int64_t* list_off, int nlist, float* lrad, hipStream_t s);
 

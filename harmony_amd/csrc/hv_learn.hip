@@ -236,8 +236,9 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 }
 
 // Serving: per query, the probe count qn = clamp(ceil(exp(y1) * nprobe_scale), min_nprobe,
-// nprobe) and the seeded bound R = expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also
-// into bound2 and bound3 when given (the search's working and reference bounds start there).
+// nprobe), raised (with list_off) to the shortest prefix of the probe list holding kmin vectors,
+// and the seeded bound R = expm1(y0) * bound_scale (FLT_MAX when bound_scale <= 0), also into
+// bound2 and bound3 when given (the search's working and reference bounds start there).
 // The features are formed here, in LDS, from the query's kNearest smallest coarse distance
 // estimates (near, ascending) and the query, with k_features' arithmetic (so the same bits);
 // pdist / probes (pc per query) are the exact probe lists the margin test walks.
@@ -253,7 +254,8 @@ void launch_mlp_predict(const float* X, int64_t n, const __bf16* w1, const __bf1
 // g^2 <= R * margin, with relative slack for the FP32 roundings of the distances, norms and
 // square roots involved. With margin >= 1, and once the search has checked that the query's k-th
 // result is <= R, no list left out holds a vector of the top k over all nprobe lists: the query's
-// result is the plain search's at nprobe. Queries whose count grew are counted in *widened.
+// result is the plain search's at nprobe. Queries whose count the margin test grew are counted in
+// *widened.
 __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ near, const float* __restrict__ pdist,
                                                    const int* __restrict__ probes,
                                                    int pc, const float* __restrict__ Qp, int d, int64_t n,
@@ -261,7 +263,8 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
                                                    const float* __restrict__ P, const float* __restrict__ norm,
                                                    float nprobe_scale, float bound_scale, int min_nprobe, int nprobe,
                                                    const float* __restrict__ lrad, const float* __restrict__ cent,
-                                                   float margin, float slack,
+                                                   float margin, float slack, const int64_t* __restrict__ list_off,
+                                                   int kmin,
                                                    int* __restrict__ qn, float* __restrict__ bound,
                                                    float* __restrict__ pred, float* __restrict__ bound2,
                                                    float* __restrict__ bound3, unsigned long long* __restrict__ widened) {
@@ -288,11 +291,27 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
     const float b =
         bound_scale > 0.f ? fminf(expm1f(fmaxf(y0, 0.f)) * bound_scale, 3.402823466e38f) : 3.402823466e38f;
     int c = qpred;
+    // enough vectors for a full result: a query whose first lists hold fewer than kmin vectors
+    // could not fill its top k (and would be redone over every list); its count grows to the
+    // shortest prefix of its probe list that holds kmin
+    if (list_off) {
+        int64_t have = 0;
+        for (int j = 0; j < c; ++j) {
+            const int l = probes[q * pc + j];
+            have += list_off[l + 1] - list_off[l];
+        }
+        while (have < kmin && c < nprobe) {
+            const int l = probes[q * pc + c];
+            have += list_off[l + 1] - list_off[l];
+            ++c;
+        }
+    }
+    const int ccov = c;
     if (margin > 0.f && b < 3.402823466e38f && lrad != nullptr) {
         const float thr = b * margin * (1.f + slack);
         const float d0 = fmaxf(pdist[q * pc], 0.f);
         const float* c0 = cent + (int64_t)probes[q * pc] * d;
-        for (int j = qpred; j < nprobe; ++j) {  // wave-uniform loop over the ranks past the prediction
+        for (int j = ccov; j < nprobe; ++j) {  // wave-uniform loop over the ranks past the prediction
             const int l = probes[q * pc + j];
             const float dl = fmaxf(pdist[q * pc + j], 0.f);
             const float* cl = cent + (int64_t)l * d;
@@ -311,7 +330,7 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
     }
     if (lane == 0) {
         qn[q] = c;
-        if (c > qpred && widened) atomicAdd(widened, 1ull);
+        if (c > ccov && widened) atomicAdd(widened, 1ull);
         bound[q] = b;
         if (bound2) bound2[q] = b;
         if (bound3) bound3[q] = b;
@@ -324,12 +343,12 @@ void launch_mlp_serve(const float* near, const float* pdist, const int* probes, 
                       int64_t n,
                       const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
                       float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
-                      float margin, float slack, int* qn, float* bound, float* pred, float* bound2, float* bound3,
-                      unsigned long long* widened, hipStream_t s) {
+                      float margin, float slack, const int64_t* list_off, int kmin, int* qn, float* bound,
+                      float* pred, float* bound2, float* bound3, unsigned long long* widened, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, near, pdist, probes, pc, Qp, d, n, w1,
-                       w2, P, norm, nprobe_scale, bound_scale, min_nprobe, nprobe, lrad, cent, margin, slack, qn, bound,
-                       pred, bound2, bound3, widened);
+                       w2, P, norm, nprobe_scale, bound_scale, min_nprobe, nprobe, lrad, cent, margin, slack, list_off,
+                       kmin, qn, bound, pred, bound2, bound3, widened);
 }
 
 // Radius of every list: the largest |x - c| over its vectors, from their squared residual norms

@@ -669,12 +669,12 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
 // ---------------------------------------------------------------------------------------
 // Exact path for the queries the search redoes (overflowed candidate buffers, learned-mode
 // fallbacks). k_exact_lists: one workgroup per (query, probed list, segment of kExactSpan
-// vectors), one candidate per thread; candidates that pass the bound and beat the current k-th
-// key are collected in LDS (up to kExactSeg per segment of the list; one LDS atomic per
-// wavefront), the k-th smallest key of the collected set is found by a block-wide radix select
-// over the bytes of the 64-bit keys, which are unique, and only the keys up to it are kept. It appends its (at most k) best
-// keys to the query's run buffer (an atomic count per query, order free). k_exact_merge: one
-// workgroup per query reduces the appended keys to the final top-k the same way and sorts them.
+// vectors), one candidate per thread; candidates that pass the bound are collected in LDS (one
+// LDS atomic per wavefront), the k-th smallest key of the collected set is found by a block-wide
+// radix select over the bytes of the 64-bit keys, which are unique, and only the keys up to it
+// are kept. It appends its (at most k) best keys to the query's run buffer (an atomic count per
+// query, order free). k_exact_merge: one workgroup per query reduces the appended keys to the
+// final top-k the same way (kExactSeg keys collected between reductions) and sorts them.
 // Same arithmetic, pruning and (distance <= bound) filter as the scan, so the result is the same.
 constexpr int kExactTop = 1024, kExactSeg = 4096, kExactSpan = 256;
 
@@ -738,11 +738,12 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
                                                      float slack, unsigned long long* __restrict__ scanned) {
     constexpr unsigned long long kPad = IP ? HV_IP_KEY_PAD : HV_KEY_PAD;
     extern __shared__ __align__(16) unsigned char s_raw[];
-    // kept keys [kExactTop] | keys collected this segment [kExactSeg] | reduction output [kExactTop]
+    // kept keys [k] | keys collected from the segment [kExactSpan] | reduction output [k]: a segment
+    // holds at most kExactSpan vectors, one per thread, so it is collected in one round
     unsigned long long* s = reinterpret_cast<unsigned long long*>(s_raw);
-    unsigned long long* s_col = s + kExactTop;
-    unsigned long long* s_out = s_col + kExactSeg;
-    float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * (2 * kExactTop + kExactSeg));
+    unsigned long long* s_col = s + k;
+    unsigned long long* s_out = s_col + kExactSpan;
+    float* qs = reinterpret_cast<float*>(s_raw + sizeof(unsigned long long) * (2 * k + kExactSpan));
     __shared__ unsigned int s_hist[256];
     __shared__ unsigned long long s_sel;
     __shared__ int s_cnt, s_top, s_base;
@@ -823,7 +824,7 @@ __global__ __launch_bounds__(256) void k_exact_lists(const int* __restrict__ qli
             __syncthreads();
             // reduce when the next round might not fit, and at the end; every thread reads the
             // count before any wavefront can append the next round's keys to it
-            const bool reduce = s_cnt > kExactSeg - (int)blockDim.x || t0 + (int)blockDim.x >= n;
+            const bool reduce = s_cnt > kExactSpan - (int)blockDim.x || t0 + (int)blockDim.x >= n;
             __syncthreads();
             if (reduce) topk_reduce(s, s_col, s_out, k, s_top, s_cnt, s_kth, s_hist, &s_sel);
         }
@@ -903,7 +904,7 @@ void launch_query_exact(const int* qlist, const int* nql_dev, int q0, int nql_ma
                         const ChunkPlan& plan, int prune, const float* bound, int k, unsigned long long* runs, float* D,
                         int64_t* I, bool counts_zeroed, hipStream_t s) {
     if (nql_max <= 0) return;
-    const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
+    const size_t lds = sizeof(unsigned long long) * (2 * k + kExactSpan) + sizeof(float) * ((d + 3) & ~3);
     const int64_t pairs = (int64_t)nql_max * nranks * nseg;
     const int64_t stride = (int64_t)nranks * nseg * k;
     int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * stride);
@@ -923,7 +924,7 @@ void launch_query_exact_ip(const int* qlist, const int* nql_dev, int q0, int nql
                            int k, unsigned long long* runs, bool append, float* D, int64_t* I,
                            unsigned long long* scanned, hipStream_t s) {
     if (nql_max <= 0 || r1 <= r0) return;
-    const size_t lds = sizeof(unsigned long long) * (2 * kExactTop + kExactSeg) + sizeof(float) * ((d + 3) & ~3);
+    const size_t lds = sizeof(unsigned long long) * (2 * k + kExactSpan) + sizeof(float) * ((d + 3) & ~3);
     const int64_t pairs = (int64_t)nql_max * (r1 - r0) * nseg;
     const int64_t stride = (int64_t)nprobe * nseg * k;  // room for every rank's runs
     int* run_cnt = reinterpret_cast<int*>(runs + (int64_t)nql_max * stride);

@@ -16,6 +16,7 @@ p such that at least ceil(target * k) of those neighbours lie in the query's p n
 (a search over those p lists returns all of them: they are the best candidates it sees).
 """
 import math
+import time
 
 import numpy as np
 
@@ -100,41 +101,55 @@ def recall_per_query(I, exact):
 
 
 def calibrate(index, feats, targets, queries, exact, nprobe, k, target_recall=0.95, margin=0.01,
-              bound_quantile=1.0, bound_margin=1.25,
+              bound_quantile=1.0, bound_margins=(1.25, 1.5, 2.0),
               scales=(0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0, 4.0),
-              widen_margins=(0.0, 0.5, 1.0)):
-    """Serving settings from held-out queries. The bound scale is bound_margin times the
-    bound_quantile of true / predicted k-th distance, so that nearly every query keeps its seeded
-    bound (a query whose seed is below its k-th distance is redone exactly over all nprobe lists,
-    ~50 us for a handful of queries, while a looser seed costs little: the compaction after each
-    scan phase lowers the bound to the k-th upper bound anyway). Then, for each widening margin
-    (0: the predicted counts alone; 1: counts widened to every list the triangle and Voronoi bounds
-    cannot exclude within the seed, so results equal the plain search's at nprobe), the smallest of
-    `scales` whose mean recall against the exact ids reaches target_recall + margin; of those the
-    setting with the fewest candidates. Leaves the index in learned mode with it; returns
-    (nprobe_scale, bound_scale, recall, mean probes); the chosen widening margin is in
-    index._params.learned_margin."""
+              widen_margins=(0.0, 0.5, 1.0), reps=5):
+    """Serving settings from held-out queries. A bound scale is a margin times the bound_quantile
+    of true / predicted k-th distance, so that nearly every query keeps its seeded bound (a query
+    whose seed is below its k-th distance is redone exactly over all nprobe lists, while a looser
+    seed costs little: the compaction after each scan phase lowers the bound to the k-th upper
+    bound anyway). For each bound margin and each widening margin (0: the predicted counts alone;
+    1: counts widened to every list the triangle and Voronoi bounds cannot exclude within the
+    seed, so results equal the plain search's at nprobe), the smallest of `scales` whose mean
+    recall against the exact ids reaches target_recall + margin qualifies; the qualifying setting
+    with the shortest measured search time on these queries (the mean of `reps` searches after
+    one) is chosen. Leaves the index in learned mode with it; returns (nprobe_scale, bound_scale,
+    recall, mean probes); the chosen widening margin is in index._params.learned_margin."""
     pred = index.mlp_predict(feats)
     ratio = np.expm1(targets[:, 0].astype(np.float64)) / np.maximum(np.expm1(pred[:, 0].astype(np.float64)), 1e-30)
-    bound_scale = float(max(1.0, bound_margin * np.quantile(ratio, bound_quantile)))
+    q_ratio = float(np.quantile(ratio, bound_quantile))
     index.nprobe = nprobe
     q = np.ascontiguousarray(queries, dtype=np.float32)
     n = q.shape[0]
     qd, Dd, Id = DeviceArray.from_host(q), DeviceArray(n * k * 4), DeviceArray(n * k * 8)
-    best, fallback = None, None
-    for wm in widen_margins:
-        for s in scales:
-            index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale, learned_margin=wm)
-            st = index.search_device(qd, n, k, Dd, Id)
-            I = Id.to_host(np.int64, (n, k))
-            rec = float(recall_per_query(I, exact).mean())
-            cand = (st["candidates"], s, wm, rec, float(index.last_nprobe().mean()))
-            fallback = cand if fallback is None or rec > fallback[3] else fallback
-            if rec >= target_recall + margin:
-                if best is None or cand[0] < best[0]:
-                    best = cand
-                break
-    pick = best if best is not None else fallback
-    _, s, wm, rec, probes = pick
-    index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=bound_scale, learned_margin=wm)
+
+    def apply(s, b, wm):
+        index.preSearch(learned=True, learned_nprobe_scale=s, learned_bound_scale=b, learned_margin=wm)
+
+    qualified, fallback = [], None
+    for bm in bound_margins:
+        bound_scale = float(max(1.0, bm * q_ratio))
+        for wm in widen_margins:
+            for s in scales:
+                apply(s, bound_scale, wm)
+                index.search_device(qd, n, k, Dd, Id)
+                I = Id.to_host(np.int64, (n, k))
+                rec = float(recall_per_query(I, exact).mean())
+                cand = (s, bound_scale, wm, rec, float(index.last_nprobe().mean()))
+                fallback = cand if fallback is None or rec > fallback[3] else fallback
+                if rec >= target_recall + margin:
+                    qualified.append(cand)
+                    break
+    best, best_t = None, None
+    for cand in qualified:
+        apply(cand[0], cand[1], cand[2])
+        index.search_device(qd, n, k, Dd, Id)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            index.search_device(qd, n, k, Dd, Id)
+        t = (time.perf_counter() - t0) / reps
+        if best_t is None or t < best_t:
+            best, best_t = cand, t
+    s, bound_scale, wm, rec, probes = best if best is not None else fallback
+    apply(s, bound_scale, wm)
     return s, bound_scale, rec, probes

@@ -96,7 +96,8 @@ typedef struct {
     int32_t tile_shape_first; /* same, for the nearest-list phase */
     /* learned probe/bound predictor (no counterpart in the reference; needs a trained model) */
     int32_t learned;              /* 1: per-query nprobe and pruning bound from the predictor */
-    float learned_nprobe_scale;   /* probes = ceil(predicted nprobe * scale), at most nprobe */
+    float learned_nprobe_scale;   /* probes = ceil(predicted nprobe * scale), at most nprobe, and at
+                                     least enough lists to hold k vectors */
     float learned_bound_scale;    /* seeded bound = predicted k-th distance * scale (<= 0: none) */
     int32_t learned_min_nprobe;   /* lower limit of the per-query probe count */
     /* FP8 form (HV_PRECISION_FP8): shortlist width factor r, k * r <= 4096 (default 4) */

@@ -134,9 +134,9 @@ def w1_model(w1):
 
 
 def test_serving_outputs_follow_predictions(w1, w1_model):
-    """The search's predictor (weights staged in LDS, several queries per wavefront) gives each
-    query qn = clamp(ceil(exp(y1) * s_np), min, nprobe) and seed = expm1(max(y0, 0)) * s_b from
-    the same network outputs as hv_mlp_predict on the query's features."""
+    """The search's predictor gives each query qn = clamp(ceil(exp(y1) * s_np), min, nprobe), raised
+    to the shortest prefix of its probe list that holds k vectors, and seed = expm1(max(y0, 0)) *
+    s_b from the same network outputs as hv_mlp_predict on the query's features."""
     xb, xq, g = w1
     g.nprobe = 10
     s_np, s_b = 1.3, 1.1
@@ -146,9 +146,15 @@ def test_serving_outputs_follow_predictions(w1, w1_model):
     qn = g.last_nprobe(xq.shape[0])
     seed = g.last_bounds(xq.shape[0])
     g.preSearch(learned=False, precision=HV_PRECISION_FP32, learned_min_nprobe=1, learned_margin=1.0)
+    probes = g.last_probes(xq.shape[0])[:, :10]
     y = g.mlp_predict(g.query_features(xq)).astype(np.float64)
     np_f = np.exp(y[:, 1]) * s_np
     ref_qn = np.where(np_f >= 10, 10, np.maximum(2, np.ceil(np_f))).clip(1, 10).astype(np.int32)
+    # then at least as many lists as hold k = 10 vectors
+    held = np.cumsum(np.diff(g.list_offsets())[probes], axis=1)
+    cover = np.minimum(np.argmax(held >= 10, axis=1) + 1, 10)
+    cover[held[:, -1] < 10] = 10
+    ref_qn = np.maximum(ref_qn, cover)
     away = np.abs(np_f - np.round(np_f)) > 1e-4  # not at an integer (last-bit exp differences)
     np.testing.assert_array_equal(qn[away], ref_qn[away])
     ref_seed = np.expm1(np.maximum(y[:, 0], 0.0)) * s_b
