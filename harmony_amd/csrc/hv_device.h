@@ -334,3 +334,55 @@ __device__ __forceinline__ void wave_sort_emit(int n, KeyOf&& key_of, Emit&& emi
     else wave_sort_emit_e<32>(n, key_of, emit);
 }
 constexpr int kWaveSortMax = 2048;
+
+// The k smallest of n (<= kWaveSortMax) unique 64-bit (distance, id) keys handed to emit(rank,
+// key) in order (ranks >= k may follow, as from wave_sort_emit), with one wavefront: for n well
+// above k, the k-th smallest distance word by wave_kth_u32, then only the keys whose distance is
+// <= it (every key of the top k, and at least k keys) gathered into s_buf (kTopkBuf keys of LDS
+// for this wavefront) and sorted; rows with few keys, or with more ties at the k-th distance than
+// the buffer holds, are sorted whole. s_tmp: 64 words of LDS for this wavefront. All 64 lanes must
+// call it with the same n and k.
+constexpr int kTopkBuf = 512;
+template <int E, typename KeyOf, typename Emit>
+__device__ __forceinline__ void wave_topk_emit_e(int n, int k, KeyOf&& key_of, unsigned int* s_tmp,
+                                                 unsigned long long* s_buf, Emit&& emit) {
+    const int lane = threadIdx.x & 63;
+    unsigned int u[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = e * 64 + lane;
+        u[e] = j < n ? (unsigned int)(key_of(j) >> 32) : 0xffffffffu;
+    }
+    const unsigned int tau = wave_kth_u32<E>(u, k, s_tmp);
+    int cnt = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = e * 64 + lane;
+        const bool in = j < n && u[e] <= tau;
+        const unsigned long long m = __ballot(in);
+        const int pos = cnt + (int)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (in && pos < kTopkBuf) s_buf[pos] = key_of(j);
+        cnt += (int)__builtin_popcountll(m);
+    }
+    if (cnt > kTopkBuf) {
+        wave_sort_emit(n, key_of, emit);
+        return;
+    }
+    __threadfence_block();  // the keys other lanes stored are read below
+    wave_sort_emit(cnt, [&](int j) { return s_buf[j]; }, emit);
+}
+
+template <typename KeyOf, typename Emit>
+__device__ __forceinline__ void wave_topk_emit(int n, int k, KeyOf&& key_of, unsigned int* s_tmp,
+                                               unsigned long long* s_buf, Emit&& emit) {
+    auto keys = [&](int j) { return j < n ? key_of(j) : HV_KEY_PAD; };
+    if (n <= 256 || n <= 2 * k || 2 * k > kTopkBuf) {
+        wave_sort_emit(max(n, k), keys, emit);
+    } else if (n <= 512) {
+        wave_topk_emit_e<8>(n, k, keys, s_tmp, s_buf, emit);
+    } else if (n <= 1024) {
+        wave_topk_emit_e<16>(n, k, keys, s_tmp, s_buf, emit);
+    } else {
+        wave_topk_emit_e<32>(n, k, keys, s_tmp, s_buf, emit);
+    }
+}

@@ -349,11 +349,18 @@ __global__ __launch_bounds__(256) void k_list_scan(const float* __restrict__ Qp,
 
     for (;;) {
         if (tid == 0) {
+            // own XCD's queue first, then the others'. A head only grows, so a plain load that
+            // already shows a queue exhausted (a stale value is only ever smaller) skips the
+            // atomic: the read-modify-writes on a head are device-scope and serialize across the
+            // XCDs, and without the check every workgroup would issue one on every queue as the
+            // work runs out.
             int it = -1;
             for (int o = 0; o < kNumXcd && it < 0; ++o) {
                 const int x = (xcd + o) % kNumXcd;
+                const int len = queue_len[x];
+                if (__hip_atomic_load(&queue_head[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
                 const int slot = atomicAdd(&queue_head[x], 1);
-                if (slot < queue_len[x]) it = queue_base[x] + slot;
+                if (slot < len) it = queue_base[x] + slot;
             }
             s_item = it;
         }

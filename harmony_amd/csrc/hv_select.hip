@@ -56,6 +56,20 @@ void launch_select_rows(const float* dist, int64_t n, int m, int p, int* out_ids
     }
 }
 
+// The k-th smallest distance of the n (<= 64 E, k <= n) keys src[0, n), by one wavefront.
+template <int E>
+__device__ __forceinline__ float kth_dist_wave(const unsigned long long* __restrict__ src, int n, int k,
+                                               unsigned int* s_tmp) {
+    const int lane = threadIdx.x & 63;
+    unsigned int u[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = e * 64 + lane;
+        u[e] = j < n ? (unsigned int)(src[j] >> 32) : 0xffffffffu;
+    }
+    return __uint_as_float(wave_kth_u32<E>(u, k, s_tmp));
+}
+
 // ---------------------------------------------------------------------------------------
 // Warm-up bound (Index::warmUpSearch, Index.cpp:999-1051; heapTops = heap top, 1303-1305):
 // the k-th smallest of the query's warm-up distances, FLT_MAX when fewer than k were seen.
@@ -64,6 +78,7 @@ __global__ __launch_bounds__(256) void k_kth_wave(const unsigned long long* __re
                                                   unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
                                                   float* __restrict__ bound_ref, float* __restrict__ bound,
                                                   const int* __restrict__ kq) {
+    __shared__ unsigned int s_tmp[4][64];
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
     const int k = hv_query_k(kq, q, k_all);
@@ -71,11 +86,13 @@ __global__ __launch_bounds__(256) void k_kth_wave(const unsigned long long* __re
     if (n > kWaveSortMax) return;  // k_kth_block
     const unsigned long long* __restrict__ src = surv + q * cap;
     if (n >= k) {
-        wave_sort_emit(
-            n, [&](int j) { return src[j]; },
-            [&](int i, unsigned long long key) {
-                if (i == k - 1) bound_ref[q] = bound[q] = hv_key_dist(key);
-            });
+        // the k-th smallest distance (of the (distance, id) keys, the k-th key's distance)
+        const int w = threadIdx.x >> 6;
+        const float b = n <= 256   ? kth_dist_wave<4>(src, n, k, s_tmp[w])
+                        : n <= 512 ? kth_dist_wave<8>(src, n, k, s_tmp[w])
+                        : n <= 1024 ? kth_dist_wave<16>(src, n, k, s_tmp[w])
+                                    : kth_dist_wave<32>(src, n, k, s_tmp[w]);
+        if ((threadIdx.x & 63) == 0) bound_ref[q] = bound[q] = b;
     } else if ((threadIdx.x & 63) == 0) {
         bound_ref[q] = bound[q] = kFltMax;
     }
@@ -200,6 +217,8 @@ __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* _
                                                      int k, float* __restrict__ D, int64_t* __restrict__ I,
                                                      int* __restrict__ overflow_list, int* __restrict__ overflow_cnt,
                                                      unsigned long long* __restrict__ stats) {
+    __shared__ unsigned int s_tmp[4][64];
+    __shared__ unsigned long long s_buf[4][kTopkBuf];
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
     const unsigned int n = surv_cnt[q];
@@ -212,8 +231,9 @@ __global__ __launch_bounds__(256) void k_select_wave(const unsigned long long* _
     const int m = max((int)n, k);
     if (m > kWaveSortMax) return;  // k_select_block
     const unsigned long long* __restrict__ src = surv + q * cap;
-    wave_sort_emit(
-        m, [&](int j) { return j < (int)n ? src[j] : HV_KEY_PAD; },
+    const int w = threadIdx.x >> 6;
+    wave_topk_emit(
+        (int)n, k, [&](int j) { return src[j]; }, s_tmp[w], s_buf[w],
         [&](int i, unsigned long long key) {
             if (i < k) {
                 D[q * k + i] = hv_key_dist(key);
