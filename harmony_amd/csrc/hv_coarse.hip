@@ -11,7 +11,7 @@
 //    |d~ - chain| (host: kappa from d), the P-th smallest upper bound tau = P-th of (d~ + err)
 //    bounds the P-th chain distance, so the P nearest all have d~ - err <= tau. Only those
 //    candidates get the exact chain; their (distance, id) keys are sorted and the first P kept.
-//    A query with more than kCoarseCand candidates is listed for k_coarse_exact_row (all
+//    A query with more candidates than the buffer holds is listed for k_coarse_exact_row (all
 //    centroids, exact), which strides over the list on the device.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -397,7 +397,8 @@ constexpr int kWaveP = 256;
 
 template <int EC>
 __device__ __forceinline__ void coarse_finish_wave(const unsigned long long* __restrict__ keys, int m, int P,
-                                                   int* __restrict__ probes, float* __restrict__ pdist) {
+                                                   int* __restrict__ probes, float* __restrict__ pdist,
+                                                   int* __restrict__ bucket_cnt, int nranks) {
     const int lane = threadIdx.x & 63;
     unsigned long long v[EC];
 #pragma unroll
@@ -410,8 +411,10 @@ __device__ __forceinline__ void coarse_finish_wave(const unsigned long long* __r
     for (int e = 0; e < EC; ++e) {
         const int r = lane * EC + e;
         if (r < P) {
-            probes[r] = (int)hv_key_id(v[e]);
+            const int id = (int)hv_key_id(v[e]);
+            probes[r] = id;
             pdist[r] = hv_key_dist(v[e]);
+            if (bucket_cnt && r < nranks) atomicAdd(&bucket_cnt[(int64_t)id * nranks + r], 1);
         }
     }
 }
@@ -424,7 +427,8 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
                                                             const float* __restrict__ cn, float kappa, int64_t nq,
                                                             int P, int* __restrict__ probes,
                                                             float* __restrict__ pdist, int* __restrict__ redo_list,
-                                                            int* __restrict__ redo_cnt, float* __restrict__ near) {
+                                                            int* __restrict__ redo_cnt, float* __restrict__ near,
+                                                            int* __restrict__ bucket_cnt, int nranks) {
     __shared__ unsigned int s_cand[4][CAND];
     __shared__ unsigned long long s_ckey[4][CAND];
     __shared__ unsigned int s_tmp[4][64];
@@ -460,29 +464,29 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
         if (in && slot < CAND) s_cand[w][slot] = (unsigned int)c;
         m += (int)__builtin_popcountll(b);
     }
-    if (m > CAND) {  // too wide a band: the exact kernel redoes this query
+    if (m > CAND) {  // too wide a band: k_coarse_exact_row redoes this query (and counts its pairs)
         if (lane == 0) redo_list[atomicAdd(redo_cnt, 1)] = (int)q;
         return;
     }
+    const float* __restrict__ qv = Q + q * d;
+    int* pq = probes + q * P;
+    float* dq = pdist + q * P;
     __threadfence_block();  // the candidates other lanes stored are read below
     // exact keys of the candidates, one chain at a time per lane
-    const float* __restrict__ qv = Q + q * d;
 #pragma unroll 1
     for (int j = lane; j < m; j += 64) {
         const unsigned int c = s_cand[w][j];
         s_ckey[w][j] = hv_key(coarse_chain(qv, C + (int64_t)c * d, d), c);
     }
     __threadfence_block();
-    int* pq = probes + q * P;
-    float* dq = pdist + q * P;
     if (m <= 64)
-        coarse_finish_wave<1>(s_ckey[w], m, P, pq, dq);
+        coarse_finish_wave<1>(s_ckey[w], m, P, pq, dq, bucket_cnt, nranks);
     else if (m <= 128)
-        coarse_finish_wave<2>(s_ckey[w], m, P, pq, dq);
+        coarse_finish_wave<2>(s_ckey[w], m, P, pq, dq, bucket_cnt, nranks);
     else if (CAND <= 256 || m <= 256)
-        coarse_finish_wave<4>(s_ckey[w], m, P, pq, dq);
+        coarse_finish_wave<4>(s_ckey[w], m, P, pq, dq, bucket_cnt, nranks);
     else
-        coarse_finish_wave<8>(s_ckey[w], m, P, pq, dq);
+        coarse_finish_wave<8>(s_ckey[w], m, P, pq, dq, bucket_cnt, nranks);
 }
 
 // Fallback: every centroid's exact chain, LDS sort, first P (one workgroup per listed query).
@@ -490,7 +494,8 @@ __global__ __launch_bounds__(256) void k_coarse_exact_row(const int* __restrict_
                                                           const int* __restrict__ redo_cnt,
                                                           const float* __restrict__ Q, const float* __restrict__ C,
                                                           int nc, int d, int P, int* __restrict__ probes,
-                                                          float* __restrict__ pdist) {
+                                                          float* __restrict__ pdist, int* __restrict__ bucket_cnt,
+                                                          int nranks) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     const int p2 = hv_next_pow2(max(nc, 2));
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(s_raw);  // [p2]
@@ -506,8 +511,10 @@ __global__ __launch_bounds__(256) void k_coarse_exact_row(const int* __restrict_
         __syncthreads();
         block_bitonic_sort(keys, p2);
         for (int r = threadIdx.x; r < P; r += blockDim.x) {
-            probes[q * P + r] = (int)hv_key_id(keys[r]);
+            const int id = (int)hv_key_id(keys[r]);
+            probes[q * P + r] = id;
             pdist[q * P + r] = hv_key_dist(keys[r]);
+            if (bucket_cnt && r < nranks) atomicAdd(&bucket_cnt[(int64_t)id * nranks + r], 1);
         }
     }
 }
@@ -605,16 +612,16 @@ void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int 
     hipLaunchKernelGGL(k_coarse_mfma, grid, dim3(256), 0, s, Q, nq, C, nc, d, qn, cn, approx);
 }
 
-void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
+bool launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
-                          int* redo_list, int* redo_cnt, float* near, hipStream_t s) {
-    if (nq <= 0) return;
+                          int* redo_list, int* redo_cnt, float* near, int* bucket_cnt, int nranks, hipStream_t s) {
+    if (nq <= 0) return bucket_cnt != nullptr;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
     // the wavefront kernel forms the features itself when P <= kNearest
     float* fused = P <= kNearest ? near : nullptr;
 #define HV_CSEL(E, CAND)                                                                                        \
     hipLaunchKernelGGL((k_coarse_select_wave<E, CAND>), wgrid, dim3(256), 0, s, approx, Q, C, nc, d, qn, cn, kappa, \
-                       nq, P, probes, pdist, redo_list, redo_cnt, fused)
+                       nq, P, probes, pdist, redo_list, redo_cnt, fused, bucket_cnt, nranks)
     const bool narrow = P <= 128;
     if (P <= kWaveP && nc <= 512) {
         if (narrow) HV_CSEL(8, 256); else HV_CSEL(8, 512);
@@ -624,15 +631,18 @@ void launch_coarse_select(const float* approx, const float* Q, const float* C, i
         if (narrow) HV_CSEL(32, 256); else HV_CSEL(32, 512);
     } else {
         fused = nullptr;
+        bucket_cnt = nullptr;  // the caller counts the pairs
         const size_t lds = sizeof(unsigned long long) * kCoarseCand + sizeof(float) * ((d + 3) & ~3);
         hipLaunchKernelGGL(k_coarse_select, dim3((unsigned)nq), dim3(256), lds, s, approx, Q, C, nc, d, qn, cn, kappa,
                            P, probes, pdist, redo_list, redo_cnt);
     }
 #undef HV_CSEL
     if (near && !fused) launch_near_rows(approx, nq, nc, qn, cn, kappa, near, s);
+    // the queries whose band overflowed its candidate buffer, exactly
     const size_t lds2 = sizeof(unsigned long long) * hv_next_pow2(nc > 2 ? nc : 2) + sizeof(float) * ((d + 3) & ~3);
     hipLaunchKernelGGL(k_coarse_exact_row, dim3((unsigned)std::min<int64_t>(nq, 256)), dim3(256), lds2, s, redo_list,
-                       redo_cnt, Q, C, nc, d, P, probes, pdist);
+                       redo_cnt, Q, C, nc, d, P, probes, pdist, bucket_cnt, nranks);
+    return bucket_cnt != nullptr;
 }
 
 }  // namespace hv

@@ -497,9 +497,11 @@ int* coarse_counter(hv_index_t* h, int64_t n) {
 }
 // counter_zeroed: the caller has cleared the re-check counter (the first slice skips its fill);
 // near (optional, [n][kNearest]): the kNearest smallest distance estimates of each query, sorted
-// (the learned predictor's features), taken from the same rows
-void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist,
-                   bool counter_zeroed = false, float* near = nullptr) {
+// (the learned predictor's features), taken from the same rows; bucket_cnt (optional): the
+// grouping's (list, rank < nranks) pair counts, added as the probes are written. Returns whether
+// they were (else the caller counts them).
+bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist,
+                   bool counter_zeroed = false, float* near = nullptr, int* bucket_cnt = nullptr, int nranks = 0) {
     const int64_t d = h->d, nlist = h->nlist;
     hipStream_t s = h->stream;
     const int64_t per = coarse_rows_per_slice(h, n);
@@ -509,18 +511,20 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
         HV_HIP(h->s_cnorm.ensure(sizeof(float) * nlist));
         int* redo_cnt = coarse_counter(h, n);
         const float kappa = hv::coarse_kappa((int)d);
+        bool counted = bucket_cnt != nullptr;
         for (int64_t r0 = 0; r0 < n; r0 += per) {
             const int64_t rows = std::min(per, n - r0);
             hv::launch_coarse_mfma(Qp + r0 * d, rows, h->d_centroids.as<float>(), (int)nlist, (int)d,
                                    h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), h->s_dist.as<float>(), s);
             if (r0 > 0 || !counter_zeroed) hv::launch_fill_i32(redo_cnt, 1, 0, s);
-            hv::launch_coarse_select(h->s_dist.as<float>(), Qp + r0 * d, h->d_centroids.as<float>(), (int)nlist,
-                                     (int)d, h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), kappa, rows, (int)P,
-                                     probes + r0 * P, pdist + r0 * P, redo_cnt + 1, redo_cnt,
-                                     near ? near + r0 * hv::kNearest : nullptr, s);
+            counted &= hv::launch_coarse_select(h->s_dist.as<float>(), Qp + r0 * d, h->d_centroids.as<float>(),
+                                                (int)nlist, (int)d, h->s_qnorm.as<float>() + r0,
+                                                h->s_cnorm.as<float>(), kappa, rows, (int)P, probes + r0 * P,
+                                                pdist + r0 * P, redo_cnt + 1, redo_cnt,
+                                                near ? near + r0 * hv::kNearest : nullptr, bucket_cnt, nranks, s);
         }
         HV_CHECK_LAUNCH("coarse");
-        return;
+        return counted;
     }
     for (int64_t r0 = 0; r0 < n; r0 += per) {
         const int64_t rows = std::min(per, n - r0);
@@ -532,6 +536,7 @@ void coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
                                  near + r0 * hv::kNearest, s);
     }
     HV_CHECK_LAUNCH("coarse");
+    return false;
 }
 
 // After a search's device work: read the counters and the redo count (the one host sync), add
@@ -1008,7 +1013,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_pdist.ensure(sizeof(float) * n * nprobe));
     int* probes = h->s_probes.as<int>();
     float* pdist = h->s_pdist.as<float>();
-    coarse_select(h, n, Qp, nprobe, probes, pdist, true, learned ? h->s_near.as<float>() : nullptr);
+    // the grouping's pair counts: by the coarse select when every query searches ranks [0, rank_end)
+    // (a learned search's predictor counts its own)
+    bool counted = coarse_select(h, n, Qp, nprobe, probes, pdist, true, learned ? h->s_near.as<float>() : nullptr,
+                                 learned || uneven ? nullptr : h->s_bucket.as<int>(), (int)rank_end);
     const int* qn = nullptr;
     if (learned) {
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));
@@ -1026,7 +1034,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                              h->s_qn.as<int>(), h->s_seed.as<float>(), h->s_pred.as<float>(),
                              learned_seed ? h->s_bound.as<float>() : nullptr,
                              learned_seed ? h->s_bound_ref.as<float>() : nullptr, h->s_stats.as<unsigned long long>() + 14,
-                             s);
+                             h->s_bucket.as<int>(), (int)rank_end, s);
+        counted = true;
         HV_CHECK_LAUNCH("predictor");
         qn = h->s_qn.as<int>();
     } else if (uneven) {
@@ -1050,7 +1059,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap));
     HV_HIP(h->s_queue.ensure(sizeof(int) * (3 * hv::kNumXcd + nlist)));
     unsigned long long* stats = h->s_stats.as<unsigned long long>();
-    hv::launch_count_pairs(probes, n, (int)nprobe, (int)rank_end, qn, h->s_bucket.as<int>(), s);
+    if (!counted) hv::launch_count_pairs(probes, n, (int)nprobe, (int)rank_end, qn, h->s_bucket.as<int>(), s);
     hv::launch_exclusive_scan(h->s_bucket.as<int>(), nb, h->s_bucket_off.as<int>(), h->s_scan_tmp.as<int>(), s);
     hv::launch_scatter_pairs(probes, n, (int)nprobe, (int)rank_end, qn, h->s_bucket_off.as<int>(), h->s_fill.as<int>(),
                              h->s_pairs.as<int>(), h->s_pair_slot.as<int>(), s);

@@ -208,7 +208,8 @@ void launch_mlp_serve(const float* near, const float* pdist, const int* probes, 
                       const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
                       float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
                       float margin, float slack, const int64_t* list_off, int kmin, int* qn, float* bound,
-                      float* pred, float* bound2, float* bound3, unsigned long long* widened, hipStream_t s);
+                      float* pred, float* bound2, float* bound3, unsigned long long* widened, int* bucket_cnt,
+                      int nranks, hipStream_t s);
 // lrad[l] = max over list l of sqrt(rnorm) (the list's radius around its centroid)
 void launch_list_radius(const float* rnorm, const int64_t* list_off, int nlist, float* lrad, hipStream_t s);
 
@@ -225,10 +226,12 @@ void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int 
 void launch_near_rows(const float* v, int64_t n, int nc, const float* qn, const float* cn, float kappa, float* out,
                       hipStream_t s);
 // P nearest per query, exact (distance, id) order; redo_cnt must be 0 on entry. near (optional):
-// the kNearest smallest upper bounds of each query, sorted, as launch_near_rows forms them
-void launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
+// the kNearest smallest upper bounds of each query, sorted, as launch_near_rows forms them.
+// bucket_cnt (optional): the (list, rank < nranks) pair counts of the grouping are added as the
+// probes are written; returns whether they were (the workgroup kernel leaves them to the caller)
+bool launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,
                           const float* cn, float kappa, int64_t nq, int P, int* probes, float* pdist,
-                          int* redo_list, int* redo_cnt, float* near, hipStream_t s);
+                          int* redo_list, int* redo_cnt, float* near, int* bucket_cnt, int nranks, hipStream_t s);
 
 // --- inner-product metric (hv_ip.hip; the IP list scan is launch_query_exact_ip)
 void launch_ip_rows(const float* X, int64_t n, const float* C, int64_t nc, int d, float* out, int64_t ldo,

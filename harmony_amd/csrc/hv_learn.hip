@@ -267,7 +267,8 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
                                                    int kmin,
                                                    int* __restrict__ qn, float* __restrict__ bound,
                                                    float* __restrict__ pred, float* __restrict__ bound2,
-                                                   float* __restrict__ bound3, unsigned long long* __restrict__ widened) {
+                                                   float* __restrict__ bound3, unsigned long long* __restrict__ widened,
+                                                   int* __restrict__ bucket_cnt, int nranks) {
     __shared__ float s_f[4][kIn];
     const int w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -328,6 +329,9 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
             if (g <= 0.f || g * g <= thr) c = j + 1;
         }
     }
+    // the grouping's (list, rank) pair counts of the ranks this query searches
+    if (bucket_cnt)
+        for (int j = lane; j < min(c, nranks); j += 64) atomicAdd(&bucket_cnt[(int64_t)probes[q * pc + j] * nranks + j], 1);
     if (lane == 0) {
         qn[q] = c;
         if (c > ccov && widened) atomicAdd(widened, 1ull);
@@ -344,11 +348,12 @@ void launch_mlp_serve(const float* near, const float* pdist, const int* probes, 
                       const __bf16* w1, const __bf16* w2, const float* P, const float* norm, float nprobe_scale,
                       float bound_scale, int min_nprobe, int nprobe, const float* lrad, const float* cent,
                       float margin, float slack, const int64_t* list_off, int kmin, int* qn, float* bound,
-                      float* pred, float* bound2, float* bound3, unsigned long long* widened, hipStream_t s) {
+                      float* pred, float* bound2, float* bound3, unsigned long long* widened, int* bucket_cnt,
+                      int nranks, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_mlp_serve, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, near, pdist, probes, pc, Qp, d, n, w1,
                        w2, P, norm, nprobe_scale, bound_scale, min_nprobe, nprobe, lrad, cent, margin, slack, list_off,
-                       kmin, qn, bound, pred, bound2, bound3, widened);
+                       kmin, qn, bound, pred, bound2, bound3, widened, bucket_cnt, nranks);
 }
 
 // Radius of every list: the largest |x - c| over its vectors, from their squared residual norms

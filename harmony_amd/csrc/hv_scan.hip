@@ -159,11 +159,66 @@ __global__ void k_scan_add(int* __restrict__ out, int64_t n, const int* __restri
 
 int64_t scan_tmp_ints(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
+// Up to kScanOne ints in one workgroup (one launch instead of three): the ints are staged in LDS
+// with coalesced loads (one pad word per 16, so the 16-int runs the threads read next fall in
+// distinct banks), each thread scans its run of 16, one block scan of the 1024 run sums (a
+// wavefront scan and the 16 wavefront totals), and the results leave through LDS coalesced.
+constexpr int kScanOne = 16384;
+__global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, int n, int* __restrict__ out) {
+    constexpr int R = kScanOne / 1024;  // 16: the run of one thread
+    __shared__ int s_v[kScanOne + kScanOne / R];
+    __shared__ int s_wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+        const int i = c * 1024 + t;
+        s_v[i + i / R] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    int v[R], sum = 0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        v[j] = s_v[t * (R + 1) + j];
+        sum += v[j];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    __syncthreads();
+    int run = incl - sum, total = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int x = s_wsum[i];
+        run += i < w ? x : 0;
+        total += x;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        s_v[t * (R + 1) + j] = run;
+        run += v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+        const int i = c * 1024 + t;
+        if (i < n) out[i] = s_v[i + i / R];
+    }
+    if (t == 0) out[n] = total;
+}
+
 void launch_exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s) {
+    if (n <= kScanOne) {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, in, (int)n, out);
+        return;
+    }
     const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    if (n > 0) hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(256), 0, s, in, n, out, tmp);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(256), 0, s, in, n, out, tmp);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, tmp, (int)ntiles, out + n);
-    if (n > 0) hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n, tmp);
 }
 
 // ---------------------------------------------------------------------------------------
