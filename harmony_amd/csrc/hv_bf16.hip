@@ -1092,13 +1092,13 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
     uint32_t pos = lane < c ? (uint32_t)(surv[(int64_t)q * cap + j0 + lane] & 0xffffffffu) : 0u;
     float b = bound ? bound[q] : 3.402823466e38f;
     const int last = plan.n - 1;
+    float cur[kChunkDims], nxt[kChunkDims];
+    load_chunk<V4>(rows_by_pos + (int64_t)pos * d, plan.lo[0], d, cur);
     while (true) {
         const bool valid = lane < c;
         const uint32_t id = ids[pos];
         const float* __restrict__ xv = rows_by_pos + (int64_t)pos * d;
         const float* __restrict__ qv = Qp + (int64_t)q * d;
-        float cur[kChunkDims], nxt[kChunkDims];
-        load_chunk<V4>(xv, plan.lo[0], d, cur);
         const int itn = it + nw;
         int qn = q, j0n = 0, cn = 0;
         uint32_t posn = 0u;
@@ -1112,8 +1112,8 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
             bn = bound ? bound[qn] : 3.402823466e38f;
         }
         float dis = 0.f, p = 0.f;
-        // the next chunk's loads are in flight during each chain (the last chunk is peeled so
-        // that every loop iteration issues them)
+        // the next chunk's loads are in flight during each chain (the last chunk is peeled: its
+        // chain overlaps the next item's first chunk)
         for (int ch = 0; ch < last; ++ch) {
             load_chunk<V4>(xv, plan.lo[ch + 1], d, nxt);
             chain_chunk<V4>(qv, plan.lo[ch], plan.len[ch], d, cur, p);
@@ -1124,6 +1124,8 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
 #pragma unroll
             for (int t = 0; t < kChunkDims; ++t) cur[t] = nxt[t];
         }
+        // the next item's first chunk is in flight during the last chain of this one
+        if (itn < ni) load_chunk<V4>(rows_by_pos + (int64_t)posn * d, plan.lo[0], d, nxt);
         chain_chunk<V4>(qv, plan.lo[last], plan.len[last], d, cur, p);
         dis += p;  // the last chunk closes the last block
         if (valid) kept[(int64_t)q * cap + j0 + lane] = dis <= b ? hv_key(dis, id) : HV_KEY_PAD;
@@ -1134,6 +1136,8 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
         c = cn;
         pos = posn;
         b = bn;
+#pragma unroll
+        for (int t = 0; t < kChunkDims; ++t) cur[t] = nxt[t];
     }
 }
 
