@@ -185,20 +185,26 @@ void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s) {
 }
 
 // One wavefront per row: lane j owns hidden units j and j + 64; the two outputs are reduced
-// across the wavefront. Returns the outputs in raw target space (every lane gets them).
+// across the wavefront. Returns the outputs in raw target space (every lane gets them). The
+// standardized inputs are formed once, one per lane (lane 0 also the last), into xs (kIn floats
+// of LDS for this wavefront), instead of by every lane for every input.
 __device__ __forceinline__ void mlp_eval_bf16(const float* __restrict__ f, int lane, const __bf16* __restrict__ w1,
                                               const __bf16* __restrict__ w2, const float* __restrict__ P,
-                                              const float* __restrict__ norm, float& y0, float& y1) {
+                                              const float* __restrict__ norm, float* xs, float& y0, float& y1) {
     const float* fmean = norm;
     const float* fstd = norm + kIn;
     const float* tmean = norm + 2 * kIn;
     const float* tstd = tmean + kOut;
+    static_assert(kIn <= 128, "two inputs per lane at most");
+    xs[lane] = (f[lane] - fmean[lane]) / fstd[lane];
+    if (lane + 64 < kIn) xs[lane + 64] = (f[lane + 64] - fmean[lane + 64]) / fstd[lane + 64];
+    __threadfence_block();  // every lane reads every input below
     float h0 = P[kOffB1 + lane], h1 = P[kOffB1 + lane + 64];
     // unrolled so that the loads of 13 inputs are in flight together (the sums keep their order)
     static_assert(kIn % 13 == 0, "unroll factor");
 #pragma unroll 13
     for (int i = 0; i < kIn; ++i) {
-        const float x = (f[i] - fmean[i]) / fstd[i];
+        const float x = xs[i];
         h0 = __builtin_fmaf(x, (float)w1[i * kHid + lane], h0);
         h1 = __builtin_fmaf(x, (float)w1[i * kHid + lane + 64], h1);
     }
@@ -218,11 +224,13 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ X
                                                      const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
                                                      const float* __restrict__ P, const float* __restrict__ norm,
                                                      float* __restrict__ out) {
-    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ float s_x[4][kIn];
+    const int w = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= n) return;
     const int lane = threadIdx.x & 63;
     float y0, y1;
-    mlp_eval_bf16(X + q * kIn, lane, w1, w2, P, norm, y0, y1);
+    mlp_eval_bf16(X + q * kIn, lane, w1, w2, P, norm, s_x[w], y0, y1);
     if (lane == 0) {
         out[q * 2] = y0;
         out[q * 2 + 1] = y1;
@@ -270,6 +278,7 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
                                                    float* __restrict__ bound3, unsigned long long* __restrict__ widened,
                                                    int* __restrict__ bucket_cnt, int nranks) {
     __shared__ float s_f[4][kIn];
+    __shared__ float s_x[4][kIn];
     const int w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= n) return;
@@ -284,7 +293,7 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
     if (lane == 0) s_f[w][kNearest] = log1pf(s);
     __threadfence_block();  // every lane reads the whole row its wavefront wrote
     float y0, y1;
-    mlp_eval_bf16(s_f[w], lane, w1, w2, P, norm, y0, y1);
+    mlp_eval_bf16(s_f[w], lane, w1, w2, P, norm, s_x[w], y0, y1);
     // every lane holds y0, y1 (reduced across the wavefront), so every lane forms the same values
     const float np = expf(y1) * nprobe_scale;
     const int c0 = np >= (float)nprobe ? nprobe : max(min_nprobe, (int)ceilf(np));
