@@ -58,7 +58,8 @@ class SearchParams(ctypes.Structure):
                 ("tile_shape_first", ctypes.c_int32), ("learned", ctypes.c_int32),
                 ("learned_nprobe_scale", ctypes.c_float), ("learned_bound_scale", ctypes.c_float),
                 ("learned_min_nprobe", ctypes.c_int32), ("rerank_r", ctypes.c_int32),
-                ("graph", ctypes.c_int32), ("scan_vsplit", ctypes.c_int32), ("learned_margin", ctypes.c_float)]
+                ("graph", ctypes.c_int32), ("scan_vsplit", ctypes.c_int32), ("learned_margin", ctypes.c_float),
+                ("nearest_pass", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -636,6 +636,348 @@ int bf16_grid_size(int device) {
 }
 
 // ---------------------------------------------------------------------------------------
+// The BF16 form's nearest-list pass: each query's nearest list scanned and reduced in one kernel,
+// in place of the direct-slot scan of the nearest lists (every pair written out) and the
+// compaction that reads them back. A workgroup takes an item (list l, up to 16 of the queries
+// whose nearest list l is, a range of up to kNearSplit vectors); each wavefront sweeps blocks w,
+// w + 4, ... of the range with the filter scan's arithmetic (the queries' residuals in registers,
+// the codes of its next two blocks in flight) and writes every pair's lower and upper bound into a
+// fixed LDS slot, as 16-bit fixed-point values on the row's scale (lower bounds rounded down, upper
+// bounds up, so both stay bounds; the scale spans the largest distance the list allows, (|q - c| +
+// the list's radius)^2, to a few units of resolution). The sweep has no atomics and no barriers.
+// Then each query's wavefront finds the k-th smallest upper bound of the range exactly, and the
+// query's bound drops to it (when lower); the pairs with lower bound <= the bound are written to
+// the query's survivor buffer. A list held by one item writes them from slot 0 with the count, the
+// bound and the compaction count (`last`): what the nearest-list scan and its compaction leave. A
+// list split into several items appends (atomics on the count), lowers the bound by an atomic
+// minimum and leaves `last`, so the compaction after the pass merges the parts (each part keeps
+// every pair of its range whose upper bound is among the k smallest of the list).
+constexpr int kNearRows = kNearQueries;  // queries per item: one MFMA query block
+static_assert(kNearRows == 16, "an item's queries are one 16-row MFMA block");
+constexpr int kNearStride = kNearSplit + 16;  // LDS words per row (padded: the rows' banks differ)
+constexpr unsigned int kNearTop = 65535u;     // code of "at or above the scale's end"
+
+// The r-th smallest (1-based, r <= the number of values) of the 16-bit values a wavefront holds E
+// per lane (padding 0xffffffff, not counted), exact: a radix select over the high byte, then the
+// low byte of the values in the selected high-byte bin, each pass a 256-bin histogram in LDS
+// (hist: 256 words of this wavefront's own) built with atomic adds and read back with one prefix
+// scan over the lanes (4 bins per lane). All 64 lanes must call it.
+template <int E>
+__device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&v)[E], int r, unsigned int* hist) {
+    const int lane = threadIdx.x & 63;
+    unsigned int prefix = 0u;  // the selected high byte (pass 2 counts only values with it)
+    int rr = r;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        *reinterpret_cast<uint4*>(hist + 4 * lane) = make_uint4(0u, 0u, 0u, 0u);
+        lds_fence();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const bool in = v[e] != 0xffffffffu && (pass == 0 || (v[e] >> 8) == prefix);
+            if (in) atomicAdd(&hist[pass == 0 ? v[e] >> 8 : v[e] & 255u], 1u);
+        }
+        lds_fence();
+        const uint4 h4 = *reinterpret_cast<const uint4*>(hist + 4 * lane);
+        const unsigned int h[4] = {h4.x, h4.y, h4.z, h4.w};
+        const unsigned int sum = h[0] + h[1] + h[2] + h[3];
+        unsigned int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const unsigned int excl = incl - sum;
+        const bool mine = excl < (unsigned int)rr && (unsigned int)rr <= incl;
+        unsigned int bin = 0u, below = 0u;
+        if (mine) {
+            unsigned int run = excl;
+            bin = 4u * lane + 3u;
+            below = run + h[0] + h[1] + h[2];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (run + h[t] >= (unsigned int)rr) {
+                    bin = 4u * lane + t;
+                    below = run;
+                    break;
+                }
+                run += h[t];
+            }
+        }
+        const int src = __builtin_ctzll(__ballot(mine) | (1ull << 63));
+        bin = (unsigned int)__shfl((int)bin, src, 64);
+        below = (unsigned int)__shfl((int)below, src, 64);
+        rr -= (int)below;
+        prefix = pass == 0 ? bin : (prefix << 8 | bin);
+        lds_fence();  // the histogram is cleared again only after every lane has read it
+    }
+    return prefix;
+}
+
+template <int NKS>
+__global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__ qres, const float* __restrict__ qerr,
+                                                      const float* __restrict__ pdist, int nprobe,
+                                                      const __bf16* __restrict__ rcodes,
+                                                      const int64_t* __restrict__ roff,
+                                                      const float* __restrict__ rnorm,
+                                                      const float* __restrict__ rerr,
+                                                      const float* __restrict__ lrad,
+                                                      const int64_t* __restrict__ list_off,
+                                                      float* __restrict__ bound, const int* __restrict__ pair_slot,
+                                                      const ScanItem* __restrict__ items,
+                                                      const int* __restrict__ queue, float kappa1, float kappa_g,
+                                                      float kappa2, int k_all, const int* __restrict__ kq,
+                                                      unsigned long long* __restrict__ surv, float* __restrict__ aux,
+                                                      unsigned int* __restrict__ surv_cnt,
+                                                      unsigned int* __restrict__ last, int cap,
+                                                      unsigned long long* __restrict__ stats) {
+    typedef ElemBf16::vec vec;
+    __shared__ int s_q[kNearRows], s_kr[kNearRows];
+    __shared__ __align__(16) float s_nrq[kNearRows];   // |rq|^2
+    __shared__ __align__(16) float s_srq[kNearRows];   // |rq| + |eq|
+    __shared__ __align__(16) float s_qsc[kNearRows];   // kappa1 |eq|
+    __shared__ __align__(16) float s_inv[kNearRows];   // 1 / the row's fixed-point step
+    __shared__ float s_step[kNearRows], s_b0[kNearRows];
+    __shared__ unsigned int s_code[kNearRows][kNearStride];  // lower bound << 16 | upper bound
+    __shared__ __align__(16) unsigned int s_hist[4][256];  // each wavefront's radix histogram
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, col16 = lane & 15;
+    const int xcd = blockIdx.x % kNumXcd;
+    const int per_xcd = max(1, (int)gridDim.x / kNumXcd);
+    const int qlen = queue[xcd], qbase = queue[2 * kNumXcd + xcd];
+    unsigned long long pairs_done = 0;
+
+    for (int slot_i = (int)blockIdx.x / kNumXcd; slot_i < qlen; slot_i += per_xcd) {
+        const ScanItem item = items[qbase + slot_i];
+        const int nQ = min(item.pair_end - item.pair_begin, kNearRows);
+        const int l = item.list;
+        const int64_t lbeg = list_off[l];
+        const int nl_all = (int)(list_off[l + 1] - lbeg);
+        const bool split = nl_all > kNearSplit;  // the list is cut into several items
+        const int v_lo = item.vec_begin;
+        const int nl = min(nl_all, v_lo + kNearSplit);  // end of the item's range
+        const int nv = nl - v_lo;
+        const int blk_lo = v_lo >> 4, nblk = ((nl + 15) >> 4) - blk_lo;
+        const uint32_t pos0 = (uint32_t)(lbeg + v_lo);
+        const vec* __restrict__ R = reinterpret_cast<const vec*>(rcodes + roff[l]) + lane;
+        // the rows' state (lane r of wavefront 0 for query row r) and the A operands (query row
+        // col16, dims s*32 + g*8 .. +8; rows past nQ read the first query's slot)
+        int q = 0, kr = 0;
+        float nr = 0.f, b = 0.f, qe = 0.f;
+        if (tid < nQ) {
+            const int slot = pair_slot[item.pair_begin + tid];
+            q = slot / nprobe;
+            nr = pdist[slot];
+            b = bound[q];
+            qe = qerr[slot];
+            kr = hv_query_k(kq, q, k_all);
+        }
+        const float rad = lrad[l];
+        vec a[NKS];
+        {
+            const int slot = pair_slot[item.pair_begin + (col16 < nQ ? col16 : 0)];
+            const vec* __restrict__ qa = reinterpret_cast<const vec*>(qres) + (int64_t)slot * (NKS * 4) + g;
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) a[s] = qa[s * 4];
+        }
+        // Wavefront w takes the range's blocks j = w, w + 4, w + 8, ..., two at a time, from two
+        // buffers A and B used in turn (blocks 8t + w into A, 8t + 4 + w into B), each refilled
+        // with its next block right after it is computed, so up to two blocks per wavefront are in
+        // flight and no register copy waits for a load. Every refill is issued (past the range's
+        // end as a repeat of its last block; an empty list reads the first list's first block), so
+        // the number of loads behind each one is the same on every path and the waits for a block
+        // are exact. Large K (more registers per block) keeps one buffer, loaded right before its
+        // block.
+        constexpr bool kTwo = NKS <= 4;
+        vec bufA[NKS], bufB[kTwo ? NKS : 1];
+        float nxA = 0.f, exA = 0.f, nxB = 0.f, exB = 0.f;
+        const bool any_blk = nblk > 0;
+        const vec* __restrict__ R0 = any_blk ? R : reinterpret_cast<const vec*>(rcodes) + lane;
+        auto load_block = [&](int bk, vec* dst, float& nxd, float& exd) {
+            bk = any_blk ? min(bk, nblk - 1) : -blk_lo;
+            const int64_t blk = (int64_t)(blk_lo + bk) * NKS * 64;
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) dst[s] = R0[blk + s * 64];
+            const int64_t v = any_blk ? lbeg + min((blk_lo + bk) * 16 + col16, nl_all - 1) : 0;
+            nxd = rnorm[v];
+            exd = rerr[v];
+        };
+        if (kTwo) {
+            load_block(w, bufA, nxA, exA);  // in flight during the item's set-up
+            load_block(w + 4, bufB, nxB, exB);
+        }
+        if (tid < kNearRows) {
+            // the row's scale: every exact distance of the list is <= (|rq| + radius)^2; 1/8 more
+            // covers the estimate's error, a few ulps the FP32 rounding of the scale itself
+            const float top = (__builtin_amdgcn_sqrtf(nr) + rad) * (__builtin_amdgcn_sqrtf(nr) + rad) * 1.125f + 1e-30f;
+            const float step = top / 65000.f;
+            s_q[tid] = q;
+            s_kr[tid] = kr;
+            s_nrq[tid] = nr;
+            s_srq[tid] = __builtin_amdgcn_sqrtf(nr) + qe;
+            s_qsc[tid] = kappa1 * qe;
+            s_step[tid] = step;
+            s_inv[tid] = 1.f / step;
+            s_b0[tid] = b;
+        }
+        lds_barrier();  // the rows' state is in (the first blocks' loads stay in flight)
+
+        // one 16-vector block: estimates and bounds of its 16 x 16 pairs, coded into LDS
+        auto do_block = [&](int b0, const vec* bc, float nx, float ex) {
+            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) acc = ElemBf16::mma(a[s], bc[s], acc);
+            const int v = (blk_lo + b0) * 16 + col16;
+            // err = kappa1 |eq| sx + (|rq| + |eq|) vx + kappa2 (|rq|^2 + |rx|^2),
+            // vx = kappa1 |ex| + kappa_g (sx + |ex|)
+            const float sx = __builtin_amdgcn_sqrtf(nx);  // within 1 ulp (covered)
+            const float vx = __builtin_fmaf(kappa1, ex, kappa_g * (sx + ex));
+            pairs_done += 256;
+            int r0 = g * 4;
+            asm volatile("" : "+v"(r0));
+            const float4 nr4 = *reinterpret_cast<const float4*>(&s_nrq[r0]);
+            const float4 q4 = *reinterpret_cast<const float4*>(&s_qsc[r0]);
+            const float4 sr4 = *reinterpret_cast<const float4*>(&s_srq[r0]);
+            const float4 iv4 = *reinterpret_cast<const float4*>(&s_inv[r0]);
+            const float nrq[4] = {nr4.x, nr4.y, nr4.z, nr4.w};
+            const float qa[4] = {q4.x, q4.y, q4.z, q4.w};
+            const float sqp[4] = {sr4.x, sr4.y, sr4.z, sr4.w};
+            const float inv[4] = {iv4.x, iv4.y, iv4.z, iv4.w};
+            const int slot = v - v_lo;  // < kNearSplit: the range's blocks stay inside it
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float nsum = nrq[i] + nx;
+                const float dt = __builtin_fmaf(-2.f, acc[i], nsum);
+                const float er = __builtin_fmaf(qa[i], sx, __builtin_fmaf(sqp[i], vx, kappa2 * nsum));
+                // codes: floor(lb / step) - 2 (>= 0) and floor(ub / step) + 2: the margins of 2
+                // steps cover the rounding of the products and of the decoding
+                // (an upper bound at or past the scale's end, or NaN, codes as kNearTop: none)
+                const float fl = fminf(fmaxf((dt - er) * inv[i], 0.f), 65000.f);
+                const float fu = (dt + er) * inv[i];
+                const unsigned int cl = (unsigned int)max((int)fl - 2, 0);
+                const unsigned int cu = fu < 65000.f ? (unsigned int)fmaxf(fu, 0.f) + 2u : kNearTop;
+                s_code[r0 + i][slot] = cl << 16 | cu;
+            }
+        };
+        // everything the item loaded so far (the A operands, the first blocks) is in before the
+        // sweep: the sweep then starts each step with exactly the two refills of the previous one
+        // in flight, and the waits for a block cover its own loads only
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // s_waitcnt vmcnt(0)
+        const int nsteps = (nblk + 7) >> 3;
+        for (int t = 0; t < nsteps; ++t) {
+            const int bA = 8 * t + w, bB = bA + 4;
+            if (kTwo) {
+                if (bA < nblk) do_block(bA, bufA, nxA, exA);
+                load_block(bA + 8, bufA, nxA, exA);
+                if (bB < nblk) do_block(bB, bufB, nxB, exB);
+                load_block(bB + 8, bufB, nxB, exB);
+            } else {
+                if (bA < nblk) {
+                    load_block(bA, bufA, nxA, exA);
+                    do_block(bA, bufA, nxA, exA);
+                }
+                if (bB < nblk) {
+                    load_block(bB, bufA, nxA, exA);
+                    do_block(bB, bufA, nxA, exA);
+                }
+            }
+        }
+        lds_barrier();  // every pair's codes are in
+
+        // per query row (wavefront w takes rows w, w + 4, ...): the exact k-th smallest upper
+        // bound of the range, then the pairs with lower bound <= the query's bound
+        constexpr int E = kNearSplit / 64;
+        for (int r = w; r < nQ; r += 4) {
+            const int qq = s_q[r], k_r = s_kr[r];
+            const float step = s_step[r];
+            float T = s_b0[r];
+            unsigned int c[E], u[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int j = e * 64 + lane;
+                c[e] = j < nv ? s_code[r][j] : 0xffffffffu;
+                u[e] = j < nv ? (c[e] & 0xffffu) : 0xffffffffu;
+            }
+            if (nv >= k_r) {
+                const unsigned int kth = wave_kth_u16<E>(u, k_r, s_hist[w]);
+                if (kth < kNearTop) T = fminf(T, (float)kth * step);  // >= the k-th upper bound
+            }
+            unsigned long long keep_m[E];
+            unsigned int m = 0u;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const bool keep = e * 64 + lane < nv && (float)(c[e] >> 16) * step <= T;
+                keep_m[e] = __ballot(keep);
+                m += (unsigned int)__builtin_popcountll(keep_m[e]);
+            }
+            unsigned int at = 0u;
+            if (split) {
+                // parts of one list append; the compaction after the pass merges them
+                if (lane == 0) {
+                    if (m) at = atomicAdd(&surv_cnt[qq], m);
+                    atomicMin(reinterpret_cast<unsigned int*>(bound) + qq, __float_as_uint(T));
+                }
+                at = (unsigned int)__shfl((int)at, 0, 64);
+            } else if (lane == 0) {
+                surv_cnt[qq] = m;
+                if (last) last[qq] = m;
+                bound[qq] = T;
+            }
+            unsigned long long* __restrict__ sq = surv + (int64_t)qq * cap;
+            unsigned int* __restrict__ aq = reinterpret_cast<unsigned int*>(aux) + (int64_t)qq * cap;
+            unsigned int o = at;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if ((keep_m[e] >> lane) & 1ull) {
+                    const unsigned int slot = o + (unsigned int)__builtin_popcountll(keep_m[e] & ((1ull << lane) - 1ull));
+                    if (slot < (unsigned int)cap) {
+                        const unsigned int cu = c[e] & 0xffffu;
+                        sq[slot] = hv_key((float)(c[e] >> 16) * step, pos0 + (unsigned int)(e * 64 + lane));
+                        aq[slot] = cu >= kNearTop ? 0x7f800000u : __float_as_uint((float)cu * step);
+                    }
+                }
+                o += (unsigned int)__builtin_popcountll(keep_m[e]);
+            }
+        }
+        lds_barrier();  // the rows are free for the next item (the stores need not wait)
+    }
+    if (lane == 0 && pairs_done) atomicAdd(&stats[3], pairs_done);
+}
+
+void launch_mfma_nearest(int grid, const __bf16* qres, const float* qerr, int d, const float* pdist, int nprobe,
+                         const __bf16* rcodes, const int64_t* roff, const float* rnorm, const float* rerr,
+                         const float* lrad, const int64_t* list_off, float* bound, const int* pair_slot,
+                         const ScanItem* items, const int* queue, float kappa1, float kappa_g, float kappa2, int k,
+                         const int* kq, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
+                         unsigned int* last, int cap, unsigned long long* stats, hipStream_t s) {
+#define HV_NEAR_CASE(K)                                                                                              \
+    case K:                                                                                                          \
+        hipLaunchKernelGGL((k_mfma_nearest<K>), dim3((unsigned)grid), dim3(256), 0, s, qres, qerr, pdist, nprobe,    \
+                           rcodes, roff, rnorm, rerr, lrad, list_off, bound, pair_slot, items, queue, kappa1,        \
+                           kappa_g, kappa2, k, kq, surv, aux, surv_cnt, last, cap, stats);                           \
+        break;
+    switch (bf16_ksteps(d)) {
+        HV_NEAR_CASE(1)
+        HV_NEAR_CASE(2)
+        HV_NEAR_CASE(3)
+        HV_NEAR_CASE(4)
+        HV_NEAR_CASE(5)
+        HV_NEAR_CASE(6)
+        HV_NEAR_CASE(7)
+        HV_NEAR_CASE(8)
+        HV_NEAR_CASE(9)
+        HV_NEAR_CASE(10)
+        HV_NEAR_CASE(11)
+        HV_NEAR_CASE(12)
+        HV_NEAR_CASE(13)
+        HV_NEAR_CASE(14)
+        HV_NEAR_CASE(15)
+        HV_NEAR_CASE(16)
+        default: break;
+    }
+#undef HV_NEAR_CASE
+}
+
+// ---------------------------------------------------------------------------------------
 // After the BF16 scan of each query's nearest list: the k-th smallest upper bound bounds the
 // k-th exact distance, so the query's bound drops to it when tighter; the stored pairs whose
 // lower bound exceeds the new bound cannot be results and are dropped (in-place compaction, order

@@ -858,26 +858,33 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         int64_t r0, r1;
         int vmin, vmax;
         bool main;
+        bool nearest;  // the nearest-list pass: items of kNearQueries queries over whole lists
     };
+    // BF16 form: each query's nearest list scanned and reduced in one pass (candidates in LDS)
+    const bool near_pass = bf16 && h->sp.nearest_pass != 0 && k <= hv::kNearMaxK;
     std::vector<PhaseRange> ranges;
     if (mfma) {
         if (learned_seed) {
             for (int64_t r0 = 0; r0 < rank_end;) {
                 const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
-                ranges.push_back({r0, r1, 0, INT32_MAX, r0 != 0});
+                ranges.push_back({r0, r1, 0, INT32_MAX, r0 != 0, near_pass && r0 == 0});
                 r0 = r1;
             }
         } else {
             const int vdirect = (int)(cap / 16 * 16);
-            if (vdirect > 0) ranges.push_back({0, 1, 0, vdirect, false});
-            if (h->max_list > vdirect) ranges.push_back({0, 1, vdirect, INT32_MAX, false});
+            if (near_pass) {
+                ranges.push_back({0, 1, 0, INT32_MAX, false, true});
+            } else {
+                if (vdirect > 0) ranges.push_back({0, 1, 0, vdirect, false, false});
+                if (h->max_list > vdirect) ranges.push_back({0, 1, vdirect, INT32_MAX, false, false});
+            }
             if (!fp8 && nprobe <= kBf16SinglePhase) {
-                if (nprobe > 1) ranges.push_back({1, nprobe, 0, INT32_MAX, true});
+                if (nprobe > 1) ranges.push_back({1, nprobe, 0, INT32_MAX, true, false});
             } else {
                 const int64_t end = fp8 ? rank_end : nprobe;
                 for (int64_t r0 = 1; r0 < end;) {
                     const int64_t r1 = std::min<int64_t>(end, r0 * 4);
-                    ranges.push_back({r0, r1, 0, INT32_MAX, true});
+                    ranges.push_back({r0, r1, 0, INT32_MAX, true, false});
                     r0 = r1;
                 }
             }
@@ -1150,7 +1157,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     size_t mph_next = 0;
     if (mfma) {
         for (const PhaseRange& r : ranges)
-            mph.push_back(hv::PlanSpec{(int)r.r0, (int)r.r1, r.vmin, r.vmax, nullptr, nullptr, r.main ? stats + 8 : stats});
+            mph.push_back(hv::PlanSpec{(int)r.r0, (int)r.r1, r.vmin, r.vmax, nullptr, nullptr, r.main ? stats + 8 : stats,
+                                       r.nearest ? hv::kNearQueries : 0, r.nearest ? hv::kNearSplit : 0});
         HV_HIP(h->s_items.ensure(sizeof(hv::ScanItem) * qcap * std::max<size_t>(1, mph.size())));
         HV_HIP(h->s_queue.ensure(sizeof(int) * qlen * std::max<size_t>(1, mph.size())));
         for (size_t i = 0; i < mph.size(); ++i) {
@@ -1184,6 +1192,26 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         scan_ev(true);
         if (main_phase) rec(7);
     };
+    // the nearest-list pass of the BF16 form (rank 0, planned with its own query tile)
+    auto near_phase = [&](unsigned long long* st_slot) {
+        require(mph_next < mph.size() && mph[mph_next].r0 == 0 && mph[mph_next].r1 == 1 &&
+                    mph[mph_next].qtile == hv::kNearQueries && mph[mph_next].stats == st_slot,
+                HV_ERR_STATE, "nearest-list pass out of plan");
+        const hv::PlanSpec& ph = mph[mph_next++];
+        // at most n / 16 + nlist + ntotal / kNearSplit items: one workgroup each, the dispatcher
+        // balancing their lengths
+        const int grid = (int)std::min<int64_t>(
+            1 << 15, (n / hv::kNearQueries + nlist + h->ntotal / hv::kNearSplit + hv::kNumXcd) / hv::kNumXcd * hv::kNumXcd);
+        scan_ev(false);
+        hv::launch_mfma_nearest(grid, h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), (int)d, pdist, (int)nprobe,
+                                h->d_rcodes.as<__bf16>(), h->d_roff.as<int64_t>(), h->d_rnorm.as<float>(),
+                                h->d_rerr.as<float>(), h->d_lrad.as<float>(), h->d_list_off.as<int64_t>(),
+                                h->s_bound.as<float>(),
+                                h->s_pair_slot.as<int>(), ph.items, ph.queue, kappa1, kappa_g, kappa2, (int)k, kq, surv,
+                                aux, surv_cnt, last, (int)cap, st_slot, s);
+        HV_CHECK_LAUNCH("nearest-list pass");
+        scan_ev(true);
+    };
     // 4. bound: the reference's warm-up bound (heapTops), or the caller's. The BF16 form needs no
     //    warm-up scan: it scans each query's nearest list unpruned and takes the bound from there.
     float* bound = h->s_bound.as<float>();
@@ -1211,7 +1239,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         // whose lower bound exceeds it
         for (int64_t r0 = 0; r0 < rank_end;) {
             const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
-            mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, r0 == 0 ? stats : stats + 8);
+            if (r0 == 0 && near_pass)
+                near_phase(stats);  // compacts as it goes; the compaction below merges split lists and spills
+            else
+                mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, r0 == 0 ? stats : stats + 8);
             hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s);
             r0 = r1;
         }
@@ -1221,13 +1252,21 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         // bound (FP8: the kr-th smallest estimate) then lowers the bound and the pairs whose lower
         // bound exceeds it are dropped (vector ranges start on 16-vector blocks)
         const int vdirect = (int)(cap / 16 * 16);
-        if (vdirect > 0) mfma_phase(0, 1, 0, vdirect, 1, stats);
-        hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
-        hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, last, kq, (int)(kr / k), s);
-        HV_CHECK_LAUNCH("compact");
+        if (near_pass) {
+            near_phase(stats);
+            // the queries whose nearest list was split into several items or that spilled out of
+            // the pass's buffers (the others are skipped)
+            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s);
+            HV_CHECK_LAUNCH("compact");
+        } else {
+            if (vdirect > 0) mfma_phase(0, 1, 0, vdirect, 1, stats);
+            hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
+            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, last, kq, (int)(kr / k), s);
+            HV_CHECK_LAUNCH("compact");
+        }
         HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
-        if (h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
+        if (!near_pass && h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
         if (!fp8 && nprobe <= kBf16SinglePhase) {
             if (nprobe > 1) mfma_phase(1, (int)nprobe, 0, INT32_MAX, 0, stats + 8);
         } else if (!fp8) {
@@ -1815,6 +1854,7 @@ void hv_default_search_params(hv_search_params* p) {
     p->learned_bound_scale = 1.0f;
     p->learned_min_nprobe = 1;
     p->learned_margin = 1.0f;
+    p->nearest_pass = 1;
 }
 
 int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
@@ -1841,6 +1881,7 @@ int hv_set_search_params(hv_index_t* h, const hv_search_params* p) {
         require(p->learned_nprobe_scale > 0.f && p->learned_min_nprobe >= 1, HV_ERR_INVALID_ARG,
                 "learned_nprobe_scale must be > 0 and learned_min_nprobe >= 1");
         require(p->learned_margin >= 0.f, HV_ERR_INVALID_ARG, "learned_margin must be >= 0");
+        require(p->nearest_pass == 0 || p->nearest_pass == 1, HV_ERR_INVALID_ARG, "nearest_pass must be 0 or 1");
         h->sp = *p;
     });
 }

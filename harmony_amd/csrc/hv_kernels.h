@@ -81,6 +81,7 @@ struct PlanSpec {
     int* queue;
     ScanItem* items;
     unsigned long long* stats;
+    int qtile = 0, vsplit = 0;  // queries / vectors per item of this phase (0: the batch's)
 };
 void launch_plan_phases(const int* bucket_off, int nlist, int nbr, int qtile, int vsplit,
                         const int* list_rank_order, const int64_t* list_off, int64_t qcap, const PlanSpec* phases,
@@ -140,6 +141,22 @@ void launch_mfma_scan(int grid, int elem, const void* qres, const float* qaux, i
                       const ScanItem* items, const int* queue, int vmax, int vsplit, int mode, float kappa1,
                       float kappa_g, float kappa2, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
                       int cap, unsigned long long* stats, hipStream_t s);
+// The BF16 form's nearest-list pass: per item (list, <= kNearQueries queries whose nearest list it
+// is, <= kNearSplit vectors; planned with that query tile and vector split), the filter scan with
+// every pair's bounds kept in LDS, the k-th smallest upper bound (k per query with kq) and the
+// pairs below it. A list held by one item leaves each query's survivors, count, bound and
+// compaction count (last) as the direct-slot scan and its compaction would; parts of split lists
+// append to the survivor buffers and leave `last`, for the compaction the caller runs after the
+// pass. lrad: each list's radius (the largest |x - c|), which sets the bounds' fixed-point scale.
+constexpr int kNearQueries = 16;
+constexpr int kNearSplit = 1024;  // vectors per item: longer lists are cut into several items
+constexpr int kNearMaxK = 1024;  // a query's k fits the range's bounds up to k <= kNearSplit
+void launch_mfma_nearest(int grid, const __bf16* qres, const float* qerr, int d, const float* pdist, int nprobe,
+                         const __bf16* rcodes, const int64_t* roff, const float* rnorm, const float* rerr,
+                         const float* lrad, const int64_t* list_off, float* bound, const int* pair_slot,
+                         const ScanItem* items, const int* queue, float kappa1, float kappa_g, float kappa2, int k,
+                         const int* kq, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
+                         unsigned int* last, int cap, unsigned long long* stats, hipStream_t s);
 // FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place;
 // bound[q] drops to the kr-th estimate
 // with kq: each query keeps min(kr, kmul * k_q)

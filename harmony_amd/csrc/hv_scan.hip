@@ -246,7 +246,7 @@ struct PlanBatch {
     PlanSpec ph[kMaxPlanPhases];
     __device__ PlanArgs args(int i) const {
         return PlanArgs{bucket_off, rank_order, list_off, nlist, nprobe, ph[i].r0, ph[i].r1, ph[i].vmin,
-                        ph[i].vmax, qtile, vsplit};
+                        ph[i].vmax, ph[i].qtile > 0 ? ph[i].qtile : qtile, ph[i].vsplit > 0 ? ph[i].vsplit : vsplit};
     }
 };
 
@@ -345,7 +345,7 @@ void launch_plan_phases(const int* bucket_off, int nlist, int nprobe, int qtile,
 void launch_plan_items(const int* bucket_off, int nlist, int nprobe, int r0, int r1, int vmin, int vmax, int qtile,
                        int vsplit, const int* list_rank_order, const int64_t* list_off, int64_t qcap, ScanItem* items,
                        int* queue, unsigned long long* stats, hipStream_t s) {
-    const PlanSpec p{r0, r1, vmin, vmax, queue, items, stats};
+    const PlanSpec p{r0, r1, vmin, vmax, queue, items, stats, 0, 0};
     launch_plan_phases(bucket_off, nlist, nprobe, qtile, vsplit, list_rank_order, list_off, qcap, &p, 1, s);
 }
 

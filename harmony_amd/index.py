@@ -119,7 +119,7 @@ class Index:
     def preSearch(self, warmup_list=10, warmup_list_size=0, pruning=True, dim_blocks=4, bound_mode=HV_BOUND_WARMUP,
                   survivor_cap=0, refine_bound=True, tile_shape=0, tile_shape_first=0, precision=None,
                   learned=None, learned_nprobe_scale=None, learned_bound_scale=None, learned_min_nprobe=None,
-                  rerank_r=None, graph=None, scan_vsplit=None, learned_margin=None):
+                  rerank_r=None, graph=None, scan_vsplit=None, learned_margin=None, nearest_pass=None):
         """Warm-up and pruning knobs (Index::preSearch, Index.h:90; Param::pruning, Index.h:59).
 
         precision: HV_PRECISION_FP32 (default), HV_PRECISION_BF16 (MFMA filter + FP32 distances
@@ -153,6 +153,8 @@ class Index:
             p.graph = 1 if graph else 0
         if scan_vsplit is not None:
             p.scan_vsplit = int(scan_vsplit)
+        if nearest_pass is not None:
+            p.nearest_pass = 1 if nearest_pass else 0
         p.refine_bound = 1 if refine_bound else 0
         p.tile_shape = int(tile_shape)
         p.tile_shape_first = int(tile_shape_first)

@@ -114,6 +114,11 @@ typedef struct {
      * 1): with the search's check that the k-th result is <= the seed (else the query is redone
      * over all nprobe lists), every learned result equals the plain search's at nprobe. */
     float learned_margin;
+    /* BF16 form, k <= 128: 1 (default) scans each query's nearest list in one pass that keeps the
+     * query's candidates in on-chip memory and reduces them to the k-th smallest upper bound
+     * before anything is written; 0 writes every pair of the nearest list out and reduces it in a
+     * second kernel. Same results either way. */
+    int32_t nearest_pass;
 } hv_search_params;
 
 /* Per-search report (what the reference's Stats + query.cpp timing report, stats.h:14-134). */
