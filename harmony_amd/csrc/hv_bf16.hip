@@ -170,12 +170,13 @@ __device__ __forceinline__ void load8_diff(const float* __restrict__ qv, const f
 template <int G>
 __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp, const float* __restrict__ centroids,
                                                     const int* __restrict__ probes, const int* __restrict__ qn,
-                                                    int64_t nitems, int nprobe, int nranks, int d, int dpad,
+                                                    int64_t nitems, int nprobe, int r0, int nranks, int d, int dpad,
                                                     bf16x8* __restrict__ out, float* __restrict__ qerr) {
-    // item = (query, rank < nranks); its slot in the [nq][nprobe] arrays
+    // item = (query, rank in [r0, nranks)); its slot in the [nq][nprobe] arrays
     const int64_t it = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
-    const int64_t qi = it / nranks;
-    const int j = (int)(it - qi * nranks);
+    const int nr = nranks - r0;
+    const int64_t qi = it / nr;
+    const int j = r0 + (int)(it - qi * nr);
     const int64_t slot = qi * nprobe + j;
     const int g = threadIdx.x % G;
     const bool live = it < nitems && !(qn && j >= qn[qi]);
@@ -200,13 +201,13 @@ __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp
 }
 
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                       int nprobe, int nranks, int d, int dpad, __bf16* out, float* qerr, hipStream_t s) {
-    const int64_t nitems = nq * nranks;
+                       int nprobe, int r0, int nranks, int d, int dpad, __bf16* out, float* qerr, hipStream_t s) {
+    const int64_t nitems = nq * (nranks - r0);
     if (nitems <= 0) return;
     const int words = dpad / 8;  // <= 64
 #define HV_RESID(G)                                                                                          \
     hipLaunchKernelGGL(k_pair_resid<G>, dim3((unsigned)((nitems + (256 / G) - 1) / (256 / G))), dim3(256), 0, s, \
-                       Qp, centroids, probes, qn, nitems, nprobe, nranks, d, dpad, reinterpret_cast<bf16x8*>(out), qerr)
+                       Qp, centroids, probes, qn, nitems, nprobe, r0, nranks, d, dpad, reinterpret_cast<bf16x8*>(out), qerr)
     if (words <= 4)
         HV_RESID(4);
     else if (words <= 8)
@@ -714,7 +715,8 @@ __device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&v)[E],
 }
 
 template <int NKS>
-__global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__ qres, const float* __restrict__ qerr,
+__global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ Qp,
+                                                      const float* __restrict__ centroids, int d,
                                                       const float* __restrict__ pdist, int nprobe,
                                                       const __bf16* __restrict__ rcodes,
                                                       const int64_t* __restrict__ roff,
@@ -722,7 +724,8 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__
                                                       const float* __restrict__ rerr,
                                                       const float* __restrict__ lrad,
                                                       const int64_t* __restrict__ list_off,
-                                                      float* __restrict__ bound, const int* __restrict__ pair_slot,
+                                                      float* __restrict__ bound, float* __restrict__ bound_ref,
+                                                      const int* __restrict__ pair_slot,
                                                       const ScanItem* __restrict__ items,
                                                       const int* __restrict__ queue, float kappa1, float kappa_g,
                                                       float kappa2, int k_all, const int* __restrict__ kq,
@@ -760,7 +763,13 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__
         const uint32_t pos0 = (uint32_t)(lbeg + v_lo);
         const vec* __restrict__ R = reinterpret_cast<const vec*>(rcodes + roff[l]) + lane;
         // the rows' state (lane r of wavefront 0 for query row r) and the A operands (query row
-        // col16, dims s*32 + g*8 .. +8; rows past nQ read the first query's slot)
+        // col16, dims s*32 + g*8 .. +8; rows past nQ read the first query's slot): the residual
+        // r = q - c formed here in FP32 and split into BF16 parts hi + lo (kSplit: r - hi is exact
+        // in FP32 and lo is its BF16 rounding, so what is left, r - hi - lo, is about 2^-16 |r|,
+        // and both parts are multiplied into the codes), else its BF16 rounding alone; qe = the
+        // norm of what the parts leave out, plus 2^-23 |r| for the FP32 subtraction (the four
+        // lanes of a row sum theirs)
+        constexpr bool kSplit = NKS <= 8;
         int q = 0, kr = 0;
         float nr = 0.f, b = 0.f, qe = 0.f;
         if (tid < nQ) {
@@ -768,16 +777,36 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__
             q = slot / nprobe;
             nr = pdist[slot];
             b = bound[q];
-            qe = qerr[slot];
             kr = hv_query_k(kq, q, k_all);
         }
         const float rad = lrad[l];
-        vec a[NKS];
+        vec a[NKS], alo[kSplit ? NKS : 1];
         {
             const int slot = pair_slot[item.pair_begin + (col16 < nQ ? col16 : 0)];
-            const vec* __restrict__ qa = reinterpret_cast<const vec*>(qres) + (int64_t)slot * (NKS * 4) + g;
+            const float* __restrict__ qv = Qp + (int64_t)(slot / nprobe) * d;
+            const float* __restrict__ cv = centroids + (int64_t)l * d;
+            float se = 0.f, sr = 0.f;
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) a[s] = qa[s * 4];
+            for (int s = 0; s < NKS; ++s) {
+                float v[8];
+                load8_diff(qv, cv, s * 32 + g * 8, d, v);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    a[s][e] = (__bf16)v[e];
+                    float er = v[e] - (float)a[s][e];
+                    if (kSplit) {
+                        alo[s][e] = (__bf16)er;
+                        er -= (float)alo[s][e];
+                    }
+                    se = __builtin_fmaf(er, er, se);
+                    sr = __builtin_fmaf(v[e], v[e], sr);
+                }
+            }
+            se += __shfl_xor(se, 16, 64);
+            sr += __shfl_xor(sr, 16, 64);
+            se += __shfl_xor(se, 32, 64);
+            sr += __shfl_xor(sr, 32, 64);
+            if (tid < nQ) qe = __builtin_sqrtf(se) + 0x1p-23f * __builtin_sqrtf(sr);  // lane tid: row tid
         }
         // Wavefront w takes the range's blocks j = w, w + 4, w + 8, ..., two at a time, from two
         // buffers A and B used in turn (blocks 8t + w into A, 8t + 4 + w into B), each refilled
@@ -825,12 +854,17 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__
         auto do_block = [&](int b0, const vec* bc, float nx, float ex) {
             floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < NKS; ++s) acc = ElemBf16::mma(a[s], bc[s], acc);
+            for (int s = 0; s < NKS; ++s) {
+                acc = ElemBf16::mma(a[s], bc[s], acc);
+                if (kSplit) acc = ElemBf16::mma(alo[s], bc[s], acc);
+            }
             const int v = (blk_lo + b0) * 16 + col16;
             // err = kappa1 |eq| sx + (|rq| + |eq|) vx + kappa2 (|rq|^2 + |rx|^2),
             // vx = kappa1 |ex| + kappa_g (sx + |ex|)
+            // (with the split, 2d exact products are accumulated: the FP32 accumulation term
+            // doubles, and 5 % more covers |hi| + |lo| against |r|)
             const float sx = __builtin_amdgcn_sqrtf(nx);  // within 1 ulp (covered)
-            const float vx = __builtin_fmaf(kappa1, ex, kappa_g * (sx + ex));
+            const float vx = __builtin_fmaf(kappa1, ex, (kSplit ? 2.1f : 1.f) * kappa_g * (sx + ex));
             pairs_done += 256;
             int r0 = g * 4;
             asm volatile("" : "+v"(r0));
@@ -921,6 +955,7 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__
                 surv_cnt[qq] = m;
                 if (last) last[qq] = m;
                 bound[qq] = T;
+                if (bound_ref) bound_ref[qq] = T;
             }
             unsigned long long* __restrict__ sq = surv + (int64_t)qq * cap;
             unsigned int* __restrict__ aq = reinterpret_cast<unsigned int*>(aux) + (int64_t)qq * cap;
@@ -943,17 +978,18 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const __bf16* __restrict__
     if (lane == 0 && pairs_done) atomicAdd(&stats[3], pairs_done);
 }
 
-void launch_mfma_nearest(int grid, const __bf16* qres, const float* qerr, int d, const float* pdist, int nprobe,
+void launch_mfma_nearest(int grid, const float* Qp, const float* centroids, int d, const float* pdist, int nprobe,
                          const __bf16* rcodes, const int64_t* roff, const float* rnorm, const float* rerr,
-                         const float* lrad, const int64_t* list_off, float* bound, const int* pair_slot,
-                         const ScanItem* items, const int* queue, float kappa1, float kappa_g, float kappa2, int k,
-                         const int* kq, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
-                         unsigned int* last, int cap, unsigned long long* stats, hipStream_t s) {
+                         const float* lrad, const int64_t* list_off,
+                         float* bound, float* bound_ref, const int* pair_slot, const ScanItem* items,
+                         const int* queue, float kappa1, float kappa_g, float kappa2, int k, const int* kq,
+                         unsigned long long* surv, float* aux, unsigned int* surv_cnt, unsigned int* last, int cap,
+                         unsigned long long* stats, hipStream_t s) {
 #define HV_NEAR_CASE(K)                                                                                              \
     case K:                                                                                                          \
-        hipLaunchKernelGGL((k_mfma_nearest<K>), dim3((unsigned)grid), dim3(256), 0, s, qres, qerr, pdist, nprobe,    \
-                           rcodes, roff, rnorm, rerr, lrad, list_off, bound, pair_slot, items, queue, kappa1,        \
-                           kappa_g, kappa2, k, kq, surv, aux, surv_cnt, last, cap, stats);                           \
+        hipLaunchKernelGGL((k_mfma_nearest<K>), dim3((unsigned)grid), dim3(256), 0, s, Qp, centroids, d,             \
+                           pdist, nprobe, rcodes, roff, rnorm, rerr, lrad, list_off, bound, bound_ref, pair_slot,    \
+                           items, queue, kappa1, kappa_g, kappa2, k, kq, surv, aux, surv_cnt, last, cap, stats);     \
         break;
     switch (bf16_ksteps(d)) {
         HV_NEAR_CASE(1)
@@ -996,7 +1032,8 @@ constexpr int kCompactWaveBig = 4096;  // then with 64 per lane (same kernel); l
 __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __restrict__ surv,
                                                       float* __restrict__ aux,
                                                       unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
-                                                      float* __restrict__ bound, unsigned int* __restrict__ last,
+                                                      float* __restrict__ bound, float* __restrict__ bound_ref,
+                                                      unsigned int* __restrict__ last,
                                                       const int* __restrict__ kq, int kmul) {
     __shared__ unsigned int s_hist[256];
     __shared__ unsigned int s_red[2][4];
@@ -1098,6 +1135,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     if (c > (unsigned int)cap) {
         if (t == 0) {
             bound[q] = b;
+            if (bound_ref) bound_ref[q] = b;
             if (last) last[q] = c;
         }
         return;
@@ -1122,6 +1160,7 @@ __global__ __launch_bounds__(256) void k_bf16_compact(unsigned long long* __rest
     if (t == 0) {
         cnt[q] = s_out;
         bound[q] = b;
+        if (bound_ref) bound_ref[q] = b;
         if (last) last[q] = s_out;
     }
 }
@@ -1133,8 +1172,8 @@ template <int E>
 __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restrict__ buf,
                                                     unsigned int* __restrict__ ubits, int n, unsigned int c,
                                                     int cap, int k, float b, unsigned int* __restrict__ cnt_q,
-                                                    float* __restrict__ bound_q, unsigned int* __restrict__ last_q,
-                                                    unsigned int* s_tmp) {
+                                                    float* __restrict__ bound_q, float* __restrict__ bref_q,
+                                                    unsigned int* __restrict__ last_q, unsigned int* s_tmp) {
     const int lane = threadIdx.x & 63;
     if (n >= k) {
         unsigned int u[E];
@@ -1153,6 +1192,7 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
     if (c > (unsigned int)cap) {
         if (lane == 0) {
             *bound_q = b;
+            if (bref_q) *bref_q = b;
             if (last_q) *last_q = c;
         }
         return;
@@ -1186,6 +1226,7 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
     if (lane == 0) {
         *cnt_q = out;
         *bound_q = b;
+        if (bref_q) *bref_q = b;
         if (last_q) *last_q = out;
     }
 }
@@ -1193,7 +1234,8 @@ __device__ __forceinline__ void bf16_compact_wave_e(unsigned long long* __restri
 __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* __restrict__ surv,
                                                            float* __restrict__ aux,
                                                            unsigned int* __restrict__ cnt, int64_t nq, int cap, int k_all,
-                                                           float* __restrict__ bound, unsigned int* __restrict__ last,
+                                                           float* __restrict__ bound, float* __restrict__ bound_ref,
+                                                           unsigned int* __restrict__ last,
                                                            const int* __restrict__ kq, int kmul) {
     __shared__ unsigned int s_tmp[4][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1209,25 +1251,26 @@ __global__ __launch_bounds__(256) void k_bf16_compact_wave(unsigned long long* _
     const float b = bound[q];
     unsigned int* lq = last ? last + q : nullptr;
     if (n <= 256)
-        bf16_compact_wave_e<4>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
+        bf16_compact_wave_e<4>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, bound_ref ? bound_ref + q : nullptr, lq, s_tmp[w]);
     else if (n <= 512)
-        bf16_compact_wave_e<8>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
+        bf16_compact_wave_e<8>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, bound_ref ? bound_ref + q : nullptr, lq, s_tmp[w]);
     else if (n <= 1024)
-        bf16_compact_wave_e<16>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
+        bf16_compact_wave_e<16>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, bound_ref ? bound_ref + q : nullptr, lq, s_tmp[w]);
     else if (n <= kCompactWaveMax)
-        bf16_compact_wave_e<32>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
+        bf16_compact_wave_e<32>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, bound_ref ? bound_ref + q : nullptr, lq, s_tmp[w]);
     else
-        bf16_compact_wave_e<kCompactWaveBig / 64>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, lq, s_tmp[w]);
+        bf16_compact_wave_e<kCompactWaveBig / 64>(buf, ubits, n, c, cap, k, b, cnt + q, bound + q, bound_ref ? bound_ref + q : nullptr, lq, s_tmp[w]);
 }
 
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
-                         float* bound, unsigned int* last, const int* kq, int kmul, hipStream_t s) {
+                         float* bound, unsigned int* last, const int* kq, int kmul, hipStream_t s, float* bound_ref) {
     if (nq <= 0) return;
     const dim3 wgrid((unsigned)((nq + 3) / 4));
-    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, last, kq, kmul);
+    hipLaunchKernelGGL(k_bf16_compact_wave, wgrid, dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound, bound_ref, last,
+                       kq, kmul);
     if (cap > kCompactWaveBig)
         hipLaunchKernelGGL(k_bf16_compact, dim3((unsigned)nq), dim3(256), 0, s, surv, aux, cnt, nq, cap, k, bound,
-                           last, kq, kmul);
+                           bound_ref, last, kq, kmul);
 }
 
 // FP8 shortlist: each query keeps its kr smallest estimates (keys' distance bits; every key tied

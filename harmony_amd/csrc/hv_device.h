@@ -319,58 +319,6 @@ __device__ __forceinline__ unsigned int wave_kth_u32(const unsigned int (&v)[E],
     return (unsigned int)__shfl((int)(unsigned int)x[0], r - c_lo - 1, 64);
 }
 
-// A value at or above the r-th smallest (1-based) of the 64*E values (padding 0xffffffff as in
-// wave_kth_u32): wave_kth_u32's bracket search stopped after `steps` probes, returning the upper
-// end of the bracket, which count(<= it) >= r makes an upper bound of the r-th smallest. Cheaper
-// than the exact value (no gather, no sort) where any bound at or above it is valid. All 64 lanes
-// must call it.
-template <int E>
-__device__ __forceinline__ unsigned int wave_kth_upper(const unsigned int (&v)[E], int r, int steps) {
-    static_assert(E <= 64, "per-lane counts must stay below 128");
-    unsigned int mn = 0xffffffffu, mx = 0u, npad = 0u;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        mn = min(mn, v[e]);
-        mx = v[e] == 0xffffffffu ? mx : max(mx, v[e]);
-        npad += v[e] == 0xffffffffu ? 1u : 0u;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        mn = min(mn, (unsigned int)__shfl_xor((int)mn, o, 64));
-        mx = max(mx, (unsigned int)__shfl_xor((int)mx, o, 64));
-    }
-    const int c_valid = 64 * E - wave_sum_small(npad);
-    if (r > c_valid) return 0xffffffffu;
-    long long lo = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mn) - 1;
-    long long hi = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mx);
-    int c_lo = 0, c_hi = c_valid;
-    bool interp = true;
-    for (int it = 0; it < steps && c_hi > r && hi - lo > 1; ++it) {
-        long long mid;
-        if (interp) {
-            const float f = (float)(r - c_lo) / (float)(c_hi - c_lo);
-            mid = lo + (long long)(f * (float)(hi - lo));
-            mid = mid <= lo ? lo + 1 : (mid >= hi ? hi - 1 : mid);
-        } else {
-            mid = lo + ((hi - lo) >> 1);
-        }
-        interp = !interp;
-        const unsigned int t = (unsigned int)mid;
-        unsigned int cl = 0u;
-#pragma unroll
-        for (int e = 0; e < E; ++e) cl += v[e] <= t ? 1u : 0u;
-        const int c = wave_sum_small(cl);
-        if (c >= r) {
-            hi = mid;
-            c_hi = c;
-        } else {
-            lo = mid;
-            c_lo = c;
-        }
-    }
-    return (unsigned int)hi;
-}
-
 // Sorts the n (<= 2048) keys produced by key_of(j), j in [0, n), with one wavefront and hands
 // every element of the sorted order to emit(rank, key); ranks >= n hold HV_KEY_PAD. All 64 lanes
 // of the wavefront must call it with the same n.

@@ -539,19 +539,19 @@ bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
     return false;
 }
 
-// After a search's device work: read the counters and the redo count (the one host sync), add
-// redo batches beyond the first, and fill the report (phase times only when the phases were
-// recorded, i.e. not from a captured graph).
+// After a search's device work: read the counters and the redo count (one copy, the one host
+// sync when nothing needs redoing), run the exact redo batches when some query needs one (the
+// redo kernels are launched only then), and fill the report (phase times only when the phases
+// were recorded, i.e. not from a captured graph).
 void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, bool timed,
                    std::chrono::steady_clock::time_point t0) {
     hipStream_t s = h->stream;
-    unsigned long long hs[15] = {};
-    int nredo = 0;
+    unsigned long long hs[16] = {};  // [15]: the redo count (low word) and the re-rank item count
     HV_HIP(hipMemcpyAsync(hs, p.stats, sizeof(hs), hipMemcpyDeviceToHost, s));
-    HV_HIP(hipMemcpyAsync(&nredo, p.redo_cnt, sizeof(int), hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
-    if (nredo > p.batch) {
-        for (int q0 = p.batch; q0 < nredo; q0 += p.batch) p.redo(q0);
+    const int nredo = (int)(hs[15] & 0xffffffffull);
+    if (nredo > 0) {
+        for (int q0 = 0; q0 < nredo; q0 += p.batch) p.redo(q0);
         if (p.truncate) p.truncate();
         HV_CHECK_LAUNCH("exact redo");
         HV_HIP(hipStreamSynchronize(s));
@@ -862,11 +862,18 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     };
     // BF16 form: each query's nearest list scanned and reduced in one pass (candidates in LDS)
     const bool near_pass = bf16 && h->sp.nearest_pass != 0 && k <= hv::kNearMaxK;
+    // learned BF16: the nearest lists, then the other searched ranks in one group when there are
+    // few of them (as the plain BF16 form takes up to kBf16SinglePhase probes), else in groups of
+    // growing width [1, 4), [4, 16), ..., each followed by a compaction that lowers the bound
+    auto learned_rank_group_end = [&](int64_t r0) -> int64_t {
+        if (r0 == 0) return 1;
+        return rank_end <= kBf16SinglePhase ? rank_end : std::min<int64_t>(rank_end, r0 * 4);
+    };
     std::vector<PhaseRange> ranges;
     if (mfma) {
         if (learned_seed) {
             for (int64_t r0 = 0; r0 < rank_end;) {
-                const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
+                const int64_t r1 = learned_rank_group_end(r0);
                 ranges.push_back({r0, r1, 0, INT32_MAX, r0 != 0, near_pass && r0 == 0});
                 r0 = r1;
             }
@@ -898,7 +905,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (!h->perm.empty()) need(h->s_qp, 4 * n * d);
         need(h->s_bucket, 4 * nb);
         need(h->s_fill, 4 * nb);
-        need(h->s_stats, 8 * 15);
+        need(h->s_stats, 8 * 16);
         need(h->s_surv_cnt, 4 * n);
         need(h->s_bound, 4 * n);
         need(h->s_bound_ref, 4 * n);
@@ -982,8 +989,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_fill.ensure(sizeof(int) * nb));
     // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
     // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos, [14] learned
-    // queries whose probe count the margin test widened
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 15));
+    // queries whose probe count the margin test widened, [15] the redo list's count (low word) and
+    // the re-rank item count (high word): all read back by one copy
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 16));
     HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
     HV_HIP(h->s_bound.ensure(sizeof(float) * n));
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
@@ -1000,9 +1008,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         };
         add(h->s_bucket.p, nb, 0);
         add(h->s_fill.p, nb, 0);
-        add(h->s_stats.p, 30, 0);
+        add(h->s_stats.p, 32, 0);
         add(h->s_surv_cnt.p, n, 0);
-        add(h->s_overflow.p, 2, 0);  // redo count, re-rank item count
         if (coarse_on_mfma(h, nprobe)) add(coarse_counter(h, n), 1, 0);
         if (mfma) add(h->s_last.p, n, -1);  // no compaction yet: counts never match
         add(h->s_runs.as<unsigned long long>() + batch * run_keys, batch, 0);  // first redo batch's counts
@@ -1131,9 +1138,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
         HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
         HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
-        hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, (int)rank_end, (int)d,
-                              h->dpad, h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), s);
-        HV_CHECK_LAUNCH("pair residuals");
+        // (the nearest-list pass forms the rank-0 residuals itself)
+        const int r_first = near_pass ? 1 : 0;
+        if (rank_end > r_first) {
+            hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, r_first, (int)rank_end,
+                                  (int)d, h->dpad, h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), s);
+            HV_CHECK_LAUNCH("pair residuals");
+        }
     }
     if (fp8) {
         if (!h->fp8_built) {
@@ -1203,10 +1214,10 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         const int grid = (int)std::min<int64_t>(
             1 << 15, (n / hv::kNearQueries + nlist + h->ntotal / hv::kNearSplit + hv::kNumXcd) / hv::kNumXcd * hv::kNumXcd);
         scan_ev(false);
-        hv::launch_mfma_nearest(grid, h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), (int)d, pdist, (int)nprobe,
+        hv::launch_mfma_nearest(grid, Qp, h->d_centroids.as<float>(), (int)d, pdist, (int)nprobe,
                                 h->d_rcodes.as<__bf16>(), h->d_roff.as<int64_t>(), h->d_rnorm.as<float>(),
                                 h->d_rerr.as<float>(), h->d_lrad.as<float>(), h->d_list_off.as<int64_t>(),
-                                h->s_bound.as<float>(),
+                                h->s_bound.as<float>(), learned_seed ? nullptr : h->s_bound_ref.as<float>(),
                                 h->s_pair_slot.as<int>(), ph.items, ph.queue, kappa1, kappa_g, kappa2, (int)k, kq, surv,
                                 aux, surv_cnt, last, (int)cap, st_slot, s);
         HV_CHECK_LAUNCH("nearest-list pass");
@@ -1238,7 +1249,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         // to the k-th smallest upper bound so far (when below the seed) and drops the pairs
         // whose lower bound exceeds it
         for (int64_t r0 = 0; r0 < rank_end;) {
-            const int64_t r1 = r0 == 0 ? 1 : std::min<int64_t>(rank_end, r0 * 4);
+            const int64_t r1 = learned_rank_group_end(r0);
             if (r0 == 0 && near_pass)
                 near_phase(stats);  // compacts as it goes; the compaction below merges split lists and spills
             else
@@ -1253,18 +1264,19 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         // bound exceeds it are dropped (vector ranges start on 16-vector blocks)
         const int vdirect = (int)(cap / 16 * 16);
         if (near_pass) {
+            // the pass and this compaction (of the queries whose nearest list was split into
+            // several items; the others are skipped) leave bound_ref = bound, the nearest list's
+            // bound
             near_phase(stats);
-            // the queries whose nearest list was split into several items or that spilled out of
-            // the pass's buffers (the others are skipped)
-            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s);
+            hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s, bound_ref);
             HV_CHECK_LAUNCH("compact");
         } else {
             if (vdirect > 0) mfma_phase(0, 1, 0, vdirect, 1, stats);
             hv::launch_first_counts(probes, n, (int)nprobe, h->d_list_off.as<int64_t>(), vdirect, surv_cnt, s);
             hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)kr, bound, last, kq, (int)(kr / k), s);
             HV_CHECK_LAUNCH("compact");
+            HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         }
-        HV_HIP(hipMemcpyAsync(bound_ref, bound, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         // the rest of the nearest lists (longer than the direct part), then the other probed lists
         if (!near_pass && h->max_list > vdirect) mfma_phase(0, 1, vdirect, INT32_MAX, 0, stats);
         if (!fp8 && nprobe <= kBf16SinglePhase) {
@@ -1307,9 +1319,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     rec(4);
     // 6. top-k selection. The BF16 form re-ranks its survivors (the plain path's FP32 arithmetic)
     //    and selects in the same pass.
-    int* redo_cnt = h->s_overflow.as<int>();  // cleared with the counters above
+    int* redo_cnt = reinterpret_cast<int*>(stats + 15);  // cleared with the counters above
     unsigned int* n_items = reinterpret_cast<unsigned int*>(redo_cnt + 1);
-    int* redo_list = redo_cnt + 2;
+    int* redo_list = h->s_overflow.as<int>();
     if (mfma) {
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
@@ -1331,9 +1343,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     }
     // 7. the exact path redoes the queries whose survivor buffer overflowed (rare) and, in learned
     //    mode, those whose k-th result is missing or above the seeded bound (a neighbour may have
-    //    been pruned), over the same probe lists, with no bound in learned mode. The list and its
-    //    count stay on the device: the kernels take the first batch by themselves, and the host
-    //    adds batches only when the final count exceeds one.
+    //    been pruned), over the same probe lists, with no bound in learned mode. The list is built
+    //    on the device; the host reads its count with the other counters and launches the redo
+    //    only when it is not empty (finish_search).
     // (the check is against the bound the seed set, bound_ref: later compactions only tighten
     //  the working bound towards the k-th distance itself)
     // (a learned query that is redone is redone over all nprobe_cap lists: its result is then the
@@ -1360,8 +1372,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                                h->d_ids.as<uint32_t>(), probes, (int)nprobe, (int)nprobe_cap, redo_qn, nseg, plan,
                                prune, redo_bound, (int)k, runs, D, I, q0 == 0, s);  // the first batch's counts: cleared above
     };
-    post.redo(0);
-    HV_CHECK_LAUNCH("exact redo");
     // per-query k: each row cut to its query's k (again after any redo batch the host adds)
     post.truncate = [=]() { hv::launch_truncate_rows(D, I, n, (int)k, kq, s); };
     post.truncate();

@@ -120,10 +120,10 @@ void launch_build_resid(const float* x, int d, int dpad, const int* order, const
                         float* rerr, hipStream_t s);
 // rows in list order: by_pos[p] = rows[order[p]]
 void launch_gather_rows(const float* rows, int64_t n, int d, const int* order, float* by_pos, hipStream_t s);
-// BF16 (q - c) rows of dpad dims for every (query, probe rank) slot, and the norm of each
-// slot's rounding error (+ 2^-23 of its norm) in qerr
+// BF16 (q - c) rows of dpad dims for every (query, probe rank in [r0, nranks)) slot, and the
+// norm of each slot's rounding error (+ 2^-23 of its norm) in qerr
 void launch_pair_resid(const float* Qp, const float* centroids, const int* probes, const int* qn, int64_t nq,
-                       int nprobe, int nranks, int d, int dpad, __bf16* out, float* qerr, hipStream_t s);
+                       int nprobe, int r0, int nranks, int d, int dpad, __bf16* out, float* qerr, hipStream_t s);
 // FP8 (e4m3) codes of every list from the rows in list order, scaled per list by a power of two
 // (inverse in inv_scale[l]), in the BF16 codes' order, one byte per value
 void launch_build_resid_fp8(const float* rows_by_pos, int d, int dpad, const int64_t* list_off,
@@ -151,12 +151,15 @@ void launch_mfma_scan(int grid, int elem, const void* qres, const float* qaux, i
 constexpr int kNearQueries = 16;
 constexpr int kNearSplit = 1024;  // vectors per item: longer lists are cut into several items
 constexpr int kNearMaxK = 1024;  // a query's k fits the range's bounds up to k <= kNearSplit
-void launch_mfma_nearest(int grid, const __bf16* qres, const float* qerr, int d, const float* pdist, int nprobe,
-                         const __bf16* rcodes, const int64_t* roff, const float* rnorm, const float* rerr,
-                         const float* lrad, const int64_t* list_off, float* bound, const int* pair_slot,
-                         const ScanItem* items, const int* queue, float kappa1, float kappa_g, float kappa2, int k,
-                         const int* kq, unsigned long long* surv, float* aux, unsigned int* surv_cnt,
-                         unsigned int* last, int cap, unsigned long long* stats, hipStream_t s);
+// The pass forms the rank-0 query residuals itself from Qp and the centroids (BF16 hi + lo parts
+// for d <= 256). bound_ref (optional): the bound of every query the pass finishes is written there
+// too.
+void launch_mfma_nearest(int grid, const float* Qp, const float* centroids, int d, const float* pdist, int nprobe, const __bf16* rcodes, const int64_t* roff,
+                         const float* rnorm, const float* rerr, const float* lrad, const int64_t* list_off,
+                         float* bound, float* bound_ref, const int* pair_slot, const ScanItem* items,
+                         const int* queue, float kappa1, float kappa_g, float kappa2, int k, const int* kq,
+                         unsigned long long* surv, float* aux, unsigned int* surv_cnt, unsigned int* last, int cap,
+                         unsigned long long* stats, hipStream_t s);
 // FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place;
 // bound[q] drops to the kr-th estimate
 // with kq: each query keeps min(kr, kmul * k_q)
@@ -168,8 +171,10 @@ void launch_first_counts(const int* probes, int64_t nq, int nprobe, const int64_
 // last (optional, nq counts; start at 0xffffffff): the count each query's previous compaction
 // left; a query whose count still equals it is skipped
 // with kq: the bound at each query's min(k, kmul * k_q)-th upper bound
+// bound_ref (optional): each compacted query's new bound is written there too
 void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt, int64_t nq, int cap, int k,
-                         float* bound, unsigned int* last, const int* kq, int kmul, hipStream_t s);
+                         float* bound, unsigned int* last, const int* kq, int kmul, hipStream_t s,
+                         float* bound_ref = nullptr);
 // re-rank + final top-k of the MFMA forms, in three launches:
 //  launch_rerank_items: work items of 64 stored pairs (items: nq * ceil(cap/64) entries, n_items
 //    zeroed by the caller); overflowed queries listed when overflow_list is given; stats[2]

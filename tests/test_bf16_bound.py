@@ -46,19 +46,36 @@ def kappas(d, B, factor=2.0):
     return (np.float32(1.01 * factor), np.float32(1.01 * 2.0 * 2.0 * gd), np.float32(1.01 * (4.0 * d + 2.0 * B + 32.0) * U))
 
 
-def bounds(q, x, c, B=4, factor=2.0):
-    """(lb, ub, exact chain distance) of the pairs (q[i], x[i]) in the list with centroid c."""
+def split_err_norm(r):
+    """|r - hi - lo| + 2^-23 |r| in FP32 for the split r ~ hi + lo (hi = bf16(r), lo = bf16(r - hi))."""
+    hi = bf16(r)
+    lo = bf16((r - hi).astype(np.float32))
+    e = ((r - hi).astype(np.float32) - lo).astype(np.float32)
+    return (np.sqrt(chain(e, np.zeros_like(e))) + np.float32(2.0 ** -23) * np.sqrt(chain(r, np.zeros_like(r)))
+            ).astype(np.float32)
+
+
+def bounds(q, x, c, B=4, factor=2.0, split=False):
+    """(lb, ub, exact chain distance) of the pairs (q[i], x[i]) in the list with centroid c. With
+    split, the query residual is carried as two BF16 parts hi + lo (the nearest-list pass for
+    d <= 256): both are multiplied into the codes, 2d exact products accumulated in FP32, and the
+    accumulation term of the bound is taken 2.1 times."""
     d = q.shape[1]
     rq = (q - c).astype(np.float32)
     rx = (x - c).astype(np.float32)
     nrq = chain(q, np.broadcast_to(c, q.shape))
     nx = chain(x, np.broadcast_to(c, x.shape))
     bq, bx = bf16(rq), bf16(rx)
+    lo = bf16((rq - bq).astype(np.float32)) if split else np.zeros_like(rq)
     dot = np.zeros(q.shape[0], np.float32)
     for k in range(d):  # exact BF16 products, FP32 accumulation
         dot = (dot + bq[:, k] * bx[:, k]).astype(np.float32)
-    eq, ex = err_norm(rq), err_norm(rx)
+        if split:
+            dot = (dot + lo[:, k] * bx[:, k]).astype(np.float32)
+    eq, ex = (split_err_norm(rq) if split else err_norm(rq)), err_norm(rx)
     k1, kg, k2 = kappas(d, B, factor)
+    if split:
+        kg = np.float32(2.1) * kg
     sq, sx = np.sqrt(nrq), np.sqrt(nx)
     sqp = (sq + eq).astype(np.float32)
     err = (k1 * (eq * sx + sqp * ex) + kg * sqp * (sx + ex) + k2 * (nrq + nx)).astype(np.float32)
@@ -106,3 +123,28 @@ def test_bound_holds_on_aligned_rounding(d, swap):
     # the cross term is reached: a bound without its factor 2 is violated
     lb1, ub1, exact1, _ = bounds(q, x, c, factor=1.0)
     assert np.any(lb1 > exact1) or np.any(exact1 > ub1)
+
+
+@pytest.mark.parametrize("d", [8, 33, 128, 256])
+@pytest.mark.parametrize("scale", [1e-3, 1.0, 3e4])
+def test_split_query_bound_holds_and_halves(d, scale):
+    """The query residual as hi + lo: the bound still holds, and with the query's rounding term
+    gone it is about half the single-part bound."""
+    rng = np.random.default_rng(d + 7)
+    c = (rng.standard_normal(d) * scale).astype(np.float32)
+    q = (c + rng.standard_normal((400, d)) * scale).astype(np.float32)
+    x = (c + rng.standard_normal((400, d)) * scale * rng.uniform(0.01, 3, size=(400, 1))).astype(np.float32)
+    lb, ub, exact, err_s = bounds(q, x, c, split=True)
+    assert np.all(lb <= exact) and np.all(exact <= ub)
+    _, _, _, err = bounds(q, x, c)
+    assert np.median(err_s / err) < 0.65
+
+
+@pytest.mark.parametrize("d", [16, 128])
+def test_split_query_bound_holds_on_aligned_rounding(d):
+    """Codes whose rounding error is near half an ulp, queries parallel to it: the code's term
+    (|rq| + |eq|) |ex| is reached and the split bound must still cover it."""
+    q, x, c = adversarial(300, d, seed=d + 1, scale=1.0)
+    q, x = x, q
+    lb, ub, exact, _ = bounds(q, x, c, split=True)
+    assert np.all(lb <= exact) and np.all(exact <= ub)
