@@ -322,6 +322,7 @@ struct hv_index {
     std::vector<float> ext_bounds;
     std::vector<int32_t> query_nprobe;  // per-query probe counts (empty: nprobe for all)
     PinnedBuf qn_host;                  // the clamped counts of the current search
+    PinnedBuf post_host;                // the search's counters, read back after its device work
     std::vector<int32_t> qn_stage;      // the same for a no-allocation search (sized when set)
     std::vector<int32_t> query_k;       // per-query k (empty: the search's k for all)
     PinnedBuf qk_host;                  // their clamped values for the current search
@@ -546,9 +547,14 @@ bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
 void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, bool timed,
                    std::chrono::steady_clock::time_point t0) {
     hipStream_t s = h->stream;
-    unsigned long long hs[16] = {};  // [15]: the redo count (low word) and the re-rank item count
-    HV_HIP(hipMemcpyAsync(hs, p.stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    // [15]: the redo count (low word) and the re-rank item count. Copied into the handle's pinned
+    // buffer (allocated with the handle): no copy is ever left in flight into a stack frame that
+    // an error unwinds.
+    unsigned long long* hs_dev_copy = static_cast<unsigned long long*>(h->post_host.p);
+    HV_HIP(hipMemcpyAsync(hs_dev_copy, p.stats, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost, s));
     HV_HIP(hipStreamSynchronize(s));
+    unsigned long long hs[16];
+    std::memcpy(hs, hs_dev_copy, sizeof(hs));
     const int nredo = (int)(hs[15] & 0xffffffffull);
     if (nredo > 0) {
         for (int q0 = 0; q0 < nredo; q0 += p.batch) p.redo(q0);
@@ -1432,6 +1438,11 @@ hv_index_t* hv_create(int64_t d, int64_t nlist, int32_t metric, int32_t opt_leve
         hv_destroy(h);
         return nullptr;
     }
+    if (h->post_host.ensure(sizeof(unsigned long long) * 16) != hipSuccess) {
+        g_create_error = "hipHostMalloc failed";
+        hv_destroy(h);
+        return nullptr;
+    }
     {
         // the no-allocation search's scratch on this device (current here); a failure only
         // disables that entry point, which reports it
@@ -1461,6 +1472,7 @@ void hv_destroy(hv_index_t* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->graph.reset();
     h->qn_host.release();
+    h->post_host.release();
     h->qk_host.release();
     DevBuf* bufs[] = {&h->d_perm, &h->d_centroids, &h->d_codes, &h->d_list_off, &h->d_code_off, &h->d_list_stride,
                       &h->d_ids, &h->d_rank_order, &h->s_q, &h->s_qp, &h->s_D, &h->s_I, &h->s_dist, &h->s_probes,
