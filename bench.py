@@ -26,14 +26,17 @@ PCIe included). config.learned_gain compares the learned form with the fixed-npr
 equal recall for higher targets (--gain-targets).
 
 Prints one JSON line. `roofline` is for the headline form's dominant kernel, the one with the
-larger summed device time per step:
+longest average launch:
   - the candidate scan: FP32 VALU for the plain form (3d FLOPs per unpruned pair against 157.3
     TFLOP/s); for the MFMA forms whichever bound is larger of HBM (the codes, norms and error
     norms of every touched list, once per step, plus the query residuals) and MFMA (2d FLOPs per
     pair);
   - the exact re-rank of the MFMA forms: bytes per re-ranked pair (the FP32 row, 4d, plus its
     id, 4, and the key read and written, 16) against the HBM peak;
-each divided by the kernel's summed duration from HIP events on the library's stream.
+each divided by the kernel's summed duration from HIP events on the library's stream. The MFMA
+forms' scan figure sums the nearest-list pass (k_mfma_nearest) and the scans of the other ranks
+(k_mfma_scan): its per-launch average is not one kernel's, so it is reported under
+`config.forms.*.kernels` and only the re-rank, a single launch, is compared for `roofline`.
 `cpu_baseline` is the oracle's restatement of the reference CPU scan (built with the reference's
 Release flags) at the plain operating point, timed on this host: this GPU's share of the cores
 (pruning on and off) and one thread.
@@ -330,13 +333,14 @@ def roofline_of(name, st, d, dpad, touched_vecs, slots):
             t_mfma = flops / (peak_f * 1e12)
             if t_hbm >= t_mfma:
                 ach = nbytes / t_scan / 1e9
-                cands["scan"] = {"bound": "hbm", "kernel": "k_mfma_scan", "achieved": round(ach, 1),
+                cands["scan"] = {"bound": "hbm", "kernel": "k_mfma_nearest+k_mfma_scan", "achieved": round(ach, 1),
                                  "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 5),
                                  "algorithmic_bytes_per_step": nbytes, "touched_vectors": touched_vecs,
                                  "mfma_frac": round(flops / t_scan / 1e12 / peak_f, 6)}
             else:
                 ach = flops / t_scan / 1e12
-                cands["scan"] = {"bound": "mfma", "kernel": "k_mfma_scan", "achieved": round(ach, 3), "peak": peak_f,
+                cands["scan"] = {"bound": "mfma", "kernel": "k_mfma_nearest+k_mfma_scan", "achieved": round(ach, 3),
+                                 "peak": peak_f,
                                  "unit": "TFLOP/s", "frac": round(ach / peak_f, 5), "algorithmic_flops_per_step": flops}
             cands["scan"].update({"kernel_ms": round(st["ms_scan_kernels"], 4), "launches": int(st["scan_launches"]),
                                   "pairs": int(st["scan_pairs"])})
@@ -349,7 +353,10 @@ def roofline_of(name, st, d, dpad, touched_vecs, slots):
                            "launches": 1, "pairs": int(st["rerank_pairs"])}
     if not cands:
         return None, {}
-    dom = max(cands, key=lambda c: cands[c]["kernel_ms"])
+    # the dominant kernel: one kernel's launches (the plain scan's warm-up and main launches are
+    # one kernel; the MFMA forms' scan figure mixes two, so the re-rank stands for those forms)
+    single = {c: v for c, v in cands.items() if plain or c != "scan"} or cands
+    dom = max(single, key=lambda c: single[c]["kernel_ms"] / max(1, single[c]["launches"]))
     return dict(cands[dom]), cands
 
 
