@@ -659,12 +659,13 @@ constexpr int kNearStride = kNearSplit + 16;  // LDS words per row (padded: the 
 constexpr unsigned int kNearTop = 65535u;     // code of "at or above the scale's end"
 
 // The r-th smallest (1-based, r <= the number of values) of the 16-bit values a wavefront holds E
-// per lane (padding 0xffffffff, not counted), exact: a radix select over the high byte, then the
-// low byte of the values in the selected high-byte bin, each pass a 256-bin histogram in LDS
-// (hist: 256 words of this wavefront's own) built with atomic adds and read back with one prefix
-// scan over the lanes (4 bins per lane). All 64 lanes must call it.
+// per lane, exact. The values are the low halves of packed words (0xffffffff: no value, not
+// counted). A radix select over the high byte, then the low byte of the values in the selected
+// high-byte bin, each pass a 256-bin histogram in LDS (hist: 256 words of this wavefront's own)
+// built with atomic adds and read back with one prefix scan over the lanes (4 bins per lane). All
+// 64 lanes must call it.
 template <int E>
-__device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&v)[E], int r, unsigned int* hist) {
+__device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&w)[E], int r, unsigned int* hist) {
     const int lane = threadIdx.x & 63;
     unsigned int prefix = 0u;  // the selected high byte (pass 2 counts only values with it)
     int rr = r;
@@ -674,19 +675,15 @@ __device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&v)[E],
         lds_fence();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const bool in = v[e] != 0xffffffffu && (pass == 0 || (v[e] >> 8) == prefix);
-            if (in) atomicAdd(&hist[pass == 0 ? v[e] >> 8 : v[e] & 255u], 1u);
+            const unsigned int v = w[e] & 0xffffu;
+            const bool in = w[e] != 0xffffffffu && (pass == 0 || (v >> 8) == prefix);
+            if (in) atomicAdd(&hist[pass == 0 ? v >> 8 : v & 255u], 1u);
         }
         lds_fence();
         const uint4 h4 = *reinterpret_cast<const uint4*>(hist + 4 * lane);
         const unsigned int h[4] = {h4.x, h4.y, h4.z, h4.w};
         const unsigned int sum = h[0] + h[1] + h[2] + h[3];
-        unsigned int incl = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned int y = (unsigned int)__shfl_up((int)incl, o, 64);
-            if (lane >= o) incl += y;
-        }
+        const unsigned int incl = wave_incl_scan(sum);
         const unsigned int excl = incl - sum;
         const bool mine = excl < (unsigned int)rr && (unsigned int)rr <= incl;
         unsigned int bin = 0u, below = 0u;
@@ -705,8 +702,8 @@ __device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&v)[E],
             }
         }
         const int src = __builtin_ctzll(__ballot(mine) | (1ull << 63));
-        bin = (unsigned int)__shfl((int)bin, src, 64);
-        below = (unsigned int)__shfl((int)below, src, 64);
+        bin = (unsigned int)__builtin_amdgcn_readlane((int)bin, src);
+        below = (unsigned int)__builtin_amdgcn_readlane((int)below, src);
         rr -= (int)below;
         prefix = pass == 0 ? bin : (prefix << 8 | bin);
         lds_fence();  // the histogram is cleared again only after every lane has read it
@@ -924,53 +921,53 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
             const int qq = s_q[r], k_r = s_kr[r];
             const float step = s_step[r];
             float T = s_b0[r];
-            unsigned int c[E], u[E];
+            unsigned int c[E];  // the pairs' packed codes (0xffffffff past the range)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int j = e * 64 + lane;
                 c[e] = j < nv ? s_code[r][j] : 0xffffffffu;
-                u[e] = j < nv ? (c[e] & 0xffffu) : 0xffffffffu;
             }
             if (nv >= k_r) {
-                const unsigned int kth = wave_kth_u16<E>(u, k_r, s_hist[w]);
+                const unsigned int kth = wave_kth_u16<E>(c, k_r, s_hist[w]);
                 if (kth < kNearTop) T = fminf(T, (float)kth * step);  // >= the k-th upper bound
             }
-            unsigned long long keep_m[E];
-            unsigned int m = 0u;
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const bool keep = e * 64 + lane < nv && (float)(c[e] >> 16) * step <= T;
-                keep_m[e] = __ballot(keep);
-                m += (unsigned int)__builtin_popcountll(keep_m[e]);
-            }
+            // the kept pairs (lower bound <= T), one ballot per 64 at a time (no mask array held)
+            auto keep_of = [&](int e) { return e * 64 + lane < nv && (float)(c[e] >> 16) * step <= T; };
             unsigned int at = 0u;
             if (split) {
                 // parts of one list append; the compaction after the pass merges them
+                unsigned int m = 0u;
+#pragma unroll
+                for (int e = 0; e < E; ++e) m += (unsigned int)__builtin_popcountll(__ballot(keep_of(e)));
                 if (lane == 0) {
                     if (m) at = atomicAdd(&surv_cnt[qq], m);
                     atomicMin(reinterpret_cast<unsigned int*>(bound) + qq, __float_as_uint(T));
                 }
-                at = (unsigned int)__shfl((int)at, 0, 64);
-            } else if (lane == 0) {
-                surv_cnt[qq] = m;
-                if (last) last[qq] = m;
-                bound[qq] = T;
-                if (bound_ref) bound_ref[qq] = T;
+                at = (unsigned int)__builtin_amdgcn_readfirstlane((int)at);
             }
             unsigned long long* __restrict__ sq = surv + (int64_t)qq * cap;
             unsigned int* __restrict__ aq = reinterpret_cast<unsigned int*>(aux) + (int64_t)qq * cap;
             unsigned int o = at;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                if ((keep_m[e] >> lane) & 1ull) {
-                    const unsigned int slot = o + (unsigned int)__builtin_popcountll(keep_m[e] & ((1ull << lane) - 1ull));
+                const bool keep = keep_of(e);
+                const unsigned long long km = __ballot(keep);
+                if (keep) {
+                    const unsigned int slot =
+                        o + __builtin_amdgcn_mbcnt_hi((unsigned int)(km >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)km, 0u));
                     if (slot < (unsigned int)cap) {
                         const unsigned int cu = c[e] & 0xffffu;
                         sq[slot] = hv_key((float)(c[e] >> 16) * step, pos0 + (unsigned int)(e * 64 + lane));
                         aq[slot] = cu >= kNearTop ? 0x7f800000u : __float_as_uint((float)cu * step);
                     }
                 }
-                o += (unsigned int)__builtin_popcountll(keep_m[e]);
+                o += (unsigned int)__builtin_popcountll(km);
+            }
+            if (!split && lane == 0) {
+                surv_cnt[qq] = o;
+                if (last) last[qq] = o;
+                bound[qq] = T;
+                if (bound_ref) bound_ref[qq] = T;
             }
         }
         lds_barrier();  // the rows are free for the next item (the stores need not wait)

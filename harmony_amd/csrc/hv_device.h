@@ -242,6 +242,21 @@ __device__ inline unsigned long long block_radix_kth(const unsigned long long* a
     return prefix;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront (all lanes active) with DPP moves only:
+// row_shr 1, 2, 4, 8 scan each 16-lane row, then row_bcast:15 adds row 0's total to row 1 and row
+// 2's to row 3, and row_bcast:31 adds the total of rows 0-1 to rows 2 and 3. Lanes whose source
+// lies outside their row, and rows outside the row mask, add the old value 0. Six VALU moves
+// instead of six LDS permutes.
+__device__ __forceinline__ unsigned int wave_incl_scan(unsigned int x) {
+    x += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 // Sum over the wavefront of a per-lane count below 128, as a wave-uniform value: one ballot per
 // bit of the count, no cross-lane data movement.
 __device__ __forceinline__ int wave_sum_small(unsigned int x) {

@@ -125,7 +125,6 @@ struct SearchPost {
     bool main_recorded = false, mfma = false, per_query = false;
     int B = 1;
     int64_t nprobe = 0, nprobe_cap = 0;
-    unsigned long long* stats = nullptr;
     int* redo_cnt = nullptr;
     std::function<void(int)> redo;
     std::function<void()> truncate;
@@ -547,14 +546,12 @@ bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
 void finish_search(hv_index_t* h, const SearchPost& p, int64_t n, hv_stats* st, bool timed,
                    std::chrono::steady_clock::time_point t0) {
     hipStream_t s = h->stream;
-    // [15]: the redo count (low word) and the re-rank item count. Copied into the handle's pinned
-    // buffer (allocated with the handle): no copy is ever left in flight into a stack frame that
-    // an error unwinds.
-    unsigned long long* hs_dev_copy = static_cast<unsigned long long*>(h->post_host.p);
-    HV_HIP(hipMemcpyAsync(hs_dev_copy, p.stats, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost, s));
+    // [15]: the redo count (low word) and the re-rank item count. The redo-list kernel, the last
+    // of the device work, wrote the 16 words into the handle's pinned buffer (allocated with the
+    // handle); the synchronization makes them visible here.
     HV_HIP(hipStreamSynchronize(s));
     unsigned long long hs[16];
-    std::memcpy(hs, hs_dev_copy, sizeof(hs));
+    std::memcpy(hs, h->post_host.p, sizeof(hs));
     const int nredo = (int)(hs[15] & 0xffffffffull);
     if (nredo > 0) {
         for (int q0 = 0; q0 < nredo; q0 += p.batch) p.redo(q0);
@@ -911,7 +908,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         if (!h->perm.empty()) need(h->s_qp, 4 * n * d);
         need(h->s_bucket, 4 * nb);
         need(h->s_fill, 4 * nb);
-        need(h->s_stats, 8 * 16);
+        need(h->s_stats, 8 * 17);
         need(h->s_surv_cnt, 4 * n);
         need(h->s_bound, 4 * n);
         need(h->s_bound_ref, 4 * n);
@@ -996,8 +993,9 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // stats: [0..3] counters of the scan phases, [4..7] of the warm-up, [8..11] of the main phase
     // (the remaining probed lists), [12] overflowed buffers, [13] learned-mode redos, [14] learned
     // queries whose probe count the margin test widened, [15] the redo list's count (low word) and
-    // the re-rank item count (high word): all read back by one copy
-    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 16));
+    // the re-rank item count (high word): all published to the host by the redo-list kernel, which
+    // counts its finished workgroups in [16]
+    HV_HIP(h->s_stats.ensure(sizeof(unsigned long long) * 17));
     HV_HIP(h->s_surv_cnt.ensure(sizeof(unsigned int) * n));
     HV_HIP(h->s_bound.ensure(sizeof(float) * n));
     HV_HIP(h->s_bound_ref.ensure(sizeof(float) * n));
@@ -1014,7 +1012,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         };
         add(h->s_bucket.p, nb, 0);
         add(h->s_fill.p, nb, 0);
-        add(h->s_stats.p, 32, 0);
+        add(h->s_stats.p, 34, 0);
         add(h->s_surv_cnt.p, n, 0);
         if (coarse_on_mfma(h, nprobe)) add(coarse_counter(h, n), 1, 0);
         if (mfma) add(h->s_last.p, n, -1);  // no compaction yet: counts never match
@@ -1358,7 +1356,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     //  plain search's at nprobe_cap, and its reported probe count says so)
     hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
                          redo_cnt, stats + 12, learned ? h->s_qn.as<int>() : nullptr, (int)nprobe_cap,
-                         learned && !fp8 ? bound : nullptr, s);
+                         learned && !fp8 ? bound : nullptr, stats, reinterpret_cast<unsigned int*>(stats + 16),
+                         static_cast<unsigned long long*>(h->post_host.p), s);
     const int* redo_qn = learned ? nullptr : qn;
     // learned (FP32, BF16): each redone query's own k-th result (written above); FP8: no bound
     const float* redo_bound = fp8 ? nullptr : bound;
@@ -1369,7 +1368,6 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     post.B = B;
     post.nprobe = nprobe;
     post.nprobe_cap = nprobe_cap;
-    post.stats = stats;
     post.redo_cnt = redo_cnt;
     unsigned long long* runs = h->s_runs.as<unsigned long long>();
     post.redo = [=](int q0) {

@@ -200,7 +200,8 @@ void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt,
 // bound_out (optional): its bound to the k-th result it has (FLT_MAX when overflowed or short)
 void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
                       int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, int* qn_out,
-                      int qn_full, float* bound_out, hipStream_t s);
+                      int qn_full, float* bound_out, const unsigned long long* stats, unsigned int* done,
+                      unsigned long long* publish, hipStream_t s);
 // the workgroup select alone: queries whose done flag is 0 (with done), else those with more than
 // kWaveSortMax survivors
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,

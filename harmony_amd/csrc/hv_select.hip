@@ -302,12 +302,19 @@ __global__ __launch_bounds__(256) void k_select_block(const unsigned long long* 
     }
 }
 
+// The queries to redo, listed on the device, and the search's counters published to the host:
+// the last workgroup to finish (a done count, cleared with the counters) copies the 16 counter
+// words, the redo count included, into host-visible memory, so the host needs no copy of its own
+// after the device work (publish: the words, pinned host memory; done: the count; both or none).
 __global__ __launch_bounds__(256) void k_redo_list(const unsigned int* __restrict__ cnt, int cap,
                                                    const float* __restrict__ D, const int64_t* __restrict__ I, int k,
                                                    const float* __restrict__ bound, int check_bound, int64_t n,
                                                    int* __restrict__ list, int* __restrict__ list_cnt,
                                                    unsigned long long* __restrict__ counts, int* __restrict__ qn_out,
-                                                   int qn_full, float* __restrict__ bound_out) {
+                                                   int qn_full, float* __restrict__ bound_out,
+                                                   const unsigned long long* stats, unsigned int* done,
+                                                   unsigned long long* __restrict__ publish) {
+    __shared__ unsigned int s_last;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     bool ovf = false, redo = false;
@@ -316,28 +323,40 @@ __global__ __launch_bounds__(256) void k_redo_list(const unsigned int* __restric
         redo = ovf || (check_bound && (I[q * k + k - 1] < 0 || D[q * k + k - 1] > bound[q]));
     }
     const unsigned long long m = __ballot(redo), mo = __ballot(ovf);
-    if (!m) return;
-    int base = 0;
-    if (lane == 0) {
-        base = atomicAdd(list_cnt, (int)__builtin_popcountll(m));
-        if (mo) atomicAdd(&counts[0], (unsigned long long)__builtin_popcountll(mo));
-        if (m != mo) atomicAdd(&counts[1], (unsigned long long)__builtin_popcountll(m & ~mo));
+    if (m) {
+        int base = 0;
+        if (lane == 0) {
+            base = atomicAdd(list_cnt, (int)__builtin_popcountll(m));
+            if (mo) atomicAdd(&counts[0], (unsigned long long)__builtin_popcountll(mo));
+            if (m != mo) atomicAdd(&counts[1], (unsigned long long)__builtin_popcountll(m & ~mo));
+        }
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (redo) {
+            list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
+            if (qn_out) qn_out[q] = qn_full;
+            // the query's own k-th result bounds the k-th over a superset of its lists
+            if (bound_out) bound_out[q] = (!ovf && I[q * k + k - 1] >= 0) ? D[q * k + k - 1] : 3.402823466e38f;
+        }
     }
-    base = __shfl(base, 0, 64);
-    if (redo) {
-        list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
-        if (qn_out) qn_out[q] = qn_full;
-        // the query's own k-th result bounds the k-th over a superset of its lists
-        if (bound_out) bound_out[q] = (!ovf && I[q * k + k - 1] >= 0) ? D[q * k + k - 1] : 3.402823466e38f;
-    }
+    if (!publish) return;
+    // every counter update of this workgroup is performed (the fence waits for the atomics issued
+    // above) before the workgroup is counted
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_last = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    __syncthreads();
+    if (s_last && threadIdx.x < 16)
+        publish[threadIdx.x] = __hip_atomic_load(stats + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
                       int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, int* qn_out,
-                      int qn_full, float* bound_out, hipStream_t s) {
+                      int qn_full, float* bound_out, const unsigned long long* stats, unsigned int* done,
+                      unsigned long long* publish, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_redo_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cnt, cap, D, I, k, bound,
-                       check_bound, n, list, list_cnt, counts, qn_out, qn_full, bound_out);
+                       check_bound, n, list, list_cnt, counts, qn_out, qn_full, bound_out, stats, done, publish);
 }
 
 static size_t select_block_lds(int cap, int k) {
