@@ -652,7 +652,12 @@ int bf16_grid_size(int device) {
 // bound and the compaction count (`last`): what the nearest-list scan and its compaction leave. A
 // list split into several items appends (atomics on the count), lowers the bound by an atomic
 // minimum and leaves `last`, so the compaction after the pass merges the parts (each part keeps
-// every pair of its range whose upper bound is among the k smallest of the list).
+// every pair of its range whose upper bound is among the k smallest of the list). A fused pass
+// (`fused`: the items of several probe ranks in one launch, so a query's items may run at once)
+// appends every row the same way; the k-th smallest upper bound over any part of a query's
+// candidates bounds its k-th distance, so each part may lower the bound. There the nearest list of
+// a query held by one item records its count in `last`, and the compaction leaves the query alone
+// when nothing else was appended.
 constexpr int kNearRows = kNearQueries;  // queries per item: one MFMA query block
 static_assert(kNearRows == 16, "an item's queries are one 16-row MFMA block");
 constexpr int kNearStride = kNearSplit + 16;  // LDS words per row (padded: the rows' banks differ)
@@ -728,10 +733,10 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
                                                       float kappa2, int k_all, const int* __restrict__ kq,
                                                       unsigned long long* __restrict__ surv, float* __restrict__ aux,
                                                       unsigned int* __restrict__ surv_cnt,
-                                                      unsigned int* __restrict__ last, int cap,
+                                                      unsigned int* __restrict__ last, int cap, int fused,
                                                       unsigned long long* __restrict__ stats) {
     typedef ElemBf16::vec vec;
-    __shared__ int s_q[kNearRows], s_kr[kNearRows];
+    __shared__ int s_q[kNearRows], s_kr[kNearRows], s_first[kNearRows];
     __shared__ __align__(16) float s_nrq[kNearRows];   // |rq|^2
     __shared__ __align__(16) float s_srq[kNearRows];   // |rq| + |eq|
     __shared__ __align__(16) float s_qsc[kNearRows];   // kappa1 |eq|
@@ -767,11 +772,12 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
         // norm of what the parts leave out, plus 2^-23 |r| for the FP32 subtraction (the four
         // lanes of a row sum theirs)
         constexpr bool kSplit = NKS <= 8;
-        int q = 0, kr = 0;
+        int q = 0, kr = 0, first = 0;
         float nr = 0.f, b = 0.f, qe = 0.f;
         if (tid < nQ) {
             const int slot = pair_slot[item.pair_begin + tid];
             q = slot / nprobe;
+            first = slot - q * nprobe == 0;  // the query's nearest list
             nr = pdist[slot];
             b = bound[q];
             kr = hv_query_k(kq, q, k_all);
@@ -838,6 +844,7 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
             const float step = top / 65000.f;
             s_q[tid] = q;
             s_kr[tid] = kr;
+            s_first[tid] = first;
             s_nrq[tid] = nr;
             s_srq[tid] = __builtin_amdgcn_sqrtf(nr) + qe;
             s_qsc[tid] = kappa1 * qe;
@@ -934,14 +941,18 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
             // the kept pairs (lower bound <= T), one ballot per 64 at a time (no mask array held)
             auto keep_of = [&](int e) { return e * 64 + lane < nv && (float)(c[e] >> 16) * step <= T; };
             unsigned int at = 0u;
-            if (split) {
-                // parts of one list append; the compaction after the pass merges them
+            const bool append = split || fused;
+            if (append) {
+                // parts of one list, and every row of a fused pass, append (the compaction after
+                // the pass merges them); in a fused pass the nearest list of a query held by one
+                // item records its count, so a query nothing else appended to is left alone
                 unsigned int m = 0u;
 #pragma unroll
                 for (int e = 0; e < E; ++e) m += (unsigned int)__builtin_popcountll(__ballot(keep_of(e)));
                 if (lane == 0) {
                     if (m) at = atomicAdd(&surv_cnt[qq], m);
                     atomicMin(reinterpret_cast<unsigned int*>(bound) + qq, __float_as_uint(T));
+                    if (!split && s_first[r] && last) last[qq] = m;
                 }
                 at = (unsigned int)__builtin_amdgcn_readfirstlane((int)at);
             }
@@ -963,7 +974,7 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
                 }
                 o += (unsigned int)__builtin_popcountll(km);
             }
-            if (!split && lane == 0) {
+            if (!append && lane == 0) {
                 surv_cnt[qq] = o;
                 if (last) last[qq] = o;
                 bound[qq] = T;
@@ -981,12 +992,13 @@ void launch_mfma_nearest(int grid, const float* Qp, const float* centroids, int 
                          float* bound, float* bound_ref, const int* pair_slot, const ScanItem* items,
                          const int* queue, float kappa1, float kappa_g, float kappa2, int k, const int* kq,
                          unsigned long long* surv, float* aux, unsigned int* surv_cnt, unsigned int* last, int cap,
-                         unsigned long long* stats, hipStream_t s) {
+                         int fused, unsigned long long* stats, hipStream_t s) {
 #define HV_NEAR_CASE(K)                                                                                              \
     case K:                                                                                                          \
         hipLaunchKernelGGL((k_mfma_nearest<K>), dim3((unsigned)grid), dim3(256), 0, s, Qp, centroids, d,             \
                            pdist, nprobe, rcodes, roff, rnorm, rerr, lrad, list_off, bound, bound_ref, pair_slot,    \
-                           items, queue, kappa1, kappa_g, kappa2, k, kq, surv, aux, surv_cnt, last, cap, stats);     \
+                           items, queue, kappa1, kappa_g, kappa2, k, kq, surv, aux, surv_cnt, last, cap, fused,      \
+                           stats);                                                                                   \
         break;
     switch (bf16_ksteps(d)) {
         HV_NEAR_CASE(1)

@@ -865,11 +865,19 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     };
     // BF16 form: each query's nearest list scanned and reduced in one pass (candidates in LDS)
     const bool near_pass = bf16 && h->sp.nearest_pass != 0 && k <= hv::kNearMaxK;
-    // learned BF16: the nearest lists, then the other searched ranks in one group when there are
-    // few of them (as the plain BF16 form takes up to kBf16SinglePhase probes), else in groups of
-    // growing width [1, 4), [4, 16), ..., each followed by a compaction that lowers the bound
+    // learned BF16 with the nearest-list pass: the pass takes ranks [0, near_end) at once (every
+    // row appends and lowers its query's bound; near_end <= cap / (2k), so what the items of one
+    // query keep, about k or a little more each, fits its buffer)
+    const int64_t near_end =
+        near_pass && learned_seed
+            ? std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(rank_end, kBf16SinglePhase), cap / (2 * k)))
+            : 1;
+    // learned BF16: the nearest lists (or the pass's ranks), then the other searched ranks in one
+    // group when there are few of them (as the plain BF16 form takes up to kBf16SinglePhase
+    // probes), else in groups of growing width [1, 4), [4, 16), ..., each followed by a compaction
+    // that lowers the bound
     auto learned_rank_group_end = [&](int64_t r0) -> int64_t {
-        if (r0 == 0) return 1;
+        if (r0 == 0) return near_pass ? near_end : 1;
         return rank_end <= kBf16SinglePhase ? rank_end : std::min<int64_t>(rank_end, r0 * 4);
     };
     std::vector<PhaseRange> ranges;
@@ -1142,8 +1150,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         kappa2 = (float)(1.01 * (4.0 * d + 2.0 * B + 32.0) * u);
         HV_HIP(h->s_qres.ensure(sizeof(__bf16) * n * nprobe * h->dpad));
         HV_HIP(h->s_qscale.ensure(sizeof(float) * n * nprobe));
-        // (the nearest-list pass forms the rank-0 residuals itself)
-        const int r_first = near_pass ? 1 : 0;
+        // (the nearest-list pass forms its ranks' residuals itself)
+        const int r_first = near_pass ? (int)near_end : 0;
         if (rank_end > r_first) {
             hv::launch_pair_resid(Qp, h->d_centroids.as<float>(), probes, qn, n, (int)nprobe, r_first, (int)rank_end,
                                   (int)d, h->dpad, h->s_qres.as<__bf16>(), h->s_qscale.as<float>(), s);
@@ -1207,14 +1215,14 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
         scan_ev(true);
         if (main_phase) rec(7);
     };
-    // the nearest-list pass of the BF16 form (rank 0, planned with its own query tile)
+    // the nearest-list pass of the BF16 form (ranks [0, near_end), planned with its own query tile)
     auto near_phase = [&](unsigned long long* st_slot) {
-        require(mph_next < mph.size() && mph[mph_next].r0 == 0 && mph[mph_next].r1 == 1 &&
+        require(mph_next < mph.size() && mph[mph_next].r0 == 0 && mph[mph_next].r1 == near_end &&
                     mph[mph_next].qtile == hv::kNearQueries && mph[mph_next].stats == st_slot,
                 HV_ERR_STATE, "nearest-list pass out of plan");
         const hv::PlanSpec& ph = mph[mph_next++];
-        // at most n / 16 + nlist + ntotal / kNearSplit items: one workgroup each, the dispatcher
-        // balancing their lengths
+        // rank 0 alone: at most n / 16 + nlist + ntotal / kNearSplit items, one workgroup each,
+        // the dispatcher balancing their lengths (more ranks: the workgroups loop over the rest)
         const int grid = (int)std::min<int64_t>(
             1 << 15, (n / hv::kNearQueries + nlist + h->ntotal / hv::kNearSplit + hv::kNumXcd) / hv::kNumXcd * hv::kNumXcd);
         scan_ev(false);
@@ -1223,7 +1231,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                                 h->d_rerr.as<float>(), h->d_lrad.as<float>(), h->d_list_off.as<int64_t>(),
                                 h->s_bound.as<float>(), learned_seed ? nullptr : h->s_bound_ref.as<float>(),
                                 h->s_pair_slot.as<int>(), ph.items, ph.queue, kappa1, kappa_g, kappa2, (int)k, kq, surv,
-                                aux, surv_cnt, last, (int)cap, st_slot, s);
+                                aux, surv_cnt, last, (int)cap, near_end > 1 ? 1 : 0, st_slot, s);
         HV_CHECK_LAUNCH("nearest-list pass");
         scan_ev(true);
     };
@@ -1249,13 +1257,13 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     const int prune = B > 1 ? 1 : 0;
     if (mfma && learned_seed) {
         // learned, BF16: the seed filters from the nearest lists on; the ranks go in groups
-        // [0, 1), [1, 4), [4, 16), ..., each followed by the compaction, which lowers the bound
-        // to the k-th smallest upper bound so far (when below the seed) and drops the pairs
-        // whose lower bound exceeds it
+        // ([0, near_end) in the nearest-list pass, or [0, 1)), then [.., 4), [4, 16), ..., each
+        // followed by the compaction, which lowers the bound to the k-th smallest upper bound so
+        // far (when below the seed) and drops the pairs whose lower bound exceeds it
         for (int64_t r0 = 0; r0 < rank_end;) {
             const int64_t r1 = learned_rank_group_end(r0);
             if (r0 == 0 && near_pass)
-                near_phase(stats);  // compacts as it goes; the compaction below merges split lists and spills
+                near_phase(stats);  // reduces each item as it goes; the compaction below merges
             else
                 mfma_phase((int)r0, (int)r1, 0, INT32_MAX, 0, r0 == 0 ? stats : stats + 8);
             hv::launch_bf16_compact(surv, aux, surv_cnt, n, (int)cap, (int)k, bound, last, kq, 1, s);

@@ -159,7 +159,7 @@ void launch_mfma_nearest(int grid, const float* Qp, const float* centroids, int 
                          float* bound, float* bound_ref, const int* pair_slot, const ScanItem* items,
                          const int* queue, float kappa1, float kappa_g, float kappa2, int k, const int* kq,
                          unsigned long long* surv, float* aux, unsigned int* surv_cnt, unsigned int* last, int cap,
-                         unsigned long long* stats, hipStream_t s);
+                         int fused, unsigned long long* stats, hipStream_t s);
 // FP8 shortlist: each query keeps its kr smallest estimates (ties at the kr-th included), in place;
 // bound[q] drops to the kr-th estimate
 // with kq: each query keeps min(kr, kmul * k_q)

@@ -181,12 +181,7 @@ __global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, i
         v[j] = s_v[t * (R + 1) + j];
         sum += v[j];
     }
-    int incl = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    const int incl = (int)wave_incl_scan((unsigned int)sum);
     if (lane == 63) s_wsum[w] = incl;
     __syncthreads();
     int run = incl - sum, total = 0;
@@ -264,10 +259,10 @@ __global__ __launch_bounds__(1024) void k_plan_count(PlanBatch pb) {
     const PlanArgs a = pb.args(blockIdx.y);
     int* __restrict__ queue = pb.ph[blockIdx.y].queue;
     unsigned long long* __restrict__ stats = pb.ph[blockIdx.y].stats;
-    __shared__ int s_scan[kNumXcd][1024];
+    __shared__ int s_wsum[16][kNumXcd];  // each wavefront's total per queue
     __shared__ int s_base[kNumXcd];
-    const int t = threadIdx.x;
-    int cnt[kNumXcd];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int cnt[kNumXcd], incl[kNumXcd];
     unsigned long long cand = 0;
 #pragma unroll
     for (int x = 0; x < kNumXcd; ++x) {
@@ -279,33 +274,35 @@ __global__ __launch_bounds__(1024) void k_plan_count(PlanBatch pb) {
             cnt[x] = nqt * nvr;
             cand += (unsigned long long)(ge - gb) * (unsigned long long)nlv;
         }
-        s_scan[x][t] = cnt[x];
+        // inclusive scan over the threads, per queue: within the wavefront by DPP moves, then
+        // the totals of the wavefronts before
+        incl[x] = (int)wave_incl_scan((unsigned int)cnt[x]);
+        if (lane == 63) s_wsum[wv][x] = incl[x];
     }
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v[kNumXcd];
 #pragma unroll
-        for (int x = 0; x < kNumXcd; ++x) v[x] = t >= off ? s_scan[x][t - off] : 0;
-        __syncthreads();
-#pragma unroll
-        for (int x = 0; x < kNumXcd; ++x) s_scan[x][t] += v[x];
-        __syncthreads();
+    for (int x = 0; x < kNumXcd; ++x) {
+        int before = 0;
+        for (int v = 0; v < wv; ++v) before += s_wsum[v][x];
+        incl[x] += before;
     }
     if (t == 0) {
         int b = 0;
         for (int x = 0; x < kNumXcd; ++x) {
+            int tot = 0;
+            for (int v = 0; v < 16; ++v) tot += s_wsum[v][x];
             s_base[x] = b;
-            queue[x] = s_scan[x][1023];
+            queue[x] = tot;
             queue[kNumXcd + x] = 0;
             queue[2 * kNumXcd + x] = b;
-            b += s_scan[x][1023];
+            b += tot;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < kNumXcd; ++x) {
         const int r = t * kNumXcd + x;
-        if (r < a.nlist) queue[3 * kNumXcd + r] = s_base[x] + s_scan[x][t] - cnt[x];
+        if (r < a.nlist) queue[3 * kNumXcd + r] = s_base[x] + incl[x] - cnt[x];
     }
     // candidates = sum over pairs of the probed list's (capped) size
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_xor(cand, off, 64);

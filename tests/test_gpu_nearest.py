@@ -4,15 +4,16 @@ the default), against the oracle and against the two-kernel path it replaces (ne
 every pair of the nearest list written out, then compacted). The shapes exercise its branches:
 lists of thousands of vectors (several in-kernel compactions per query), more than 16 queries per
 list (several work items per list), lists shorter than k, per-query k, float data with d = 420
-(the kernel's unpipelined form), and massive ties that keep a query's candidates above the
-buffer's headroom (the query spills to its survivor buffer and the compaction after the pass
-finishes it)."""
+(the kernel's unpipelined form), massive ties that keep a query's candidates above the buffer's
+headroom (the query spills to its survivor buffer and the compaction after the pass finishes it),
+and the learned search's fused pass (every searched rank in one launch, each item appending and
+lowering its query's bound)."""
 import sys
 
 import numpy as np
 import pytest
 
-from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP32, datagen
+from harmony_amd import HV_PRECISION_BF16, HV_PRECISION_FP32, datagen, learned
 from harmony_amd.index import Index
 from oracle import oracle as O
 
@@ -156,3 +157,32 @@ def test_float_wide_dims():
     for near, (D, I) in out.items():
         np.testing.assert_array_equal(I, oI, err_msg="nearest_pass=%d" % near)
         np.testing.assert_array_equal(D, oD, err_msg="nearest_pass=%d" % near)
+
+
+def test_fused_learned_pass_matches_oracle(long_lists):
+    """The learned BF16 search runs its searched ranks [0, 4) in one nearest-list pass: items of
+    several ranks of one query (and the parts of its split lists, about 2,500 vectors each) run in
+    one launch, each keeping what its own k-th upper bound allows and lowering the query's bound
+    by an atomic minimum. With every query at the cap the results are the oracle's at nprobe 4,
+    with the predictor's seed, a seed far too low (every query redone exactly) and no seed."""
+    g, o, xq = long_lists
+    g.nprobe = 4
+    say("fused learned pass: traces and a short training on the 400 queries")
+    feats, targets, _ = learned.collect_traces(g, xq, k=100)
+    learned.train(g, feats, targets, epochs=5, batch=128, lr=3e-3, seed=0)
+    oD, oI, _, _ = O.search(o, xq, 100, nprobe=4, dim_blocks=4)
+    try:
+        for bscale in (1.0, 1e-6, 0.0):
+            g.preSearch(learned=True, learned_nprobe_scale=1e6, learned_bound_scale=bscale,
+                        precision=HV_PRECISION_BF16, dim_blocks=4)
+            D, I, st = g.search(xq, 100, return_stats=True)
+            say("bound scale %g: survivors/query %.1f, overflowed %d, redone %d"
+                % (bscale, st["survivors"] / len(xq), st["overflow_queries"], st["fallback_queries"]))
+            np.testing.assert_array_equal(I, oI, err_msg="bound scale %g" % bscale)
+            np.testing.assert_array_equal(D, oD, err_msg="bound scale %g" % bscale)
+            assert np.all(g.last_nprobe(len(xq)) == 4)
+            assert st["overflow_queries"] == 0
+            if bscale == 1e-6:
+                assert st["fallback_queries"] == len(xq)
+    finally:
+        g.preSearch(learned=False, precision=HV_PRECISION_FP32)
