@@ -662,13 +662,17 @@ constexpr int kNearRows = kNearQueries;  // queries per item: one MFMA query blo
 static_assert(kNearRows == 16, "an item's queries are one 16-row MFMA block");
 constexpr int kNearStride = kNearSplit + 16;  // LDS words per row (padded: the rows' banks differ)
 constexpr unsigned int kNearTop = 65535u;     // code of "at or above the scale's end"
+constexpr int kNearHist = 256 + 64;           // LDS words per wavefront: 256 bins, a word per lane
+constexpr unsigned int kNearStage = 256u;     // kept positions a row stages in those words
 
 // The r-th smallest (1-based, r <= the number of values) of the 16-bit values a wavefront holds E
 // per lane, exact. The values are the low halves of packed words (0xffffffff: no value, not
 // counted). A radix select over the high byte, then the low byte of the values in the selected
-// high-byte bin, each pass a 256-bin histogram in LDS (hist: 256 words of this wavefront's own)
-// built with atomic adds and read back with one prefix scan over the lanes (4 bins per lane). All
-// 64 lanes must call it.
+// high-byte bin, each pass a 256-bin histogram in LDS (hist: kNearHist words of this wavefront's
+// own, bins 0..255 and one word per lane past them) built with atomic adds and read back with one
+// prefix scan over the lanes (4 bins per lane). Every lane adds for every value, a value that is
+// not counted into its lane's own word past the bins, so the adds need no change of the
+// execution mask. All 64 lanes must call it.
 template <int E>
 __device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&w)[E], int r, unsigned int* hist) {
     const int lane = threadIdx.x & 63;
@@ -682,7 +686,7 @@ __device__ __forceinline__ unsigned int wave_kth_u16(const unsigned int (&w)[E],
         for (int e = 0; e < E; ++e) {
             const unsigned int v = w[e] & 0xffffu;
             const bool in = w[e] != 0xffffffffu && (pass == 0 || (v >> 8) == prefix);
-            if (in) atomicAdd(&hist[pass == 0 ? v >> 8 : v & 255u], 1u);
+            atomicAdd(&hist[in ? (pass == 0 ? v >> 8 : v & 255u) : 256u + (unsigned int)lane], 1u);
         }
         lds_fence();
         const uint4 h4 = *reinterpret_cast<const uint4*>(hist + 4 * lane);
@@ -743,7 +747,7 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
     __shared__ __align__(16) float s_inv[kNearRows];   // 1 / the row's fixed-point step
     __shared__ float s_step[kNearRows], s_b0[kNearRows];
     __shared__ unsigned int s_code[kNearRows][kNearStride];  // lower bound << 16 | upper bound
-    __shared__ __align__(16) unsigned int s_hist[4][256];  // each wavefront's radix histogram
+    __shared__ __align__(16) unsigned int s_hist[4][kNearHist];  // each wavefront's radix histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, col16 = lane & 15;
     const int xcd = blockIdx.x % kNumXcd;
@@ -938,17 +942,35 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
                 const unsigned int kth = wave_kth_u16<E>(c, k_r, s_hist[w]);
                 if (kth < kNearTop) T = fminf(T, (float)kth * step);  // >= the k-th upper bound
             }
-            // the kept pairs (lower bound <= T), one ballot per 64 at a time (no mask array held)
-            auto keep_of = [&](int e) { return e * 64 + lane < nv && (float)(c[e] >> 16) * step <= T; };
+            // the kept pairs (lower bound <= T): each lane's kept positions staged, lane by lane,
+            // in this wavefront's LDS words (s_hist[w], free after the select; a position past
+            // the stage goes to the lane's own word past it, so every lane writes for every value
+            // and the execution mask never changes), then written out coalesced; a row with more
+            // than kNearStage kept pairs writes them directly instead
+            // (a bitwise and of two tests: no short-circuit branch; past the range c = 0xffffffff)
+            auto keep_of = [&](int e) { return (c[e] != 0xffffffffu) & ((float)(c[e] >> 16) * step <= T); };
+            unsigned int* __restrict__ stage = s_hist[w];
+            unsigned int kb = 0u;  // the lane's kept values, bit e for value e
+#pragma unroll
+            for (int e = 0; e < E; ++e) kb |= keep_of(e) ? 1u << e : 0u;
+            const unsigned int kcnt = (unsigned int)__builtin_popcount(kb);
+            const unsigned int kincl = wave_incl_scan(kcnt);
+            const unsigned int m = (unsigned int)__builtin_amdgcn_readlane((int)kincl, 63);
+            {
+                unsigned int at_l = kincl - kcnt;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const unsigned int bit = (kb >> e) & 1u;
+                    stage[bit && at_l < kNearStage ? at_l : kNearStage + (unsigned int)lane] = (unsigned int)(e * 64 + lane);
+                    at_l += bit;
+                }
+            }
             unsigned int at = 0u;
             const bool append = split || fused;
             if (append) {
                 // parts of one list, and every row of a fused pass, append (the compaction after
                 // the pass merges them); in a fused pass the nearest list of a query held by one
                 // item records its count, so a query nothing else appended to is left alone
-                unsigned int m = 0u;
-#pragma unroll
-                for (int e = 0; e < E; ++e) m += (unsigned int)__builtin_popcountll(__ballot(keep_of(e)));
                 if (lane == 0) {
                     if (m) at = atomicAdd(&surv_cnt[qq], m);
                     atomicMin(reinterpret_cast<unsigned int*>(bound) + qq, __float_as_uint(T));
@@ -958,25 +980,37 @@ __global__ __launch_bounds__(256) void k_mfma_nearest(const float* __restrict__ 
             }
             unsigned long long* __restrict__ sq = surv + (int64_t)qq * cap;
             unsigned int* __restrict__ aq = reinterpret_cast<unsigned int*>(aux) + (int64_t)qq * cap;
-            unsigned int o = at;
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const bool keep = keep_of(e);
-                const unsigned long long km = __ballot(keep);
-                if (keep) {
-                    const unsigned int slot =
-                        o + __builtin_amdgcn_mbcnt_hi((unsigned int)(km >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)km, 0u));
-                    if (slot < (unsigned int)cap) {
-                        const unsigned int cu = c[e] & 0xffffu;
-                        sq[slot] = hv_key((float)(c[e] >> 16) * step, pos0 + (unsigned int)(e * 64 + lane));
-                        aq[slot] = cu >= kNearTop ? 0x7f800000u : __float_as_uint((float)cu * step);
-                    }
+            // one kept pair: its key (lower bound, position) and upper bound
+            auto put = [&](unsigned int slot, unsigned int cc, unsigned int j) {
+                if (slot < (unsigned int)cap) {
+                    const unsigned int cu = cc & 0xffffu;
+                    sq[slot] = hv_key((float)(cc >> 16) * step, pos0 + j);
+                    aq[slot] = cu >= kNearTop ? 0x7f800000u : __float_as_uint((float)cu * step);
                 }
-                o += (unsigned int)__builtin_popcountll(km);
+            };
+            if (m <= kNearStage) {
+                lds_fence();  // the staged positions of every lane are in
+                for (unsigned int i = (unsigned int)lane; i < m; i += 64u) {
+                    const unsigned int j = stage[i];
+                    put(at + i, s_code[r][j], j);
+                }
+            } else {
+                unsigned int o = at;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const bool keep = keep_of(e);
+                    const unsigned long long km = __ballot(keep);
+                    if (keep)
+                        put(o + __builtin_amdgcn_mbcnt_hi((unsigned int)(km >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned int)km, 0u)),
+                            c[e], (unsigned int)(e * 64 + lane));
+                    o += (unsigned int)__builtin_popcountll(km);
+                }
             }
+            lds_fence();  // the staging words are read before the next row's select clears them
             if (!append && lane == 0) {
-                surv_cnt[qq] = o;
-                if (last) last[qq] = o;
+                surv_cnt[qq] = m;
+                if (last) last[qq] = m;
                 bound[qq] = T;
                 if (bound_ref) bound_ref[qq] = T;
             }
@@ -1449,6 +1483,15 @@ __device__ __forceinline__ void chain_chunk(const float* __restrict__ qv, int lo
                                             const float (&x)[kChunkDims], float& p) {
     float qr[kChunkDims];
     load_chunk<V4>(qv, lo, d, qr);
+    if (len >= kChunkDims) {
+        // a full chunk (len is wavefront-uniform): no per-dimension test
+#pragma unroll
+        for (int t = 0; t < kChunkDims; ++t) {
+            const float u = qr[t] - x[t];
+            p = __builtin_fmaf(u, u, p);
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < kChunkDims; ++t) {
         if (t < len) {
