@@ -125,6 +125,9 @@ class Index:
         precision: HV_PRECISION_FP32 (default), HV_PRECISION_BF16 (MFMA filter + FP32 distances
         of the survivors; same results) or HV_PRECISION_FP8 (FP8 MFMA estimates, the k * rerank_r
         smallest per query re-ranked in FP32; approximate). None keeps the current setting.
+        nearest_pass (BF16): 1 (the default) filters and reduces each query's nearest list in one
+        kernel; 0 writes every pair of the nearest list out and compacts it in a second; the
+        results are the same. None keeps the current setting.
 
         learned (None keeps the current settings): per-query probe count and pruning bound from
         the trained predictor (mlp_*): probes = ceil(predicted nprobe * learned_nprobe_scale),
