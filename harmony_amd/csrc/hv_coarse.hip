@@ -91,11 +91,23 @@ void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t 
 // multiplied, and the squared norms of the 64 + 64 rows are summed from the same loads (8 threads
 // per row, any order: kappa covers it): the workgroups of the first column write the query norms,
 // those of the first row the centroid norms, for the select. With d % 4 != 0 the norms come from
-// k_row_norms, launched before.
+// k_row_norms, launched before. The search's counters and buffers that start cleared (fill) are
+// cleared by the same grid first (a grid-stride loop; no kernel of this launch reads them), so a
+// search needs no launch of its own for them.
 __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q, int64_t nq,
                                                      const float* __restrict__ C, int nc, int d,
                                                      float* __restrict__ qn, float* __restrict__ cn,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out, FillSegs fill) {
+    if (fill.nseg > 0) {
+        const int64_t stride = (int64_t)gridDim.x * gridDim.y * blockDim.x;
+        const int64_t g0 = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+        for (int j = 0; j < fill.nseg; ++j) {
+            int* p = fill.p[j];
+            const int64_t n = fill.n[j];
+            const int v = fill.v[j];
+            for (int64_t i = g0; i < n; i += stride) p[i] = v;
+        }
+    }
     constexpr int KC = 32, LS = KC + 1;  // odd row stride: the 16 rows a K-step reads hit 16 banks
     __shared__ float Qs[64 * LS];
     __shared__ float Cs[64 * LS];
@@ -602,14 +614,19 @@ float coarse_kappa(int d) {
 }
 
 void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, float* qn, float* cn,
-                        float* approx, hipStream_t s) {
-    if (nq <= 0) return;
+                        float* approx, hipStream_t s, const FillSegs* fill) {
+    FillSegs f{};
+    if (fill) f = *fill;
+    if (nq <= 0) {
+        launch_fill_multi(f, s);
+        return;
+    }
     if ((d & 3) != 0) {  // the kernel sums the norms itself only from its 16-byte loads
         launch_row_norms(C, nc, d, cn, s);
         launch_row_norms(Q, nq, d, qn, s);
     }
     const dim3 grid((unsigned)((nc + 63) / 64), (unsigned)((nq + 63) / 64));
-    hipLaunchKernelGGL(k_coarse_mfma, grid, dim3(256), 0, s, Q, nq, C, nc, d, qn, cn, approx);
+    hipLaunchKernelGGL(k_coarse_mfma, grid, dim3(256), 0, s, Q, nq, C, nc, d, qn, cn, approx, f);
 }
 
 bool launch_coarse_select(const float* approx, const float* Q, const float* C, int nc, int d, const float* qn,

@@ -500,8 +500,11 @@ int* coarse_counter(hv_index_t* h, int64_t n) {
 // (the learned predictor's features), taken from the same rows; bucket_cnt (optional): the
 // grouping's (list, rank < nranks) pair counts, added as the probes are written. Returns whether
 // they were (else the caller counts them).
+// fill: the search's buffers that start cleared, cleared by the coarse GEMM's first launch (or by
+// a fill launch of their own before the other coarse path)
 bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* probes, float* pdist,
-                   bool counter_zeroed = false, float* near = nullptr, int* bucket_cnt = nullptr, int nranks = 0) {
+                   bool counter_zeroed = false, float* near = nullptr, int* bucket_cnt = nullptr, int nranks = 0,
+                   const hv::FillSegs* fill = nullptr) {
     const int64_t d = h->d, nlist = h->nlist;
     hipStream_t s = h->stream;
     const int64_t per = coarse_rows_per_slice(h, n);
@@ -515,7 +518,8 @@ bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
         for (int64_t r0 = 0; r0 < n; r0 += per) {
             const int64_t rows = std::min(per, n - r0);
             hv::launch_coarse_mfma(Qp + r0 * d, rows, h->d_centroids.as<float>(), (int)nlist, (int)d,
-                                   h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), h->s_dist.as<float>(), s);
+                                   h->s_qnorm.as<float>() + r0, h->s_cnorm.as<float>(), h->s_dist.as<float>(), s,
+                                   r0 == 0 ? fill : nullptr);
             if (r0 > 0 || !counter_zeroed) hv::launch_fill_i32(redo_cnt, 1, 0, s);
             counted &= hv::launch_coarse_select(h->s_dist.as<float>(), Qp + r0 * d, h->d_centroids.as<float>(),
                                                 (int)nlist, (int)d, h->s_qnorm.as<float>() + r0,
@@ -526,6 +530,7 @@ bool coarse_select(hv_index_t* h, int64_t n, const float* Qp, int64_t P, int* pr
         HV_CHECK_LAUNCH("coarse");
         return counted;
     }
+    if (fill) hv::launch_fill_multi(*fill, s);
     for (int64_t r0 = 0; r0 < n; r0 += per) {
         const int64_t rows = std::min(per, n - r0);
         hv::launch_pair_l2_full(Qp + r0 * d, rows, h->d_centroids.as<float>(), nlist, (int)d, h->s_dist.as<float>(),
@@ -1010,8 +1015,8 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     HV_HIP(h->s_overflow.ensure(sizeof(int) * (n + 2)));
     if (mfma) HV_HIP(h->s_last.ensure(sizeof(unsigned int) * n));
     HV_HIP(h->s_runs.ensure((sizeof(unsigned long long) * run_keys + sizeof(int)) * batch));
+    hv::FillSegs f{};
     {
-        hv::FillSegs f{};
         auto add = [&](void* p, int64_t words, int v) {
             f.p[f.nseg] = static_cast<int*>(p);
             f.n[f.nseg] = words;
@@ -1032,7 +1037,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
             add(h->s_bound.p, n, fm);
             add(h->s_bound_ref.p, n, fm);
         }
-        hv::launch_fill_multi(f, s);
+        // (cleared by the coarse step's first launch)
     }
     // 2. coarse: full distance rows + nprobe select (Index.cpp:967-997)
     HV_HIP(h->s_probes.ensure(sizeof(int) * n * nprobe));
@@ -1042,7 +1047,7 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     // the grouping's pair counts: by the coarse select when every query searches ranks [0, rank_end)
     // (a learned search's predictor counts its own)
     bool counted = coarse_select(h, n, Qp, nprobe, probes, pdist, true, learned ? h->s_near.as<float>() : nullptr,
-                                 learned || uneven ? nullptr : h->s_bucket.as<int>(), (int)rank_end);
+                                 learned || uneven ? nullptr : h->s_bucket.as<int>(), (int)rank_end, &f);
     const int* qn = nullptr;
     if (learned) {
         HV_HIP(h->s_qn.ensure(sizeof(int) * n));

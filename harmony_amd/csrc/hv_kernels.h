@@ -241,7 +241,7 @@ void launch_row_norms(const float* X, int64_t n, int d, float* out, hipStream_t 
 float coarse_kappa(int d);
 // also writes the squared norms of the rows (qn [nq], cn [nc]) that the select uses
 void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, float* qn, float* cn,
-                        float* approx, hipStream_t s);
+                        float* approx, hipStream_t s, const FillSegs* fill = nullptr);
 // the kNearest smallest distance values of each row of v [n][nc] ascending, a short row's largest
 // repeated, into out [n][kNearest] (nc <= kMaxSelectRow): with qn / cn, v holds coarse estimates
 // and the values are their upper bounds d~ + kappa (|q|^2 + |c|^2); without, v holds exact
