@@ -1716,7 +1716,7 @@ void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint3
 }
 
 void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt, int64_t nq, int cap, int k, float* D,
-                        int64_t* I, unsigned int* done, hipStream_t s) {
+                        int64_t* I, unsigned int* done, hipStream_t s, const RedoArgs* redo) {
     if (nq <= 0) return;
     const dim3 grid((unsigned)((nq + 3) / 4));
     const int sel = k <= 256 ? 512 : kSelLds;
@@ -1724,7 +1724,10 @@ void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt,
         hipLaunchKernelGGL(k_select_kept<512>, grid, dim3(256), 0, s, kept, cnt, nq, cap, k, D, I, done);
     else
         hipLaunchKernelGGL(k_select_kept<kSelLds>, grid, dim3(256), 0, s, kept, cnt, nq, cap, k, D, I, done);
-    if (cap > sel || k > sel) launch_select_block(kept, cnt, nq, cap, k, D, I, done, s);
+    if (cap > sel || k > sel)
+        launch_select_block(kept, cnt, nq, cap, k, D, I, done, s, redo);
+    else if (redo)
+        launch_redo_list(*redo, s);
 }
 
 }  // namespace hv

@@ -1339,6 +1339,20 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
     int* redo_cnt = reinterpret_cast<int*>(stats + 15);  // cleared with the counters above
     unsigned int* n_items = reinterpret_cast<unsigned int*>(redo_cnt + 1);
     int* redo_list = h->s_overflow.as<int>();
+    // 7. (listed in the select's last kernel) the exact path redoes the queries whose survivor
+    //    buffer overflowed (rare) and, in learned mode, those whose k-th result is missing or above
+    //    the seeded bound (a neighbour may have been pruned), over the same probe lists, with no
+    //    bound in learned mode. The list is built on the device, which also publishes the counters
+    //    to the host; the host launches the redo only when the list is not empty (finish_search).
+    // (the check is against the bound the seed set, bound_ref: later compactions only tighten
+    //  the working bound towards the k-th distance itself)
+    // (a learned query that is redone is redone over all nprobe_cap lists: its result is then the
+    //  plain search's at nprobe_cap, and its reported probe count says so)
+    const hv::RedoArgs redo_args{surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
+                                 redo_cnt, stats + 12, learned ? h->s_qn.as<int>() : nullptr, (int)nprobe_cap,
+                                 learned && !fp8 ? bound : nullptr, stats,
+                                 reinterpret_cast<unsigned int*>(stats + 16),
+                                 static_cast<unsigned long long*>(h->post_host.p)};
     if (mfma) {
         HV_HIP(h->s_ritems.ensure(sizeof(int2) * n * ((cap + 63) / 64)));
         HV_HIP(h->s_kept.ensure(sizeof(unsigned long long) * n * cap));
@@ -1352,25 +1366,12 @@ void do_search(hv_index_t* h, int64_t n, const float* q_dev, int64_t k, float* D
                           fp8 ? nullptr : bound, h->s_ritems.as<int2>(), n_items, h->s_kept.as<unsigned long long>(), s);
         rec(9);
         hv::launch_select_kept(h->s_kept.as<unsigned long long>(), surv_cnt, n, (int)cap, (int)k, D, I,
-                               h->s_done.as<unsigned int>(), s);
+                               h->s_done.as<unsigned int>(), s, &redo_args);
         HV_CHECK_LAUNCH("rerank + select");
     } else {
-        hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, nullptr, nullptr, stats, s);
+        hv::launch_select_survivors(surv, surv_cnt, n, (int)cap, (int)k, D, I, nullptr, nullptr, stats, s, &redo_args);
         HV_CHECK_LAUNCH("select");
     }
-    // 7. the exact path redoes the queries whose survivor buffer overflowed (rare) and, in learned
-    //    mode, those whose k-th result is missing or above the seeded bound (a neighbour may have
-    //    been pruned), over the same probe lists, with no bound in learned mode. The list is built
-    //    on the device; the host reads its count with the other counters and launches the redo
-    //    only when it is not empty (finish_search).
-    // (the check is against the bound the seed set, bound_ref: later compactions only tighten
-    //  the working bound towards the k-th distance itself)
-    // (a learned query that is redone is redone over all nprobe_cap lists: its result is then the
-    //  plain search's at nprobe_cap, and its reported probe count says so)
-    hv::launch_redo_list(surv_cnt, (int)cap, D, I, (int)k, bound_ref, learned && !fp8 ? 1 : 0, n, redo_list,
-                         redo_cnt, stats + 12, learned ? h->s_qn.as<int>() : nullptr, (int)nprobe_cap,
-                         learned && !fp8 ? bound : nullptr, stats, reinterpret_cast<unsigned int*>(stats + 16),
-                         static_cast<unsigned long long*>(h->post_host.p), s);
     const int* redo_qn = learned ? nullptr : qn;
     // learned (FP32, BF16): each redone query's own k-th result (written above); FP8: no bound
     const float* redo_bound = fp8 ? nullptr : bound;

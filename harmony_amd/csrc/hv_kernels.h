@@ -95,9 +95,35 @@ void launch_list_scan(int shape, const float* Qp, int d, const float* codes, con
                       const ScanItem* items, int* queue, int vmax, int vsplit, int accept_all,
                       unsigned long long* surv, unsigned int* surv_cnt, int cap, unsigned long long* stats, int grid,
                       hipStream_t s);
+// the queries the exact path redoes -> list / list_cnt (both zeroed by the caller): overflowed
+// survivor buffers (cnt[q] > cap; counts[0]) and, with check_bound, queries whose k-th result is
+// missing or above bound[q] (counts[1]); qn_out (optional): each listed query's probe count is set
+// to qn_full (its redo covers them all); bound_out (optional): its bound to the k-th result it has
+// (FLT_MAX when overflowed or short). publish (optional, pinned host memory): the last workgroup
+// of the grid copies stats[0..16) there; done counts the workgroups (cleared by the caller).
+// The select launchers take it and launch k_redo_list after their last kernel.
+struct RedoArgs {
+    const unsigned int* cnt;
+    int cap;
+    const float* D;
+    const int64_t* I;
+    int k;
+    const float* bound;
+    int check_bound;
+    int64_t n;
+    int* list;
+    int* list_cnt;
+    unsigned long long* counts;
+    int* qn_out;
+    int qn_full;
+    float* bound_out;
+    const unsigned long long* stats;
+    unsigned int* done;
+    unsigned long long* publish;
+};
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap,
                              int k, float* D, int64_t* I, int* overflow_list, int* overflow_cnt,
-                             unsigned long long* stats, hipStream_t s);
+                             unsigned long long* stats, hipStream_t s, const RedoArgs* redo = nullptr);
 // exact path for the queries qlist[q0 .. q0 + min(*nql_dev - q0, nql_max)) (the count read on the
 // device) over their probe ranks [0, nranks) (and below qn[q] when given; probe rows are nprobe
 // wide), lists cut into exact_segments(max list length) segments: per-(query, rank, segment)
@@ -192,20 +218,12 @@ void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint3
                    const unsigned long long* surv, int64_t nq, int cap, const float* bound, const int2* items,
                    const unsigned int* n_items, unsigned long long* kept, hipStream_t s);
 void launch_select_kept(const unsigned long long* kept, const unsigned int* cnt, int64_t nq, int cap, int k, float* D,
-                        int64_t* I, unsigned int* done, hipStream_t s);
-// the queries the exact path redoes -> list / list_cnt (both zeroed by the caller): overflowed
-// survivor buffers (cnt[q] > cap; counts[0]) and, with check_bound, queries whose k-th result is
-// missing or above bound[q] (counts[1])
-// qn_out (optional): each listed query's probe count is set to qn_full (its redo covers them all);
-// bound_out (optional): its bound to the k-th result it has (FLT_MAX when overflowed or short)
-void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
-                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, int* qn_out,
-                      int qn_full, float* bound_out, const unsigned long long* stats, unsigned int* done,
-                      unsigned long long* publish, hipStream_t s);
+                        int64_t* I, unsigned int* done, hipStream_t s, const RedoArgs* redo = nullptr);
+void launch_redo_list(const RedoArgs& ra, hipStream_t s);
 // the workgroup select alone: queries whose done flag is 0 (with done), else those with more than
-// kWaveSortMax survivors
+// kWaveSortMax survivors; with redo, then the redo list and the counters (see RedoArgs)
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
-                         float* D, int64_t* I, const unsigned int* done, hipStream_t s);
+                         float* D, int64_t* I, const unsigned int* done, hipStream_t s, const RedoArgs* redo = nullptr);
 
 // --- learned probe/bound predictor (hv_learn.hip)
 constexpr int kNearest = 64;                                            // nearest-centroid distances

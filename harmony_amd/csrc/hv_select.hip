@@ -302,61 +302,61 @@ __global__ __launch_bounds__(256) void k_select_block(const unsigned long long* 
     }
 }
 
-// The queries to redo, listed on the device, and the search's counters published to the host:
-// the last workgroup to finish (a done count, cleared with the counters) copies the 16 counter
-// words, the redo count included, into host-visible memory, so the host needs no copy of its own
-// after the device work (publish: the words, pinned host memory; done: the count; both or none).
-__global__ __launch_bounds__(256) void k_redo_list(const unsigned int* __restrict__ cnt, int cap,
-                                                   const float* __restrict__ D, const int64_t* __restrict__ I, int k,
-                                                   const float* __restrict__ bound, int check_bound, int64_t n,
-                                                   int* __restrict__ list, int* __restrict__ list_cnt,
-                                                   unsigned long long* __restrict__ counts, int* __restrict__ qn_out,
-                                                   int qn_full, float* __restrict__ bound_out,
-                                                   const unsigned long long* stats, unsigned int* done,
-                                                   unsigned long long* __restrict__ publish) {
-    __shared__ unsigned int s_last;
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The queries to redo (overflowed buffers; with check_bound, a k-th result missing or above the
+// query's bound), listed on the device: thread q checks query q (q < 0: none), one list append
+// per wavefront. All 64 lanes of the wavefront must call it.
+__device__ __forceinline__ void redo_check(const RedoArgs& ra, int64_t q) {
     const int lane = threadIdx.x & 63;
     bool ovf = false, redo = false;
-    if (q < n) {
-        ovf = cnt[q] > (unsigned int)cap;
-        redo = ovf || (check_bound && (I[q * k + k - 1] < 0 || D[q * k + k - 1] > bound[q]));
+    if (q >= 0 && q < ra.n) {
+        ovf = ra.cnt[q] > (unsigned int)ra.cap;
+        redo = ovf || (ra.check_bound && (ra.I[q * ra.k + ra.k - 1] < 0 || ra.D[q * ra.k + ra.k - 1] > ra.bound[q]));
     }
     const unsigned long long m = __ballot(redo), mo = __ballot(ovf);
-    if (m) {
-        int base = 0;
-        if (lane == 0) {
-            base = atomicAdd(list_cnt, (int)__builtin_popcountll(m));
-            if (mo) atomicAdd(&counts[0], (unsigned long long)__builtin_popcountll(mo));
-            if (m != mo) atomicAdd(&counts[1], (unsigned long long)__builtin_popcountll(m & ~mo));
-        }
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (redo) {
-            list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
-            if (qn_out) qn_out[q] = qn_full;
-            // the query's own k-th result bounds the k-th over a superset of its lists
-            if (bound_out) bound_out[q] = (!ovf && I[q * k + k - 1] >= 0) ? D[q * k + k - 1] : 3.402823466e38f;
-        }
+    if (!m) return;
+    int base = 0;
+    if (lane == 0) {
+        base = atomicAdd(ra.list_cnt, (int)__builtin_popcountll(m));
+        if (mo) atomicAdd(&ra.counts[0], (unsigned long long)__builtin_popcountll(mo));
+        if (m != mo) atomicAdd(&ra.counts[1], (unsigned long long)__builtin_popcountll(m & ~mo));
     }
-    if (!publish) return;
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (redo) {
+        ra.list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (int)q;
+        if (ra.qn_out) ra.qn_out[q] = ra.qn_full;
+        // the query's own k-th result bounds the k-th over a superset of its lists
+        if (ra.bound_out)
+            ra.bound_out[q] = (!ovf && ra.I[q * ra.k + ra.k - 1] >= 0) ? ra.D[q * ra.k + ra.k - 1] : 3.402823466e38f;
+    }
+}
+
+// The search's counters published to the host: the last workgroup of the grid to get here (a done
+// count, cleared with the counters) copies the 16 counter words, the redo count included, into
+// host-visible memory (ra.publish, pinned), so the host needs no copy of its own after the device
+// work. Every thread of every workgroup must call it, after its last counter update.
+__device__ __forceinline__ void redo_publish(const RedoArgs& ra, unsigned int* s_last) {
+    if (!ra.publish) return;
     // every counter update of this workgroup is performed (the fence waits for the atomics issued
-    // above) before the workgroup is counted
+    // before) before the workgroup is counted
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0)
-        s_last = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+        *s_last = __hip_atomic_fetch_add(ra.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                  gridDim.x * gridDim.y - 1;
     __syncthreads();
-    if (s_last && threadIdx.x < 16)
-        publish[threadIdx.x] = __hip_atomic_load(stats + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (*s_last && threadIdx.x < 16)
+        ra.publish[threadIdx.x] = __hip_atomic_load(ra.stats + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-void launch_redo_list(const unsigned int* cnt, int cap, const float* D, const int64_t* I, int k, const float* bound,
-                      int check_bound, int64_t n, int* list, int* list_cnt, unsigned long long* counts, int* qn_out,
-                      int qn_full, float* bound_out, const unsigned long long* stats, unsigned int* done,
-                      unsigned long long* publish, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_redo_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cnt, cap, D, I, k, bound,
-                       check_bound, n, list, list_cnt, counts, qn_out, qn_full, bound_out, stats, done, publish);
+__global__ __launch_bounds__(256) void k_redo_list(RedoArgs ra) {
+    __shared__ unsigned int s_last;
+    redo_check(ra, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    redo_publish(ra, &s_last);
+}
+
+void launch_redo_list(const RedoArgs& ra, hipStream_t s) {
+    if (ra.n <= 0) return;
+    hipLaunchKernelGGL(k_redo_list, dim3((unsigned)((ra.n + 255) / 256)), dim3(256), 0, s, ra);
 }
 
 static size_t select_block_lds(int cap, int k) {
@@ -364,24 +364,25 @@ static size_t select_block_lds(int cap, int k) {
 }
 
 void launch_select_block(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
-                         float* D, int64_t* I, const unsigned int* done, hipStream_t s) {
+                         float* D, int64_t* I, const unsigned int* done, hipStream_t s, const RedoArgs* redo) {
     if (nq <= 0) return;
     const dim3 grid((unsigned)std::min<int64_t>(nq, 1024));
     hipLaunchKernelGGL(k_select_block, grid, dim3(256), select_block_lds(cap, k), s, surv, surv_cnt, nq, cap, k, D, I,
                        done);
+    // (a launch of its own: few workgroups, each with one agent-scope release for the counters)
+    if (redo) launch_redo_list(*redo, s);
 }
 
 void launch_select_survivors(const unsigned long long* surv, const unsigned int* surv_cnt, int64_t nq, int cap, int k,
                              float* D, int64_t* I, int* overflow_list, int* overflow_cnt, unsigned long long* stats,
-                             hipStream_t s) {
+                             hipStream_t s, const RedoArgs* redo) {
     if (nq <= 0) return;
     hipLaunchKernelGGL(k_select_wave, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, surv, surv_cnt, nq, cap, k, D,
                        I, overflow_list, overflow_cnt, stats);
-    if (cap > kWaveSortMax || k > kWaveSortMax) {
-        const dim3 grid((unsigned)std::min<int64_t>(nq, 1024));
-        hipLaunchKernelGGL(k_select_block, grid, dim3(256), select_block_lds(cap, k), s, surv, surv_cnt, nq, cap, k, D,
-                           I, nullptr);
-    }
+    if (cap > kWaveSortMax || k > kWaveSortMax)
+        launch_select_block(surv, surv_cnt, nq, cap, k, D, I, nullptr, s, redo);
+    else if (redo)
+        launch_redo_list(*redo, s);
 }
 
 }  // namespace hv
