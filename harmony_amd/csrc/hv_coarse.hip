@@ -5,7 +5,8 @@
 // centroid id): the plain path's probe lists and their distances are compared bit for bit.
 //
 // 1. k_coarse_mfma: d~(q, c) = |q|^2 + |c|^2 - 2 q.c for every (query, centroid), the dot
-//    products on v_mfma_f32_16x16x4f32 (64 x 64 tiles, 32-dim LDS chunks).
+//    products as split BF16 products on v_mfma_f32_16x16x32_bf16 (64 x 64 tiles, 32-dim LDS
+//    chunks).
 // 2. k_coarse_select_wave (one wavefront per query; k_coarse_select, one workgroup per query,
 //    for wide rows or large P): with err = kappa (|q|^2 + |c|^2) bounding
 //    |d~ - chain| (host: kappa from d), the P-th smallest upper bound tau = P-th of (d~ + err)
@@ -14,6 +15,7 @@
 //    A query with more candidates than the buffer holds is listed for k_coarse_exact_row (all
 //    centroids, exact), which strides over the list on the device.
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <stdint.h>
 
 #include "hv_device.h"
@@ -22,6 +24,7 @@
 namespace hv {
 
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 constexpr int kCoarseCand = 1024;  // exact re-checks per query in k_coarse_select
 
 // Bits of the upper bound d~ + kappa (|q|^2 + |c|^2) of a chain distance (one FMA, so every
@@ -108,9 +111,11 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
             for (int64_t i = g0; i < n; i += stride) p[i] = v;
         }
     }
-    constexpr int KC = 32, LS = KC + 1;  // odd row stride: the 16 rows a K-step reads hit 16 banks
-    __shared__ float Qs[64 * LS];
-    __shared__ float Cs[64 * LS];
+    // row stride 36 floats: a row starts 16-byte aligned, and the 8 rows one 16-byte LDS read
+    // serves at a time start 4 banks apart (36 mod 32), so each such read is conflict-free
+    constexpr int KC = 32, LS = KC + 4;
+    __shared__ __align__(16) float Qs[64 * LS];
+    __shared__ __align__(16) float Cs[64 * LS];
     __shared__ float s_qn[64], s_cn[64];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int64_t q0 = (int64_t)blockIdx.y * 64;
@@ -118,15 +123,30 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
     floatx4 acc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // One chunk of 32 dims is one K-step of v_mfma_f32_16x16x32_bf16: lane l holds dims
+    // 8 (l / 16) .. +8 of its row (2 x 16 bytes from LDS), split into BF16 parts v = hi + lo + e
+    // (hi the BF16 rounding of v, lo that of v - hi, which is exact in FP32; |e| <= 2^-16 |v|),
+    // and q.c is taken as qh.ch + qh.cl + ql.ch: three MFMAs whose products are exact in FP32
+    // (coarse_kappa bounds what is left out and the FP32 accumulation)
+    auto split8 = [](const float4* p, bf16x8& hi, bf16x8& lo) {
+        const float4 x0 = p[0], x1 = p[1];
+        const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            hi[e] = (__bf16)v[e];
+            lo[e] = (__bf16)(v[e] - (float)hi[e]);
+        }
+    };
     auto mult = [&]() {
+        bf16x8 ah, al;
+        split8(reinterpret_cast<const float4*>(Qs + (16 * w + (l & 15)) * LS + 8 * (l >> 4)), ah, al);
 #pragma unroll
-        for (int s = 0; s < KC / 4; ++s) {
-            const float a = Qs[(16 * w + (l & 15)) * LS + 4 * s + (l >> 4)];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float b = Cs[(16 * j + (l & 15)) * LS + 4 * s + (l >> 4)];
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
-            }
+        for (int j = 0; j < 4; ++j) {
+            bf16x8 bh, bl;
+            split8(reinterpret_cast<const float4*>(Cs + (16 * j + (l & 15)) * LS + 8 * (l >> 4)), bh, bl);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[j], 0, 0, 0);
         }
     };
     if ((d & 3) == 0) {
@@ -150,18 +170,10 @@ __global__ __launch_bounds__(256) void k_coarse_mfma(const float* __restrict__ Q
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int e = t + 256 * i, r = e >> 3, kk = 4 * (e & 7);
-                float* qs = Qs + r * LS + kk;
-                float* cs = Cs + r * LS + kk;
                 sq[i] += qv[i].x * qv[i].x + qv[i].y * qv[i].y + qv[i].z * qv[i].z + qv[i].w * qv[i].w;
                 sc[i] += cv[i].x * cv[i].x + cv[i].y * cv[i].y + cv[i].z * cv[i].z + cv[i].w * cv[i].w;
-                qs[0] = qv[i].x;
-                qs[1] = qv[i].y;
-                qs[2] = qv[i].z;
-                qs[3] = qv[i].w;
-                cs[0] = cv[i].x;
-                cs[1] = cv[i].y;
-                cs[2] = cv[i].z;
-                cs[3] = cv[i].w;
+                *reinterpret_cast<float4*>(Qs + r * LS + kk) = qv[i];
+                *reinterpret_cast<float4*>(Cs + r * LS + kk) = cv[i];
             }
             __syncthreads();
             if (k0 + KC < d) load(k0 + KC);  // next chunk in flight during the MFMAs
@@ -606,11 +618,19 @@ void launch_near_rows(const float* v, int64_t n, int nc, const float* qn, const 
 }
 
 float coarse_kappa(int d) {
-    // |d~ - chain| <= kappa (|q|^2 + |c|^2): the MFMA dot product (gamma_2d |q||c|), the FP32
-    // norms (gamma_d each), the chain's own error against the real distance (2 gamma_2d of
-    // 2(|q|^2 + |c|^2)), the three roundings of d~; 2% slack for the rounding of this bound.
+    // |d~ - chain| <= kappa (|q|^2 + |c|^2), every term taken against |q||c| <= (|q|^2 + |c|^2) / 2
+    // where it multiplies |q||c|:
+    // - the split dot product (BF16 parts: |ql| <= 2^-8 |q|, |eq| <= 2^-16 |q|, and so for c):
+    //   3d exact products accumulated in FP32 in the MFMA's order (gamma_6d, twice gamma_3d for
+    //   the unknown order) over their absolute sum (<= 1.016 |q||c|), and the parts left out,
+    //   ql.cl + (qh + ql).ec + eq.c (<= 3.01 2^-16 |q||c|), both doubled by the factor 2 of the
+    //   dot product: 6.1 d u + 3.1 2^-16;
+    // - the FP32 norms (gamma_d each): d u;
+    // - the chain's own error against the real distance (2 gamma_2d of 2 (|q|^2 + |c|^2)): 8 d u;
+    // - the three roundings of d~ and slack: (d + 8) u;
+    // and 2 % for the rounding of this bound.
     const double u = 1.0 / 16777216.0;
-    return (float)(1.02 * (12.0 * d + 8.0) * u);
+    return (float)(1.02 * ((16.2 * d + 8.0) * u + 3.1 * std::ldexp(1.0, -16)));
 }
 
 void launch_coarse_mfma(const float* Q, int64_t nq, const float* C, int nc, int d, float* qn, float* cn,

@@ -108,7 +108,7 @@ def test_query_features_match_oracle(w1):
     _, pd = O.coarse(xq, g.centroids(), 64)
     ref = M.features_from_coarse(pd, xq).astype(np.float64)
     d = xq.shape[1]
-    kappa = 1.02 * (12 * d + 8) / 2.0 ** 24
+    kappa = 1.02 * ((16.2 * d + 8) / 2.0 ** 24 + 3.1 * 2.0 ** -16)  # coarse_kappa (split BF16 product)
     c = g.centroids().astype(np.float64)
     err = kappa * ((xq.astype(np.float64) ** 2).sum(1) + (c ** 2).sum(1).max())
     pd64 = pd.astype(np.float64)
