@@ -2247,7 +2247,7 @@ namespace {
 constexpr int kNormLen = 2 * hv::kMlpIn + 2 * hv::kMlpOut;
 
 void mlp_refresh_serving(hv_index_t* h) {
-    HV_HIP(h->m_w1b.ensure(sizeof(__bf16) * hv::kMlpIn * hv::kMlpHidden));
+    HV_HIP(h->m_w1b.ensure(sizeof(__bf16) * hv::kMlpW1Serve));
     HV_HIP(h->m_w2b.ensure(sizeof(__bf16) * hv::kMlpHidden * hv::kMlpOut));
     hv::launch_mlp_to_bf16(h->m_params.as<float>(), h->m_w1b.as<__bf16>(), h->m_w2b.as<__bf16>(), h->stream);
     HV_CHECK_LAUNCH("mlp weights");

@@ -229,6 +229,7 @@ void launch_select_block(const unsigned long long* surv, const unsigned int* sur
 constexpr int kNearest = 64;                                            // nearest-centroid distances
 constexpr int kMlpIn = kNearest + 1, kMlpHidden = 128, kMlpOut = 2;
 constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpOut + kMlpOut;
+constexpr int kMlpW1Serve = (kMlpIn + 3) / 4 * 4 * kMlpHidden;  // BF16 serving copy of W1 (padded)
 // norm: fmean [65], fstd [65], tmean [2], tstd [2]
 void launch_features(const float* pdist, int P, const float* Qp, int d, int64_t n, float* feat, hipStream_t s);
 void launch_mlp_step(const float* X, const float* T, const int* idx, int B, float* P, const float* norm, float* Xn,

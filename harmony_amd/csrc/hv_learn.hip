@@ -21,6 +21,7 @@ namespace hv {
 
 constexpr int kIn = kMlpIn, kHid = kMlpHidden, kOut = kMlpOut;
 constexpr int kOffB1 = kIn * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kOut;
+constexpr int kInPad = (kIn + 3) / 4 * 4;  // the serving copy's inputs, in groups of 4
 static_assert(kOffB2 + kOut == kMlpParams, "parameter layout");
 
 // ---------------------------------------------------------------------------------------
@@ -174,20 +175,29 @@ void launch_adam(float* P, const float* G, float* M, float* V, float lr, float b
 
 // ---------------------------------------------------------------------------------------
 // Serving weights: BF16 copies of W1 and W2 (biases stay FP32).
+// The serving copies: W1 in groups of 4 inputs (kInPad, the last padded with zeros), lane j of a
+// group holding hidden units 2j and 2j + 1: element [g][j][k][h] = W1[4g + k][2j + h], so one
+// 16-byte load gives a lane its 8 weights of a group and a wavefront's load of a group is one
+// contiguous 1 KB; W2 as it is, [hidden][output].
 __global__ void k_mlp_to_bf16(const float* __restrict__ P, __bf16* __restrict__ w1, __bf16* __restrict__ w2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < kOffB1) w1[p] = (__bf16)P[p];
+    if (p < kMlpW1Serve) {
+        const int g = p >> 9, j = (p >> 3) & 63, k = (p >> 1) & 3, hh = p & 1;
+        const int i = 4 * g + k, u = 2 * j + hh;
+        w1[p] = (__bf16)(i < kIn ? P[i * kHid + u] : 0.f);
+    }
     if (p < kHid * kOut) w2[p] = (__bf16)P[kOffW2 + p];
 }
 
 void launch_mlp_to_bf16(const float* P, __bf16* w1, __bf16* w2, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_to_bf16, dim3((kOffB1 + 255) / 256), dim3(256), 0, s, P, w1, w2);
+    hipLaunchKernelGGL(k_mlp_to_bf16, dim3((kMlpW1Serve + 255) / 256), dim3(256), 0, s, P, w1, w2);
 }
 
-// One wavefront per row: lane j owns hidden units j and j + 64; the two outputs are reduced
-// across the wavefront. Returns the outputs in raw target space (every lane gets them). The
-// standardized inputs are formed once, one per lane (lane 0 also the last), into xs (kIn floats
-// of LDS for this wavefront), instead of by every lane for every input.
+// One wavefront per row: lane j owns hidden units 2j and 2j + 1 (the serving layout of W1, one
+// 16-byte load per group of 4 inputs); the two outputs are reduced across the wavefront. Returns
+// the outputs in raw target space (every lane gets them). The standardized inputs are formed
+// once, one per lane (lanes 0..3 also the last and the padding zeros), into xs (kInPad floats of
+// LDS for this wavefront), instead of by every lane for every input.
 __device__ __forceinline__ void mlp_eval_bf16(const float* __restrict__ f, int lane, const __bf16* __restrict__ w1,
                                               const __bf16* __restrict__ w2, const float* __restrict__ P,
                                               const float* __restrict__ norm, float* xs, float& y0, float& y1) {
@@ -195,23 +205,34 @@ __device__ __forceinline__ void mlp_eval_bf16(const float* __restrict__ f, int l
     const float* fstd = norm + kIn;
     const float* tmean = norm + 2 * kIn;
     const float* tstd = tmean + kOut;
-    static_assert(kIn <= 128, "two inputs per lane at most");
+    static_assert(kInPad <= 128 && kHid == 128 && kOut == 2, "two inputs and two hidden units per lane");
     xs[lane] = (f[lane] - fmean[lane]) / fstd[lane];
-    if (lane + 64 < kIn) xs[lane + 64] = (f[lane + 64] - fmean[lane + 64]) / fstd[lane + 64];
+    if (lane + 64 < kIn)
+        xs[lane + 64] = (f[lane + 64] - fmean[lane + 64]) / fstd[lane + 64];
+    else if (lane + 64 < kInPad)
+        xs[lane + 64] = 0.f;  // padding inputs (their weights are 0 too)
     __threadfence_block();  // every lane reads every input below
-    float h0 = P[kOffB1 + lane], h1 = P[kOffB1 + lane + 64];
-    // unrolled so that the loads of 13 inputs are in flight together (the sums keep their order)
-    static_assert(kIn % 13 == 0, "unroll factor");
-#pragma unroll 13
-    for (int i = 0; i < kIn; ++i) {
-        const float x = xs[i];
-        h0 = __builtin_fmaf(x, (float)w1[i * kHid + lane], h0);
-        h1 = __builtin_fmaf(x, (float)w1[i * kHid + lane + 64], h1);
+    const float2 b1 = *reinterpret_cast<const float2*>(P + kOffB1 + 2 * lane);
+    float h0 = b1.x, h1 = b1.y;
+    // each hidden unit sums its inputs in order; the 17 group loads are independent
+    const uint4* __restrict__ wq = reinterpret_cast<const uint4*>(w1) + lane;
+#pragma unroll
+    for (int g = 0; g < kInPad / 4; ++g) {
+        const uint4 wv = wq[g * 64];
+        const unsigned int wk[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float x = xs[4 * g + k];
+            h0 = __builtin_fmaf(x, __uint_as_float(wk[k] << 16), h0);          // unit 2j
+            h1 = __builtin_fmaf(x, __uint_as_float(wk[k] & 0xffff0000u), h1);  // unit 2j + 1
+        }
     }
     h0 = fmaxf(h0, 0.f);
     h1 = fmaxf(h1, 0.f);
-    float a = h0 * (float)w2[lane * kOut] + h1 * (float)w2[(lane + 64) * kOut];
-    float c = h0 * (float)w2[lane * kOut + 1] + h1 * (float)w2[(lane + 64) * kOut + 1];
+    // W2 rows 2j and 2j + 1 (outputs 0, 1 each): 4 BF16 values, one 8-byte load
+    const uint2 w2v = *reinterpret_cast<const uint2*>(w2 + 4 * lane);
+    float a = h0 * __uint_as_float(w2v.x << 16) + h1 * __uint_as_float(w2v.y << 16);
+    float c = h0 * __uint_as_float(w2v.x & 0xffff0000u) + h1 * __uint_as_float(w2v.y & 0xffff0000u);
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o, 64);
         c += __shfl_xor(c, o, 64);
@@ -224,7 +245,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ X
                                                      const __bf16* __restrict__ w1, const __bf16* __restrict__ w2,
                                                      const float* __restrict__ P, const float* __restrict__ norm,
                                                      float* __restrict__ out) {
-    __shared__ float s_x[4][kIn];
+    __shared__ float s_x[4][kInPad];
     const int w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= n) return;
@@ -278,7 +299,7 @@ __global__ __launch_bounds__(256) void k_mlp_serve(const float* __restrict__ nea
                                                    float* __restrict__ bound3, unsigned long long* __restrict__ widened,
                                                    int* __restrict__ bucket_cnt, int nranks) {
     __shared__ float s_f[4][kIn];
-    __shared__ float s_x[4][kIn];
+    __shared__ float s_x[4][kInPad];
     const int w = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= n) return;
