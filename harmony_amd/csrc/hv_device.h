@@ -80,9 +80,27 @@ __host__ __device__ inline int hv_next_pow2(int v) {
     return p;
 }
 
+// The value of lane ^ m, for m a power of two known at compile time after unrolling: a DPP
+// quad permutation for 1 and 2 (a VALU move), a bit-mode swizzle within 32 lanes for 4, 8 and 16
+// (new lane = lane ^ m, no address operand), a permute across the halves for 32.
+__device__ __forceinline__ unsigned int shfl_xor_pow2(unsigned int x, int m) {
+    switch (m) {
+        case 1: return (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xf, 0xf, false);  // [1,0,3,2]
+        case 2: return (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xf, 0xf, false);  // [2,3,0,1]
+        case 4: return (unsigned int)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (4 << 10));
+        case 8: return (unsigned int)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (8 << 10));
+        case 16: return (unsigned int)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (16 << 10));
+        default: return (unsigned int)__shfl_xor((int)x, m, 64);
+    }
+}
+__device__ __forceinline__ unsigned long long shfl_xor_pow2(unsigned long long x, int m) {
+    const unsigned int lo = shfl_xor_pow2((unsigned int)x, m), hi = shfl_xor_pow2((unsigned int)(x >> 32), m);
+    return (unsigned long long)hi << 32 | lo;
+}
+
 // Wavefront-wide ascending bitonic sort of 64*E keys held in registers: element i lives in
 // lane i / E, slot i % E. Strides below E exchange within a lane; larger strides exchange with
-// lane ^ (stride / E) through cross-lane shuffles. No LDS, no barriers.
+// lane ^ (stride / E) through cross-lane moves (shfl_xor_pow2). No LDS, no barriers.
 template <int E>
 __device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
     const int lane = threadIdx.x & 63;
@@ -96,7 +114,7 @@ __device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const int i = lane * E + e;
-                    const unsigned long long o = __shfl_xor(v[e], lx, 64);
+                    const unsigned long long o = shfl_xor_pow2(v[e], lx);
                     const bool up = (i & size) == 0;
                     const bool lower = (i & stride) == 0;
                     const unsigned long long mn = o < v[e] ? o : v[e];
