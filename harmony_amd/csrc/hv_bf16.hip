@@ -125,7 +125,7 @@ __global__ void k_gather_rows(const float* __restrict__ rows, int64_t n, int d, 
                               float* __restrict__ by_pos) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * d) return;
-    const int64_t p = e / d;
+    const int64_t p = hv_idiv(e, d);
     const int k = (int)(e - p * d);
     by_pos[e] = rows[(int64_t)order[p] * d + k];
 }
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp
     // item = (query, rank in [r0, nranks)); its slot in the [nq][nprobe] arrays
     const int64_t it = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
     const int nr = nranks - r0;
-    const int64_t qi = it / nr;
+    const int64_t qi = hv_idiv(it, nr);
     const int j = r0 + (int)(it - qi * nr);
     const int64_t slot = qi * nprobe + j;
     const int g = threadIdx.x % G;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_pair_resid(const float* __restrict__ Qp
     const int k0 = g * 8;
     float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, se = 0.f, sr = 0.f;
     bf16x8 r;
-    if (live) load8_diff(Qp + (slot / nprobe) * d, centroids + (int64_t)probes[slot] * d, k0, d, v);
+    if (live) load8_diff(Qp + qi * d, centroids + (int64_t)probes[slot] * d, k0, d, v);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         r[e] = (__bf16)v[e];
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict_
                                                         int64_t nitems, int nprobe, int nranks, int d, int dpad,
                                                         long* __restrict__ out, float* __restrict__ inv_scale) {
     const int64_t it = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;  // (query, rank < nranks)
-    const int64_t qi = it / nranks;
+    const int64_t qi = hv_idiv(it, nranks);
     const int j = (int)(it - qi * nranks);
     const int64_t slot = qi * nprobe + j;
     const int g = threadIdx.x % G;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void k_pair_resid_fp8(const float* __restrict_
     float r[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, m = 0.f;
     const int k0 = g * 8;
     if (live) {
-        load8_diff(Qp + (slot / nprobe) * d, centroids + (int64_t)probes[slot] * d, k0, d, r);
+        load8_diff(Qp + qi * d, centroids + (int64_t)probes[slot] * d, k0, d, r);
 #pragma unroll
         for (int t = 0; t < 8; ++t) m = fmaxf(m, fabsf(r[t]));
     }

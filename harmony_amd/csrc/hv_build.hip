@@ -108,20 +108,19 @@ void launch_argmin_rows(const float* dist, int64_t n, int m, int* labels, float*
 }
 
 // ---------------------------------------------------------------------------------------
-// dst[r][i] = src[r][perm[i]]: applies the index's dimension order to incoming vectors.
+// dst[r][i] = src[r][perm[i]]: applies the index's dimension order to incoming vectors. One
+// workgroup per row (no division per element), the row's gathers served by the cache.
 __global__ void k_permute_rows(const float* __restrict__ src, int64_t n, int d, const int* __restrict__ perm,
                                float* __restrict__ dst) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * d) return;
-    const int64_t r = e / d;
-    const int i = (int)(e - r * d);
-    dst[e] = src[r * d + perm[i]];
+    const int64_t r = blockIdx.x;
+    const float* __restrict__ sr = src + r * d;
+    float* __restrict__ dr = dst + r * d;
+    for (int i = threadIdx.x; i < d; i += blockDim.x) dr[i] = sr[perm[i]];
 }
 
 void launch_permute_rows(const float* src, int64_t n, int d, const int* perm, float* dst, hipStream_t s) {
-    const int64_t tot = n * d;
-    if (tot <= 0) return;
-    hipLaunchKernelGGL(k_permute_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, src, n, d, perm, dst);
+    if (n <= 0 || d <= 0) return;
+    hipLaunchKernelGGL(k_permute_rows, dim3((unsigned)n), dim3(d <= 128 ? 128 : 256), 0, s, src, n, d, perm, dst);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -69,6 +69,12 @@ __device__ __forceinline__ void lds_barrier() {
 // without waiting for its global loads.
 __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local"); }
 
+// e / den for an index e >= 0 and 0 < den < 2^31: the 32-bit unsigned division (a short
+// reciprocal-based sequence) when e < 2^32, the 64-bit one (a long software sequence) otherwise.
+__device__ __forceinline__ int64_t hv_idiv(int64_t e, int den) {
+    return ((uint64_t)e >> 32) == 0 ? (int64_t)((uint32_t)e / (uint32_t)den) : e / (int64_t)den;
+}
+
 // A query's own k when per-query counts are given (clamped to [1, k]), else k.
 __device__ __forceinline__ int hv_query_k(const int* kq, int64_t q, int k) {
     return kq ? max(1, min(kq[q], k)) : k;

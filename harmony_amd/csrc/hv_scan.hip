@@ -55,7 +55,7 @@ __global__ void k_count_pairs(const int* __restrict__ probes, int64_t total, int
                               const int* __restrict__ qn, int* __restrict__ cnt) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const int64_t q = e / nranks;
+    const int64_t q = hv_idiv(e, nranks);
     const int j = (int)(e - q * nranks);
     if (qn && j >= qn[q]) return;
     atomicAdd(&cnt[(int64_t)probes[q * nprobe + j] * nranks + j], 1);
@@ -76,7 +76,7 @@ __global__ void k_scatter_pairs(const int* __restrict__ probes, int64_t total, i
                                 int* __restrict__ pairs, int* __restrict__ pair_slot) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const int64_t q = e / nranks;
+    const int64_t q = hv_idiv(e, nranks);
     const int j = (int)(e - q * nranks);
     if (qn && j >= qn[q]) return;
     const int64_t slot = q * nprobe + j;

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_truncate_rows(float* __restrict__ D, in
                                                        const int* __restrict__ kq) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * k) return;
-    const int64_t q = e / k;
+    const int64_t q = hv_idiv(e, k);
     if ((int)(e - q * k) >= hv_query_k(kq, q, k)) {
         D[e] = kFltMax;
         I[e] = -1;
