@@ -35,35 +35,39 @@ __device__ __forceinline__ unsigned int coarse_ub_bits(float approx, float qq, f
 }
 
 // The F (<= 64) smallest of the 64 E values a wavefront holds E per lane (pad 0xffffffff),
-// ascending, as floats into out[0, kNearest) (lanes past F repeat the F-th): the F-th smallest by
-// wave_kth_u32, the values below it and then enough equal to it gathered in LDS (s_tmp, 64 words
-// of this wavefront), one register sort. Returns the lane's value of that order. All 64 lanes
-// must call it.
+// ascending, as floats into out[0, kNearest) (lanes past F repeat the F-th): a bracket (lo, hi]
+// around the F-th smallest (wave_kth_bracket), then one gather into LDS (s_tmp, 128 words of this
+// wavefront) and one register sort of 128. When the bracket holds at most 64 values, the values
+// <= hi are gathered (fewer than F below it, so at most 127); otherwise every value of the bracket
+// equals hi (or F exceeds the values other than the padding), and the values <= lo are gathered
+// with the rest of the 128 set to hi. Returns the lane's value of that order. All 64 lanes must
+// call it.
 template <int E>
 __device__ __forceinline__ unsigned int sorted_smallest_u32(const unsigned int (&u)[E], int F, unsigned int* s_tmp,
                                                             float* __restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const unsigned int thr = wave_kth_u32<E>(u, F, s_tmp);
-    int base = 0;
+    const KthBracket b = wave_kth_bracket<E>(u, F);
+    const bool upto_hi = b.c_hi - b.c_lo <= 64 && b.c_hi <= 128;
+    const long long lim = upto_hi ? b.hi : b.lo;
+    const unsigned int pad = upto_hi ? 0xffffffffu : (unsigned int)b.hi;
+    int n = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const bool in = u[e] < thr;
+        const bool in = (long long)u[e] <= lim;
         const unsigned long long m = __ballot(in);
-        if (in) s_tmp[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = u[e];
-        base += __builtin_popcountll(m);
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const bool eq = u[e] == thr;
-        const unsigned long long m = __ballot(eq);
-        const int pos = base + __builtin_popcountll(m & ((1ull << lane) - 1ull));
-        if (eq && pos < F) s_tmp[pos] = u[e];
-        base += __builtin_popcountll(m);
+        if (in) s_tmp[n + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = u[e];
+        n += __builtin_popcountll(m);
     }
     __threadfence_block();  // the words other lanes stored are read below
-    unsigned long long x[1] = {lane < F ? (unsigned long long)s_tmp[lane] : ~0ull};
-    wave_bitonic_sort<1>(x);
-    const unsigned int val = (unsigned int)__shfl((int)(unsigned int)x[0], min(lane, F - 1), 64);
+    unsigned int x[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) x[e] = 2 * lane + e < n ? s_tmp[2 * lane + e] : pad;
+    wave_bitonic_sort<2>(x);
+    // position p of the order lives in lane p / 2, slot p % 2
+    const int p = min(lane, F - 1);
+    const unsigned int y0 = (unsigned int)__shfl((int)x[0], p >> 1, 64);
+    const unsigned int y1 = (unsigned int)__shfl((int)x[1], p >> 1, 64);
+    const unsigned int val = (p & 1) ? y1 : y0;
     out[lane] = __uint_as_float(val);
     return val;
 }
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
                                                             int* __restrict__ bucket_cnt, int nranks) {
     __shared__ unsigned int s_cand[4][CAND];
     __shared__ unsigned long long s_ckey[4][CAND];
-    __shared__ unsigned int s_tmp[4][64];
+    __shared__ unsigned int s_tmp[4][128];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -463,10 +467,13 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
     const float* __restrict__ row = approx + q * nc;
     const float qq = qn[q];
     unsigned int ub[E];
+    float lbf[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int c = e * 64 + lane;
-        ub[e] = c < nc ? coarse_ub_bits(row[c], qq, cn[c], kappa) : 0xffffffffu;
+        const float a = c < nc ? row[c] : 0.f, cc = c < nc ? cn[c] : 0.f;
+        ub[e] = c < nc ? coarse_ub_bits(a, qq, cc, kappa) : 0xffffffffu;
+        lbf[e] = c < nc ? a - kappa * (qq + cc) : __builtin_inff();
     }
     float tau;
     if (near) {
@@ -477,12 +484,13 @@ __global__ __launch_bounds__(256) void k_coarse_select_wave(const float* __restr
     } else {
         tau = __uint_as_float(wave_kth_u32<E>(ub, P, s_tmp[w]));
     }
-    // lower bounds from the row again (cache hits) rather than held through the search
+    // the candidates: lower bound <= tau (the bounds were kept in registers through the search:
+    // re-reading the row costs more than the occupancy they take)
     int m = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int c = e * 64 + lane;
-        const bool in = c < nc && row[c] - kappa * (qq + cn[c]) <= tau;
+        const bool in = c < nc && lbf[e] <= tau;
         const unsigned long long b = __ballot(in);
         const int slot = m + (int)__builtin_popcountll(b & ((1ull << lane) - 1ull));
         if (in && slot < CAND) s_cand[w][slot] = (unsigned int)c;
@@ -553,7 +561,7 @@ template <int E>
 __global__ __launch_bounds__(256) void k_near_rows_wave(const float* __restrict__ v, int64_t n, int nc,
                                                         const float* __restrict__ qn, const float* __restrict__ cn,
                                                         float kappa, float* __restrict__ out) {
-    __shared__ unsigned int s_tmp[4][64];
+    __shared__ unsigned int s_tmp[4][128];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t r = (int64_t)blockIdx.x * 4 + w;

@@ -106,9 +106,10 @@ __device__ __forceinline__ unsigned long long shfl_xor_pow2(unsigned long long x
 
 // Wavefront-wide ascending bitonic sort of 64*E keys held in registers: element i lives in
 // lane i / E, slot i % E. Strides below E exchange within a lane; larger strides exchange with
-// lane ^ (stride / E) through cross-lane moves (shfl_xor_pow2). No LDS, no barriers.
-template <int E>
-__device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
+// lane ^ (stride / E) through cross-lane moves (shfl_xor_pow2). No LDS, no barriers. T is
+// unsigned int or unsigned long long.
+template <int E, typename T>
+__device__ __forceinline__ void wave_bitonic_sort(T (&v)[E]) {
     const int lane = threadIdx.x & 63;
     constexpr int N = 64 * E;
 #pragma unroll
@@ -120,11 +121,11 @@ __device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const int i = lane * E + e;
-                    const unsigned long long o = shfl_xor_pow2(v[e], lx);
+                    const T o = shfl_xor_pow2(v[e], lx);
                     const bool up = (i & size) == 0;
                     const bool lower = (i & stride) == 0;
-                    const unsigned long long mn = o < v[e] ? o : v[e];
-                    const unsigned long long mx = o < v[e] ? v[e] : o;
+                    const T mn = o < v[e] ? o : v[e];
+                    const T mx = o < v[e] ? v[e] : o;
                     v[e] = (lower == up) ? mn : mx;
                 }
             } else {
@@ -133,7 +134,7 @@ __device__ __forceinline__ void wave_bitonic_sort(unsigned long long (&v)[E]) {
                     if (e & stride) continue;
                     const int e2 = e | stride;
                     const bool up = ((lane * E + e) & size) == 0;
-                    const unsigned long long a = v[e], b = v[e2];
+                    const T a = v[e], b = v[e2];
                     const bool sw = (a > b) == up;
                     v[e] = sw ? b : a;
                     v[e2] = sw ? a : b;
@@ -290,17 +291,22 @@ __device__ __forceinline__ int wave_sum_small(unsigned int x) {
     return s;
 }
 
-// The r-th smallest (1-based, 1 <= r <= 64*E, E <= 64) of the 64*E unsigned values a wavefront
-// holds E per lane (callers pad with 0xffffffff, which then counts among the values). Exact: a
-// search over the value range keeps a bracket (lo, hi] with count(<= lo) < r <= count(<= hi)
-// until at most 64 values lie inside it; those are gathered (s_tmp: 64 words of LDS for this
-// wavefront alone) and sorted across the wavefront. The probe alternates between interpolation
-// on the counts and bisection, so it never needs more than twice the bisection's steps. All 64
-// lanes must call it.
+// A bracket (lo, hi] around the r-th smallest (1-based, 1 <= r <= 64*E, E <= 64) of the 64*E
+// unsigned values a wavefront holds E per lane (callers pad with 0xffffffff, which then counts
+// among the values): c_lo = count(<= lo) < r <= c_hi = count(<= hi), and either c_hi - c_lo <= 64
+// or hi - lo <= 1 (every value in the bracket equals hi). A search over the value range that
+// alternates between interpolation on the counts (the probe's fraction from a hardware
+// reciprocal: any probe inside the bracket is valid) and bisection, so it never needs more than
+// twice the bisection's steps; r = 1 is the minimum at once. r above the count of values other
+// than the padding gives lo = their maximum, hi = 0xffffffff, c_lo = that count. All 64 lanes
+// must call it.
+struct KthBracket {
+    long long lo, hi;
+    int c_lo, c_hi;
+};
 template <int E>
-__device__ __forceinline__ unsigned int wave_kth_u32(const unsigned int (&v)[E], int r, unsigned int* s_tmp) {
+__device__ __forceinline__ KthBracket wave_kth_bracket(const unsigned int (&v)[E], int r) {
     static_assert(E <= 64, "per-lane counts must stay below 128");
-    const int lane = threadIdx.x & 63;
     // range of the values other than the padding (which would stretch it to 2^32)
     unsigned int mn = 0xffffffffu, mx = 0u, npad = 0u;
 #pragma unroll
@@ -310,24 +316,33 @@ __device__ __forceinline__ unsigned int wave_kth_u32(const unsigned int (&v)[E],
         npad += v[e] == 0xffffffffu ? 1u : 0u;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        mn = min(mn, (unsigned int)__shfl_xor((int)mn, o, 64));
-        mx = max(mx, (unsigned int)__shfl_xor((int)mx, o, 64));
+    for (int o = 1; o < 64; o <<= 1) {
+        mn = min(mn, shfl_xor_pow2(mn, o));
+        mx = max(mx, shfl_xor_pow2(mx, o));
     }
     const int c_valid = 64 * E - wave_sum_small(npad);
-    if (r > c_valid) return 0xffffffffu;
-    long long lo = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mn) - 1;
-    long long hi = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mx);
-    int c_lo = 0, c_hi = c_valid;
+    KthBracket b;
+    b.lo = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mn) - 1;
+    b.hi = (long long)(unsigned int)__builtin_amdgcn_readfirstlane(mx);
+    b.c_lo = 0;
+    b.c_hi = c_valid;
+    if (r > c_valid) {
+        b.lo = b.hi;
+        b.hi = 0xffffffffll;
+        b.c_lo = c_valid;
+        b.c_hi = 64 * E;
+        return b;
+    }
+    if (r == 1) b.hi = b.lo + 1;  // the minimum
     bool interp = true;
-    while (c_hi - c_lo > 64 && hi - lo > 1) {
+    while (b.c_hi - b.c_lo > 64 && b.hi - b.lo > 1) {
         long long mid;
         if (interp) {
-            const float f = (float)(r - c_lo) / (float)(c_hi - c_lo);
-            mid = lo + (long long)(f * (float)(hi - lo));
-            mid = mid <= lo ? lo + 1 : (mid >= hi ? hi - 1 : mid);
+            const float f = (float)(r - b.c_lo) * __builtin_amdgcn_rcpf((float)(b.c_hi - b.c_lo));
+            mid = b.lo + (long long)(f * (float)(b.hi - b.lo));
+            mid = mid <= b.lo ? b.lo + 1 : (mid >= b.hi ? b.hi - 1 : mid);
         } else {
-            mid = lo + ((hi - lo) >> 1);
+            mid = b.lo + ((b.hi - b.lo) >> 1);
         }
         interp = !interp;
         const unsigned int t = (unsigned int)mid;  // lo < mid < hi, so 0 <= mid < 2^32
@@ -336,26 +351,37 @@ __device__ __forceinline__ unsigned int wave_kth_u32(const unsigned int (&v)[E],
         for (int e = 0; e < E; ++e) cl += v[e] <= t ? 1u : 0u;
         const int c = wave_sum_small(cl);
         if (c >= r) {
-            hi = mid;
-            c_hi = c;
+            b.hi = mid;
+            b.c_hi = c;
         } else {
-            lo = mid;
-            c_lo = c;
+            b.lo = mid;
+            b.c_lo = c;
         }
     }
-    if (hi - lo <= 1) return (unsigned int)hi;  // every value in the bracket equals hi
+    return b;
+}
+
+// The r-th smallest (1-based, 1 <= r <= 64*E) of the values as above (0xffffffff when r exceeds
+// the count of values other than the padding): the bracket's values, at most 64, are gathered
+// (s_tmp: 64 words of LDS for this wavefront alone) and sorted across the wavefront.
+template <int E>
+__device__ __forceinline__ unsigned int wave_kth_u32(const unsigned int (&v)[E], int r, unsigned int* s_tmp) {
+    const int lane = threadIdx.x & 63;
+    const KthBracket b = wave_kth_bracket<E>(v, r);
+    // every value in the bracket equals hi (or r exceeds the values: hi is the padding)
+    if (b.hi - b.lo <= 1 || b.c_hi - b.c_lo > 64) return (unsigned int)b.hi;
     int base = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const bool in = (long long)v[e] > lo && (long long)v[e] <= hi;
+        const bool in = (long long)v[e] > b.lo && (long long)v[e] <= b.hi;
         const unsigned long long m = __ballot(in);
         if (in) s_tmp[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = v[e];
         base += __builtin_popcountll(m);
     }
     __threadfence_block();  // the words other lanes stored are read below
-    unsigned long long x[1] = {lane < base ? (unsigned long long)s_tmp[lane] : ~0ull};
+    unsigned int x[1] = {lane < base ? s_tmp[lane] : 0xffffffffu};
     wave_bitonic_sort<1>(x);
-    return (unsigned int)__shfl((int)(unsigned int)x[0], r - c_lo - 1, 64);
+    return (unsigned int)__shfl((int)x[0], r - b.c_lo - 1, 64);
 }
 
 // Sorts the n (<= 2048) keys produced by key_of(j), j in [0, n), with one wavefront and hands

@@ -139,12 +139,15 @@ def test_ties_spill_to_survivor_buffer():
     assert s1["survivors"] >= 200 * len(xq) and s0["survivors"] >= 200 * len(xq)
 
 
-def test_float_wide_dims():
+@pytest.mark.parametrize("nq,nlist,nprobe", [(60, 12, 3), (400, 4, 2)])
+def test_float_wide_dims(nq, nlist, nprobe):
     """Float data, d = 420 (14 MFMA steps: the kernel's form without code prefetch), variance
-    order, 8 blocks."""
-    xb, xq = datagen.make(3, nb=6000, nq=60)
+    order, 8 blocks. 400 queries over 4 lists at nprobe 2: about 200 queries per list, so most
+    items hold four query blocks and the scan stages their code blocks in LDS."""
+    say("d=420: %d queries, nlist %d, nprobe %d" % (nq, nlist, nprobe))
+    xb, xq = datagen.make(3, nb=6000, nq=nq)
     perm = datagen.variance_order(xb)
-    g = Index(420, 12, nprobe=3, dim_order=perm)
+    g = Index(420, nlist, nprobe=nprobe, dim_order=perm)
     g.train(xb, niter=4, seed=0, max_points_per_centroid=256)
     g.add(xb)
     o = O.IndexData.from_gpu_index(g)
@@ -153,7 +156,7 @@ def test_float_wide_dims():
     for near in (1, 0):
         g.preSearch(dim_blocks=8, precision=HV_PRECISION_BF16, nearest_pass=near)
         out[near] = g.search(xq, 50)
-    oD, oI, _, _ = O.search(o, xq[:, perm], 50, nprobe=3, dim_blocks=8)
+    oD, oI, _, _ = O.search(o, xq[:, perm], 50, nprobe=nprobe, dim_blocks=8)
     for near, (D, I) in out.items():
         np.testing.assert_array_equal(I, oI, err_msg="nearest_pass=%d" % near)
         np.testing.assert_array_equal(D, oD, err_msg="nearest_pass=%d" % near)
