@@ -1578,6 +1578,124 @@ __global__ __launch_bounds__(256) void k_rerank_flat(const float* __restrict__ Q
     }
 }
 
+// The same re-rank with the rows staged through LDS (every chunk start a multiple of 4 dims, d a
+// multiple of 4). Each chunk of an item's 64 rows is loaded with eight lanes per row (up to 128
+// bytes of one row per eight lanes, so a load instruction touches 8 rows rather than 64: the
+// texture path's cost follows the rows an instruction touches), written to this wavefront's tile
+// (row stride kTileStride floats: conflict-free 16-byte row reads and chunk writes), and each lane
+// then chains its own row from the tile in dimension order, as k_rerank_flat does (the same
+// kept keys). The next chunk, or the next item's first, is in flight in registers during each
+// chain; the rows' positions come from the lanes that hold them (cross-lane reads, no loads).
+// Registers capped for 4 wavefronts per SIMD (the tiles allow 4 workgroups per CU).
+constexpr int kTileStride = kChunkDims + 4;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rerank_tile(
+    const float* __restrict__ Qp, int d, const float* __restrict__ rows_by_pos, const uint32_t* __restrict__ ids,
+    ChunkPlan plan, const unsigned long long* __restrict__ surv, int cap, const int2* __restrict__ items,
+    const unsigned int* __restrict__ n_items, const float* __restrict__ bound, unsigned long long* __restrict__ kept) {
+    __shared__ __align__(16) float s_tile[4][64 * kTileStride];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* __restrict__ tile = s_tile[w];
+    const int xcd = (int)blockIdx.x % kNumXcd;
+    const int nw = (int)gridDim.x / kNumXcd * 4;  // wavefronts per XCD
+    const int64_t nall = (int64_t)*n_items;
+    const int ni = (int)(nall * (xcd + 1) / kNumXcd);  // end of this XCD's items
+    int it = (int)(nall * xcd / kNumXcd) + (int)blockIdx.x / kNumXcd * 4 + w;
+    if (it >= ni) return;
+    // this lane loads dims f4 .. f4 + 3 of every chunk of rows rsub + 8 t, t < 8
+    const int rsub = lane >> 3, f4 = (lane & 7) * 4;
+    // an item: (query, first pair | pairs << 16)
+    int2 item = items[it];
+    int q = __builtin_amdgcn_readfirstlane(item.x);
+    int j0 = __builtin_amdgcn_readfirstlane(item.y & 0xffff), c = __builtin_amdgcn_readfirstlane(item.y >> 16);
+    uint32_t pos = lane < c ? (uint32_t)(surv[(int64_t)q * cap + j0 + lane] & 0xffffffffu) : 0u;
+    float b = bound ? bound[q] : 3.402823466e38f;
+    const int last = plan.n - 1;
+    // the rows this lane loads (rows rsub + 8 t of the item), for this item and the next
+    uint32_t rp[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) rp[t] = (uint32_t)__shfl((int)pos, rsub + 8 * t, 64);
+    float x[32];  // plain floats (a float4 array carried around the loop stayed in scratch)
+    {
+        const int col = min(plan.lo[0] + f4, d - 4);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float4 v = *reinterpret_cast<const float4*>(rows_by_pos + (int64_t)rp[t] * d + col);
+            x[4 * t] = v.x;
+            x[4 * t + 1] = v.y;
+            x[4 * t + 2] = v.z;
+            x[4 * t + 3] = v.w;
+        }
+    }
+    while (true) {
+        const bool valid = lane < c;
+        const uint32_t id = ids[pos];
+        const float* __restrict__ qv = Qp + (int64_t)q * d;
+        const int itn = it + nw;
+        int qn = q, j0n = 0, cn = 0;
+        uint32_t posn = 0u;
+        float bn = 0.f;
+        if (itn < ni) {
+            const int2 nitem = items[itn];
+            qn = __builtin_amdgcn_readfirstlane(nitem.x);
+            j0n = __builtin_amdgcn_readfirstlane(nitem.y & 0xffff);
+            cn = __builtin_amdgcn_readfirstlane(nitem.y >> 16);
+            posn = lane < cn ? (uint32_t)(surv[(int64_t)qn * cap + j0n + lane] & 0xffffffffu) : 0u;
+            bn = bound ? bound[qn] : 3.402823466e38f;
+        }
+        uint32_t rpn[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) rpn[t] = (uint32_t)__shfl((int)posn, rsub + 8 * t, 64);
+        float dis = 0.f, p = 0.f;
+        for (int ch = 0; ch <= last; ++ch) {
+            // the chunk's rows into the tile (every lane's reads of the previous chunk are done)
+            lds_fence();
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                *reinterpret_cast<float4*>(tile + (rsub + 8 * t) * kTileStride + f4) =
+                    make_float4(x[4 * t], x[4 * t + 1], x[4 * t + 2], x[4 * t + 3]);
+            lds_fence();
+            float cur[kChunkDims];
+#pragma unroll
+            for (int t = 0; t < kChunkDims / 4; ++t) {
+                const float4 v = *reinterpret_cast<const float4*>(tile + lane * kTileStride + 4 * t);
+                cur[4 * t] = v.x;
+                cur[4 * t + 1] = v.y;
+                cur[4 * t + 2] = v.z;
+                cur[4 * t + 3] = v.w;
+            }
+            // the next chunk of this item, or the next item's first (row 0 when there is none), in
+            // flight during this chain
+            const bool nxt = ch == last;
+            const int col = min((nxt ? plan.lo[0] : plan.lo[ch + 1]) + f4, d - 4);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const float4 v =
+                    *reinterpret_cast<const float4*>(rows_by_pos + (int64_t)(nxt ? rpn[t] : rp[t]) * d + col);
+                x[4 * t] = v.x;
+                x[4 * t + 1] = v.y;
+                x[4 * t + 2] = v.z;
+                x[4 * t + 3] = v.w;
+            }
+            chain_chunk<true>(qv, plan.lo[ch], plan.len[ch], d, cur, p);
+            if (plan.ends_block[ch] || ch == last) {  // the last chunk closes the last block
+                dis += p;
+                p = 0.f;
+            }
+        }
+        if (valid) kept[(int64_t)q * cap + j0 + lane] = dis <= b ? hv_key(dis, id) : HV_KEY_PAD;
+        if (itn >= ni) break;
+        it = itn;
+        q = qn;
+        j0 = j0n;
+        c = cn;
+        pos = posn;
+        b = bn;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) rp[t] = rpn[t];
+    }
+}
+
 // Final top-k of a re-ranked buffer (kept keys in place, padding for dropped pairs): one
 // wavefront per query compacts the kept keys into LDS (ballot prefix), sorts them in registers and
 // emits the first k. Buffers of up to kSelLds pairs are loaded at once; longer ones are read in
@@ -1707,10 +1825,12 @@ void launch_rerank(const float* Qp, int d, const float* rows_by_pos, const uint3
     const int64_t max_items = nq * ((cap + 63) / 64);
     const int64_t wgs = std::min<int64_t>((max_items + 3) / 4, kRerankGrid);
     const dim3 grid((unsigned)((wgs + kNumXcd - 1) / kNumXcd * kNumXcd));  // every XCD gets workgroups
-    if (v4)
-        hipLaunchKernelGGL(k_rerank_flat<true>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
+    if (v4) {
+        const int64_t twgs = std::min<int64_t>((max_items + 3) / 4, kRerankTileGrid);
+        const dim3 tgrid((unsigned)((twgs + kNumXcd - 1) / kNumXcd * kNumXcd));
+        hipLaunchKernelGGL(k_rerank_tile, tgrid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
                            n_items, bound, kept);
-    else
+    } else
         hipLaunchKernelGGL(k_rerank_flat<false>, grid, dim3(256), 0, s, Qp, d, rows_by_pos, ids, plan, surv, cap, items,
                            n_items, bound, kept);
 }

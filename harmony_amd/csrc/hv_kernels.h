@@ -211,6 +211,7 @@ void launch_bf16_compact(unsigned long long* surv, float* aux, unsigned int* cnt
 //  launch_select_kept: the first k kept keys of every non-overflowed query into D/I (done: nq
 //    flags of scratch).
 constexpr int kRerankGrid = 1536;  // the re-rank's resident workgroups (76 VGPRs: 6 wavefronts per SIMD)
+constexpr int kRerankTileGrid = 1024;  // the LDS-staged re-rank's (36 KB of LDS: 4 workgroups per CU)
 void launch_rerank_items(const unsigned int* cnt, int64_t nq, int cap, const int* pairs, const int* pair_slot,
                          const int* npairs, int64_t max_pairs, int nprobe, int2* items, unsigned int* n_items,
                          int* overflow_list, int* overflow_cnt, unsigned long long* stats, hipStream_t s);
