@@ -326,6 +326,9 @@ def roofline_of(name, st, d, dpad, touched_vecs, slots):
                              "launches": int(st["scan_launches"]), "pairs": int(st["scan_pairs"])}
         else:
             s = 1 if fp8 else 2
+            # one launch is the nearest-list pass alone (every searched rank in one pass); more add
+            # the filter scans of the other ranks
+            scan_kernel = "k_mfma_nearest" if int(st["scan_launches"]) == 1 else "k_mfma_nearest+k_mfma_scan"
             nbytes = touched_vecs * (s * dpad + (4 if fp8 else 8)) + slots * s * dpad
             flops = 2.0 * d * st["scan_pairs"]
             peak_f = 5000.0 if fp8 else PEAK_BF16_TFLOPS
@@ -333,13 +336,13 @@ def roofline_of(name, st, d, dpad, touched_vecs, slots):
             t_mfma = flops / (peak_f * 1e12)
             if t_hbm >= t_mfma:
                 ach = nbytes / t_scan / 1e9
-                cands["scan"] = {"bound": "hbm", "kernel": "k_mfma_nearest+k_mfma_scan", "achieved": round(ach, 1),
+                cands["scan"] = {"bound": "hbm", "kernel": scan_kernel, "achieved": round(ach, 1),
                                  "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 5),
                                  "algorithmic_bytes_per_step": nbytes, "touched_vectors": touched_vecs,
                                  "mfma_frac": round(flops / t_scan / 1e12 / peak_f, 6)}
             else:
                 ach = flops / t_scan / 1e12
-                cands["scan"] = {"bound": "mfma", "kernel": "k_mfma_nearest+k_mfma_scan", "achieved": round(ach, 3),
+                cands["scan"] = {"bound": "mfma", "kernel": scan_kernel, "achieved": round(ach, 3),
                                  "peak": peak_f,
                                  "unit": "TFLOP/s", "frac": round(ach / peak_f, 5), "algorithmic_flops_per_step": flops}
             cands["scan"].update({"kernel_ms": round(st["ms_scan_kernels"], 4), "launches": int(st["scan_launches"]),
@@ -347,15 +350,18 @@ def roofline_of(name, st, d, dpad, touched_vecs, slots):
     if st["ms_rerank"] > 0:
         rbytes = (4.0 * d + 20.0) * st["rerank_pairs"]
         gbs = rbytes / (st["ms_rerank"] / 1000.0) / 1e9
-        cands["rerank"] = {"bound": "hbm", "kernel": "k_rerank_flat", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+        # rows staged through LDS when d and every chunk start are multiples of 4 (the dims here)
+        rk = "k_rerank_tile" if d % 4 == 0 else "k_rerank_flat"
+        cands["rerank"] = {"bound": "hbm", "kernel": rk, "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
                            "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 5),
                            "algorithmic_bytes_per_step": rbytes, "kernel_ms": round(st["ms_rerank"], 4),
                            "launches": 1, "pairs": int(st["rerank_pairs"])}
     if not cands:
         return None, {}
     # the dominant kernel: one kernel's launches (the plain scan's warm-up and main launches are
-    # one kernel; the MFMA forms' scan figure mixes two, so the re-rank stands for those forms)
-    single = {c: v for c, v in cands.items() if plain or c != "scan"} or cands
+    # one kernel; an MFMA form's scan figure of several launches mixes two kernels, so the re-rank
+    # stands for it then)
+    single = {c: v for c, v in cands.items() if plain or c != "scan" or v["launches"] == 1} or cands
     dom = max(single, key=lambda c: single[c]["kernel_ms"] / max(1, single[c]["launches"]))
     return dict(cands[dom]), cands
 
