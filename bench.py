@@ -159,6 +159,7 @@ def bench_skewed(args):
         "config": {"workload": "skewed-420d: %d x %d fp32, per-dimension scale exp(N(0,1)), nlist %d, %d queries, "
                                "nprobe %d, k %d, B=%d in variance order" % (nb, d, nlist, nq, nprobe, k, B),
                    "form": best, "forms": forms},
+        "roofline": forms[best].get("roofline"),
     }
     print(json.dumps(out), flush=True)
 
@@ -209,16 +210,36 @@ def bench_uneven(args):
             times.append(time.perf_counter() - t0)
         ms = 1000.0 * float(np.mean(times))
         res = I_dev.to_host(np.int64, (nq, kmax))
+        # kernel timings from eager calls with the same tuned shapes (graph replays report the
+        # device work as one figure), for the roofline of the step's dominant kernel
+        index._params.graph = 0
+        index._push()
+        ests = [index.search_device(q_dev, nq, kmax, D_dev, I_dev) for _ in range(3)]
+        index._params.graph = 1
+        index._push()
+        avg = {kk: float(np.mean([e[kk] for e in ests])) for kk in ests[0]}
+        tv, slots = touched(index, sizes)
+        roof, cands = roofline_of(name, avg, d, (d + 31) // 32 * 32, tv, slots)
+        if cands:
+            # here the grouped rank phases are most of the step: the kernel taking the most time,
+            # even where its figure sums the nearest-list pass and the filter scans' launches
+            roof = dict(max(cands.values(), key=lambda c: c["kernel_ms"]))
         if name == "plain-fp32":
             ref = res
         rec = float(np.mean([metrics.recall(res[k_q == kk][:, :kk], ref[k_q == kk][:, :kk])
                              for kk in np.unique(k_q)]))
         forms[name] = {"qps": round(nq / (ms / 1000.0), 1), "ms_per_step": round(ms, 3),
-                       "recall_vs_plain": round(rec, 6), "autotuned": tuned, "calls_per_step": 1}
+                       "recall_vs_plain": round(rec, 6), "autotuned": tuned, "calls_per_step": 1,
+                       "roofline": roof, "kernels": cands, "touched_vectors": tv,
+                       "stage_ms": {kk: round(avg[kk], 4) for kk in ("ms_coarse", "ms_group", "ms_scan",
+                                                                     "ms_select", "ms_scan_kernels", "ms_rerank")}}
         if prec == HV_PRECISION_FP8:
             forms[name]["rerank_r"] = r
         log("%s: %.3f ms per step (%d queries, one graph launch), %.0f QPS, recall vs plain %.5f"
             % (name, ms, nq, nq / (ms / 1000.0), rec))
+        if roof:
+            log("%s: dominant %s %.3f ms (%.1f%% of its peak)" % (name, roof["kernel"], roof["kernel_ms"],
+                                                               100.0 * roof["frac"]))
     index.set_query_nprobe(None)
     index.set_query_k(None)
     best = max(forms, key=lambda f: forms[f]["qps"] if forms[f]["recall_vs_plain"] >= 0.95 else 0)
@@ -232,6 +253,7 @@ def bench_uneven(args):
                                "round(exp U[ln 8, ln 512]) (mean %.1f) and k in {10, 100} in one search call, "
                                "B=%d, graph mode" % (nb, d, nlist, nq, float(nprobe_q.mean()), B),
                    "form": best, "forms": forms},
+        "roofline": forms[best].get("roofline"),
     }
     print(json.dumps(out), flush=True)
 
